@@ -1,0 +1,413 @@
+"""Python binding of libgba_amd.so (MI355X bundle-adjustment LM solver).
+
+Mirrors the reference's C++ API (gba::BALProblem, gba::LMOptions,
+gba::LMResult, gba::solve_bal_gpu, ...) over the C ABI in include/gba_c.h,
+with numpy arrays for the problem's structure of arrays.  The library must be
+built (gpu-bundle-adjustment_amd/build.py); loading fails loudly otherwise.
+"""
+import ctypes
+import dataclasses
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgba_amd.so")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+CONVERGENCE, MAX_ITERATIONS, FAILURE = 0, 1, 2
+TERMINATION = {CONVERGENCE: "CONVERGENCE", MAX_ITERATIONS: "MAX_ITERATIONS", FAILURE: "FAILURE"}
+PRECISION_FP64, PRECISION_MIXED = 0, 1
+TRACE_STRIDE = 6
+TRIAL_STATUS = {0: "accept", 1: "reject", 2: "nonspd_v", 3: "chol_fail", 4: "denom", 5: "nonfinite"}
+KERNELS = {"edge_point": 0, "edge_camera": 1, "schur": 2, "factor": 3, "solve": 4, "backsub": 5,
+           "cost": 6}
+
+
+class ProblemView(ctypes.Structure):
+    _fields_ = [("num_cameras", ctypes.c_int), ("num_points", ctypes.c_int),
+                ("num_observations", ctypes.c_int), ("cameras", _dp), ("points", _dp),
+                ("obs_camera", _ip), ("obs_point", _ip), ("obs_pixel", _dp)]
+
+
+class LMOptionsC(ctypes.Structure):
+    _fields_ = [("max_iterations", ctypes.c_int), ("initial_lambda", ctypes.c_double),
+                ("rel_decrease_tol", ctypes.c_double), ("max_inner_trials", ctypes.c_int),
+                ("verbose", ctypes.c_int)]
+
+
+class LMResultC(ctypes.Structure):
+    _fields_ = [("initial_cost", ctypes.c_double), ("final_cost", ctypes.c_double),
+                ("iterations", ctypes.c_int), ("termination", ctypes.c_int),
+                ("anomalies", ctypes.c_ulonglong), ("total_time_s", ctypes.c_double),
+                ("trials", ctypes.c_int), ("solve_time_s", ctypes.c_double),
+                ("setup_time_s", ctypes.c_double), ("refine_iterations", ctypes.c_int),
+                ("max_refine_residual", ctypes.c_double)]
+
+
+class SolverConfigC(ctypes.Structure):
+    _fields_ = [("precision", ctypes.c_int), ("refine_tol", ctypes.c_double),
+                ("refine_max_iters", ctypes.c_int), ("trace", _dp), ("trace_capacity", ctypes.c_int)]
+
+
+class SynthSpecC(ctypes.Structure):
+    _fields_ = [("num_cameras", ctypes.c_int), ("num_points", ctypes.c_int),
+                ("num_observations", ctypes.c_int), ("kmin", ctypes.c_int), ("kmax", ctypes.c_int),
+                ("law", ctypes.c_int), ("alpha", ctypes.c_double), ("ring_radius", ctypes.c_double),
+                ("pixel_noise", ctypes.c_double), ("perturb", ctypes.c_double)]
+
+
+# exported symbols and their prototypes (include/gba_c.h)
+_PROTOS = {
+    "gba_default_options": (None, [ctypes.POINTER(LMOptionsC)]),
+    "gba_default_config": (None, [ctypes.POINTER(SolverConfigC)]),
+    "gba_solve_bal": (ctypes.c_int, [ctypes.POINTER(ProblemView), ctypes.POINTER(LMOptionsC),
+                                     ctypes.POINTER(SolverConfigC), ctypes.POINTER(LMResultC)]),
+    "gba_solver_create": (ctypes.c_int, [ctypes.POINTER(ProblemView), ctypes.POINTER(SolverConfigC),
+                                         ctypes.POINTER(ctypes.c_void_p)]),
+    "gba_solver_solve": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(LMOptionsC),
+                                        ctypes.POINTER(LMResultC)]),
+    "gba_solver_params": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp]),
+    "gba_solver_time_kernel": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_double,
+                                              ctypes.c_int, _dp]),
+    "gba_solver_stats": (ctypes.c_int, [ctypes.c_void_p, _dp, _ip, _ip]),
+    "gba_solver_destroy": (None, [ctypes.c_void_p]),
+    "gba_linearize": (ctypes.c_int, [ctypes.POINTER(ProblemView), ctypes.c_int, _dp, _dp, _dp, _dp,
+                                     _dp, _dp, ctypes.POINTER(ctypes.c_ulonglong)]),
+    "gba_schur_step": (ctypes.c_int, [ctypes.POINTER(ProblemView), ctypes.c_int, ctypes.c_double, _dp,
+                                      _dp, _dp, _ip, _dp, _dp]),
+    "gba_cost": (ctypes.c_int, [ctypes.POINTER(ProblemView), _dp, ctypes.POINTER(ctypes.c_ulonglong)]),
+    "gba_dense_spd_solve": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _dp, _dp, _ip, _ip]),
+    "gba_batch_chol9_factor_solve": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                                    ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]),
+    "gba_batch_inverse3_spd": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                              ctypes.c_int, ctypes.c_void_p]),
+    "gba_snavely_residual": (None, [_dp, _dp, _dp, _dp]),
+    "gba_total_squared_error": (ctypes.c_double, [ctypes.POINTER(ProblemView)]),
+    "gba_edge_residual_jacobian_host": (None, [_dp, _dp, _dp, _dp, _dp, _dp]),
+    "gba_bal_header": (ctypes.c_int, [ctypes.c_char_p, _ip, _ip, _ip]),
+    "gba_load_bal": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ProblemView)]),
+    "gba_write_bal": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ProblemView)]),
+    "gba_synth_preset": (ctypes.c_int, [ctypes.c_int, ctypes.c_ulonglong, ctypes.POINTER(SynthSpecC)]),
+    "gba_synth_generate": (ctypes.c_int, [ctypes.POINTER(SynthSpecC), ctypes.c_ulonglong,
+                                          ctypes.POINTER(ProblemView)]),
+    "gba_synth_launch_bounds": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_ulonglong,
+                                               ctypes.POINTER(ProblemView)]),
+    "gba_last_error": (ctypes.c_char_p, []),
+    "gba_version": (ctypes.c_char_p, []),
+}
+
+_lib = None
+
+
+def lib():
+    """The loaded libgba_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with gpu-bundle-adjustment_amd/build.py")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _PROTOS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+class GBAError(RuntimeError):
+    pass
+
+
+def _check(rc):
+    if rc != 0:
+        raise GBAError(lib().gba_last_error().decode())
+
+
+def _ptr(a, ct=ctypes.c_double):
+    return a.ctypes.data_as(ctypes.POINTER(ct))
+
+
+@dataclasses.dataclass
+class BALProblem:
+    """Structure of arrays, as gba::BALProblem (reference bal_problem.hpp:15-31)."""
+    cameras: np.ndarray     # (C, 9) float64
+    points: np.ndarray      # (P, 3) float64
+    obs_camera: np.ndarray  # (O,) int32
+    obs_point: np.ndarray   # (O,) int32
+    obs_pixel: np.ndarray   # (O, 2) float64
+
+    @property
+    def num_cameras(self):
+        return self.cameras.shape[0]
+
+    @property
+    def num_points(self):
+        return self.points.shape[0]
+
+    @property
+    def num_observations(self):
+        return self.obs_camera.shape[0]
+
+    def copy(self):
+        return BALProblem(*(np.array(a, copy=True) for a in dataclasses.astuple(self)))
+
+    def normalize(self):
+        self.cameras = np.ascontiguousarray(self.cameras, dtype=np.float64).reshape(-1, 9)
+        self.points = np.ascontiguousarray(self.points, dtype=np.float64).reshape(-1, 3)
+        self.obs_camera = np.ascontiguousarray(self.obs_camera, dtype=np.int32).reshape(-1)
+        self.obs_point = np.ascontiguousarray(self.obs_point, dtype=np.int32).reshape(-1)
+        self.obs_pixel = np.ascontiguousarray(self.obs_pixel, dtype=np.float64).reshape(-1, 2)
+        return self
+
+    def view(self):
+        self.normalize()
+        return ProblemView(self.num_cameras, self.num_points, self.num_observations,
+                           _ptr(self.cameras), _ptr(self.points), _ptr(self.obs_camera, ctypes.c_int),
+                           _ptr(self.obs_point, ctypes.c_int), _ptr(self.obs_pixel))
+
+    @staticmethod
+    def empty(C, P, O):
+        return BALProblem(np.zeros((C, 9)), np.zeros((P, 3)), np.zeros(O, np.int32),
+                          np.zeros(O, np.int32), np.zeros((O, 2)))
+
+
+@dataclasses.dataclass
+class LMOptions:
+    """gba::LMOptions (reference lm.hpp:13-19)."""
+    max_iterations: int = 50
+    initial_lambda: float = 1e-3
+    rel_decrease_tol: float = 1e-6
+    max_inner_trials: int = 16
+    verbose: bool = False
+
+    def c(self):
+        return LMOptionsC(self.max_iterations, self.initial_lambda, self.rel_decrease_tol,
+                          self.max_inner_trials, int(self.verbose))
+
+
+@dataclasses.dataclass
+class LMResult:
+    """gba::LMResult (reference lm.hpp:21-29) plus timing / refinement fields."""
+    initial_cost: float
+    final_cost: float
+    iterations: int
+    termination: str
+    anomalies: int
+    total_time_s: float
+    trials: int = 0
+    solve_time_s: float = 0.0
+    setup_time_s: float = 0.0
+    refine_iterations: int = 0
+    max_refine_residual: float = 0.0
+    trace: np.ndarray = None
+
+    @staticmethod
+    def from_c(r, trace=None):
+        return LMResult(r.initial_cost, r.final_cost, r.iterations, TERMINATION[r.termination],
+                        r.anomalies, r.total_time_s, r.trials, r.solve_time_s, r.setup_time_s,
+                        r.refine_iterations, r.max_refine_residual, trace)
+
+
+def _config(precision, trace_capacity=0, refine_tol=1e-10, refine_max_iters=12):
+    c = SolverConfigC()
+    lib().gba_default_config(ctypes.byref(c))
+    c.precision = precision
+    c.refine_tol = refine_tol
+    c.refine_max_iters = refine_max_iters
+    trace = None
+    if trace_capacity:
+        trace = np.full((trace_capacity, TRACE_STRIDE), np.nan)
+        c.trace = _ptr(trace)
+        c.trace_capacity = trace_capacity
+    return c, trace
+
+
+def _trim_trace(trace):
+    if trace is None:
+        return None
+    keep = ~np.isnan(trace[:, 4])
+    return trace[keep]
+
+
+def solve_bal_gpu(prob, opts=None, precision=PRECISION_FP64, trace_capacity=1024):
+    """gba::solve_bal_gpu: updates prob.cameras / prob.points in place."""
+    opts = opts or LMOptions()
+    cfg, trace = _config(precision, trace_capacity)
+    res = LMResultC()
+    v = prob.view()
+    _check(lib().gba_solve_bal(ctypes.byref(v), ctypes.byref(opts.c()), ctypes.byref(cfg),
+                               ctypes.byref(res)))
+    return LMResult.from_c(res, _trim_trace(trace))
+
+
+class Solver:
+    """Device-resident solver handle: upload once, solve repeatedly."""
+
+    def __init__(self, prob, precision=PRECISION_FP64, trace_capacity=1024):
+        self._cfg, self._trace = _config(precision, trace_capacity)
+        self._h = ctypes.c_void_p()
+        self._prob = prob
+        v = prob.view()
+        _check(lib().gba_solver_create(ctypes.byref(v), ctypes.byref(self._cfg), ctypes.byref(self._h)))
+
+    def solve(self, opts=None):
+        opts = opts or LMOptions()
+        if self._trace is not None:
+            self._trace[:] = np.nan
+        res = LMResultC()
+        _check(lib().gba_solver_solve(self._h, ctypes.byref(opts.c()), ctypes.byref(res)))
+        return LMResult.from_c(res, _trim_trace(None if self._trace is None else self._trace.copy()))
+
+    def params(self):
+        cams = np.zeros((self._prob.num_cameras, 9))
+        pts = np.zeros((self._prob.num_points, 3))
+        _check(lib().gba_solver_params(self._h, _ptr(cams), _ptr(pts)))
+        return cams, pts
+
+    def time_kernel(self, kernel, lam=1e-4, reps=10):
+        ms = ctypes.c_double()
+        _check(lib().gba_solver_time_kernel(self._h, KERNELS[kernel], lam, reps, ctypes.byref(ms)))
+        return ms.value
+
+    def stats(self):
+        pc, n, npad = ctypes.c_double(), ctypes.c_int(), ctypes.c_int()
+        _check(lib().gba_solver_stats(self._h, ctypes.byref(pc), ctypes.byref(n), ctypes.byref(npad)))
+        return {"pair_count": pc.value, "n": n.value, "n_padded": npad.value}
+
+    def close(self):
+        if self._h:
+            lib().gba_solver_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def linearize(prob, precision=PRECISION_FP64):
+    """GPU linearisation in the reference layout: U (C,9,9 lower), V (P,3,3 lower),
+    W (O,9,3), bc (C,9), bp (P,3), sse, anomalies."""
+    C, P, O = prob.num_cameras, prob.num_points, prob.num_observations
+    U, V, W = np.zeros((C, 9, 9)), np.zeros((P, 3, 3)), np.zeros((O, 9, 3))
+    bc, bp = np.zeros((C, 9)), np.zeros((P, 3))
+    sse, an = ctypes.c_double(), ctypes.c_ulonglong()
+    v = prob.view()
+    _check(lib().gba_linearize(ctypes.byref(v), precision, _ptr(U), _ptr(V), _ptr(W), _ptr(bc), _ptr(bp),
+                               ctypes.byref(sse), ctypes.byref(an)))
+    return dict(U=U, V=V, W=W, bc=bc, bp=bp, sse=sse.value, anomalies=an.value)
+
+
+def schur_step(prob, lam, precision=PRECISION_FP64, want_S=False):
+    C, P = prob.num_cameras, prob.num_points
+    n = 9 * C
+    dc, dp = np.zeros(n), np.zeros((P, 3))
+    den, st = ctypes.c_double(), ctypes.c_int()
+    S = np.zeros((n, n), order="F") if want_S else None
+    rhs = np.zeros(n) if want_S else None
+    v = prob.view()
+    _check(lib().gba_schur_step(ctypes.byref(v), precision, lam, _ptr(dc), _ptr(dp), ctypes.byref(den),
+                                ctypes.byref(st), _ptr(S) if want_S else None,
+                                _ptr(rhs) if want_S else None))
+    return dict(dc=dc, dp=dp, denom=den.value, status=TRIAL_STATUS[st.value], S=S, rhs=rhs)
+
+
+def cost(prob):
+    sse, an = ctypes.c_double(), ctypes.c_ulonglong()
+    v = prob.view()
+    _check(lib().gba_cost(ctypes.byref(v), ctypes.byref(sse), ctypes.byref(an)))
+    return sse.value, an.value
+
+
+def dense_spd_solve(A, b, precision=PRECISION_FP64):
+    A = np.asfortranarray(A, dtype=np.float64)
+    x = np.array(b, dtype=np.float64, copy=True)
+    ok, it = ctypes.c_int(), ctypes.c_int()
+    _check(lib().gba_dense_spd_solve(A.shape[0], precision, _ptr(A), _ptr(x), ctypes.byref(ok),
+                                     ctypes.byref(it)))
+    return x, bool(ok.value), it.value
+
+
+def batch_chol9_factor_solve(A, b):
+    """A (n,9,9), b (n,9) torch CUDA float64 tensors -> (x, ok)."""
+    import torch
+    n = A.shape[0]
+    x = torch.zeros_like(b)
+    ok = torch.zeros(n, dtype=torch.int32, device=A.device)
+    stream = torch.cuda.current_stream(A.device).cuda_stream
+    _check(lib().gba_batch_chol9_factor_solve(A.data_ptr(), b.data_ptr(), x.data_ptr(), ok.data_ptr(),
+                                              n, stream))
+    return x, ok
+
+
+def batch_inverse3_spd(A):
+    import torch
+    n = A.shape[0]
+    Ainv = torch.zeros_like(A)
+    ok = torch.zeros(n, dtype=torch.int32, device=A.device)
+    stream = torch.cuda.current_stream(A.device).cuda_stream
+    _check(lib().gba_batch_inverse3_spd(A.data_ptr(), Ainv.data_ptr(), ok.data_ptr(), n, stream))
+    return Ainv, ok
+
+
+def snavely_residual(cam, X, obs):
+    out = np.zeros(2)
+    cam, X, obs = (np.ascontiguousarray(a, dtype=np.float64) for a in (cam, X, obs))
+    lib().gba_snavely_residual(_ptr(cam), _ptr(X), _ptr(obs), _ptr(out))
+    return out
+
+
+def total_squared_error(prob):
+    v = prob.view()
+    return lib().gba_total_squared_error(ctypes.byref(v))
+
+
+def edge_residual_jacobian_host(cam, X, obs):
+    r, Jc, Jp = np.zeros(2), np.zeros((2, 9)), np.zeros((2, 3))
+    cam, X, obs = (np.ascontiguousarray(a, dtype=np.float64) for a in (cam, X, obs))
+    lib().gba_edge_residual_jacobian_host(_ptr(cam), _ptr(X), _ptr(obs), _ptr(r), _ptr(Jc), _ptr(Jp))
+    return r, Jc, Jp
+
+
+def load_bal(path):
+    C, P, O = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    _check(lib().gba_bal_header(str(path).encode(), ctypes.byref(C), ctypes.byref(P), ctypes.byref(O)))
+    prob = BALProblem.empty(C.value, P.value, O.value)
+    v = prob.view()
+    _check(lib().gba_load_bal(str(path).encode(), ctypes.byref(v)))
+    return prob
+
+
+def write_bal(path, prob):
+    v = prob.view()
+    _check(lib().gba_write_bal(str(path).encode(), ctypes.byref(v)))
+
+
+def synth_spec(config, seed=1):
+    s = SynthSpecC()
+    _check(lib().gba_synth_preset(config, seed, ctypes.byref(s)))
+    return s
+
+
+def generate(config=1, seed=1, spec=None):
+    """Seeded synthetic problem of a preset shape (1..5) or of a given spec."""
+    s = spec if spec is not None else synth_spec(config, seed)
+    prob = BALProblem.empty(s.num_cameras, s.num_points, s.num_observations)
+    v = prob.view()
+    _check(lib().gba_synth_generate(ctypes.byref(s), seed, ctypes.byref(v)))
+    return prob
+
+
+def generate_launch_bounds(C=8, P=400000, seed=7):
+    prob = BALProblem.empty(C, P, 2 * P)
+    v = prob.view()
+    _check(lib().gba_synth_launch_bounds(C, P, seed, ctypes.byref(v)))
+    return prob
+
+
+def version():
+    return lib().gba_version().decode()

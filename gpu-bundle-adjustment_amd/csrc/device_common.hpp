@@ -1,0 +1,159 @@
+// device_common.hpp -- device layout of a problem and wave64 helpers.
+//
+// HBM layout of one problem (all arrays device resident for the whole solve):
+//   observations sorted by point (stable in the caller's order):
+//     scam[O] int, spt[O] int, spx[O] double2, perm[O] (sorted -> caller index)
+//     pt_off[P+1]           edges of point p are [pt_off[p], pt_off[p+1])
+//   point-pass work units (int2 edge ranges): either whole points packed into
+//     at most 64 edges (one wave, one lane per edge) or a single point with more
+//     than 64 edges (one wave looping over it), so every per-point reduction
+//     stays inside one wavefront: no atomics, fixed summation order.
+//   camera-major index: cam_list[O] (sorted-edge indices grouped by camera,
+//     ascending point), cam_off[C+1]; camera chunks (int4: camera, begin,
+//     end, 0) of at most kCamChunk entries of cam_list, cam_chunk_off[C+1].
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace gba_amd {
+
+constexpr int kWave = 64;
+constexpr int kCamChunk = 256;  // camera-pass edges per wave
+constexpr int kTile = 64;       // dense factor tile edge
+
+struct DevProblem {
+    int C, P, O;
+    int n, npad;  // 9C and 9C rounded up to kTile
+    int num_units, num_chunks;
+    const int* scam;
+    const int* spt;
+    const double2* spx;
+    const int* perm;
+    const int* pt_off;
+    const int2* units;
+    const int* cam_list;
+    const int* cam_off;
+    const int4* chunks;
+    const int* cam_chunk_off;
+};
+
+// Per-trial device flags, cleared by a fill kernel at the start of a trial.
+struct TrialFlags {
+    int nonspd;     // damped V blocks that are not SPD
+    int chol_fail;  // dense factor met a pivot that is not > 0
+    int solve_err;  // triangular solve gave up waiting (never expected)
+    int pad;
+};
+
+// Reduced results of a pass, copied to pinned host memory once per trial.
+struct PassStatus {
+    double sse;        // sum of squared residuals (finite edges)
+    double denom;      // gain-ratio denominator (trial pass only)
+    double res_norm2;  // refinement residual norm^2 (mixed precision)
+    double rhs_norm2;  // refinement right-hand side norm^2
+    unsigned long long anomalies;  // non-finite edges in this pass
+    TrialFlags flags;
+};
+
+__device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, kWave);
+    return v;
+}
+__device__ __forceinline__ unsigned wave_sum_u(unsigned v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, kWave);
+    return v;
+}
+
+// Segmented suffix sum over lanes with non-decreasing keys inside the active
+// prefix (inactive lanes carry a key no active lane has): afterwards the first
+// lane of every segment holds the segment total.
+template <int N>
+__device__ __forceinline__ void seg_suffix_sum(double (&v)[N], int key) {
+    const int lane = lane_id();
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const int okey = __shfl_down(key, d, kWave);
+        const bool take = (lane + d < kWave) && okey == key;
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            const double o = __shfl_down(v[i], d, kWave);
+            if (take) v[i] += o;
+        }
+    }
+}
+
+// Segmented all-reduce: every lane ends with its segment's total.
+template <int N>
+__device__ __forceinline__ void seg_allreduce(double (&v)[N], int key) {
+    const int lane = lane_id();
+    double suf[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) suf[i] = v[i];
+    seg_suffix_sum<N>(suf, key);
+    // prefix (exclusive of suffix overlap): inclusive prefix minus own value
+    double pre[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) pre[i] = v[i];
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const int okey = __shfl_up(key, d, kWave);
+        const bool take = (lane >= d) && okey == key;
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            const double o = __shfl_up(pre[i], d, kWave);
+            if (take) pre[i] += o;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < N; ++i) v[i] = pre[i] + suf[i] - v[i];
+}
+
+__device__ __forceinline__ double damp_diag(double x) { return x > 0.0 ? x : 1e-12; }
+
+// Inverse of a symmetric 3x3 given by its lower triangle
+// (a00, a10, a11, a20, a21, a22) through A^-1 = L^-T L^-1; returns false when a
+// pivot is not > 0 (the reference's SPD test, include/gba/schur.cuh:66-96).
+__device__ __forceinline__ bool spd3_inverse(const double* a, double* inv6) {
+    const double l00s = a[0];
+    if (!(l00s > 0.0)) return false;
+    const double l00 = sqrt(l00s);
+    const double l10 = a[1] / l00;
+    const double l20 = a[3] / l00;
+    const double l11s = a[2] - l10 * l10;
+    if (!(l11s > 0.0)) return false;
+    const double l11 = sqrt(l11s);
+    const double l21 = (a[4] - l20 * l10) / l11;
+    const double l22s = a[5] - l20 * l20 - l21 * l21;
+    if (!(l22s > 0.0)) return false;
+    const double l22 = sqrt(l22s);
+    // M = L^-1 (lower)
+    const double m00 = 1.0 / l00, m11 = 1.0 / l11, m22 = 1.0 / l22;
+    const double m10 = -l10 * m00 / l11;
+    const double m21 = -l21 * m11 / l22;
+    const double m20 = -(l20 * m00 + l21 * m10) / l22;
+    // A^-1 = M^T M
+    inv6[0] = m00 * m00 + m10 * m10 + m20 * m20;
+    inv6[1] = m11 * m10 + m21 * m20;
+    inv6[2] = m11 * m11 + m21 * m21;
+    inv6[3] = m22 * m20;
+    inv6[4] = m22 * m21;
+    inv6[5] = m22 * m22;
+    return true;
+}
+
+// Bounded wait for a device-scope flag written by another workgroup.
+// Returns false when the limit is hit (the caller records an error and exits).
+__device__ __forceinline__ bool wait_flag(const int* flag, int want) {
+    for (long spins = 0; spins < (1L << 20); ++spins) {
+        if (__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) >= want) return true;
+        __builtin_amdgcn_s_sleep(2);
+    }
+    return false;
+}
+
+}  // namespace gba_amd

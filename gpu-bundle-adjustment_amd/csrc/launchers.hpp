@@ -1,0 +1,76 @@
+// launchers.hpp -- host-side launch wrappers for the HIP kernels.  Every
+// wrapper enqueues on the given stream and returns immediately.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "device_common.hpp"
+
+namespace gba_amd {
+
+// k_linearize.hip
+void launch_lin_points(hipStream_t st, const DevProblem& pb, const double* cams, const double* pts,
+                       double* W64, float* W32, double* VB, double* unit_sse, unsigned* unit_anom);
+void launch_lin_cameras(hipStream_t st, const DevProblem& pb, const double* cams, const double* pts,
+                        double* gram_part);
+void launch_lin_reduce(hipStream_t st, const DevProblem& pb, const double* gram_part,
+                       const double* unit_sse, const unsigned* unit_anom, double* U, double* bc,
+                       PassStatus* status);
+void launch_cost(hipStream_t st, const DevProblem& pb, const double* cams, const double* pts,
+                 double* blk_sse, unsigned* blk_anom, PassStatus* status);
+int cost_partials(const DevProblem& pb);
+
+// k_blocks.hip
+void launch_clear_flags(hipStream_t st, TrialFlags* flags);
+void launch_fill_int(hipStream_t st, int* p, int n, int v);
+void launch_fill_double(hipStream_t st, double* p, size_t n, double v);
+void launch_damp_invert_v(hipStream_t st, int P, const double* VB, double lambda, double* Vinv6,
+                          TrialFlags* flags);
+void launch_batch_chol9(hipStream_t st, const double* A, const double* b, double* x, int* ok, int n);
+void launch_batch_inverse3(hipStream_t st, const double* A, double* Ainv, int* ok, int n);
+void launch_identity_pad(hipStream_t st, double* S64, float* S32, int ld, int n, int npad);
+
+// k_schur.hip
+void launch_schur(hipStream_t st, const DevProblem& pb, const double* W64, const float* W32,
+                  const double* Vinv6, const double* VB, const double* U, const double* bc,
+                  double lambda, const double* dscale, double* S64, float* S32fac, float* S32full,
+                  int ld, double* rhs, const TrialFlags* flags);
+void launch_diag_scale(hipStream_t st, int n, int npad, const double* U, double lambda,
+                       double* dscale);
+
+// k_dense.hip
+// Factors in place; the diagonal tiles of L go to D (nblk tiles of 64x64,
+// row-major, upper part zero), the off-diagonal tiles replace S's.
+void launch_cholesky(hipStream_t st, double* S64, float* S32, int ld, int nblk, double* D64,
+                     float* D32, TrialFlags* flags);
+void launch_cholesky_panel(hipStream_t st, double* S64, float* S32, int ld, int nblk, int k,
+                           double* D64, float* D32, TrialFlags* flags);
+void launch_cholesky_update(hipStream_t st, double* S64, float* S32, int ld, int nblk, int k,
+                            TrialFlags* flags);
+// Solves L L^T x = b (b, x length nblk*kTile; b and x may alias).  sync holds
+// 2*nblk + 2 ints of scratch (done flags and tickets).
+void launch_chol_solve(hipStream_t st, const double* L64, const float* L32, int ld, int nblk,
+                       const double* D64, const float* D32, const double* b, double* y, double* x,
+                       int* sync, TrialFlags* flags);
+void launch_residual(hipStream_t st, const float* A, int ld, int npad, const double* b,
+                     const double* x, double* r, double* blk_norm2, PassStatus* status,
+                     bool also_rhs_norm, const TrialFlags* flags);
+int residual_partials(int npad);
+void launch_vec_axpy(hipStream_t st, int n, double* x, const double* d);
+void launch_vec_scale(hipStream_t st, int n, int npad, const double* in, const double* s,
+                      double* out);
+
+// k_backsub.hip
+void launch_trial_cams(hipStream_t st, int C, const double* cams, const double* dc, double* tcams);
+void launch_backsub(hipStream_t st, const DevProblem& pb, const double* W64, const float* W32,
+                    const double* Vinv6, const double* VB, const double* dc, const double* pts,
+                    const double* tcams, double* tpts, double lambda, double* unit_sse,
+                    double* unit_den, unsigned* unit_anom, const TrialFlags* flags);
+void launch_trial_finalize(hipStream_t st, const DevProblem& pb, const double* unit_sse,
+                           const double* unit_den, const unsigned* unit_anom, const double* dc,
+                           const double* U, const double* bc, double lambda, PassStatus* status,
+                           const TrialFlags* flags);
+// dp for the first 3P entries of out (test helper): recomputes the point step.
+void launch_point_step(hipStream_t st, const DevProblem& pb, const double* W64, const float* W32,
+                       const double* Vinv6, const double* VB, const double* dc, double* dp);
+
+}  // namespace gba_amd

@@ -1,0 +1,559 @@
+// solver.cpp -- host driver of the device-resident LM solve (replaces
+// gba::solve_bal_gpu, reference src/lm.cu:371-724).
+//
+// Everything per iteration stays in HBM: linearisation, damped V inverse,
+// Schur assembly, dense factor, solves, back-substitution and trial cost run
+// as kernels on one stream; kernels after a failed stage exit early on device
+// flags, so a trial is enqueued without host round trips and the host reads one
+// small status block per trial (plus one per refinement step in mixed
+// precision).  The accept / reject logic, lambda and nu follow the reference
+// exactly (src/lm.cu:455-463, :672-714).
+#include "solver.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+
+#include "launchers.hpp"
+
+namespace gba_amd {
+
+void hip_check(hipError_t e, const char* what, const char* file, int line) {
+    if (e != hipSuccess)
+        throw Error(std::string("HIP error at ") + file + ":" + std::to_string(line) + ": " +
+                    hipGetErrorString(e) + " (" + what + ")");
+}
+
+namespace {
+
+double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+template <typename T>
+void upload(DevBuf<T>& b, const std::vector<T>& v, hipStream_t st) {
+    b.alloc(v.size());
+    if (!v.empty()) GBA_HIP(hipMemcpyAsync(b.p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, st));
+}
+
+}  // namespace
+
+Solver::Solver(const gba_problem_view& prob, const gba_solver_config& cfg) : cfg_(cfg) {
+    mixed_ = cfg.precision == GBA_PRECISION_MIXED;
+    const int C = prob.num_cameras, P = prob.num_points, O = prob.num_observations;
+    if (C <= 0 || P <= 0 || O <= 0) throw Error("problem sizes must be positive");
+    if (static_cast<long long>(9) * C > (1 << 20)) throw Error("too many cameras for a dense S");
+    for (int e = 0; e < O; ++e)
+        if (prob.obs_camera[e] < 0 || prob.obs_camera[e] >= C || prob.obs_point[e] < 0 ||
+            prob.obs_point[e] >= P)
+            throw Error("observation index out of range");
+    pb_.C = C;
+    pb_.P = P;
+    pb_.O = O;
+    pb_.n = 9 * C;
+    pb_.npad = (pb_.n + kTile - 1) / kTile * kTile;
+    nblk_ = pb_.npad / kTile;
+    init_cams_.assign(prob.cameras, prob.cameras + 9 * static_cast<size_t>(C));
+    init_pts_.assign(prob.points, prob.points + 3 * static_cast<size_t>(P));
+
+    // ---- host-side layout: point-major sort, work units, camera index ----
+    std::vector<int> pt_off(static_cast<size_t>(P) + 1, 0);
+    for (int e = 0; e < O; ++e) pt_off[prob.obs_point[e] + 1]++;
+    for (int p = 0; p < P; ++p) pt_off[p + 1] += pt_off[p];
+    std::vector<int> perm(static_cast<size_t>(O)), scam(static_cast<size_t>(O)), spt(static_cast<size_t>(O));
+    std::vector<double2> spx(static_cast<size_t>(O));
+    {
+        std::vector<int> fill(pt_off.begin(), pt_off.end() - 1);
+        for (int e = 0; e < O; ++e) {
+            const int k = fill[prob.obs_point[e]]++;
+            perm[k] = e;
+            scam[k] = prob.obs_camera[e];
+            spt[k] = prob.obs_point[e];
+            spx[k] = make_double2(prob.obs_pixel[2 * static_cast<size_t>(e)],
+                                  prob.obs_pixel[2 * static_cast<size_t>(e) + 1]);
+        }
+    }
+    std::vector<int2> units;
+    {
+        int b = -1, e_end = -1;
+        for (int p = 0; p < P; ++p) {
+            const int k0 = pt_off[p], k1 = pt_off[p + 1], k = k1 - k0;
+            if (k == 0) continue;
+            pair_count_ += 0.5 * static_cast<double>(k) * (k + 1);
+            if (k > kWave) {
+                if (b >= 0) units.push_back(make_int2(b, e_end));
+                units.push_back(make_int2(k0, k1));
+                b = -1;
+            } else if (b < 0) {
+                b = k0;
+                e_end = k1;
+            } else if (k1 - b > kWave) {
+                units.push_back(make_int2(b, e_end));
+                b = k0;
+                e_end = k1;
+            } else {
+                e_end = k1;
+            }
+        }
+        if (b >= 0) units.push_back(make_int2(b, e_end));
+    }
+    std::vector<int> cam_off(static_cast<size_t>(C) + 1, 0), cam_list(static_cast<size_t>(O));
+    for (int k = 0; k < O; ++k) cam_off[scam[k] + 1]++;
+    for (int c = 0; c < C; ++c) cam_off[c + 1] += cam_off[c];
+    {
+        std::vector<int> fill(cam_off.begin(), cam_off.end() - 1);
+        for (int k = 0; k < O; ++k) cam_list[fill[scam[k]]++] = k;
+    }
+    std::vector<int4> chunks;
+    std::vector<int> cam_chunk_off(static_cast<size_t>(C) + 1, 0);
+    for (int c = 0; c < C; ++c) {
+        cam_chunk_off[c] = static_cast<int>(chunks.size());
+        for (int b = cam_off[c]; b < cam_off[c + 1]; b += kCamChunk)
+            chunks.push_back(make_int4(c, b, std::min(b + kCamChunk, cam_off[c + 1]), 0));
+    }
+    cam_chunk_off[C] = static_cast<int>(chunks.size());
+    pb_.num_units = static_cast<int>(units.size());
+    pb_.num_chunks = static_cast<int>(chunks.size());
+
+    // ---- device buffers ----
+    GBA_HIP(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
+    upload(scam_, scam, st_);
+    upload(spt_, spt, st_);
+    upload(perm_, perm, st_);
+    upload(pt_off_, pt_off, st_);
+    upload(spx_, spx, st_);
+    upload(units_, units, st_);
+    upload(cam_list_, cam_list, st_);
+    upload(cam_off_, cam_off, st_);
+    upload(chunks_, chunks, st_);
+    upload(cam_chunk_off_, cam_chunk_off, st_);
+    pb_.scam = scam_.p;
+    pb_.spt = spt_.p;
+    pb_.spx = spx_.p;
+    pb_.perm = perm_.p;
+    pb_.pt_off = pt_off_.p;
+    pb_.units = units_.p;
+    pb_.cam_list = cam_list_.p;
+    pb_.cam_off = cam_off_.p;
+    pb_.chunks = chunks_.p;
+    pb_.cam_chunk_off = cam_chunk_off_.p;
+    for (int i = 0; i < 2; ++i) {
+        cams_[i].alloc(9 * static_cast<size_t>(C));
+        pts_[i].alloc(3 * static_cast<size_t>(P));
+    }
+    if (mixed_) W32_.alloc(27 * static_cast<size_t>(O));
+    else W64_.alloc(27 * static_cast<size_t>(O));
+    VB_.alloc(9 * static_cast<size_t>(P));
+    Vinv_.alloc(6 * static_cast<size_t>(P));
+    U_.alloc(81 * static_cast<size_t>(C));
+    bc_.alloc(9 * static_cast<size_t>(C));
+    const size_t nu = std::max(1, pb_.num_units);
+    unit_sse_.alloc(nu);
+    unit_den_.alloc(nu);
+    unit_anom_.alloc(nu);
+    gram_.alloc(55 * static_cast<size_t>(std::max(1, pb_.num_chunks)));
+    blk_sse_.alloc(static_cast<size_t>(cost_partials(pb_)) + 1);
+    blk_anom_.alloc(static_cast<size_t>(cost_partials(pb_)) + 1);
+    const size_t np = static_cast<size_t>(pb_.npad);
+    if (mixed_) {
+        S32_.alloc(np * np);
+        S32full_.alloc(np * np);
+        D32_.alloc(static_cast<size_t>(nblk_) * kTile * kTile);
+    } else {
+        S64_.alloc(np * np);
+        D64_.alloc(static_cast<size_t>(nblk_) * kTile * kTile);
+    }
+    rhs_.alloc(np);
+    y_.alloc(np);
+    x_.alloc(np);
+    dc_.alloc(np);
+    dscale_.alloc(np);
+    r_.alloc(np);
+    d_.alloc(np);
+    resblk_.alloc(2 * static_cast<size_t>(residual_partials(pb_.npad)));
+    sync_.alloc(2 * static_cast<size_t>(nblk_) + 2);
+    flags_.alloc(1);
+    status_.alloc(1);
+    GBA_HIP(hipHostMalloc(&h_status_, sizeof(PassStatus), hipHostMallocDefault));
+    std::memset(h_status_, 0, sizeof(PassStatus));
+    // points without observations keep V = 0, bp = 0; padding of S is identity;
+    // rhs padding stays zero
+    launch_fill_double(st_, VB_.p, VB_.n, 0.0);
+    launch_fill_double(st_, Vinv_.p, Vinv_.n, 0.0);
+    launch_fill_double(st_, rhs_.p, rhs_.n, 0.0);
+    launch_fill_double(st_, dc_.p, dc_.n, 0.0);
+    launch_fill_double(st_, x_.p, x_.n, 0.0);
+    if (mixed_) {
+        launch_identity_pad(st_, nullptr, S32_.p, pb_.npad, pb_.n, pb_.npad);
+        launch_identity_pad(st_, nullptr, S32full_.p, pb_.npad, pb_.n, pb_.npad);
+    } else {
+        launch_identity_pad(st_, S64_.p, nullptr, pb_.npad, pb_.n, pb_.npad);
+    }
+    launch_clear_flags(st_, flags_.p);
+    reset_params();
+    GBA_HIP(hipGetLastError());
+    GBA_HIP(hipStreamSynchronize(st_));
+}
+
+Solver::~Solver() {
+    if (st_) (void)hipStreamSynchronize(st_);
+    if (h_status_) (void)hipHostFree(h_status_);
+    if (st_) (void)hipStreamDestroy(st_);
+}
+
+void Solver::reset_params(const double* cams, const double* pts) {
+    cur_ = 0;
+    GBA_HIP(hipMemcpyAsync(cams_[0].p, cams ? cams : init_cams_.data(), sizeof(double) * cams_[0].n,
+                           hipMemcpyHostToDevice, st_));
+    GBA_HIP(hipMemcpyAsync(pts_[0].p, pts ? pts : init_pts_.data(), sizeof(double) * pts_[0].n,
+                           hipMemcpyHostToDevice, st_));
+}
+
+void Solver::sync_status() {
+    GBA_HIP(hipGetLastError());
+    GBA_HIP(hipMemcpyAsync(h_status_, status_.p, sizeof(PassStatus), hipMemcpyDeviceToHost, st_));
+    GBA_HIP(hipStreamSynchronize(st_));
+}
+
+void Solver::linearize(double* sse, unsigned long long* anomalies) {
+    launch_lin_points(st_, pb_, cams_[cur_].p, pts_[cur_].p, W64_.p, W32_.p, VB_.p, unit_sse_.p,
+                      unit_anom_.p);
+    launch_lin_cameras(st_, pb_, cams_[cur_].p, pts_[cur_].p, gram_.p);
+    launch_lin_reduce(st_, pb_, gram_.p, unit_sse_.p, unit_anom_.p, U_.p, bc_.p, status_.p);
+    sync_status();
+    *sse = h_status_->sse;
+    *anomalies = h_status_->anomalies;
+}
+
+void Solver::cost(double* sse, unsigned long long* anomalies) {
+    launch_cost(st_, pb_, cams_[cur_].p, pts_[cur_].p, blk_sse_.p, blk_anom_.p, status_.p);
+    sync_status();
+    *sse = h_status_->sse;
+    *anomalies = h_status_->anomalies;
+}
+
+void Solver::enqueue_trial_linear_algebra(double lambda, std::vector<double>* hook_S,
+                                          std::vector<double>* hook_rhs, TrialOut& out) {
+    const int ld = pb_.npad;
+    launch_clear_flags(st_, flags_.p);
+    launch_damp_invert_v(st_, pb_.P, VB_.p, lambda, Vinv_.p, flags_.p);
+    if (mixed_) launch_diag_scale(st_, pb_.n, pb_.npad, U_.p, lambda, dscale_.p);
+    launch_schur(st_, pb_, W64_.p, W32_.p, Vinv_.p, VB_.p, U_.p, bc_.p, lambda,
+                 mixed_ ? dscale_.p : nullptr, S64_.p, S32_.p, S32full_.p, ld, rhs_.p, flags_.p);
+    if (hook_S || hook_rhs) {
+        GBA_HIP(hipGetLastError());
+        GBA_HIP(hipStreamSynchronize(st_));
+        const size_t np = static_cast<size_t>(pb_.npad), n = static_cast<size_t>(pb_.n);
+        std::vector<double> scale(np, 1.0);
+        if (mixed_)
+            GBA_HIP(hipMemcpy(scale.data(), dscale_.p, sizeof(double) * np, hipMemcpyDeviceToHost));
+        if (hook_S) {
+            std::vector<double> S(np * np);
+            if (mixed_) {
+                std::vector<float> Sf(np * np);
+                GBA_HIP(hipMemcpy(Sf.data(), S32_.p, sizeof(float) * np * np, hipMemcpyDeviceToHost));
+                for (size_t i = 0; i < np * np; ++i) S[i] = Sf[i];
+            } else {
+                GBA_HIP(hipMemcpy(S.data(), S64_.p, sizeof(double) * np * np, hipMemcpyDeviceToHost));
+            }
+            hook_S->assign(n * n, 0.0);
+            for (size_t r = 0; r < n; ++r)
+                for (size_t c = 0; c <= r && c < n; ++c)  // lower triangle, column-major out
+                    (*hook_S)[r + c * n] = S[r * np + c] / (scale[r] * scale[c]);
+        }
+        if (hook_rhs) {
+            std::vector<double> rr(np);
+            GBA_HIP(hipMemcpy(rr.data(), rhs_.p, sizeof(double) * np, hipMemcpyDeviceToHost));
+            hook_rhs->assign(n, 0.0);
+            for (size_t i = 0; i < n; ++i) (*hook_rhs)[i] = rr[i] / scale[i];
+        }
+    }
+    if (!mixed_) {
+        launch_cholesky(st_, S64_.p, nullptr, ld, nblk_, D64_.p, nullptr, flags_.p);
+        launch_chol_solve(st_, S64_.p, nullptr, ld, nblk_, D64_.p, nullptr, rhs_.p, y_.p, dc_.p,
+                          sync_.p, flags_.p);
+        return;
+    }
+    // mixed: FP32 factor of D S D, FP64 refinement against the FP32 matrix
+    launch_cholesky(st_, nullptr, S32_.p, ld, nblk_, nullptr, D32_.p, flags_.p);
+    launch_chol_solve(st_, nullptr, S32_.p, ld, nblk_, nullptr, D32_.p, rhs_.p, y_.p, x_.p, sync_.p,
+                      flags_.p);
+    launch_residual(st_, S32full_.p, ld, pb_.npad, rhs_.p, x_.p, r_.p, resblk_.p, status_.p, true,
+                    flags_.p);
+    sync_status();
+    out.refine_iters = 0;
+    const TrialFlags f = h_status_->flags;
+    double rhs2 = h_status_->rhs_norm2, res2 = h_status_->res_norm2;
+    if (f.nonspd == 0 && f.chol_fail == 0 && f.solve_err == 0) {
+        const double tol = cfg_.refine_tol > 0 ? cfg_.refine_tol : 1e-10;
+        const int max_it = cfg_.refine_max_iters > 0 ? cfg_.refine_max_iters : 12;
+        double prev = res2;
+        while (out.refine_iters < max_it && res2 > tol * tol * rhs2 && std::isfinite(res2)) {
+            launch_chol_solve(st_, nullptr, S32_.p, ld, nblk_, nullptr, D32_.p, r_.p, y_.p, d_.p,
+                              sync_.p, flags_.p);
+            launch_vec_axpy(st_, pb_.npad, x_.p, d_.p);
+            launch_residual(st_, S32full_.p, ld, pb_.npad, rhs_.p, x_.p, r_.p, resblk_.p, status_.p,
+                            false, flags_.p);
+            sync_status();
+            ++out.refine_iters;
+            res2 = h_status_->res_norm2;
+            if (!(res2 < prev)) break;  // stagnation: keep the last iterate
+            prev = res2;
+        }
+    }
+    out.refine_rel = rhs2 > 0 ? std::sqrt(res2 / rhs2) : 0.0;
+    launch_vec_scale(st_, pb_.n, pb_.npad, x_.p, dscale_.p, dc_.p);
+}
+
+TrialOut Solver::trial(double lambda, std::vector<double>* hook_S, std::vector<double>* hook_rhs) {
+    TrialOut out;
+    enqueue_trial_linear_algebra(lambda, hook_S, hook_rhs, out);
+    const int nxt = 1 - cur_;
+    launch_trial_cams(st_, pb_.C, cams_[cur_].p, dc_.p, cams_[nxt].p);
+    launch_backsub(st_, pb_, W64_.p, W32_.p, Vinv_.p, VB_.p, dc_.p, pts_[cur_].p, cams_[nxt].p,
+                   pts_[nxt].p, lambda, unit_sse_.p, unit_den_.p, unit_anom_.p, flags_.p);
+    launch_trial_finalize(st_, pb_, unit_sse_.p, unit_den_.p, unit_anom_.p, dc_.p, U_.p, bc_.p, lambda,
+                          status_.p, flags_.p);
+    sync_status();
+    out.flags = h_status_->flags;
+    out.trial_sse = h_status_->sse;
+    out.denom = h_status_->denom;
+    out.anomalies = h_status_->anomalies;
+    if (out.flags.solve_err) throw Error("triangular solve timed out waiting for a tile row");
+    return out;
+}
+
+void Solver::accept_trial() { cur_ = 1 - cur_; }
+
+void Solver::download_params(double* cams, double* pts) {
+    GBA_HIP(hipMemcpyAsync(cams, cams_[cur_].p, sizeof(double) * cams_[cur_].n, hipMemcpyDeviceToHost, st_));
+    GBA_HIP(hipMemcpyAsync(pts, pts_[cur_].p, sizeof(double) * pts_[cur_].n, hipMemcpyDeviceToHost, st_));
+    GBA_HIP(hipStreamSynchronize(st_));
+}
+
+void Solver::download_linearization(double* U, double* V, double* W, double* bc, double* bp) {
+    const size_t C = pb_.C, P = pb_.P, O = pb_.O;
+    std::vector<double> hU(81 * C), hVB(9 * P), hbc(9 * C);
+    std::vector<int> hperm(O);
+    GBA_HIP(hipStreamSynchronize(st_));
+    GBA_HIP(hipMemcpy(hU.data(), U_.p, sizeof(double) * hU.size(), hipMemcpyDeviceToHost));
+    GBA_HIP(hipMemcpy(hVB.data(), VB_.p, sizeof(double) * hVB.size(), hipMemcpyDeviceToHost));
+    GBA_HIP(hipMemcpy(hbc.data(), bc_.p, sizeof(double) * hbc.size(), hipMemcpyDeviceToHost));
+    GBA_HIP(hipMemcpy(hperm.data(), perm_.p, sizeof(int) * O, hipMemcpyDeviceToHost));
+    if (U)
+        for (size_t c = 0; c < C; ++c)
+            for (int i = 0; i < 9; ++i)
+                for (int j = 0; j < 9; ++j) U[81 * c + 9 * i + j] = j <= i ? hU[81 * c + 9 * i + j] : 0.0;
+    if (bc) std::memcpy(bc, hbc.data(), sizeof(double) * hbc.size());
+    for (size_t p = 0; p < P; ++p) {
+        const double* vb = &hVB[9 * p];
+        if (V) {
+            double* o = V + 9 * p;
+            o[0] = vb[0]; o[1] = 0.0;   o[2] = 0.0;
+            o[3] = vb[1]; o[4] = vb[2]; o[5] = 0.0;
+            o[6] = vb[3]; o[7] = vb[4]; o[8] = vb[5];
+        }
+        if (bp) {
+            bp[3 * p] = vb[6];
+            bp[3 * p + 1] = vb[7];
+            bp[3 * p + 2] = vb[8];
+        }
+    }
+    if (W) {
+        std::vector<double> hW(27 * O);
+        if (mixed_) {
+            std::vector<float> hw(27 * O);
+            GBA_HIP(hipMemcpy(hw.data(), W32_.p, sizeof(float) * hw.size(), hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < hw.size(); ++i) hW[i] = hw[i];
+        } else {
+            GBA_HIP(hipMemcpy(hW.data(), W64_.p, sizeof(double) * hW.size(), hipMemcpyDeviceToHost));
+        }
+        for (size_t k = 0; k < O; ++k)
+            std::memcpy(W + 27 * static_cast<size_t>(hperm[k]), &hW[27 * k], 27 * sizeof(double));
+    }
+}
+
+void Solver::download_step(double* dc, double* dp) {
+    if (dp) {
+        DevBuf<double> ddp;
+        ddp.alloc(3 * static_cast<size_t>(pb_.P));
+        launch_fill_double(st_, ddp.p, ddp.n, 0.0);
+        launch_point_step(st_, pb_, W64_.p, W32_.p, Vinv_.p, VB_.p, dc_.p, ddp.p);
+        GBA_HIP(hipGetLastError());
+        GBA_HIP(hipStreamSynchronize(st_));
+        GBA_HIP(hipMemcpy(dp, ddp.p, sizeof(double) * ddp.n, hipMemcpyDeviceToHost));
+    }
+    if (dc) {
+        GBA_HIP(hipStreamSynchronize(st_));
+        GBA_HIP(hipMemcpy(dc, dc_.p, sizeof(double) * pb_.n, hipMemcpyDeviceToHost));
+    }
+}
+
+double Solver::time_kernel(int kernel, double lambda, int reps) {
+    hipEvent_t a, b;
+    GBA_HIP(hipEventCreate(&a));
+    GBA_HIP(hipEventCreate(&b));
+    const int ld = pb_.npad;
+    auto enqueue = [&]() {
+        switch (kernel) {
+            case GBA_KERNEL_EDGE_POINT:
+                launch_lin_points(st_, pb_, cams_[cur_].p, pts_[cur_].p, W64_.p, W32_.p, VB_.p,
+                                  unit_sse_.p, unit_anom_.p);
+                break;
+            case GBA_KERNEL_EDGE_CAMERA:
+                launch_lin_cameras(st_, pb_, cams_[cur_].p, pts_[cur_].p, gram_.p);
+                break;
+            case GBA_KERNEL_SCHUR:
+                launch_schur(st_, pb_, W64_.p, W32_.p, Vinv_.p, VB_.p, U_.p, bc_.p, lambda,
+                             mixed_ ? dscale_.p : nullptr, S64_.p, S32_.p, S32full_.p, ld, rhs_.p,
+                             flags_.p);
+                break;
+            case GBA_KERNEL_FACTOR:
+                launch_cholesky(st_, S64_.p, S32_.p, ld, nblk_, D64_.p, D32_.p, flags_.p);
+                break;
+            case GBA_KERNEL_SOLVE:
+                launch_chol_solve(st_, S64_.p, S32_.p, ld, nblk_, D64_.p, D32_.p, rhs_.p, y_.p, x_.p,
+                                  sync_.p, flags_.p);
+                break;
+            case GBA_KERNEL_BACKSUB:
+                launch_backsub(st_, pb_, W64_.p, W32_.p, Vinv_.p, VB_.p, dc_.p, pts_[cur_].p,
+                               cams_[1 - cur_].p, pts_[1 - cur_].p, lambda, unit_sse_.p, unit_den_.p,
+                               unit_anom_.p, flags_.p);
+                break;
+            case GBA_KERNEL_COST:
+                launch_cost(st_, pb_, cams_[cur_].p, pts_[cur_].p, blk_sse_.p, blk_anom_.p, status_.p);
+                break;
+            default:
+                throw Error("unknown kernel id");
+        }
+    };
+    // prepare a valid state: a linearisation and one trial at lambda
+    double sse;
+    unsigned long long an;
+    linearize(&sse, &an);
+    (void)trial(lambda);
+    if (kernel == GBA_KERNEL_FACTOR) {
+        // refactor the same matrix each repetition: re-assemble first
+        double total = 0.0;
+        for (int r = 0; r < reps; ++r) {
+            launch_clear_flags(st_, flags_.p);
+            launch_damp_invert_v(st_, pb_.P, VB_.p, lambda, Vinv_.p, flags_.p);
+            if (mixed_) launch_diag_scale(st_, pb_.n, pb_.npad, U_.p, lambda, dscale_.p);
+            launch_schur(st_, pb_, W64_.p, W32_.p, Vinv_.p, VB_.p, U_.p, bc_.p, lambda,
+                         mixed_ ? dscale_.p : nullptr, S64_.p, S32_.p, S32full_.p, ld, rhs_.p, flags_.p);
+            GBA_HIP(hipEventRecord(a, st_));
+            enqueue();
+            GBA_HIP(hipEventRecord(b, st_));
+            GBA_HIP(hipEventSynchronize(b));
+            float ms = 0.f;
+            GBA_HIP(hipEventElapsedTime(&ms, a, b));
+            total += ms;
+        }
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        return total / reps;
+    }
+    enqueue();  // warm-up
+    GBA_HIP(hipEventRecord(a, st_));
+    for (int r = 0; r < reps; ++r) enqueue();
+    GBA_HIP(hipEventRecord(b, st_));
+    GBA_HIP(hipEventSynchronize(b));
+    GBA_HIP(hipGetLastError());
+    float ms = 0.f;
+    GBA_HIP(hipEventElapsedTime(&ms, a, b));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return static_cast<double>(ms) / reps;
+}
+
+void Solver::solve(const gba_lm_options& opts, gba_lm_result* res, double t_entry) {
+    const double t_solve0 = now_s();
+    reset_params();
+    double sse = 0.0;
+    unsigned long long an = 0, counter0 = 0;
+    linearize(&sse, &an);
+    counter0 += an;
+    res->initial_cost = sse / 2.0;
+    res->iterations = 0;
+    res->trials = 0;
+    res->refine_iterations = 0;
+    res->max_refine_residual = 0.0;
+    double lambda = opts.initial_lambda;
+    bool converged = false, failed = false;
+    double rel_decrease = 0.0;
+    int ntrace = 0;
+    while (res->iterations < opts.max_iterations && !converged && !failed) {
+        double nu = 2.0;
+        bool accepted = false;
+        for (int trial_i = 0; trial_i < opts.max_inner_trials; ++trial_i) {
+            const unsigned long long before0 = counter0;
+            const double lam_used = lambda;
+            TrialOut t = trial(lambda);
+            res->trials++;
+            res->refine_iterations += t.refine_iters;
+            res->max_refine_residual = std::max(res->max_refine_residual, t.refine_rel);
+            int code;
+            double rho = NAN, tcost = NAN;
+            if (t.flags.nonspd) {
+                code = GBA_TRIAL_NONSPD_V;
+            } else if (t.flags.chol_fail) {
+                code = GBA_TRIAL_CHOL_FAIL;
+            } else if (!(t.denom > 0.0)) {
+                code = GBA_TRIAL_DENOM;
+            } else {
+                counter0 += t.anomalies;
+                tcost = t.trial_sse / 2.0;
+                if (counter0 != before0 || !std::isfinite(t.trial_sse)) {
+                    code = GBA_TRIAL_NONFINITE;
+                } else {
+                    rho = (sse - t.trial_sse) / (0.5 * t.denom);
+                    code = (rho > 0.0 && t.trial_sse < sse) ? GBA_TRIAL_ACCEPT : GBA_TRIAL_REJECT;
+                }
+            }
+            if (opts.verbose)
+                std::fprintf(stderr, "  iter %d trial: lambda %.3g cost %.9g rho %.3g %s\n",
+                             res->iterations, lam_used, tcost, rho,
+                             code == GBA_TRIAL_ACCEPT ? "accept" : "reject");
+            if (code == GBA_TRIAL_ACCEPT) {
+                accept_trial();
+                rel_decrease = (sse - t.trial_sse) / std::max(sse, 1.0);
+                sse = t.trial_sse;
+                lambda *= std::max(1.0 / 3.0, 1.0 - std::pow(2.0 * rho - 1.0, 3.0));
+                accepted = true;
+            }
+            if (cfg_.trace && ntrace < cfg_.trace_capacity) {
+                double* tr = cfg_.trace + GBA_TRACE_STRIDE * static_cast<size_t>(ntrace);
+                tr[0] = res->iterations;
+                tr[1] = lam_used;
+                tr[2] = tcost;
+                tr[3] = rho;
+                tr[4] = code;
+                tr[5] = now_s() - t_entry;
+                ++ntrace;
+            }
+            if (accepted) break;
+            lambda *= nu;
+            nu *= 2.0;
+        }
+        if (!accepted) {
+            failed = true;
+            break;
+        }
+        ++res->iterations;
+        if (rel_decrease < opts.rel_decrease_tol) {
+            converged = true;
+            break;
+        }
+        double lin_sse;  // the current cost stays the accepted trial cost
+        linearize(&lin_sse, &an);
+        counter0 += an;
+    }
+    res->final_cost = sse / 2.0;
+    res->anomalies = counter0;
+    res->termination = converged ? GBA_CONVERGENCE : (failed ? GBA_FAILURE : GBA_MAX_ITERATIONS);
+    res->solve_time_s = now_s() - t_solve0;
+}
+
+}  // namespace gba_amd

@@ -1,0 +1,109 @@
+// solver.hpp -- host driver of the device-resident LM solve.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "device_common.hpp"
+#include "gba_c.h"
+#include "gba_internal.hpp"
+
+namespace gba_amd {
+
+void hip_check(hipError_t e, const char* what, const char* file, int line);
+#define GBA_HIP(x) ::gba_amd::hip_check((x), #x, __FILE__, __LINE__)
+
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    void alloc(size_t count) {
+        release();
+        n = count;
+        if (count) GBA_HIP(hipMalloc(&p, sizeof(T) * count));
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    ~DevBuf() { release(); }
+};
+
+// Result of one enqueued-and-synchronised LM trial.
+struct TrialOut {
+    TrialFlags flags;
+    double trial_sse = 0.0;
+    double denom = 0.0;
+    unsigned long long anomalies = 0;
+    int refine_iters = 0;
+    double refine_rel = 0.0;
+};
+
+class Solver {
+public:
+    Solver(const gba_problem_view& prob, const gba_solver_config& cfg);
+    ~Solver();
+    Solver(const Solver&) = delete;
+    Solver& operator=(const Solver&) = delete;
+
+    // Uploads the parameters the solver was created with (or the given ones).
+    void reset_params(const double* cams = nullptr, const double* pts = nullptr);
+    // Linearisation at the current parameters; returns SSE / anomalies.
+    void linearize(double* sse, unsigned long long* anomalies);
+    // One LM trial at lambda (enqueue + synchronise).  When hook_S is set, S
+    // (column-major lower, unscaled, before factorisation) and rhs are copied
+    // out after assembly.
+    TrialOut trial(double lambda, std::vector<double>* hook_S = nullptr,
+                   std::vector<double>* hook_rhs = nullptr);
+    // Residual-only cost at the current parameters.
+    void cost(double* sse, unsigned long long* anomalies);
+    void accept_trial();  // the trial parameters become current
+    void solve(const gba_lm_options& opts, gba_lm_result* res, double t_entry);
+    void download_params(double* cams, double* pts);
+    // Test helpers (host arrays in the reference layout).
+    void download_linearization(double* U, double* V, double* W, double* bc, double* bp);
+    void download_step(double* dc, double* dp);
+    double time_kernel(int kernel, double lambda, int reps);
+    double pair_count() const { return pair_count_; }
+    int n() const { return pb_.n; }
+    int npad() const { return pb_.npad; }
+
+private:
+    void enqueue_trial_linear_algebra(double lambda, std::vector<double>* hook_S,
+                                      std::vector<double>* hook_rhs, TrialOut& out);
+    void sync_status();
+
+    gba_solver_config cfg_;
+    bool mixed_;
+    DevProblem pb_{};
+    int nblk_ = 0;
+    hipStream_t st_ = nullptr;
+    std::vector<double> init_cams_, init_pts_;
+    double pair_count_ = 0.0;
+    int cur_ = 0;  // index of the current parameter buffers
+
+    DevBuf<int> scam_, spt_, perm_, pt_off_, cam_list_, cam_off_, cam_chunk_off_;
+    DevBuf<double2> spx_;
+    DevBuf<int2> units_;
+    DevBuf<int4> chunks_;
+    DevBuf<double> cams_[2], pts_[2];
+    DevBuf<double> W64_;
+    DevBuf<float> W32_;
+    DevBuf<double> VB_, Vinv_, U_, bc_;
+    DevBuf<double> unit_sse_, unit_den_, gram_, blk_sse_;
+    DevBuf<unsigned> unit_anom_, blk_anom_;
+    DevBuf<double> S64_, D64_;
+    DevBuf<float> S32_, S32full_, D32_;
+    DevBuf<double> rhs_, y_, x_, dc_, dscale_, r_, d_, resblk_;
+    DevBuf<int> sync_;
+    DevBuf<TrialFlags> flags_;
+    DevBuf<PassStatus> status_;
+    PassStatus* h_status_ = nullptr;
+};
+
+}  // namespace gba_amd

@@ -1,0 +1,226 @@
+/*
+ * gba_c.h -- C ABI of libgba_amd.so, the MI355X (gfx950) bundle-adjustment
+ * LM solver.  Plain C: pointers, sizes and POD structs only.
+ *
+ * The reference has no FFI; its boundary is the C++ library API of
+ * FireHorseAutumn/gpu-bundle-adjustment (gba::solve_bal_gpu and friends).
+ * Each entry point below names the reference interface it replaces.  The
+ * C++ mirror of that API (include/gba/lm.hpp, include/gba/bal_problem.hpp,
+ * include/gba/schur.hpp) is implemented on top of these functions inside the
+ * same library.
+ *
+ * Conventions: every int-returning function returns 0 on success and a
+ * non-zero code on failure; gba_last_error() then holds a message (thread
+ * local).  GPU runtime failures are errors; numerical failure of the LM
+ * solve is not (it is reported through gba_lm_result.termination).
+ */
+#ifndef GBA_C_H
+#define GBA_C_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- problem ---------------------------------------------------------- */
+
+/* Mirrors gba::BALProblem (reference include/gba/bal_problem.hpp:15-31):
+ * structure of arrays; cameras are [rodrigues(3), t(3), f, k1, k2].  The
+ * solver reads the observation arrays and overwrites cameras/points with the
+ * last accepted parameters. */
+typedef struct gba_problem_view {
+    int num_cameras;
+    int num_points;
+    int num_observations;
+    double* cameras;   /* 9 * num_cameras */
+    double* points;    /* 3 * num_points */
+    int* obs_camera;   /* num_observations */
+    int* obs_point;    /* num_observations */
+    double* obs_pixel; /* 2 * num_observations, x/y interleaved */
+} gba_problem_view;
+
+/* ---- LM solve ----------------------------------------------------------- */
+
+/* Mirrors gba::LMOptions (reference include/gba/lm.hpp:13-19). */
+typedef struct gba_lm_options {
+    int max_iterations;      /* default 50 */
+    double initial_lambda;   /* default 1e-3 (Marquardt diagonal damping) */
+    double rel_decrease_tol; /* default 1e-6 */
+    int max_inner_trials;    /* default 16 */
+    int verbose;             /* per-trial log to stderr */
+} gba_lm_options;
+
+enum gba_termination { GBA_CONVERGENCE = 0, GBA_MAX_ITERATIONS = 1, GBA_FAILURE = 2 };
+
+/* Mirrors gba::LMResult (reference include/gba/lm.hpp:21-29); termination is
+ * the enum above instead of the reference's string.  The fields after
+ * total_time_s have no reference counterpart. */
+typedef struct gba_lm_result {
+    double initial_cost;
+    double final_cost;
+    int iterations;               /* accepted outer iterations */
+    int termination;              /* enum gba_termination */
+    unsigned long long anomalies; /* cumulative non-finite edge count */
+    double total_time_s;          /* includes device allocation and upload */
+    int trials;                   /* LM trials evaluated */
+    double solve_time_s;          /* device-resident part (after upload) */
+    double setup_time_s;          /* allocation, upload, observation sort */
+    int refine_iterations;        /* mixed precision: refinement steps, summed */
+    double max_refine_residual;   /* mixed precision: worst final relative residual */
+} gba_lm_result;
+
+/* Arithmetic of the reduced camera solve. */
+enum gba_precision {
+    GBA_PRECISION_FP64 = 0,  /* plain drop-in: FP64 throughout */
+    GBA_PRECISION_MIXED = 1  /* FP32 W / S / MFMA factor, FP64 refinement, cost, gain ratio */
+};
+
+/* Solver configuration beyond gba_lm_options (no reference counterpart). */
+typedef struct gba_solver_config {
+    int precision;          /* enum gba_precision */
+    double refine_tol;      /* mixed: relative residual target (default 1e-10) */
+    int refine_max_iters;   /* mixed: refinement cap per trial (default 12) */
+    double* trace;          /* optional, GBA_TRACE_STRIDE doubles per trial */
+    int trace_capacity;     /* trials that fit in trace */
+} gba_solver_config;
+
+/* Trace record per LM trial: iteration, lambda used, trial cost, rho,
+ * status (enum below), seconds since solve entry when the trial finished. */
+#define GBA_TRACE_STRIDE 6
+enum gba_trial_status {
+    GBA_TRIAL_ACCEPT = 0,
+    GBA_TRIAL_REJECT = 1,
+    GBA_TRIAL_NONSPD_V = 2,
+    GBA_TRIAL_CHOL_FAIL = 3,
+    GBA_TRIAL_DENOM = 4,
+    GBA_TRIAL_NONFINITE = 5
+};
+
+void gba_default_options(gba_lm_options* opts);
+void gba_default_config(gba_solver_config* cfg);
+
+/* Replaces gba::solve_bal_gpu (reference include/gba/lm.hpp:34,
+ * src/lm.cu:371-724).  cfg may be NULL (plain FP64).  Blocking; uses its own
+ * HIP stream on the current device. */
+int gba_solve_bal(gba_problem_view* prob, const gba_lm_options* opts,
+                  const gba_solver_config* cfg, gba_lm_result* res);
+
+/* Device-resident solver handle (no reference counterpart): uploads and
+ * sorts the observations once, then solves repeatedly from the uploaded
+ * initial parameters.  Used by the bench to separate setup from solve. */
+typedef struct gba_solver gba_solver;
+int gba_solver_create(const gba_problem_view* prob, const gba_solver_config* cfg,
+                      gba_solver** out);
+int gba_solver_solve(gba_solver* s, const gba_lm_options* opts, gba_lm_result* res);
+int gba_solver_params(gba_solver* s, double* cameras, double* points);
+/* Times one kernel family on the solver's stream with HIP events at the
+ * solver's current parameters: reps launches, average milliseconds per
+ * launch in *avg_ms.  kernel: enum gba_kernel_id. */
+enum gba_kernel_id {
+    GBA_KERNEL_EDGE_POINT = 0,  /* linearisation, point pass (residual, J, W, V, bp) */
+    GBA_KERNEL_EDGE_CAMERA = 1, /* linearisation, camera pass (U, bc) */
+    GBA_KERNEL_SCHUR = 2,       /* reduced camera system assembly */
+    GBA_KERNEL_FACTOR = 3,      /* dense Cholesky of S (all its launches) */
+    GBA_KERNEL_SOLVE = 4,       /* triangular solves */
+    GBA_KERNEL_BACKSUB = 5,     /* point back-substitution + trial cost */
+    GBA_KERNEL_COST = 6         /* residual-only cost pass */
+};
+int gba_solver_time_kernel(gba_solver* s, int kernel, double lambda, int reps, double* avg_ms);
+/* Shape facts used for roofline accounting: sum_p k_p(k_p+1)/2, matrix
+ * order and padded order of S, and the workgroup counts of the edge passes. */
+int gba_solver_stats(gba_solver* s, double* pair_count, int* n, int* n_padded);
+void gba_solver_destroy(gba_solver* s);
+
+/* ---- pieces of the path, host arrays in / out (parity tests) ----------- */
+
+/* Linearisation at the given parameters (reference src/lm.cu:205-252 and
+ * accumulate() :398-422).  Outputs in the reference layout: U 81C and V 9P
+ * with the lower triangle written and the upper zero, W 27O in observation
+ * order (FP32-stored W is widened in mixed precision), bc 9C, bp 3P. */
+int gba_linearize(const gba_problem_view* prob, int precision, double* U, double* V, double* W,
+                  double* bc, double* bp, double* sse, unsigned long long* anomalies);
+
+/* One LM trial's linear algebra at lambda (reference src/lm.cu:469-643):
+ * damped V inverse, Schur assembly, dense factor + solve, back-substitution,
+ * gain denominator.  *status is enum gba_trial_status (ACCEPT = solved).
+ * S_out (9C x 9C, column-major, lower triangle, before factorisation) and
+ * rhs_out (9C) may be NULL; in mixed precision S_out is the FP32 matrix
+ * widened, without the internal diagonal scaling. */
+int gba_schur_step(const gba_problem_view* prob, int precision, double lambda, double* dc,
+                   double* dp, double* denom, int* status, double* S_out, double* rhs_out);
+
+/* Residual-only SSE at the given parameters (reference src/lm.cu:255-281). */
+int gba_cost(const gba_problem_view* prob, double* sse, unsigned long long* anomalies);
+
+/* Dense SPD factor + solve of an n x n column-major matrix on the GPU with
+ * the solver's blocked MFMA Cholesky (replaces Eigen::LLT at
+ * src/lm.cu:597-604).  b (n) is overwritten with the solution; *ok = 0 when
+ * the matrix is not positive definite.  precision FP64 or MIXED (FP32
+ * factor + FP64 refinement; *refine_iters receives the step count). */
+int gba_dense_spd_solve(int n, int precision, const double* A, double* b, int* ok,
+                        int* refine_iters);
+
+/* ---- batched small blocks, device pointers (reference include/gba/schur.cuh) */
+
+/* Replaces the batch_chol9_factor_solve kernel (schur.cuh:104-109): factor
+ * each symmetric 9x9 (lower triangle read) and solve; ok[i] = 0 if not SPD.
+ * A, b, x, ok are device pointers; stream is a hipStream_t (NULL = default). */
+int gba_batch_chol9_factor_solve(const double* A, const double* b, double* x, int* ok, int n,
+                                 void* stream);
+/* Replaces the batch_inverse3_spd kernel (schur.cuh:112-116). */
+int gba_batch_inverse3_spd(const double* A, double* Ainv, int* ok, int n, void* stream);
+
+/* ---- host camera model and I/O ----------------------------------------- */
+
+/* Replaces gba::snavely_residual (reference bal_problem.hpp:45-46,
+ * src/bal_reader.cpp:100-128). */
+void gba_snavely_residual(const double* camera9, const double* point3, const double* observed2,
+                          double* out2);
+/* Replaces gba::total_squared_error (bal_problem.hpp:50, bal_reader.cpp:130-139). */
+double gba_total_squared_error(const gba_problem_view* prob);
+/* Replaces gba::edge_residual_jacobian_host (reference lm.hpp:40-42,
+ * src/lm.cu:365-369): the residual + analytic Jacobian the GPU kernels use. */
+void gba_edge_residual_jacobian_host(const double* camera9, const double* point3,
+                                     const double* observed2, double* out_residual2,
+                                     double* out_Jc, double* out_Jp);
+/* Replaces gba::load_bal (bal_problem.hpp:36, src/bal_reader.cpp:66-98), in
+ * two calls: the header, then the arrays into caller-owned storage.  .bz2
+ * files are streamed through bzcat as the reference does. */
+int gba_bal_header(const char* path, int* num_cameras, int* num_points, int* num_observations);
+int gba_load_bal(const char* path, gba_problem_view* out);
+/* BAL text writer (the reference has one only in tools/quantize_bal.cu:99-116). */
+int gba_write_bal(const char* path, const gba_problem_view* prob);
+
+/* ---- synthetic problems (no reference counterpart) ---------------------- */
+
+enum gba_visibility_law { GBA_LAW_GEOMETRIC = 0, GBA_LAW_PARETO = 1 };
+
+typedef struct gba_synth_spec {
+    int num_cameras, num_points, num_observations;
+    int kmin, kmax;      /* observations per point */
+    int law;             /* enum gba_visibility_law */
+    double alpha;        /* Pareto tail index */
+    double ring_radius;  /* camera ring radius (default 30) */
+    double pixel_noise;  /* pixel noise sigma (1 px) */
+    double perturb;      /* relative parameter perturbation (0.01) */
+} gba_synth_spec;
+
+/* Preset shapes: 1 (C16/P22106/O83718), 2 (C49/P7776/O31843),
+ * 3 (C356/P226730/O1255268), 4 (C1936/P525000/O3400000), 5 (batched: C, P,
+ * O drawn from the seed: C~U{8..64}, P~U{2000..20000}, O = round(4.5 P)). */
+int gba_synth_preset(int config, unsigned long long seed, gba_synth_spec* out);
+int gba_synth_generate(const gba_synth_spec* spec, unsigned long long seed, gba_problem_view* out);
+/* The reference's data-free end-to-end problem (tests/test_launch_bounds.cpp
+ * :28-80): C cameras, P points, 2P noise-free observations, std::mt19937_64. */
+int gba_synth_launch_bounds(int num_cameras, int num_points, unsigned long long seed,
+                            gba_problem_view* out);
+
+/* ---- misc ---------------------------------------------------------------- */
+const char* gba_last_error(void);
+const char* gba_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GBA_C_H */

@@ -1,0 +1,198 @@
+"""Parity of the HIP path (through the C ABI of libgba_amd.so) with the CPU
+oracle on the same seeded inputs.  Tolerances: FP64 pieces agree to 1e-12
+relative (summation order only), the FP64 step / final cost to 1e-9; the mixed
+form's final cost to 1e-6 of the FP64 path."""
+import numpy as np
+import pytest
+
+from conftest import rel
+
+pytestmark = pytest.mark.gpu
+
+
+def heavy(gba, seed=3):
+    # points with more than 64 observations exercise the looping work units
+    s = gba.synth_spec(3)
+    s.num_cameras, s.num_points, s.num_observations, s.kmax = 120, 600, 6000, 120
+    return gba.generate(spec=s, seed=seed)
+
+
+def arel(a, b):
+    """max |a-b| relative to max |b| (per array)."""
+    return float(np.max(np.abs(np.asarray(a) - np.asarray(b))) / max(1e-300, np.max(np.abs(b))))
+
+
+@pytest.mark.parametrize("which", ["cfg2", "heavy"])
+def test_linearize_fp64_matches_oracle(gba, orc, which):
+    prob = gba.generate(2, 1) if which == "cfg2" else heavy(gba)
+    g = gba.linearize(prob)
+    o = orc.linearize(prob)
+    for k in ("U", "V", "W", "bc", "bp"):
+        assert arel(g[k], o[k]) < 1e-12, k
+    assert abs(g["sse"] - o["sse"]) <= 1e-12 * o["sse"]
+    assert g["anomalies"] == 0
+
+
+def test_linearize_mixed_stores_fp32_w(gba, orc):
+    prob = gba.generate(2, 2)
+    g = gba.linearize(prob, gba.PRECISION_MIXED)
+    o = orc.linearize(prob)
+    assert arel(g["W"], o["W"]) < 1e-6
+    for k in ("U", "V", "bc", "bp"):
+        assert arel(g[k], o[k]) < 1e-12, k
+
+
+def test_linearize_anomaly_skipped(gba, orc):
+    prob = gba.generate(2, 1)
+    prob.obs_pixel[17, 0] = np.inf
+    prob.obs_pixel[3000, 1] = np.nan
+    g = gba.linearize(prob)
+    o = orc.linearize(prob)
+    assert g["anomalies"] == o["anomalies"] == 2
+    for k in ("U", "V", "W", "bc", "bp"):
+        assert arel(g[k], o[k]) < 1e-12, k
+    s, an = gba.cost(prob)
+    so, ano = orc.cost(prob)
+    assert an == ano == 2 and abs(s - so) <= 1e-12 * so
+
+
+@pytest.mark.parametrize("which,lam", [("cfg2", 1e-4), ("heavy", 1e-3), ("cfg1", 1e-2)])
+def test_schur_step_fp64_matches_oracle(gba, orc, which, lam):
+    prob = {"cfg2": lambda: gba.generate(2, 1), "heavy": lambda: heavy(gba),
+            "cfg1": lambda: gba.generate(1, 1)}[which]()
+    g = gba.schur_step(prob, lam, want_S=True)
+    o = orc.schur_step(prob, orc.linearize(prob), lam, want_S=True)
+    assert g["status"] == "accept" and o["status"] == "ok"
+    assert arel(np.tril(g["S"]), np.tril(o["S"])) < 1e-12
+    assert arel(g["rhs"], o["rhs"]) < 1e-12
+    assert arel(g["dc"], o["dc"]) < 1e-9
+    assert arel(g["dp"], o["dp"]) < 1e-9
+    assert abs(g["denom"] - o["denom"]) <= 1e-9 * abs(o["denom"])
+
+
+def test_schur_step_mixed_close_to_fp64(gba, orc):
+    prob = gba.generate(2, 1)
+    g = gba.schur_step(prob, 1e-4, gba.PRECISION_MIXED, want_S=True)
+    o = orc.schur_step(prob, orc.linearize(prob), 1e-4, want_S=True)
+    assert g["status"] == "accept"
+    assert arel(np.tril(g["S"]), np.tril(o["S"])) < 1e-5
+    assert arel(g["rhs"], o["rhs"]) < 1e-12
+    assert arel(g["dc"], o["dc"]) < 1e-3
+    assert abs(g["denom"] - o["denom"]) <= 1e-4 * abs(o["denom"])
+
+
+def test_schur_step_reports_non_spd_v(gba, orc):
+    # a point without observations has V = 0; at lambda = 0 its damped block
+    # is not SPD and the trial is abandoned (src/lm.cu:477-481)
+    prob = gba.generate(2, 1)
+    prob.obs_point[prob.obs_point == 5] = 6
+    g = gba.schur_step(prob, 0.0)
+    _, bad = orc.damp_invert_v(orc.linearize(prob)["V"], 0.0)
+    assert bad >= 1
+    assert g["status"] == "nonspd_v"
+
+
+@pytest.mark.parametrize("n", [1, 63, 64, 65, 441, 1000])
+def test_dense_spd_solve_fp64(gba, n):
+    rng = np.random.default_rng(n)
+    M = rng.uniform(-1, 1, (n, n))
+    A = M.T @ M + n * 0.05 * np.eye(n) + np.eye(n)
+    b = rng.normal(size=n)
+    x, ok, _ = gba.dense_spd_solve(A, b)
+    assert ok
+    assert np.linalg.norm(A @ x - b) <= 1e-12 * np.linalg.norm(b) * np.linalg.cond(A)
+    assert arel(x, np.linalg.solve(A, b)) < 1e-10
+
+
+@pytest.mark.parametrize("n", [64, 441, 1000])
+def test_dense_spd_solve_mixed_refines_to_fp64(gba, n):
+    rng = np.random.default_rng(n + 1)
+    M = rng.uniform(-1, 1, (n, n))
+    A = (M.T @ M + n * 0.05 * np.eye(n) + np.eye(n)).astype(np.float32).astype(np.float64)
+    b = rng.normal(size=n)
+    x, ok, it = gba.dense_spd_solve(A, b, gba.PRECISION_MIXED)
+    assert ok and it >= 1
+    assert np.linalg.norm(A @ x - b) <= 1e-10 * np.linalg.norm(b)
+
+
+def test_dense_spd_solve_rejects_indefinite(gba):
+    A = np.eye(100)
+    A[70, 70] = -1.0
+    _, ok, _ = gba.dense_spd_solve(A, np.ones(100))
+    assert not ok
+
+
+def test_batch_kernels_reference_protocol(gba):
+    import torch
+    rng = np.random.default_rng(42)
+    n = 200
+    M = rng.uniform(-1, 1, (n, 9, 9))
+    A = M.transpose(0, 2, 1) @ M + 9 * np.eye(9)
+    A[7, 3, 3] = -5.0  # not SPD
+    b = rng.uniform(-1, 1, (n, 9))
+    x, ok = gba.batch_chol9_factor_solve(torch.tensor(A, device="cuda"), torch.tensor(b, device="cuda"))
+    ok = ok.cpu().numpy()
+    assert ok[7] == 0 and ok.sum() == n - 1
+    good = ok == 1
+    ref = np.linalg.solve(A[good], b[good][..., None])[..., 0]
+    assert np.max(np.abs(x.cpu().numpy()[good] - ref)) <= 1e-9
+    M3 = rng.uniform(-1, 1, (n, 3, 3))
+    A3 = M3.transpose(0, 2, 1) @ M3 + 3 * np.eye(3)
+    Ai, ok3 = gba.batch_inverse3_spd(torch.tensor(A3, device="cuda"))
+    assert ok3.cpu().numpy().all()
+    assert np.max(np.abs(Ai.cpu().numpy() - np.linalg.inv(A3))) <= 1e-9
+
+
+@pytest.mark.parametrize("cfg,lam", [(1, 1e-3), (2, 1e-4)])
+def test_lm_fp64_matches_oracle(gba, orc, cfg, lam):
+    p_gpu, p_cpu = gba.generate(cfg, 1), gba.generate(cfg, 1)
+    r = gba.solve_bal_gpu(p_gpu, gba.LMOptions(initial_lambda=lam))
+    o = orc.lm_solve(p_cpu, initial_lambda=lam)
+    assert r.termination == o["termination"] == "CONVERGENCE"
+    assert r.iterations == o["iterations"]
+    assert r.anomalies == o["anomalies"] == 0
+    assert abs(r.initial_cost - o["initial_cost"]) <= 1e-12 * o["initial_cost"]
+    assert abs(r.final_cost - o["final_cost"]) <= 1e-9 * o["final_cost"]
+    assert arel(p_gpu.cameras, p_cpu.cameras) < 1e-6
+    # accepted-cost trajectory
+    ga = r.trace[r.trace[:, 4] == 0][:, 2]
+    oa = o["trace"][o["trace"][:, 4] == 0][:, 2]
+    assert len(ga) == len(oa) and np.max(np.abs(ga - oa) / oa) < 1e-9
+
+
+def test_lm_mixed_reaches_fp64_cost(gba):
+    p64, p32 = gba.generate(2, 1), gba.generate(2, 1)
+    opts = gba.LMOptions(initial_lambda=1e-4)
+    r64 = gba.solve_bal_gpu(p64, opts)
+    r32 = gba.solve_bal_gpu(p32, opts, gba.PRECISION_MIXED)
+    assert r32.termination == "CONVERGENCE"
+    assert abs(r32.final_cost - r64.final_cost) <= 1e-6 * r64.final_cost
+    assert r32.max_refine_residual <= 1e-8
+
+
+def test_lm_failure_and_max_iterations_paths(gba):
+    p = gba.generate(2, 1)
+    r = gba.solve_bal_gpu(p, gba.LMOptions(max_inner_trials=0))
+    assert r.termination == "FAILURE" and r.iterations == 0 and r.final_cost == r.initial_cost
+    r = gba.solve_bal_gpu(gba.generate(2, 1), gba.LMOptions(max_iterations=2, rel_decrease_tol=0.0))
+    assert r.termination == "MAX_ITERATIONS" and r.iterations == 2
+
+
+def test_lm_anomalies_match_oracle(gba, orc):
+    p_gpu, p_cpu = gba.generate(2, 1), gba.generate(2, 1)
+    for p in (p_gpu, p_cpu):
+        p.obs_pixel[10, 1] = np.nan
+    r = gba.solve_bal_gpu(p_gpu, gba.LMOptions(max_iterations=3))
+    o = orc.lm_solve(p_cpu, max_iterations=3)
+    assert r.anomalies == o["anomalies"] > 0
+    assert abs(r.final_cost - o["final_cost"]) <= 1e-9 * o["final_cost"]
+
+
+def test_launch_bounds_gate_on_gpu(gba):
+    # reference tests/test_launch_bounds.cpp:82-95 (P = 400000 > 1024 * 256)
+    prob = gba.generate_launch_bounds(8, 400000, 7)
+    r = gba.solve_bal_gpu(prob)
+    assert r.initial_cost > 1.0
+    assert r.termination == "CONVERGENCE"
+    assert r.anomalies == 0
+    assert r.final_cost < 1e-4 * r.initial_cost
