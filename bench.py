@@ -57,6 +57,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-iterations", type=int, default=2)
     ap.add_argument("--kernel-reps", type=int, default=10)
+    ap.add_argument("--mode", choices=["solve", "kernels"], default="solve",
+                    help="kernels: only the per-kernel timing loops plus a 512 MiB device copy "
+                         "(short run for rocprofv3 counter passes)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this solver runs one problem on one GPU; multi-GPU sharding is "
@@ -72,6 +75,24 @@ def main():
 
     s64 = g.Solver(prob, g.PRECISION_FP64)
     s32 = g.Solver(prob, g.PRECISION_MIXED)
+    if args.mode == "kernels":
+        out = {"mode": "kernels", "config": {"cameras": C, "points": P, "observations": O}}
+        for name, s in (("fp64", s64), ("mixed", s32)):
+            out[name] = {k: round(s.time_kernel(k, args.lambda0, args.kernel_reps), 4)
+                         for k in ("edge_point", "edge_camera", "schur", "factor", "solve", "backsub")}
+        # calibration for FETCH_SIZE / WRITE_SIZE: a plain copy of 512 MiB
+        a = torch.empty(64 * 1024 * 1024, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        a.fill_(1.0)
+        torch.cuda.synchronize()
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        ev0.record()
+        b.copy_(a)
+        ev1.record()
+        torch.cuda.synchronize()
+        out["copy_512MiB_ms"] = round(ev0.elapsed_time(ev1), 4)
+        print(json.dumps(out))
+        return
     for _ in range(args.warmup):
         s64.solve(opts)
         s32.solve(opts)

@@ -95,6 +95,7 @@ _PROTOS = {
                                           ctypes.POINTER(ProblemView)]),
     "gba_synth_launch_bounds": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_ulonglong,
                                                ctypes.POINTER(ProblemView)]),
+    "gba_device_synchronize": (ctypes.c_int, []),
     "gba_last_error": (ctypes.c_char_p, []),
     "gba_version": (ctypes.c_char_p, []),
 }
@@ -332,15 +333,21 @@ def dense_spd_solve(A, b, precision=PRECISION_FP64):
     return x, bool(ok.value), it.value
 
 
+# torch ships its own HIP runtime; device pointers are shared between the two
+# runtimes of the process, stream handles are not.  The batched launchers
+# therefore run on this library's default stream, bracketed by a torch
+# synchronisation before and a library synchronisation after.
+
 def batch_chol9_factor_solve(A, b):
-    """A (n,9,9), b (n,9) torch CUDA float64 tensors -> (x, ok)."""
+    """A (n,9,9), b (n,9) torch device float64 tensors -> (x, ok)."""
     import torch
     n = A.shape[0]
     x = torch.zeros_like(b)
     ok = torch.zeros(n, dtype=torch.int32, device=A.device)
-    stream = torch.cuda.current_stream(A.device).cuda_stream
+    torch.cuda.synchronize()
     _check(lib().gba_batch_chol9_factor_solve(A.data_ptr(), b.data_ptr(), x.data_ptr(), ok.data_ptr(),
-                                              n, stream))
+                                              n, None))
+    _check(lib().gba_device_synchronize())
     return x, ok
 
 
@@ -349,8 +356,9 @@ def batch_inverse3_spd(A):
     n = A.shape[0]
     Ainv = torch.zeros_like(A)
     ok = torch.zeros(n, dtype=torch.int32, device=A.device)
-    stream = torch.cuda.current_stream(A.device).cuda_stream
-    _check(lib().gba_batch_inverse3_spd(A.data_ptr(), Ainv.data_ptr(), ok.data_ptr(), n, stream))
+    torch.cuda.synchronize()
+    _check(lib().gba_batch_inverse3_spd(A.data_ptr(), Ainv.data_ptr(), ok.data_ptr(), n, None))
+    _check(lib().gba_device_synchronize())
     return Ainv, ok
 
 

@@ -384,6 +384,13 @@ int gba_synth_launch_bounds(int C, int P, unsigned long long seed, gba_problem_v
     });
 }
 
+int gba_device_synchronize(void) {
+    return guarded([&] {
+        GBA_HIP(hipDeviceSynchronize());
+        return 0;
+    });
+}
+
 const char* gba_last_error(void) { return last_error(); }
 
 const char* gba_version(void) { return "gba_amd 0.1 (gfx950)"; }

@@ -51,6 +51,12 @@ __device__ __forceinline__ v4f mfma16(float a, float b, v4f c) {
 template <typename T> struct Acc4;
 template <> struct Acc4<double> { typedef v4d type; };
 template <> struct Acc4<float> { typedef v4f type; };
+// Row of accumulator element q held by lane group g = lane / 16 (column =
+// lane % 16): v_mfma_f32_16x16x4f32 uses row 4g + q, v_mfma_f64_16x16x4f64
+// uses row g + 4q.
+template <typename T> __device__ __forceinline__ int acc_row(int g, int q);
+template <> __device__ __forceinline__ int acc_row<float>(int g, int q) { return 4 * g + q; }
+template <> __device__ __forceinline__ int acc_row<double>(int g, int q) { return g + 4 * q; }
 
 __device__ __forceinline__ bool any_fail(const TrialFlags* f) {
     return f->nonspd != 0 || f->chol_fail != 0;
@@ -154,7 +160,7 @@ __global__ __launch_bounds__(256) void k_chol_update(T* __restrict__ S, int ld, 
         for (int c = 0; c < 2; ++c)
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                const int row = wr + 16 * r + 4 * lk + q, col = wc + 16 * c + li;
+                const int row = wr + 16 * r + acc_row<T>(lk, q), col = wc + 16 * c + li;
                 Aij[static_cast<size_t>(row) * ld + col] -= acc[r][c][q];
             }
 }

@@ -172,12 +172,12 @@ __global__ __launch_bounds__(256) void k_lin_cameras(DevProblem pb, const double
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
-    // acc[q] is G[4*(lane/16) + q][lane%16]; keep the lower triangle of the
-    // leading 10x10 block (55 values, packed row by row).
+    // v_mfma_f64_16x16x4f64 leaves G[lane/16 + 4q][lane%16] in acc[q]; keep
+    // the lower triangle of the leading 10x10 block (55 values, row by row).
     double* out = gram_part + 55 * static_cast<size_t>(chunk);
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        const int gi = 4 * rsub + q, gj = col;
+        const int gi = rsub + 4 * q, gj = col;
         if (gi < 10 && gj <= gi) out[gi * (gi + 1) / 2 + gj] = acc[q];
     }
 }

@@ -2,51 +2,57 @@
 //   S = U + lambda diag'(U) - sum_p W_p V'_p^-1 W_p^T,   rhs = bc - sum_e T_e bp_p
 // (replaces the host Phase 1 / Phase 2 / merge loops of src/lm.cu:483-594).
 //
-// Workgroup (a, chunk) owns block row a of S restricted to the camera columns
-// [chunk*kSchurCB, ...] below the diagonal: it walks camera a's observations,
-// forms T_e = W_e V'^-1 once per observation and, for every other observation f
-// of the same point whose camera b falls in the chunk, adds -T_e W_f^T into an
-// LDS accumulator of kSchurCB 9x9 blocks.  Each block of S is written by
-// exactly one workgroup with plain stores; no global atomics.  FP64 path:
-// everything double.  Mixed path: W, T and the accumulator are FP32, the
-// result is written diagonally scaled (D S D, D = (U_ii + lambda U_ii)^-1/2)
-// into the factor copy (lower blocks) and a full symmetric copy used by the
-// FP64 refinement residual; rhs stays FP64.
+// Workgroup (a, chunk) owns block row a of S restricted to a chunk of camera
+// columns b <= a wide enough that the whole chunk lives in LDS (up to 128 KB:
+// 202 cameras in FP64, 404 in FP32).  Its 1024 threads walk camera a's
+// observations (pre-sorted by the point's observation count, so a wavefront
+// gets similar work), form T_e = W_e V'^-1 once per observation and, for every
+// other observation f of the same point with camera b in the chunk (a point's
+// observations are sorted by camera: skip to lo, stop at hi), add -T_e W_f^T
+// into the LDS accumulator (ds_add_f64 / ds_add_f32).  Each block of S is
+// written by exactly one workgroup with plain stores; no global atomics.
+// FP64 path: everything double.  Mixed path: W, T and the accumulator are
+// FP32, the result is written diagonally scaled (D S D, D = (U_ii + lambda
+// U_ii)^-1/2) into the factor copy (lower blocks) and a full symmetric copy
+// used by the FP64 refinement residual; rhs stays FP64.
 #include "device_common.hpp"
 #include "launchers.hpp"
+
+#include <algorithm>
 
 namespace gba_amd {
 
 namespace {
 
-constexpr int kSchurCB = 16;
+constexpr int kSchurThreads = 1024;
+constexpr int kSchurLdsCap = 128 * 1024;  // accumulator bytes per workgroup at most
 
 template <typename WT, typename ST>
-__global__ __launch_bounds__(256) void k_schur(DevProblem pb, const WT* __restrict__ W,
-                                               const double* __restrict__ Vinv6,
-                                               const double* __restrict__ VB,
-                                               const double* __restrict__ U,
-                                               const double* __restrict__ bc, double lambda,
-                                               const double* __restrict__ dscale,
-                                               ST* __restrict__ Sfac, ST* __restrict__ Sfull, int ld,
-                                               double* __restrict__ rhs, const TrialFlags* flags) {
-    __shared__ ST acc[kSchurCB * 81];
-    __shared__ double s_rhs[4][9];
+__global__ __launch_bounds__(kSchurThreads) void k_schur(
+    DevProblem pb, const int* __restrict__ slist, const WT* __restrict__ W,
+    const double* __restrict__ Vinv6, const double* __restrict__ VB, const double* __restrict__ U,
+    const double* __restrict__ bc, double lambda, const double* __restrict__ dscale,
+    ST* __restrict__ Sfac, ST* __restrict__ Sfull, int ld, double* __restrict__ rhs,
+    const TrialFlags* flags, int CB) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    ST* acc = reinterpret_cast<ST*>(smem);
+    __shared__ double s_rhs[kSchurThreads / 64][9];
     if (flags->nonspd) return;
     const int a = blockIdx.x;
-    const int lo = blockIdx.y * kSchurCB;
+    const int lo = blockIdx.y * CB;
     if (lo > a) return;
-    const int hi = min(lo + kSchurCB, a + 1);
+    const int hi = min(lo + CB, a + 1);
     const bool own_rhs = (a >= lo && a < hi);
     const int tid = threadIdx.x;
-    for (int t = tid; t < kSchurCB * 81; t += 256) acc[t] = ST(0);
+    const int nb = hi - lo;
+    for (int t = tid; t < nb * 81; t += kSchurThreads) acc[t] = ST(0);
     __syncthreads();
     double racc[9];
 #pragma unroll
     for (int i = 0; i < 9; ++i) racc[i] = 0.0;
     const int beg = pb.cam_off[a], end = pb.cam_off[a + 1];
-    for (int idx = beg + tid; idx < end; idx += 256) {
-        const int e = pb.cam_list[idx];
+    for (int idx = beg + tid; idx < end; idx += kSchurThreads) {
+        const int e = slist[idx];
         const int p = pb.spt[e];
         const double* vi = Vinv6 + 6 * static_cast<size_t>(p);
         const double V[9] = {vi[0], vi[1], vi[3], vi[1], vi[2], vi[4], vi[3], vi[4], vi[5]};
@@ -67,10 +73,12 @@ __global__ __launch_bounds__(256) void k_schur(DevProblem pb, const WT* __restri
         ST Ts[27];
 #pragma unroll
         for (int i = 0; i < 27; ++i) Ts[i] = static_cast<ST>(T[i]);
-        const int f0 = pb.pt_off[p], f1 = pb.pt_off[p + 1];
-        for (int f = f0; f < f1; ++f) {
+        const int f1 = pb.pt_off[p + 1];
+        int f = pb.pt_off[p];
+        while (f < f1 && pb.scam[f] < lo) ++f;
+        for (; f < f1; ++f) {
             const int b = pb.scam[f];
-            if (b < lo || b >= hi) continue;
+            if (b >= hi) break;
             const WT* Wf = W + 27 * static_cast<size_t>(f);
             ST wf[27];
 #pragma unroll
@@ -96,12 +104,13 @@ __global__ __launch_bounds__(256) void k_schur(DevProblem pb, const WT* __restri
     }
     __syncthreads();
     if (own_rhs && tid < 9) {
+        double sum = 0.0;
+        for (int w = 0; w < kSchurThreads / 64; ++w) sum += s_rhs[w][tid];
         const int row = 9 * a + tid;
-        const double v = bc[row] - ((s_rhs[0][tid] + s_rhs[1][tid]) + (s_rhs[2][tid] + s_rhs[3][tid]));
+        const double v = bc[row] - sum;
         rhs[row] = dscale ? v * dscale[row] : v;
     }
-    const int nb = hi - lo;
-    for (int t = tid; t < nb * 81; t += 256) {
+    for (int t = tid; t < nb * 81; t += kSchurThreads) {
         const int bb = t / 81, ij = t - 81 * bb;
         const int i = ij / 9, j = ij - 9 * i;
         const int b = lo + bb;
@@ -136,17 +145,50 @@ __global__ void k_diag_scale(int n, int npad, const double* __restrict__ U, doub
 
 }  // namespace
 
-void launch_schur(hipStream_t st, const DevProblem& pb, const double* W64, const float* W32,
-                  const double* Vinv6, const double* VB, const double* U, const double* bc,
-                  double lambda, const double* dscale, double* S64, float* S32fac, float* S32full,
-                  int ld, double* rhs, const TrialFlags* flags) {
-    dim3 grid(pb.C, (pb.C + kSchurCB - 1) / kSchurCB);
-    if (W32)
-        hipLaunchKernelGGL((k_schur<float, float>), grid, dim3(256), 0, st, pb, W32, Vinv6, VB, U, bc,
-                           lambda, dscale, S32fac, S32full, ld, rhs, flags);
-    else
-        hipLaunchKernelGGL((k_schur<double, double>), grid, dim3(256), 0, st, pb, W64, Vinv6, VB, U,
-                           bc, lambda, dscale, S64, static_cast<double*>(nullptr), ld, rhs, flags);
+namespace {
+// Accumulator bytes a workgroup may use: the device's per-block LDS limit
+// (160 KB on gfx950) minus the static part, capped at kSchurLdsCap.
+int schur_lds_bytes() {
+    static int bytes = -1;
+    if (bytes < 0) {
+        int dev = 0, v = 0;
+        (void)hipGetDevice(&dev);
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+            v = 64 * 1024;
+        bytes = std::min(kSchurLdsCap, v - 4096);
+        if (bytes > 64 * 1024 - 4096) {
+            if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_schur<float, float>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, bytes) != hipSuccess ||
+                hipFuncSetAttribute(reinterpret_cast<const void*>(&k_schur<double, double>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, bytes) != hipSuccess)
+                bytes = 64 * 1024 - 4096;
+        }
+        (void)hipGetLastError();
+    }
+    return bytes;
+}
+}  // namespace
+
+void launch_schur(hipStream_t st, const DevProblem& pb, const int* slist, const double* W64,
+                  const float* W32, const double* Vinv6, const double* VB, const double* U,
+                  const double* bc, double lambda, const double* dscale, double* S64, float* S32fac,
+                  float* S32full, int ld, double* rhs, const TrialFlags* flags) {
+    const int bytes = schur_lds_bytes();
+    if (W32) {
+        const int CB = std::min(pb.C, bytes / static_cast<int>(81 * sizeof(float)));
+        const int nch = (pb.C + CB - 1) / CB;
+        const size_t lds = static_cast<size_t>(CB) * 81 * sizeof(float);
+        hipLaunchKernelGGL((k_schur<float, float>), dim3(pb.C, nch), dim3(kSchurThreads), lds, st, pb,
+                           slist, W32, Vinv6, VB, U, bc, lambda, dscale, S32fac, S32full, ld, rhs,
+                           flags, CB);
+    } else {
+        const int CB = std::min(pb.C, bytes / static_cast<int>(81 * sizeof(double)));
+        const int nch = (pb.C + CB - 1) / CB;
+        const size_t lds = static_cast<size_t>(CB) * 81 * sizeof(double);
+        hipLaunchKernelGGL((k_schur<double, double>), dim3(pb.C, nch), dim3(kSchurThreads), lds, st,
+                           pb, slist, W64, Vinv6, VB, U, bc, lambda, dscale, S64,
+                           static_cast<double*>(nullptr), ld, rhs, flags, CB);
+    }
 }
 
 void launch_diag_scale(hipStream_t st, int n, int npad, const double* U, double lambda,

@@ -30,10 +30,12 @@ void launch_batch_inverse3(hipStream_t st, const double* A, double* Ainv, int* o
 void launch_identity_pad(hipStream_t st, double* S64, float* S32, int ld, int n, int npad);
 
 // k_schur.hip
-void launch_schur(hipStream_t st, const DevProblem& pb, const double* W64, const float* W32,
-                  const double* Vinv6, const double* VB, const double* U, const double* bc,
-                  double lambda, const double* dscale, double* S64, float* S32fac, float* S32full,
-                  int ld, double* rhs, const TrialFlags* flags);
+// slist: camera-major observation list with each camera's entries ordered by
+// decreasing observation count of their point.
+void launch_schur(hipStream_t st, const DevProblem& pb, const int* slist, const double* W64,
+                  const float* W32, const double* Vinv6, const double* VB, const double* U,
+                  const double* bc, double lambda, const double* dscale, double* S64, float* S32fac,
+                  float* S32full, int ld, double* rhs, const TrialFlags* flags);
 void launch_diag_scale(hipStream_t st, int n, int npad, const double* U, double lambda,
                        double* dscale);
 

@@ -59,6 +59,16 @@ Solver::Solver(const gba_problem_view& prob, const gba_solver_config& cfg) : cfg
     init_pts_.assign(prob.points, prob.points + 3 * static_cast<size_t>(P));
 
     // ---- host-side layout: point-major sort, work units, camera index ----
+    // sort key (point, camera, caller index): counting sort by camera, then a
+    // stable counting sort by point, so each point's observations are
+    // contiguous and ordered by camera
+    std::vector<int> by_cam(static_cast<size_t>(O));
+    {
+        std::vector<int> coff(static_cast<size_t>(C) + 1, 0);
+        for (int e = 0; e < O; ++e) coff[prob.obs_camera[e] + 1]++;
+        for (int c = 0; c < C; ++c) coff[c + 1] += coff[c];
+        for (int e = 0; e < O; ++e) by_cam[coff[prob.obs_camera[e]]++] = e;
+    }
     std::vector<int> pt_off(static_cast<size_t>(P) + 1, 0);
     for (int e = 0; e < O; ++e) pt_off[prob.obs_point[e] + 1]++;
     for (int p = 0; p < P; ++p) pt_off[p + 1] += pt_off[p];
@@ -66,7 +76,8 @@ Solver::Solver(const gba_problem_view& prob, const gba_solver_config& cfg) : cfg
     std::vector<double2> spx(static_cast<size_t>(O));
     {
         std::vector<int> fill(pt_off.begin(), pt_off.end() - 1);
-        for (int e = 0; e < O; ++e) {
+        for (int idx = 0; idx < O; ++idx) {
+            const int e = by_cam[idx];
             const int k = fill[prob.obs_point[e]]++;
             perm[k] = e;
             scam[k] = prob.obs_camera[e];
@@ -106,6 +117,16 @@ Solver::Solver(const gba_problem_view& prob, const gba_solver_config& cfg) : cfg
         std::vector<int> fill(cam_off.begin(), cam_off.end() - 1);
         for (int k = 0; k < O; ++k) cam_list[fill[scam[k]]++] = k;
     }
+    // Schur order: each camera's entries by decreasing observation count of
+    // their point (stable), so the wavefronts of the assembly get similar work
+    std::vector<int> slist(cam_list);
+    for (int c = 0; c < C; ++c)
+        std::stable_sort(slist.begin() + cam_off[c], slist.begin() + cam_off[c + 1],
+                         [&](int x, int y) {
+                             const int kx = pt_off[spt[x] + 1] - pt_off[spt[x]];
+                             const int ky = pt_off[spt[y] + 1] - pt_off[spt[y]];
+                             return kx > ky;
+                         });
     std::vector<int4> chunks;
     std::vector<int> cam_chunk_off(static_cast<size_t>(C) + 1, 0);
     for (int c = 0; c < C; ++c) {
@@ -126,6 +147,7 @@ Solver::Solver(const gba_problem_view& prob, const gba_solver_config& cfg) : cfg
     upload(spx_, spx, st_);
     upload(units_, units, st_);
     upload(cam_list_, cam_list, st_);
+    upload(slist_, slist, st_);
     upload(cam_off_, cam_off, st_);
     upload(chunks_, chunks, st_);
     upload(cam_chunk_off_, cam_chunk_off, st_);
@@ -240,7 +262,7 @@ void Solver::enqueue_trial_linear_algebra(double lambda, std::vector<double>* ho
     launch_clear_flags(st_, flags_.p);
     launch_damp_invert_v(st_, pb_.P, VB_.p, lambda, Vinv_.p, flags_.p);
     if (mixed_) launch_diag_scale(st_, pb_.n, pb_.npad, U_.p, lambda, dscale_.p);
-    launch_schur(st_, pb_, W64_.p, W32_.p, Vinv_.p, VB_.p, U_.p, bc_.p, lambda,
+    launch_schur(st_, pb_, slist_.p, W64_.p, W32_.p, Vinv_.p, VB_.p, U_.p, bc_.p, lambda,
                  mixed_ ? dscale_.p : nullptr, S64_.p, S32_.p, S32full_.p, ld, rhs_.p, flags_.p);
     if (hook_S || hook_rhs) {
         GBA_HIP(hipGetLastError());
@@ -406,7 +428,7 @@ double Solver::time_kernel(int kernel, double lambda, int reps) {
                 launch_lin_cameras(st_, pb_, cams_[cur_].p, pts_[cur_].p, gram_.p);
                 break;
             case GBA_KERNEL_SCHUR:
-                launch_schur(st_, pb_, W64_.p, W32_.p, Vinv_.p, VB_.p, U_.p, bc_.p, lambda,
+                launch_schur(st_, pb_, slist_.p, W64_.p, W32_.p, Vinv_.p, VB_.p, U_.p, bc_.p, lambda,
                              mixed_ ? dscale_.p : nullptr, S64_.p, S32_.p, S32full_.p, ld, rhs_.p,
                              flags_.p);
                 break;
@@ -441,7 +463,7 @@ double Solver::time_kernel(int kernel, double lambda, int reps) {
             launch_clear_flags(st_, flags_.p);
             launch_damp_invert_v(st_, pb_.P, VB_.p, lambda, Vinv_.p, flags_.p);
             if (mixed_) launch_diag_scale(st_, pb_.n, pb_.npad, U_.p, lambda, dscale_.p);
-            launch_schur(st_, pb_, W64_.p, W32_.p, Vinv_.p, VB_.p, U_.p, bc_.p, lambda,
+            launch_schur(st_, pb_, slist_.p, W64_.p, W32_.p, Vinv_.p, VB_.p, U_.p, bc_.p, lambda,
                          mixed_ ? dscale_.p : nullptr, S64_.p, S32_.p, S32full_.p, ld, rhs_.p, flags_.p);
             GBA_HIP(hipEventRecord(a, st_));
             enqueue();

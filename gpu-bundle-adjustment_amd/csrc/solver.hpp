@@ -87,7 +87,7 @@ private:
     double pair_count_ = 0.0;
     int cur_ = 0;  // index of the current parameter buffers
 
-    DevBuf<int> scam_, spt_, perm_, pt_off_, cam_list_, cam_off_, cam_chunk_off_;
+    DevBuf<int> scam_, spt_, perm_, pt_off_, cam_list_, slist_, cam_off_, cam_chunk_off_;
     DevBuf<double2> spx_;
     DevBuf<int2> units_;
     DevBuf<int4> chunks_;

@@ -217,6 +217,9 @@ int gba_synth_launch_bounds(int num_cameras, int num_points, unsigned long long 
                             gba_problem_view* out);
 
 /* ---- misc ---------------------------------------------------------------- */
+/* Waits for all work this library enqueued on the current device (for callers
+ * that launched the batched kernels on the default stream). */
+int gba_device_synchronize(void);
 const char* gba_last_error(void);
 const char* gba_version(void);
 

@@ -26,6 +26,19 @@ def load_gba():
     return mod
 
 
+@pytest.fixture(scope="session", autouse=True)
+def _torch_runtime_first():
+    """torch bundles its own HIP runtime: when both runtimes live in one process
+    torch's must initialise first, or it reports no device."""
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.init()
+    except Exception:
+        pass
+    yield
+
+
 @pytest.fixture(scope="session")
 def gba():
     return load_gba()

@@ -111,7 +111,7 @@ def test_dense_spd_solve_mixed_refines_to_fp64(gba, n):
     A = (M.T @ M + n * 0.05 * np.eye(n) + np.eye(n)).astype(np.float32).astype(np.float64)
     b = rng.normal(size=n)
     x, ok, it = gba.dense_spd_solve(A, b, gba.PRECISION_MIXED)
-    assert ok and it >= 1
+    assert ok and 1 <= it <= 6  # a wrong factor (e.g. a permuted MFMA layout) would not refine this fast
     assert np.linalg.norm(A @ x - b) <= 1e-10 * np.linalg.norm(b)
 
 
