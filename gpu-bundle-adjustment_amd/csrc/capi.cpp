@@ -192,8 +192,8 @@ int gba_dense_spd_solve(int n, int precision, const double* A, double* b, int* o
         const bool mixed = precision == GBA_PRECISION_MIXED;
         hipStream_t st;
         GBA_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-        DevBuf<double> S64, D64, rhs, y, x, r, d, blk;
-        DevBuf<float> S32, S32full, D32;
+        DevBuf<double> S64, D64, rhs, y, x, work;
+        DevBuf<float> S32, S32low, D32;
         DevBuf<int> sync;
         DevBuf<TrialFlags> flags;
         DevBuf<PassStatus> status;
@@ -209,19 +209,15 @@ int gba_dense_spd_solve(int n, int precision, const double* A, double* b, int* o
         rhs.alloc(np);
         y.alloc(np);
         x.alloc(np);
-        r.alloc(np);
-        d.alloc(np);
-        blk.alloc(2 * static_cast<size_t>(residual_partials(npad)));
         sync.alloc(2 * static_cast<size_t>(nblk) + 2);
         flags.alloc(1);
         status.alloc(1);
         GBA_HIP(hipMemcpyAsync(rhs.p, hb.data(), sizeof(double) * np, hipMemcpyHostToDevice, st));
         launch_clear_flags(st, flags.p);
-        PassStatus hs{};
         int iters = 0;
         if (!mixed) {
             S64.alloc(np * np);
-            D64.alloc(static_cast<size_t>(nblk) * kTile * kTile);
+            D64.alloc(2 * static_cast<size_t>(nblk) * kTile * kTile);
             GBA_HIP(hipMemcpyAsync(S64.p, h.data(), sizeof(double) * np * np, hipMemcpyHostToDevice, st));
             launch_cholesky(st, S64.p, nullptr, npad, nblk, D64.p, nullptr, flags.p);
             launch_chol_solve(st, S64.p, nullptr, npad, nblk, D64.p, nullptr, rhs.p, y.p, x.p, sync.p,
@@ -230,30 +226,23 @@ int gba_dense_spd_solve(int n, int precision, const double* A, double* b, int* o
             std::vector<float> hf(np * np);
             for (size_t i = 0; i < np * np; ++i) hf[i] = static_cast<float>(h[i]);
             S32.alloc(np * np);
-            S32full.alloc(np * np);
-            D32.alloc(static_cast<size_t>(nblk) * kTile * kTile);
+            S32low.alloc(np * np);
+            D32.alloc(2 * static_cast<size_t>(nblk) * kTile * kTile);
+            work.alloc(pcg_work(npad));
+            PassStatus* hs = nullptr;
+            GBA_HIP(hipHostMalloc(&hs, sizeof(PassStatus), hipHostMallocDefault));
             GBA_HIP(hipMemcpyAsync(S32.p, hf.data(), sizeof(float) * np * np, hipMemcpyHostToDevice, st));
-            GBA_HIP(hipMemcpyAsync(S32full.p, hf.data(), sizeof(float) * np * np, hipMemcpyHostToDevice, st));
+            GBA_HIP(hipMemcpyAsync(S32low.p, hf.data(), sizeof(float) * np * np, hipMemcpyHostToDevice, st));
             launch_cholesky(st, nullptr, S32.p, npad, nblk, nullptr, D32.p, flags.p);
-            launch_chol_solve(st, nullptr, S32.p, npad, nblk, nullptr, D32.p, rhs.p, y.p, x.p, sync.p,
-                              flags.p);
-            launch_residual(st, S32full.p, npad, npad, rhs.p, x.p, r.p, blk.p, status.p, true, flags.p);
-            GBA_HIP(hipMemcpyAsync(&hs, status.p, sizeof hs, hipMemcpyDeviceToHost, st));
-            GBA_HIP(hipStreamSynchronize(st));
-            double res2 = hs.res_norm2, rhs2 = hs.rhs_norm2, prev = res2;
-            while (!hs.flags.chol_fail && iters < 20 && res2 > 1e-20 * rhs2 && std::isfinite(res2)) {
-                launch_chol_solve(st, nullptr, S32.p, npad, nblk, nullptr, D32.p, r.p, y.p, d.p, sync.p,
-                                  flags.p);
-                launch_vec_axpy(st, npad, x.p, d.p);
-                launch_residual(st, S32full.p, npad, npad, rhs.p, x.p, r.p, blk.p, status.p, false,
-                                flags.p);
-                GBA_HIP(hipMemcpyAsync(&hs, status.p, sizeof hs, hipMemcpyDeviceToHost, st));
-                GBA_HIP(hipStreamSynchronize(st));
-                ++iters;
-                res2 = hs.res_norm2;
-                if (!(res2 < prev)) break;
-                prev = res2;
+            double rel = 0.0;
+            try {
+                iters = pcg_solve(st, S32low.p, S32.p, D32.p, npad, npad, rhs.p, x.p, work.p, sync.p,
+                                  flags.p, status.p, hs, 1e-12, 50, &rel);
+            } catch (...) {
+                (void)hipHostFree(hs);
+                throw;
             }
+            (void)hipHostFree(hs);
         }
         TrialFlags hf{};
         GBA_HIP(hipGetLastError());

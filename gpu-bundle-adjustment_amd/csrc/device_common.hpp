@@ -50,8 +50,7 @@ struct TrialFlags {
 struct PassStatus {
     double sse;        // sum of squared residuals (finite edges)
     double denom;      // gain-ratio denominator (trial pass only)
-    double res_norm2;  // refinement residual norm^2 (mixed precision)
-    double rhs_norm2;  // refinement right-hand side norm^2
+    double dot[2];     // refinement dot products (mixed precision)
     unsigned long long anomalies;  // non-finite edges in this pass
     TrialFlags flags;
 };

@@ -7,21 +7,23 @@
 // of the padded matrix is block-diagonal [L, I].
 //
 // Right-looking blocked Cholesky on 64x64 tiles, per panel k:
-//   panel  : one wavefront factors the diagonal tile in registers (lane = row,
-//            pivots broadcast with v_readlane), every other wavefront of the
-//            launch solves one tile of the panel column against it (TRSM,
-//            lane = row, L_kk broadcast from LDS);
-//   update : A_ij -= L_ik L_jk^T for all trailing lower tiles, one workgroup per
-//            tile, four wavefronts of 2x2 MFMA 16x16 blocks
-//            (v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32), operands from LDS.
+//   potrf : one wavefront factors the diagonal tile in registers (lane = row,
+//           pivots broadcast with v_readlane) and inverts it (lane = column,
+//           forward substitution against the factor in LDS); L_kk and
+//           L_kk^-T are kept in separate tile arrays;
+//   trsm  : L_ik = A_ik L_kk^-T for every tile below, an MFMA GEMM per tile;
+//   update: A_ij -= L_ik L_jk^T for all trailing lower tiles, one workgroup per
+//           tile, four wavefronts of 2x2 MFMA 16x16 blocks
+//           (v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32), operands from LDS.
 // A pivot that is not > 0 sets TrialFlags::chol_fail (the LM then escalates
 // lambda, as the reference does on LLT failure).
 //
 // Triangular solves: one persistent launch per direction; workgroups take
 // tile rows in dependency order through an atomic ticket, so a workgroup only
 // ever waits on tile rows already owned by running workgroups; tile-row
-// results are published with release/acquire flags at agent scope.  Waits are
-// bounded (TrialFlags::solve_err on time-out), so every wave exits.
+// results are published with release/acquire flags at agent scope; the
+// diagonal block is applied as a product with the stored inverse tile.  Waits
+// are bounded (TrialFlags::solve_err on time-out), so every wave exits.
 #include "device_common.hpp"
 #include "launchers.hpp"
 
@@ -62,57 +64,107 @@ __device__ __forceinline__ bool any_fail(const TrialFlags* f) {
     return f->nonspd != 0 || f->chol_fail != 0;
 }
 
+// Factor + invert the diagonal tile k.  D[k] receives L_kk (row-major, upper
+// zero), DinvT[k] receives (L_kk^-1)^T (row c holds column c of the inverse).
+// One wavefront: lane r owns row r in registers; at step j the pivot comes from
+// lane j (v_readlane) and the scaled column j is broadcast through LDS.
 template <typename T>
-__global__ __launch_bounds__(256) void k_chol_panel(T* __restrict__ S, int ld, int nblk, int k,
-                                                    T* __restrict__ Ldiag, TrialFlags* flags) {
-    __shared__ T Lkk[kTile][kTile + 1];
-    __shared__ T invd[kTile];
+__global__ __launch_bounds__(64) void k_potrf_tile(const T* __restrict__ S, int ld, int k,
+                                                   T* __restrict__ D, T* __restrict__ DinvT,
+                                                   TrialFlags* flags) {
+    __shared__ T col[kTile];
+    __shared__ T L[kTile][kTile + 1];
     if (any_fail(flags)) return;
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    if (w == 0) {
+    const int lane = threadIdx.x;
+    bool bad = false;
+    {
         const T* row = S + static_cast<size_t>(k * kTile + lane) * ld + k * kTile;
         T a[kTile];
 #pragma unroll
         for (int c = 0; c < kTile; ++c) a[c] = row[c];
-        bool bad = false;
 #pragma unroll
         for (int j = 0; j < kTile; ++j) {
             const T d = readlane(a[j], j);
             bad = bad || !(d > T(0));
             const T piv = sqrt(d);
-            const T inv = T(1) / piv;
-            a[j] = (lane == j) ? piv : a[j] * inv;
+            const T lj = (lane == j) ? piv : a[j] / piv;
+            a[j] = lj;
+            col[lane] = lj;
+            __syncthreads();
 #pragma unroll
-            for (int m = j + 1; m < kTile; ++m) a[m] -= a[j] * readlane(a[j], m);
+            for (int m = j + 1; m < kTile; ++m) a[m] -= lj * col[m];
+            __syncthreads();
         }
+        T* drow = D + static_cast<size_t>(k) * kTile * kTile + lane * kTile;
 #pragma unroll
-        for (int c = 0; c < kTile; ++c) Lkk[lane][c] = (c <= lane) ? a[c] : T(0);
-        invd[lane] = T(1) / a[lane];
-        // every workgroup read A_kk above; only the diagonal-tile array
-        // receives L_kk, so no workgroup can observe a half-written tile
-        if (blockIdx.x == 0) {
-            T* dst = Ldiag + static_cast<size_t>(k) * kTile * kTile + lane * kTile;
-#pragma unroll
-            for (int c = 0; c < kTile; ++c) dst[c] = (c <= lane) ? a[c] : T(0);
-            if (bad && lane == 0) atomicOr(&flags->chol_fail, 1);
+        for (int c = 0; c < kTile; ++c) {
+            const T v = (c <= lane) ? a[c] : T(0);
+            L[lane][c] = v;
+            drow[c] = v;
         }
     }
+    if (bad && lane == 0) atomicOr(&flags->chol_fail, 1);
     __syncthreads();
-    const int i = k + 1 + blockIdx.x * 4 + w;
-    if (i >= nblk) return;
-    T* row = S + static_cast<size_t>(i * kTile + lane) * ld + k * kTile;
+    // column `lane` of L^-1: right-looking forward substitution on e_lane
     T x[kTile];
 #pragma unroll
-    for (int c = 0; c < kTile; ++c) x[c] = row[c];
-    // x L_kk^T = a, column by column
+    for (int i = 0; i < kTile; ++i) x[i] = (i == lane) ? T(1) : T(0);
 #pragma unroll
-    for (int j = 0; j < kTile; ++j) {
-        x[j] *= invd[j];
+    for (int i = 0; i < kTile; ++i) {
+        x[i] /= L[i][i];
+        const T xi = x[i];
 #pragma unroll
-        for (int m = j + 1; m < kTile; ++m) x[m] -= x[j] * Lkk[m][j];
+        for (int m = i + 1; m < kTile; ++m) x[m] -= L[m][i] * xi;
     }
+    T* irow = DinvT + static_cast<size_t>(k) * kTile * kTile + lane * kTile;
 #pragma unroll
-    for (int c = 0; c < kTile; ++c) row[c] = x[c];
+    for (int i = 0; i < kTile; ++i) irow[i] = (i >= lane) ? x[i] : T(0);
+}
+
+// L_ik = A_ik (L_kk^-1)^T for tile rows i > k: an MFMA GEMM per tile.
+template <typename T>
+__global__ __launch_bounds__(256) void k_trsm_tiles(T* __restrict__ S, int ld, int k,
+                                                    const T* __restrict__ DinvT,
+                                                    const TrialFlags* flags) {
+    __shared__ T Ai[kTile][kTile + 2];
+    __shared__ T Bi[kTile][kTile + 2];  // Bi[c][m] = (L_kk^-1)[c][m]
+    if (any_fail(flags)) return;
+    const int i = k + 1 + blockIdx.x;
+    T* Aik = S + static_cast<size_t>(i * kTile) * ld + k * kTile;
+    const T* inv = DinvT + static_cast<size_t>(k) * kTile * kTile;
+    for (int q = threadIdx.x; q < kTile * kTile; q += 256) {
+        const int r = q >> 6, c = q & 63;
+        Ai[r][c] = Aik[static_cast<size_t>(r) * ld + c];
+        Bi[c][r] = inv[q];  // inv[r][c] = Linv[c][r]
+    }
+    __syncthreads();
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
+    const int li = lane & 15, lk = lane >> 4;
+    typename Acc4<T>::type acc[2][2];
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int c = 0; c < 2; ++c) acc[r][c] = typename Acc4<T>::type{0, 0, 0, 0};
+#pragma unroll 4
+    for (int kk = 0; kk < kTile; kk += 4) {
+        const T a0 = Ai[wr + li][kk + lk], a1 = Ai[wr + 16 + li][kk + lk];
+        const T b0 = Bi[wc + li][kk + lk], b1 = Bi[wc + 16 + li][kk + lk];
+        acc[0][0] = mfma16(a0, b0, acc[0][0]);
+        acc[0][1] = mfma16(a0, b1, acc[0][1]);
+        acc[1][0] = mfma16(a1, b0, acc[1][0]);
+        acc[1][1] = mfma16(a1, b1, acc[1][1]);
+    }
+    __syncthreads();  // every wave has read Ai before the tile is overwritten
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int row = wr + 16 * r + acc_row<T>(lk, q), col = wc + 16 * c + li;
+                Aik[static_cast<size_t>(row) * ld + col] = acc[r][c][q];
+            }
 }
 
 template <typename T>
@@ -166,14 +218,16 @@ __global__ __launch_bounds__(256) void k_chol_update(T* __restrict__ S, int ld, 
 }
 
 // Forward solve L y = b.  Workgroup with ticket t owns tile row i = t.
+// DinvT[i] holds (L_ii^-1)^T.
 template <typename T>
 __global__ __launch_bounds__(256) void k_trsv_fwd(const T* __restrict__ L, int ld, int nblk,
-                                                  const T* __restrict__ Ldiag,
+                                                  const T* __restrict__ DinvT,
                                                   const double* __restrict__ b, double* y, int* done,
                                                   int* ticket, TrialFlags* flags) {
     __shared__ int s_blk, s_skip;
     __shared__ double s_part[4][16];
     __shared__ double s_rhs[kTile];
+    __shared__ T s_inv[kTile][kTile + 1];
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
     if (tid == 0) {
         s_blk = atomicAdd(ticket, 1);
@@ -182,17 +236,24 @@ __global__ __launch_bounds__(256) void k_trsv_fwd(const T* __restrict__ L, int l
     __syncthreads();
     if (s_skip) return;
     const int i = s_blk;
+    {
+        const T* inv = DinvT + static_cast<size_t>(i) * kTile * kTile;
+        for (int q = tid; q < kTile * kTile; q += 256) s_inv[q >> 6][q & 63] = inv[q];
+    }
     double acc[16];
 #pragma unroll
     for (int r = 0; r < 16; ++r) acc[r] = 0.0;
     for (int j = 0; j < i; ++j) {
+        const T* Lr = L + static_cast<size_t>(i * kTile + 16 * w) * ld + j * kTile + lane;
+        T lv[16];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) lv[r] = Lr[static_cast<size_t>(r) * ld];  // independent of y
         if (tid == 0 && !wait_flag(done + j, 1)) atomicOr(&flags->solve_err, 1);
         __syncthreads();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         const double yj = y[j * kTile + lane];
-        const T* Lr = L + static_cast<size_t>(i * kTile + 16 * w) * ld + j * kTile + lane;
 #pragma unroll
-        for (int r = 0; r < 16; ++r) acc[r] += static_cast<double>(Lr[static_cast<size_t>(r) * ld]) * yj;
+        for (int r = 0; r < 16; ++r) acc[r] += static_cast<double>(lv[r]) * yj;
     }
     double mine = 0.0;
 #pragma unroll
@@ -205,17 +266,10 @@ __global__ __launch_bounds__(256) void k_trsv_fwd(const T* __restrict__ L, int l
     if (tid < kTile) s_rhs[tid] = b[i * kTile + tid] - s_part[tid >> 4][tid & 15];
     __syncthreads();
     if (w == 0) {
-        const T* Lrow = Ldiag + static_cast<size_t>(i) * kTile * kTile + lane * kTile;
-        T Lr[kTile];
-#pragma unroll
-        for (int c = 0; c < kTile; ++c) Lr[c] = Lrow[c];
-        double v = s_rhs[lane];
-#pragma unroll
-        for (int j = 0; j < kTile; ++j) {
-            const double yj = readlane(v, j) / static_cast<double>(readlane(Lr[j], j));
-            if (lane == j) v = yj;
-            else if (lane > j) v -= static_cast<double>(Lr[j]) * yj;
-        }
+        // y_r = sum_c Linv[r][c] s_c = sum_c DinvT[c][r] s_c
+        double v = 0.0;
+#pragma unroll 16
+        for (int c = 0; c < kTile; ++c) v += static_cast<double>(s_inv[c][lane]) * s_rhs[c];
         y[i * kTile + lane] = v;
         if (lane == 0) __hip_atomic_store(done + i, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     }
@@ -224,11 +278,13 @@ __global__ __launch_bounds__(256) void k_trsv_fwd(const T* __restrict__ L, int l
 // Backward solve L^T x = y.  Ticket t owns tile row i = nblk - 1 - t.
 template <typename T>
 __global__ __launch_bounds__(256) void k_trsv_bwd(const T* __restrict__ L, int ld, int nblk,
-                                                  const T* __restrict__ Ldiag,
+                                                  const T* __restrict__ DinvT,
                                                   const double* __restrict__ y, double* x, int* done,
                                                   int* ticket, TrialFlags* flags) {
     __shared__ int s_blk, s_skip;
     __shared__ double s_part[4][kTile];
+    __shared__ double s_rhs[kTile];
+    __shared__ T s_inv[kTile][kTile + 1];
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
     if (tid == 0) {
         s_blk = nblk - 1 - atomicAdd(ticket, 1);
@@ -237,94 +293,158 @@ __global__ __launch_bounds__(256) void k_trsv_bwd(const T* __restrict__ L, int l
     __syncthreads();
     if (s_skip) return;
     const int i = s_blk;
+    {
+        const T* inv = DinvT + static_cast<size_t>(i) * kTile * kTile;
+        for (int q = tid; q < kTile * kTile; q += 256) s_inv[q >> 6][q & 63] = inv[q];
+    }
     double acc = 0.0;
     for (int j = nblk - 1; j > i; --j) {
+        const T* Lc = L + static_cast<size_t>(j * kTile + 16 * w) * ld + i * kTile + lane;
+        T lv[16];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) lv[r] = Lc[static_cast<size_t>(r) * ld];
         if (tid == 0 && !wait_flag(done + j, 1)) atomicOr(&flags->solve_err, 1);
         __syncthreads();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        const T* Lc = L + static_cast<size_t>(j * kTile + 16 * w) * ld + i * kTile + lane;
         const double* xj = x + j * kTile + 16 * w;
 #pragma unroll
-        for (int r = 0; r < 16; ++r) acc += static_cast<double>(Lc[static_cast<size_t>(r) * ld]) * xj[r];
+        for (int r = 0; r < 16; ++r) acc += static_cast<double>(lv[r]) * xj[r];
     }
     s_part[w][lane] = acc;
     __syncthreads();
+    if (tid < kTile)
+        s_rhs[tid] = y[i * kTile + tid] -
+                     ((s_part[0][tid] + s_part[1][tid]) + (s_part[2][tid] + s_part[3][tid]));
+    __syncthreads();
     if (w == 0) {
-        double s = y[i * kTile + lane] - ((s_part[0][lane] + s_part[1][lane]) + (s_part[2][lane] + s_part[3][lane]));
-        // lane r holds column r of the diagonal tile: Lc[j] = L[j][r], j >= r
-        const T* Lcol = Ldiag + static_cast<size_t>(i) * kTile * kTile + lane;
-        T Lc[kTile];
-#pragma unroll
-        for (int j = 0; j < kTile; ++j) Lc[j] = Lcol[j * kTile];
-#pragma unroll
-        for (int j = kTile - 1; j >= 0; --j) {
-            const double xjv = readlane(s, j) / static_cast<double>(readlane(Lc[j], j));
-            if (lane == j) s = xjv;
-            else if (lane < j) s -= static_cast<double>(Lc[j]) * xjv;
-        }
-        x[i * kTile + lane] = s;
+        // x_r = sum_c Linv[c][r] s_c = sum_c DinvT[r][c] s_c
+        double v = 0.0;
+#pragma unroll 16
+        for (int c = 0; c < kTile; ++c) v += static_cast<double>(s_inv[lane][c]) * s_rhs[c];
+        x[i * kTile + lane] = v;
         if (lane == 0) __hip_atomic_store(done + i, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
-// r = b - A x (A full symmetric FP32, FP64 accumulation), one wave per row;
-// per-workgroup partial of |r|^2 (and |b|^2 when asked).
-__global__ __launch_bounds__(256) void k_residual(const float* __restrict__ A, int ld, int npad,
-                                                  const double* __restrict__ b,
-                                                  const double* __restrict__ x, double* __restrict__ r,
-                                                  double* __restrict__ blk) {
-    __shared__ double s_r[4], s_b[4];
+// Symmetric product y = A x from the lower triangle of A (FP32, row-major,
+// FP64 accumulation), no atomics.  Pass 1: one workgroup per lower tile (i, j)
+// writes the tile's row partial A_ij x_j (block i) and column partial
+// A_ij^T x_i (block j; strictly lower part for diagonal tiles).  Pass 2: one
+// workgroup per 64-block sums its partials in a fixed order.
+__global__ __launch_bounds__(256) void k_symv_tiles(const float* __restrict__ A, int ld,
+                                                    const double* __restrict__ x,
+                                                    double* __restrict__ rp, double* __restrict__ cp) {
+    __shared__ double s_col[4][kTile];
+    __shared__ double s_row[kTile];
+    const int t = blockIdx.x;
+    int i = static_cast<int>((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+    while (i * (i + 1) / 2 > t) --i;
+    while ((i + 1) * (i + 2) / 2 <= t) ++i;
+    const int j = t - i * (i + 1) / 2;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int row = blockIdx.x * 4 + w;
-    double rr = 0.0, bb = 0.0;
-    if (row < npad) {
-        const float* Ar = A + static_cast<size_t>(row) * ld;
-        double s = 0.0;
-        for (int c = lane; c < npad; c += 64) s += static_cast<double>(Ar[c]) * x[c];
-        s = wave_sum(s);
-        const double bv = b[row];
-        const double v = bv - s;
-        if (lane == 0) r[row] = v;
-        rr = v * v;
-        bb = bv * bv;
+    const bool diag = (i == j);
+    const double xj = x[j * kTile + lane];
+    double cacc = 0.0;
+    for (int rr = 0; rr < 16; ++rr) {
+        const int r = 16 * w + rr;
+        const double v = static_cast<double>(A[static_cast<size_t>(i * kTile + r) * ld + j * kTile + lane]);
+        const double xi = x[i * kTile + r];
+        const double rs = wave_sum((!diag || lane <= r) ? v * xj : 0.0);
+        if (lane == 0) s_row[r] = rs;
+        cacc += (!diag || lane < r) ? v * xi : 0.0;
     }
-    if (lane == 0) {
-        s_r[w] = rr;
-        s_b[w] = bb;
-    }
+    s_col[w][lane] = cacc;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        blk[2 * blockIdx.x] = (s_r[0] + s_r[1]) + (s_r[2] + s_r[3]);
-        blk[2 * blockIdx.x + 1] = (s_b[0] + s_b[1]) + (s_b[2] + s_b[3]);
+    if (threadIdx.x < kTile) {
+        rp[static_cast<size_t>(t) * kTile + threadIdx.x] = s_row[threadIdx.x];
+        cp[static_cast<size_t>(t) * kTile + threadIdx.x] =
+            (s_col[0][threadIdx.x] + s_col[1][threadIdx.x]) + (s_col[2][threadIdx.x] + s_col[3][threadIdx.x]);
     }
 }
 
-__global__ __launch_bounds__(256) void k_residual_reduce(int n, const double* __restrict__ blk,
-                                                         PassStatus* status, int also_rhs,
-                                                         const TrialFlags* flags) {
-    __shared__ double s_r[256], s_b[256];
-    const int tid = threadIdx.x;
-    double a = 0.0, c = 0.0;
-    for (int u = tid; u < n; u += 256) {
-        a += blk[2 * u];
-        c += blk[2 * u + 1];
+// y_b = sum_{j <= b} rp[(b, j)] + sum_{i > b} cp[(i, b)].
+__global__ __launch_bounds__(64) void k_symv_reduce(int nblk, const double* __restrict__ rp,
+                                                    const double* __restrict__ cp,
+                                                    double* __restrict__ y) {
+    const int b = blockIdx.x, lane = threadIdx.x;
+    double s = 0.0;
+    for (int j = 0; j <= b; ++j) s += rp[static_cast<size_t>(b * (b + 1) / 2 + j) * kTile + lane];
+    for (int i = b; i < nblk; ++i) s += cp[static_cast<size_t>(i * (i + 1) / 2 + b) * kTile + lane];
+    y[b * kTile + lane] = s;
+}
+
+// Per-workgroup partials of a.b and c.d (FP64).
+__global__ __launch_bounds__(256) void k_dot2(int n, const double* __restrict__ a,
+                                              const double* __restrict__ b,
+                                              const double* __restrict__ c,
+                                              const double* __restrict__ d, double* __restrict__ blk) {
+    __shared__ double s0[4], s1[4];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    double u = 0.0, v = 0.0;
+    if (i < n) {
+        u = a[i] * b[i];
+        v = c[i] * d[i];
     }
-    s_r[tid] = a;
-    s_b[tid] = c;
+    u = wave_sum(u);
+    v = wave_sum(v);
+    if ((threadIdx.x & 63) == 0) {
+        s0[threadIdx.x >> 6] = u;
+        s1[threadIdx.x >> 6] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        blk[2 * blockIdx.x] = (s0[0] + s0[1]) + (s0[2] + s0[3]);
+        blk[2 * blockIdx.x + 1] = (s1[0] + s1[1]) + (s1[2] + s1[3]);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_dot2_reduce(int nb, const double* __restrict__ blk,
+                                                     PassStatus* status, const TrialFlags* flags) {
+    __shared__ double s0[256], s1[256];
+    const int tid = threadIdx.x;
+    double u = 0.0, v = 0.0;
+    for (int q = tid; q < nb; q += 256) {
+        u += blk[2 * q];
+        v += blk[2 * q + 1];
+    }
+    s0[tid] = u;
+    s1[tid] = v;
     __syncthreads();
     for (int h = 128; h > 0; h >>= 1) {
         if (tid < h) {
-            s_r[tid] += s_r[tid + h];
-            s_b[tid] += s_b[tid + h];
+            s0[tid] += s0[tid + h];
+            s1[tid] += s1[tid + h];
         }
         __syncthreads();
     }
     if (tid == 0) {
-        status->res_norm2 = s_r[0];
-        if (also_rhs) status->rhs_norm2 = s_b[0];
+        status->dot[0] = s0[0];
+        status->dot[1] = s1[0];
         status->flags = *flags;
     }
 }
+
+// x += alpha p ; r -= alpha q
+__global__ void k_pcg_update(int n, double alpha, double* __restrict__ x, const double* __restrict__ p,
+                             double* __restrict__ r, const double* __restrict__ q) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) {
+        x[i] += alpha * p[i];
+        r[i] -= alpha * q[i];
+    }
+}
+
+// p = z + beta p
+__global__ void k_pcg_direction(int n, double beta, const double* __restrict__ z, double* __restrict__ p) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = z[i] + beta * p[i];
+}
+
+__global__ void k_vec_copy(int n, const double* __restrict__ a, double* __restrict__ b) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) b[i] = a[i];
+}
+
 
 __global__ void k_vec_axpy(int n, double* __restrict__ x, const double* __restrict__ d) {
     const int i = blockIdx.x * 256 + threadIdx.x;
@@ -341,14 +461,20 @@ __global__ void k_vec_scale(int n, int npad, const double* __restrict__ in,
 
 void launch_cholesky_panel(hipStream_t st, double* S64, float* S32, int ld, int nblk, int k,
                            double* D64, float* D32, TrialFlags* flags) {
-    const int rest = nblk - k - 1;
-    const int grid = rest > 0 ? (rest + 3) / 4 : 1;
-    if (S64)
-        hipLaunchKernelGGL(k_chol_panel<double>, dim3(grid), dim3(256), 0, st, S64, ld, nblk, k, D64,
-                           flags);
-    else
-        hipLaunchKernelGGL(k_chol_panel<float>, dim3(grid), dim3(256), 0, st, S32, ld, nblk, k, D32,
-                           flags);
+    const size_t tile = static_cast<size_t>(kTile) * kTile;
+    if (S64) {
+        hipLaunchKernelGGL(k_potrf_tile<double>, dim3(1), dim3(64), 0, st, S64, ld, k, D64,
+                           D64 + static_cast<size_t>(nblk) * tile, flags);
+        if (k + 1 < nblk)
+            hipLaunchKernelGGL(k_trsm_tiles<double>, dim3(nblk - k - 1), dim3(256), 0, st, S64, ld, k,
+                               D64 + static_cast<size_t>(nblk) * tile, flags);
+    } else {
+        hipLaunchKernelGGL(k_potrf_tile<float>, dim3(1), dim3(64), 0, st, S32, ld, k, D32,
+                           D32 + static_cast<size_t>(nblk) * tile, flags);
+        if (k + 1 < nblk)
+            hipLaunchKernelGGL(k_trsm_tiles<float>, dim3(nblk - k - 1), dim3(256), 0, st, S32, ld, k,
+                               D32 + static_cast<size_t>(nblk) * tile, flags);
+    }
 }
 
 void launch_cholesky_update(hipStream_t st, double* S64, float* S32, int ld, int nblk, int k,
@@ -377,29 +503,57 @@ void launch_chol_solve(hipStream_t st, const double* L64, const float* L32, int 
     int* done_f = sync;
     int* done_b = sync + nblk;
     int* tickets = sync + 2 * nblk;
+    const size_t inv_off = static_cast<size_t>(nblk) * kTile * kTile;
     if (L64) {
-        hipLaunchKernelGGL(k_trsv_fwd<double>, dim3(nblk), dim3(256), 0, st, L64, ld, nblk, D64, b,
-                           y, done_f, tickets, flags);
-        hipLaunchKernelGGL(k_trsv_bwd<double>, dim3(nblk), dim3(256), 0, st, L64, ld, nblk, D64, y,
-                           x, done_b, tickets + 1, flags);
+        hipLaunchKernelGGL(k_trsv_fwd<double>, dim3(nblk), dim3(256), 0, st, L64, ld, nblk,
+                           D64 + inv_off, b, y, done_f, tickets, flags);
+        hipLaunchKernelGGL(k_trsv_bwd<double>, dim3(nblk), dim3(256), 0, st, L64, ld, nblk,
+                           D64 + inv_off, y, x, done_b, tickets + 1, flags);
     } else {
-        hipLaunchKernelGGL(k_trsv_fwd<float>, dim3(nblk), dim3(256), 0, st, L32, ld, nblk, D32, b,
-                           y, done_f, tickets, flags);
-        hipLaunchKernelGGL(k_trsv_bwd<float>, dim3(nblk), dim3(256), 0, st, L32, ld, nblk, D32, y,
-                           x, done_b, tickets + 1, flags);
+        hipLaunchKernelGGL(k_trsv_fwd<float>, dim3(nblk), dim3(256), 0, st, L32, ld, nblk,
+                           D32 + inv_off, b, y, done_f, tickets, flags);
+        hipLaunchKernelGGL(k_trsv_bwd<float>, dim3(nblk), dim3(256), 0, st, L32, ld, nblk,
+                           D32 + inv_off, y, x, done_b, tickets + 1, flags);
     }
 }
 
-int residual_partials(int npad) { return (npad + 3) / 4; }
+int residual_partials(int npad) { return (npad + 255) / 256; }
 
-void launch_residual(hipStream_t st, const float* A, int ld, int npad, const double* b,
-                     const double* x, double* r, double* blk_norm2, PassStatus* status,
-                     bool also_rhs_norm, const TrialFlags* flags) {
-    const int grid = residual_partials(npad);
-    hipLaunchKernelGGL(k_residual, dim3(grid), dim3(256), 0, st, A, ld, npad, b, x, r, blk_norm2);
-    hipLaunchKernelGGL(k_residual_reduce, dim3(1), dim3(256), 0, st, grid, blk_norm2, status,
-                       also_rhs_norm ? 1 : 0, flags);
+size_t symv_work(int npad) {
+    const size_t nblk = static_cast<size_t>(npad / kTile);
+    return 2 * (nblk * (nblk + 1) / 2) * kTile;
 }
+
+void launch_symv(hipStream_t st, const float* A, int ld, int npad, const double* x, double* y,
+                 double* work) {
+    const int nblk = npad / kTile;
+    const int tiles = nblk * (nblk + 1) / 2;
+    double* rp = work;
+    double* cp = work + static_cast<size_t>(tiles) * kTile;
+    hipLaunchKernelGGL(k_symv_tiles, dim3(tiles), dim3(256), 0, st, A, ld, x, rp, cp);
+    hipLaunchKernelGGL(k_symv_reduce, dim3(nblk), dim3(64), 0, st, nblk, rp, cp, y);
+}
+
+void launch_dot2(hipStream_t st, int n, const double* a, const double* b, const double* c,
+                 const double* d, double* blk, PassStatus* status, const TrialFlags* flags) {
+    const int grid = residual_partials(n);
+    hipLaunchKernelGGL(k_dot2, dim3(grid), dim3(256), 0, st, n, a, b, c, d, blk);
+    hipLaunchKernelGGL(k_dot2_reduce, dim3(1), dim3(256), 0, st, grid, blk, status, flags);
+}
+
+void launch_pcg_update(hipStream_t st, int n, double alpha, double* x, const double* p, double* r,
+                       const double* q) {
+    hipLaunchKernelGGL(k_pcg_update, dim3((n + 255) / 256), dim3(256), 0, st, n, alpha, x, p, r, q);
+}
+
+void launch_pcg_direction(hipStream_t st, int n, double beta, const double* z, double* p) {
+    hipLaunchKernelGGL(k_pcg_direction, dim3((n + 255) / 256), dim3(256), 0, st, n, beta, z, p);
+}
+
+void launch_vec_copy(hipStream_t st, int n, const double* a, double* b) {
+    hipLaunchKernelGGL(k_vec_copy, dim3((n + 255) / 256), dim3(256), 0, st, n, a, b);
+}
+
 
 void launch_vec_axpy(hipStream_t st, int n, double* x, const double* d) {
     hipLaunchKernelGGL(k_vec_axpy, dim3((n + 255) / 256), dim3(256), 0, st, n, x, d);

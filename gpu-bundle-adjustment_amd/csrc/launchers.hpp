@@ -35,13 +35,14 @@ void launch_identity_pad(hipStream_t st, double* S64, float* S32, int ld, int n,
 void launch_schur(hipStream_t st, const DevProblem& pb, const int* slist, const double* W64,
                   const float* W32, const double* Vinv6, const double* VB, const double* U,
                   const double* bc, double lambda, const double* dscale, double* S64, float* S32fac,
-                  float* S32full, int ld, double* rhs, const TrialFlags* flags);
+                  float* S32low, int ld, double* rhs, const TrialFlags* flags);
 void launch_diag_scale(hipStream_t st, int n, int npad, const double* U, double lambda,
                        double* dscale);
 
 // k_dense.hip
-// Factors in place; the diagonal tiles of L go to D (nblk tiles of 64x64,
-// row-major, upper part zero), the off-diagonal tiles replace S's.
+// Factors in place; D holds 2*nblk tiles of 64x64: tiles [0, nblk) receive the
+// diagonal tiles of L (row-major, upper part zero), tiles [nblk, 2 nblk) their
+// inverses transposed; the off-diagonal tiles of L replace S's.
 void launch_cholesky(hipStream_t st, double* S64, float* S32, int ld, int nblk, double* D64,
                      float* D32, TrialFlags* flags);
 void launch_cholesky_panel(hipStream_t st, double* S64, float* S32, int ld, int nblk, int k,
@@ -53,10 +54,19 @@ void launch_cholesky_update(hipStream_t st, double* S64, float* S32, int ld, int
 void launch_chol_solve(hipStream_t st, const double* L64, const float* L32, int ld, int nblk,
                        const double* D64, const float* D32, const double* b, double* y, double* x,
                        int* sync, TrialFlags* flags);
-void launch_residual(hipStream_t st, const float* A, int ld, int npad, const double* b,
-                     const double* x, double* r, double* blk_norm2, PassStatus* status,
-                     bool also_rhs_norm, const TrialFlags* flags);
+// y = A x, A symmetric given by the lower triangle of an FP32 row-major
+// matrix (FP64 accumulation); work holds symv_work(npad) doubles.
+size_t symv_work(int npad);
+void launch_symv(hipStream_t st, const float* A, int ld, int npad, const double* x, double* y,
+                 double* work);
 int residual_partials(int npad);
+// status->dot[0] = a.b, status->dot[1] = c.d; blk holds 2 residual_partials(n) doubles.
+void launch_dot2(hipStream_t st, int n, const double* a, const double* b, const double* c,
+                 const double* d, double* blk, PassStatus* status, const TrialFlags* flags);
+void launch_pcg_update(hipStream_t st, int n, double alpha, double* x, const double* p, double* r,
+                       const double* q);
+void launch_pcg_direction(hipStream_t st, int n, double beta, const double* z, double* p);
+void launch_vec_copy(hipStream_t st, int n, const double* a, double* b);
 void launch_vec_axpy(hipStream_t st, int n, double* x, const double* d);
 void launch_vec_scale(hipStream_t st, int n, int npad, const double* in, const double* s,
                       double* out);

@@ -40,6 +40,67 @@ void upload(DevBuf<T>& b, const std::vector<T>& v, hipStream_t st) {
 
 }  // namespace
 
+size_t pcg_work(int npad) {
+    return 5 * static_cast<size_t>(npad) + symv_work(npad) + 2 * static_cast<size_t>(residual_partials(npad));
+}
+
+int pcg_solve(hipStream_t st, const float* A, const float* L32, const float* D32, int ld, int npad,
+              const double* b, double* x, double* work, int* sync, TrialFlags* flags,
+              PassStatus* status, PassStatus* h_status, double tol, int max_it, double* rel) {
+    const int nblk = npad / kTile;
+    const size_t np = static_cast<size_t>(npad);
+    double* r = work;
+    double* z = work + np;
+    double* p = work + 2 * np;
+    double* q = work + 3 * np;
+    double* y = work + 4 * np;
+    double* sw = work + 5 * np;
+    double* blk = sw + symv_work(npad);
+    auto sync_status = [&]() {
+        GBA_HIP(hipGetLastError());
+        GBA_HIP(hipMemcpyAsync(h_status, status, sizeof(PassStatus), hipMemcpyDeviceToHost, st));
+        GBA_HIP(hipStreamSynchronize(st));
+    };
+    launch_fill_double(st, x, np, 0.0);
+    launch_vec_copy(st, npad, b, r);
+    launch_chol_solve(st, nullptr, L32, ld, nblk, nullptr, D32, r, y, z, sync, flags);
+    launch_vec_copy(st, npad, z, p);
+    launch_dot2(st, npad, r, z, b, b, blk, status, flags);
+    sync_status();
+    *rel = 1.0;
+    const TrialFlags f = h_status->flags;
+    if (f.nonspd || f.chol_fail || f.solve_err) return 0;
+    double rz = h_status->dot[0];
+    const double bb = h_status->dot[1];
+    if (!(bb > 0.0)) {
+        *rel = 0.0;
+        return 0;
+    }
+    double rr = bb;
+    int it = 0;
+    while (it < max_it) {
+        launch_symv(st, A, ld, npad, p, q, sw);
+        launch_dot2(st, npad, p, q, p, q, blk, status, flags);
+        sync_status();
+        const double pq = h_status->dot[0];
+        if (!(pq > 0.0) || !(rz > 0.0)) break;
+        const double alpha = rz / pq;
+        launch_pcg_update(st, npad, alpha, x, p, r, q);
+        launch_chol_solve(st, nullptr, L32, ld, nblk, nullptr, D32, r, y, z, sync, flags);
+        launch_dot2(st, npad, r, z, r, r, blk, status, flags);
+        sync_status();
+        ++it;
+        const double rz_new = h_status->dot[0];
+        rr = h_status->dot[1];
+        if (h_status->flags.solve_err || !std::isfinite(rr)) break;
+        if (rr <= tol * tol * bb) break;
+        launch_pcg_direction(st, npad, rz_new / rz, z, p);
+        rz = rz_new;
+    }
+    *rel = std::sqrt(rr / bb);
+    return it;
+}
+
 Solver::Solver(const gba_problem_view& prob, const gba_solver_config& cfg) : cfg_(cfg) {
     mixed_ = cfg.precision == GBA_PRECISION_MIXED;
     const int C = prob.num_cameras, P = prob.num_points, O = prob.num_observations;
@@ -181,20 +242,18 @@ Solver::Solver(const gba_problem_view& prob, const gba_solver_config& cfg) : cfg
     const size_t np = static_cast<size_t>(pb_.npad);
     if (mixed_) {
         S32_.alloc(np * np);
-        S32full_.alloc(np * np);
-        D32_.alloc(static_cast<size_t>(nblk_) * kTile * kTile);
+        S32low_.alloc(np * np);
+        D32_.alloc(2 * static_cast<size_t>(nblk_) * kTile * kTile);
     } else {
         S64_.alloc(np * np);
-        D64_.alloc(static_cast<size_t>(nblk_) * kTile * kTile);
+        D64_.alloc(2 * static_cast<size_t>(nblk_) * kTile * kTile);
     }
     rhs_.alloc(np);
     y_.alloc(np);
     x_.alloc(np);
     dc_.alloc(np);
     dscale_.alloc(np);
-    r_.alloc(np);
-    d_.alloc(np);
-    resblk_.alloc(2 * static_cast<size_t>(residual_partials(pb_.npad)));
+    pcgw_.alloc(pcg_work(pb_.npad));
     sync_.alloc(2 * static_cast<size_t>(nblk_) + 2);
     flags_.alloc(1);
     status_.alloc(1);
@@ -209,7 +268,7 @@ Solver::Solver(const gba_problem_view& prob, const gba_solver_config& cfg) : cfg
     launch_fill_double(st_, x_.p, x_.n, 0.0);
     if (mixed_) {
         launch_identity_pad(st_, nullptr, S32_.p, pb_.npad, pb_.n, pb_.npad);
-        launch_identity_pad(st_, nullptr, S32full_.p, pb_.npad, pb_.n, pb_.npad);
+        launch_identity_pad(st_, nullptr, S32low_.p, pb_.npad, pb_.n, pb_.npad);
     } else {
         launch_identity_pad(st_, S64_.p, nullptr, pb_.npad, pb_.n, pb_.npad);
     }
@@ -263,7 +322,7 @@ void Solver::enqueue_trial_linear_algebra(double lambda, std::vector<double>* ho
     launch_damp_invert_v(st_, pb_.P, VB_.p, lambda, Vinv_.p, flags_.p);
     if (mixed_) launch_diag_scale(st_, pb_.n, pb_.npad, U_.p, lambda, dscale_.p);
     launch_schur(st_, pb_, slist_.p, W64_.p, W32_.p, Vinv_.p, VB_.p, U_.p, bc_.p, lambda,
-                 mixed_ ? dscale_.p : nullptr, S64_.p, S32_.p, S32full_.p, ld, rhs_.p, flags_.p);
+                 mixed_ ? dscale_.p : nullptr, S64_.p, S32_.p, S32low_.p, ld, rhs_.p, flags_.p);
     if (hook_S || hook_rhs) {
         GBA_HIP(hipGetLastError());
         GBA_HIP(hipStreamSynchronize(st_));
@@ -298,34 +357,13 @@ void Solver::enqueue_trial_linear_algebra(double lambda, std::vector<double>* ho
                           sync_.p, flags_.p);
         return;
     }
-    // mixed: FP32 factor of D S D, FP64 refinement against the FP32 matrix
+    // mixed: FP32 factor of D S D; the camera system is solved in FP64 by CG
+    // preconditioned with that factor, against the FP32-assembled D S D
     launch_cholesky(st_, nullptr, S32_.p, ld, nblk_, nullptr, D32_.p, flags_.p);
-    launch_chol_solve(st_, nullptr, S32_.p, ld, nblk_, nullptr, D32_.p, rhs_.p, y_.p, x_.p, sync_.p,
-                      flags_.p);
-    launch_residual(st_, S32full_.p, ld, pb_.npad, rhs_.p, x_.p, r_.p, resblk_.p, status_.p, true,
-                    flags_.p);
-    sync_status();
-    out.refine_iters = 0;
-    const TrialFlags f = h_status_->flags;
-    double rhs2 = h_status_->rhs_norm2, res2 = h_status_->res_norm2;
-    if (f.nonspd == 0 && f.chol_fail == 0 && f.solve_err == 0) {
-        const double tol = cfg_.refine_tol > 0 ? cfg_.refine_tol : 1e-10;
-        const int max_it = cfg_.refine_max_iters > 0 ? cfg_.refine_max_iters : 12;
-        double prev = res2;
-        while (out.refine_iters < max_it && res2 > tol * tol * rhs2 && std::isfinite(res2)) {
-            launch_chol_solve(st_, nullptr, S32_.p, ld, nblk_, nullptr, D32_.p, r_.p, y_.p, d_.p,
-                              sync_.p, flags_.p);
-            launch_vec_axpy(st_, pb_.npad, x_.p, d_.p);
-            launch_residual(st_, S32full_.p, ld, pb_.npad, rhs_.p, x_.p, r_.p, resblk_.p, status_.p,
-                            false, flags_.p);
-            sync_status();
-            ++out.refine_iters;
-            res2 = h_status_->res_norm2;
-            if (!(res2 < prev)) break;  // stagnation: keep the last iterate
-            prev = res2;
-        }
-    }
-    out.refine_rel = rhs2 > 0 ? std::sqrt(res2 / rhs2) : 0.0;
+    const double tol = cfg_.refine_tol > 0 ? cfg_.refine_tol : 1e-10;
+    const int max_it = cfg_.refine_max_iters > 0 ? cfg_.refine_max_iters : 12;
+    out.refine_iters = pcg_solve(st_, S32low_.p, S32_.p, D32_.p, ld, pb_.npad, rhs_.p, x_.p, pcgw_.p,
+                                 sync_.p, flags_.p, status_.p, h_status_, tol, max_it, &out.refine_rel);
     launch_vec_scale(st_, pb_.n, pb_.npad, x_.p, dscale_.p, dc_.p);
 }
 
@@ -429,7 +467,7 @@ double Solver::time_kernel(int kernel, double lambda, int reps) {
                 break;
             case GBA_KERNEL_SCHUR:
                 launch_schur(st_, pb_, slist_.p, W64_.p, W32_.p, Vinv_.p, VB_.p, U_.p, bc_.p, lambda,
-                             mixed_ ? dscale_.p : nullptr, S64_.p, S32_.p, S32full_.p, ld, rhs_.p,
+                             mixed_ ? dscale_.p : nullptr, S64_.p, S32_.p, S32low_.p, ld, rhs_.p,
                              flags_.p);
                 break;
             case GBA_KERNEL_FACTOR:
@@ -464,7 +502,7 @@ double Solver::time_kernel(int kernel, double lambda, int reps) {
             launch_damp_invert_v(st_, pb_.P, VB_.p, lambda, Vinv_.p, flags_.p);
             if (mixed_) launch_diag_scale(st_, pb_.n, pb_.npad, U_.p, lambda, dscale_.p);
             launch_schur(st_, pb_, slist_.p, W64_.p, W32_.p, Vinv_.p, VB_.p, U_.p, bc_.p, lambda,
-                         mixed_ ? dscale_.p : nullptr, S64_.p, S32_.p, S32full_.p, ld, rhs_.p, flags_.p);
+                         mixed_ ? dscale_.p : nullptr, S64_.p, S32_.p, S32low_.p, ld, rhs_.p, flags_.p);
             GBA_HIP(hipEventRecord(a, st_));
             enqueue();
             GBA_HIP(hipEventRecord(b, st_));

@@ -76,7 +76,7 @@ def test_schur_step_mixed_close_to_fp64(gba, orc):
     o = orc.schur_step(prob, orc.linearize(prob), 1e-4, want_S=True)
     assert g["status"] == "accept"
     assert arel(np.tril(g["S"]), np.tril(o["S"])) < 1e-5
-    assert arel(g["rhs"], o["rhs"]) < 1e-12
+    assert arel(g["rhs"], o["rhs"]) < 1e-6  # FP64 arithmetic over FP32-stored W
     assert arel(g["dc"], o["dc"]) < 1e-3
     assert abs(g["denom"] - o["denom"]) <= 1e-4 * abs(o["denom"])
 
