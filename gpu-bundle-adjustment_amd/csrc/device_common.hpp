@@ -38,6 +38,14 @@ struct DevProblem {
     const int* cam_chunk_off;
 };
 
+// Schur assembly task: blocks S[a][b], lo <= b < hi <= a + 1, from the
+// observation pairs [begin, end) of the pair list (int2: e, f in sorted edge
+// indices, same point, camera(e) = a, lo <= camera(f) < hi).
+struct SchurTask {
+    int a, lo, hi, pad;
+    long long begin, end;
+};
+
 // Per-trial device flags, cleared by a fill kernel at the start of a trial.
 struct TrialFlags {
     int nonspd;     // damped V blocks that are not SPD

@@ -2,18 +2,20 @@
 //   S = U + lambda diag'(U) - sum_p W_p V'_p^-1 W_p^T,   rhs = bc - sum_e T_e bp_p
 // (replaces the host Phase 1 / Phase 2 / merge loops of src/lm.cu:483-594).
 //
-// Workgroup (a, chunk) owns block row a of S restricted to a chunk of camera
-// columns b <= a wide enough that the whole chunk lives in LDS (up to 128 KB:
-// 202 cameras in FP64, 404 in FP32).  Its 1024 threads walk camera a's
-// observations (pre-sorted by the point's observation count, so a wavefront
-// gets similar work), form T_e = W_e V'^-1 once per observation and, for every
-// other observation f of the same point with camera b in the chunk (a point's
-// observations are sorted by camera: skip to lo, stop at hi), add -T_e W_f^T
-// into the LDS accumulator (ds_add_f64; FP64 accumulation in both forms).  Each block of S is
-// written by exactly one workgroup with plain stores; no global atomics.
-// FP64 path: everything double.  Mixed path: W is FP32, the result is written diagonally scaled (D S D, D = (U_ii + lambda
-// U_ii)^-1/2) into the factor copy and a second lower copy used by the FP64
-// refinement residual; rhs stays FP64.
+// Work is a fixed list of tasks built once per problem (the sparsity never
+// changes during a solve).  Task (a, lo, hi) owns the blocks S[a][b] for
+// lo <= b < hi <= a + 1, accumulated in LDS; its observation pairs (e, f) --
+// e an observation of camera a, f an observation of the same point by a camera
+// b in [lo, hi) -- are stored contiguously, grouped by e and ordered by b.  One
+// lane per pair: the lanes of a wavefront that share e hit distinct blocks, so
+// the LDS adds of one instruction rarely meet on one address.  Per pair the
+// lane forms Z = V'^-1 W_f^T (3x9) and adds -W_e Z row by row.  Each block of
+// S is written by exactly one workgroup with plain stores; no global atomics.
+// The rhs term T_e bp comes from the pairs with e == f, in FP64.
+// FP64 path: everything double.  Mixed path: W is FP32 (widened; products and
+// LDS accumulation stay FP64), the result is written diagonally scaled
+// (D S D, D = (U_ii + lambda U_ii)^-1/2) into the factor copy and a second
+// lower copy used by the FP64 refinement residual.
 #include "device_common.hpp"
 #include "launchers.hpp"
 
@@ -24,77 +26,68 @@ namespace gba_amd {
 
 namespace {
 
-constexpr int kSchurThreads = 1024;
-constexpr int kSchurLdsCap = 128 * 1024;  // accumulator bytes per workgroup at most
-
-// WT: stored W type; AT: LDS accumulator type; ST: output type of S.
-template <typename WT, typename AT, typename ST>
-__global__ __launch_bounds__(kSchurThreads) void k_schur(
-    DevProblem pb, const int* __restrict__ slist, const WT* __restrict__ W,
-    const double* __restrict__ Vinv6, const double* __restrict__ VB, const double* __restrict__ U,
-    const double* __restrict__ bc, double lambda, const double* __restrict__ dscale,
-    ST* __restrict__ Sfac, ST* __restrict__ Slow, int ld, double* __restrict__ rhs,
-    const TrialFlags* flags, int CB) {
+// WT: stored W type; AT: LDS accumulator (and product) type; ST: type of S.
+template <int NT, typename WT, typename AT, typename ST>
+__global__ __launch_bounds__(NT) void k_schur(
+    DevProblem pb, const SchurTask* __restrict__ tasks, const int2* __restrict__ pairs,
+    const WT* __restrict__ W, const double* __restrict__ Vinv6, const double* __restrict__ VB,
+    const double* __restrict__ U, const double* __restrict__ bc, double lambda,
+    const double* __restrict__ dscale, ST* __restrict__ Sfac, ST* __restrict__ Slow, int ld,
+    double* __restrict__ rhs, const TrialFlags* flags) {
     extern __shared__ __align__(16) unsigned char smem[];
     AT* acc = reinterpret_cast<AT*>(smem);
-    __shared__ double s_rhs[kSchurThreads / 64][9];
+    __shared__ double s_rhs[NT / 64][9];
     if (flags->nonspd) return;
-    const int a = blockIdx.x;
-    const int lo = blockIdx.y * CB;
-    if (lo > a) return;
-    const int hi = min(lo + CB, a + 1);
-    const bool own_rhs = (a >= lo && a < hi);
+    const SchurTask task = tasks[blockIdx.x];
+    const int a = task.a, lo = task.lo, hi = task.hi;
+    const bool own_rhs = (hi == a + 1);
     const int tid = threadIdx.x;
     const int nb = hi - lo;
-    for (int t = tid; t < nb * 81; t += kSchurThreads) acc[t] = AT(0);
+    for (int t = tid; t < nb * 81; t += NT) acc[t] = AT(0);
     __syncthreads();
     double racc[9];
 #pragma unroll
     for (int i = 0; i < 9; ++i) racc[i] = 0.0;
-    const int beg = pb.cam_off[a], end = pb.cam_off[a + 1];
-    for (int idx = beg + tid; idx < end; idx += kSchurThreads) {
-        const int e = slist[idx];
+    for (long long q = task.begin + tid; q < task.end; q += NT) {
+        const int2 ef = pairs[q];
+        const int e = ef.x, f = ef.y;
         const int p = pb.spt[e];
+        const int b = pb.scam[f];
         const double* vi = Vinv6 + 6 * static_cast<size_t>(p);
-        const double V[9] = {vi[0], vi[1], vi[3], vi[1], vi[2], vi[4], vi[3], vi[4], vi[5]};
+        const double v00 = vi[0], v10 = vi[1], v11 = vi[2], v20 = vi[3], v21 = vi[4], v22 = vi[5];
         const WT* We = W + 27 * static_cast<size_t>(e);
-        if (own_rhs) {
+        const WT* Wf = W + 27 * static_cast<size_t>(f);
+        if (own_rhs && e == f) {
             // T_e bp = W_e (V'^-1 bp), in FP64
             const double* bpp = VB + 9 * static_cast<size_t>(p) + 6;
-            const double vb0 = V[0] * bpp[0] + V[1] * bpp[1] + V[2] * bpp[2];
-            const double vb1 = V[3] * bpp[0] + V[4] * bpp[1] + V[5] * bpp[2];
-            const double vb2 = V[6] * bpp[0] + V[7] * bpp[1] + V[8] * bpp[2];
+            const double vb0 = v00 * bpp[0] + v10 * bpp[1] + v20 * bpp[2];
+            const double vb1 = v10 * bpp[0] + v11 * bpp[1] + v21 * bpp[2];
+            const double vb2 = v20 * bpp[0] + v21 * bpp[1] + v22 * bpp[2];
 #pragma unroll
             for (int i = 0; i < 9; ++i)
-                racc[i] += static_cast<double>(We[3 * i]) * vb0 + static_cast<double>(We[3 * i + 1]) * vb1 +
+                racc[i] += static_cast<double>(We[3 * i]) * vb0 +
+                           static_cast<double>(We[3 * i + 1]) * vb1 +
                            static_cast<double>(We[3 * i + 2]) * vb2;
         }
-        AT Ts[27];  // T_e = W_e V'^-1 in the accumulation precision
+        // Z = V'^-1 W_f^T, column j = V'^-1 (row j of W_f)
+        const AT a00 = static_cast<AT>(v00), a10 = static_cast<AT>(v10), a11 = static_cast<AT>(v11),
+                 a20 = static_cast<AT>(v20), a21 = static_cast<AT>(v21), a22 = static_cast<AT>(v22);
+        AT z0[9], z1[9], z2[9];
+#pragma unroll
+        for (int j = 0; j < 9; ++j) {
+            const AT w0 = static_cast<AT>(Wf[3 * j]), w1 = static_cast<AT>(Wf[3 * j + 1]),
+                     w2 = static_cast<AT>(Wf[3 * j + 2]);
+            z0[j] = a00 * w0 + a10 * w1 + a20 * w2;
+            z1[j] = a10 * w0 + a11 * w1 + a21 * w2;
+            z2[j] = a20 * w0 + a21 * w1 + a22 * w2;
+        }
+        AT* dst = acc + (b - lo) * 81;
 #pragma unroll
         for (int i = 0; i < 9; ++i) {
-            const AT w0 = static_cast<AT>(We[3 * i]), w1 = static_cast<AT>(We[3 * i + 1]),
-                     w2 = static_cast<AT>(We[3 * i + 2]);
+            const AT x0 = static_cast<AT>(We[3 * i]), x1 = static_cast<AT>(We[3 * i + 1]),
+                     x2 = static_cast<AT>(We[3 * i + 2]);
 #pragma unroll
-            for (int j = 0; j < 3; ++j)
-                Ts[3 * i + j] = w0 * static_cast<AT>(V[j]) + w1 * static_cast<AT>(V[3 + j]) +
-                                w2 * static_cast<AT>(V[6 + j]);
-        }
-        const int f1 = pb.pt_off[p + 1];
-        int f = pb.pt_off[p];
-        while (f < f1 && pb.scam[f] < lo) ++f;
-        for (; f < f1; ++f) {
-            const int b = pb.scam[f];
-            if (b >= hi) break;
-            const WT* Wf = W + 27 * static_cast<size_t>(f);
-            AT* dst = acc + (b - lo) * 81;
-#pragma unroll
-            for (int j = 0; j < 9; ++j) {
-                const AT w0 = static_cast<AT>(Wf[3 * j]), w1 = static_cast<AT>(Wf[3 * j + 1]),
-                         w2 = static_cast<AT>(Wf[3 * j + 2]);
-#pragma unroll
-                for (int i = 0; i < 9; ++i)
-                    atomicAdd(dst + 9 * i + j, -(Ts[3 * i] * w0 + Ts[3 * i + 1] * w1 + Ts[3 * i + 2] * w2));
-            }
+            for (int j = 0; j < 9; ++j) atomicAdd(dst + 9 * i + j, -(x0 * z0[j] + x1 * z1[j] + x2 * z2[j]));
         }
     }
     if (own_rhs) {
@@ -108,12 +101,12 @@ __global__ __launch_bounds__(kSchurThreads) void k_schur(
     __syncthreads();
     if (own_rhs && tid < 9) {
         double sum = 0.0;
-        for (int w = 0; w < kSchurThreads / 64; ++w) sum += s_rhs[w][tid];
+        for (int w = 0; w < NT / 64; ++w) sum += s_rhs[w][tid];
         const int row = 9 * a + tid;
         const double v = bc[row] - sum;
         rhs[row] = dscale ? v * dscale[row] : v;
     }
-    for (int t = tid; t < nb * 81; t += kSchurThreads) {
+    for (int t = tid; t < nb * 81; t += NT) {
         const int bb = t / 81, ij = t - 81 * bb;
         const int i = ij / 9, j = ij - 9 * i;
         const int b = lo + bb;
@@ -143,57 +136,65 @@ __global__ void k_diag_scale(int n, int npad, const double* __restrict__ U, doub
     d[i] = v > 0.0 ? 1.0 / sqrt(v) : 1.0;
 }
 
-}  // namespace
-
-namespace {
-// Accumulator bytes a workgroup may use: the device's per-block LDS limit
-// (160 KB on gfx950) minus the static part, capped at kSchurLdsCap.
-int schur_lds_bytes() {
-    static int bytes = -1;
-    if (bytes < 0) {
-        int dev = 0, v = 0;
-        (void)hipGetDevice(&dev);
-        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
-            v = 64 * 1024;
-        bytes = std::min(kSchurLdsCap, v - 4096);
-        // tuning knob: accumulator LDS per workgroup in KB (smaller chunks,
-        // more workgroups per CU)
-        if (const char* env = std::getenv("GBA_SCHUR_LDS_KB")) {
-            const int kb = std::atoi(env);
-            if (kb >= 16) bytes = std::min(bytes, kb * 1024);
-        }
-        if (bytes > 64 * 1024 - 4096) {
-            if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_schur<float, double, float>),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, bytes) != hipSuccess ||
-                hipFuncSetAttribute(reinterpret_cast<const void*>(&k_schur<double, double, double>),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, bytes) != hipSuccess)
-                bytes = 64 * 1024 - 4096;
-        }
-        (void)hipGetLastError();
-    }
-    return bytes;
+template <int NT, typename WT, typename AT, typename ST>
+void launch_one(hipStream_t st, const SchurShape& sh, const DevProblem& pb, const SchurTask* tasks,
+                int num_tasks, const int2* pairs, const WT* W, const double* Vinv6,
+                const double* VB, const double* U, const double* bc, double lambda,
+                const double* dscale, ST* Sfac, ST* Slow, int ld, double* rhs,
+                const TrialFlags* flags) {
+    const size_t lds = static_cast<size_t>(sh.cb) * 81 * sizeof(AT);
+    if (lds > 64 * 1024 - 4096)
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_schur<NT, WT, AT, ST>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+    hipLaunchKernelGGL((k_schur<NT, WT, AT, ST>), dim3(num_tasks), dim3(NT), lds, st, pb, tasks,
+                       pairs, W, Vinv6, VB, U, bc, lambda, dscale, Sfac, Slow, ld, rhs, flags);
 }
+
 }  // namespace
 
-void launch_schur(hipStream_t st, const DevProblem& pb, const int* slist, const double* W64,
+SchurShape schur_shape(int C, bool /*w32*/) {
+    SchurShape sh;
+    int dev = 0, v = 0;
+    (void)hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+        v = 64 * 1024;
+    (void)hipGetLastError();
+    // FP64 accumulator LDS per workgroup: 128 KB (one workgroup of 1024
+    // threads per CU) unless GBA_SCHUR_LDS_KB asks otherwise.  FP64 LDS adds
+    // are the accumulation in both forms: on gfx950 ds_add_f32 measured about
+    // eight times slower than ds_add_f64 for this access pattern.
+    int bytes = 128 * 1024;
+    if (const char* env = std::getenv("GBA_SCHUR_LDS_KB")) {
+        const int kb = std::atoi(env);
+        if (kb >= 8) bytes = kb * 1024;
+    }
+    bytes = std::min(bytes, v - 4096);
+    sh.cb = std::max(1, std::min(C, bytes / (81 * 8)));
+    sh.threads = (static_cast<size_t>(sh.cb) * 81 * 8 > 80 * 1024) ? 1024 : 512;
+    return sh;
+}
+
+void launch_schur(hipStream_t st, const SchurShape& sh, const DevProblem& pb,
+                  const SchurTask* tasks, int num_tasks, const int2* pairs, const double* W64,
                   const float* W32, const double* Vinv6, const double* VB, const double* U,
                   const double* bc, double lambda, const double* dscale, double* S64, float* S32fac,
                   float* S32low, int ld, double* rhs, const TrialFlags* flags) {
-    const int bytes = schur_lds_bytes();
     if (W32) {
-        const int CB = std::min(pb.C, bytes / static_cast<int>(81 * sizeof(double)));
-        const int nch = (pb.C + CB - 1) / CB;
-        const size_t lds = static_cast<size_t>(CB) * 81 * sizeof(double);
-        hipLaunchKernelGGL((k_schur<float, double, float>), dim3(pb.C, nch), dim3(kSchurThreads), lds, st, pb,
-                           slist, W32, Vinv6, VB, U, bc, lambda, dscale, S32fac, S32low, ld, rhs,
-                           flags, CB);
+        if (sh.threads == 1024)
+            launch_one<1024, float, double, float>(st, sh, pb, tasks, num_tasks, pairs, W32, Vinv6, VB,
+                                                   U, bc, lambda, dscale, S32fac, S32low, ld, rhs, flags);
+        else
+            launch_one<512, float, double, float>(st, sh, pb, tasks, num_tasks, pairs, W32, Vinv6, VB, U,
+                                                  bc, lambda, dscale, S32fac, S32low, ld, rhs, flags);
     } else {
-        const int CB = std::min(pb.C, bytes / static_cast<int>(81 * sizeof(double)));
-        const int nch = (pb.C + CB - 1) / CB;
-        const size_t lds = static_cast<size_t>(CB) * 81 * sizeof(double);
-        hipLaunchKernelGGL((k_schur<double, double, double>), dim3(pb.C, nch), dim3(kSchurThreads), lds, st,
-                           pb, slist, W64, Vinv6, VB, U, bc, lambda, dscale, S64,
-                           static_cast<double*>(nullptr), ld, rhs, flags, CB);
+        if (sh.threads == 1024)
+            launch_one<1024, double, double, double>(st, sh, pb, tasks, num_tasks, pairs, W64, Vinv6, VB, U,
+                                                     bc, lambda, dscale, S64, static_cast<double*>(nullptr),
+                                                     ld, rhs, flags);
+        else
+            launch_one<512, double, double, double>(st, sh, pb, tasks, num_tasks, pairs, W64, Vinv6, VB, U,
+                                                    bc, lambda, dscale, S64, static_cast<double*>(nullptr),
+                                                    ld, rhs, flags);
     }
 }
 

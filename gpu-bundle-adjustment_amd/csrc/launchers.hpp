@@ -30,9 +30,15 @@ void launch_batch_inverse3(hipStream_t st, const double* A, double* Ainv, int* o
 void launch_identity_pad(hipStream_t st, double* S64, float* S32, int ld, int n, int npad);
 
 // k_schur.hip
-// slist: camera-major observation list with each camera's entries ordered by
-// decreasing observation count of their point.
-void launch_schur(hipStream_t st, const DevProblem& pb, const int* slist, const double* W64,
+// Shape of the assembly: camera columns per LDS chunk and workgroup size
+// (the task list is built for this chunk width).
+struct SchurShape {
+    int cb = 1;  // cameras per column chunk
+    int threads = 512;
+};
+SchurShape schur_shape(int C, bool w32);
+void launch_schur(hipStream_t st, const SchurShape& sh, const DevProblem& pb,
+                  const SchurTask* tasks, int num_tasks, const int2* pairs, const double* W64,
                   const float* W32, const double* Vinv6, const double* VB, const double* U,
                   const double* bc, double lambda, const double* dscale, double* S64, float* S32fac,
                   float* S32low, int ld, double* rhs, const TrialFlags* flags);

@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <thread>
 
 #include "launchers.hpp"
 
@@ -36,6 +37,73 @@ template <typename T>
 void upload(DevBuf<T>& b, const std::vector<T>& v, hipStream_t st) {
     b.alloc(v.size());
     if (!v.empty()) GBA_HIP(hipMemcpyAsync(b.p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, st));
+}
+
+// Runs f(i) for i in [0, n) on up to 16 host threads, interleaved.
+template <typename F>
+void parallel_for(int n, F f) {
+    const int nt = std::max(1, std::min({16, n, static_cast<int>(std::thread::hardware_concurrency())}));
+    if (nt == 1) {
+        for (int i = 0; i < n; ++i) f(i);
+        return;
+    }
+    std::vector<std::thread> th;
+    for (int t = 0; t < nt; ++t)
+        th.emplace_back([&, t]() {
+            for (int i = t; i < n; i += nt) f(i);
+        });
+    for (auto& x : th) x.join();
+}
+
+// Task and pair lists of the Schur assembly for chunks of cb camera columns:
+// task (a, c) holds the pairs (e, f) with camera(e) = a, f an observation of
+// the same point by a camera b <= a in [c cb, (c + 1) cb), grouped by e
+// (ascending point) and ordered by b.  Every (a, c) with c cb <= a gets a
+// task, empty or not, because the task also writes its blocks of S.  Tasks
+// are returned by decreasing pair count (longest first).
+void build_schur_tasks(int C, int cb, const std::vector<int>& cam_off,
+                       const std::vector<int>& cam_list, const std::vector<int>& pt_off,
+                       const std::vector<int>& scam, const std::vector<int>& spt,
+                       std::vector<SchurTask>& tasks, std::vector<int2>& pairs) {
+    std::vector<int> toff(static_cast<size_t>(C) + 1, 0);
+    for (int a = 0; a < C; ++a) toff[a + 1] = toff[a] + a / cb + 1;
+    const int ntask = toff[C];
+    std::vector<long long> cnt(static_cast<size_t>(ntask), 0);
+    parallel_for(C, [&](int a) {
+        long long* c = cnt.data() + toff[a];
+        for (int i = cam_off[a]; i < cam_off[a + 1]; ++i) {
+            const int p = spt[cam_list[i]];
+            for (int f = pt_off[p]; f < pt_off[p + 1]; ++f) {
+                const int b = scam[f];
+                if (b > a) break;
+                ++c[b / cb];
+            }
+        }
+    });
+    std::vector<long long> begin(static_cast<size_t>(ntask) + 1, 0);
+    for (int t = 0; t < ntask; ++t) begin[t + 1] = begin[t] + cnt[t];
+    pairs.resize(static_cast<size_t>(begin[ntask]));
+    parallel_for(C, [&](int a) {
+        std::vector<long long> cur(begin.begin() + toff[a], begin.begin() + toff[a + 1]);
+        for (int i = cam_off[a]; i < cam_off[a + 1]; ++i) {
+            const int e = cam_list[i];
+            const int p = spt[e];
+            for (int f = pt_off[p]; f < pt_off[p + 1]; ++f) {
+                const int b = scam[f];
+                if (b > a) break;
+                pairs[static_cast<size_t>(cur[b / cb]++)] = make_int2(e, f);
+            }
+        }
+    });
+    tasks.resize(static_cast<size_t>(ntask));
+    for (int a = 0; a < C; ++a)
+        for (int c = 0; c <= a / cb; ++c) {
+            const int t = toff[a] + c;
+            tasks[t] = SchurTask{a, c * cb, std::min((c + 1) * cb, a + 1), 0, begin[t], begin[t + 1]};
+        }
+    std::stable_sort(tasks.begin(), tasks.end(), [](const SchurTask& x, const SchurTask& y) {
+        return x.end - x.begin > y.end - y.begin;
+    });
 }
 
 }  // namespace
@@ -178,16 +246,11 @@ Solver::Solver(const gba_problem_view& prob, const gba_solver_config& cfg) : cfg
         std::vector<int> fill(cam_off.begin(), cam_off.end() - 1);
         for (int k = 0; k < O; ++k) cam_list[fill[scam[k]]++] = k;
     }
-    // Schur order: each camera's entries by decreasing observation count of
-    // their point (stable), so the wavefronts of the assembly get similar work
-    std::vector<int> slist(cam_list);
-    for (int c = 0; c < C; ++c)
-        std::stable_sort(slist.begin() + cam_off[c], slist.begin() + cam_off[c + 1],
-                         [&](int x, int y) {
-                             const int kx = pt_off[spt[x] + 1] - pt_off[spt[x]];
-                             const int ky = pt_off[spt[y] + 1] - pt_off[spt[y]];
-                             return kx > ky;
-                         });
+    sshape_ = schur_shape(C, mixed_);
+    std::vector<SchurTask> stasks;
+    std::vector<int2> spairs;
+    build_schur_tasks(C, sshape_.cb, cam_off, cam_list, pt_off, scam, spt, stasks, spairs);
+    num_stasks_ = static_cast<int>(stasks.size());
     std::vector<int4> chunks;
     std::vector<int> cam_chunk_off(static_cast<size_t>(C) + 1, 0);
     for (int c = 0; c < C; ++c) {
@@ -208,7 +271,8 @@ Solver::Solver(const gba_problem_view& prob, const gba_solver_config& cfg) : cfg
     upload(spx_, spx, st_);
     upload(units_, units, st_);
     upload(cam_list_, cam_list, st_);
-    upload(slist_, slist, st_);
+    upload(stasks_, stasks, st_);
+    upload(spairs_, spairs, st_);
     upload(cam_off_, cam_off, st_);
     upload(chunks_, chunks, st_);
     upload(cam_chunk_off_, cam_chunk_off, st_);
@@ -315,14 +379,19 @@ void Solver::cost(double* sse, unsigned long long* anomalies) {
     *anomalies = h_status_->anomalies;
 }
 
+void Solver::enqueue_schur(double lambda) {
+    launch_schur(st_, sshape_, pb_, stasks_.p, num_stasks_, spairs_.p, W64_.p, W32_.p, Vinv_.p,
+                 VB_.p, U_.p, bc_.p, lambda, mixed_ ? dscale_.p : nullptr, S64_.p, S32_.p,
+                 S32low_.p, pb_.npad, rhs_.p, flags_.p);
+}
+
 void Solver::enqueue_trial_linear_algebra(double lambda, std::vector<double>* hook_S,
                                           std::vector<double>* hook_rhs, TrialOut& out) {
     const int ld = pb_.npad;
     launch_clear_flags(st_, flags_.p);
     launch_damp_invert_v(st_, pb_.P, VB_.p, lambda, Vinv_.p, flags_.p);
     if (mixed_) launch_diag_scale(st_, pb_.n, pb_.npad, U_.p, lambda, dscale_.p);
-    launch_schur(st_, pb_, slist_.p, W64_.p, W32_.p, Vinv_.p, VB_.p, U_.p, bc_.p, lambda,
-                 mixed_ ? dscale_.p : nullptr, S64_.p, S32_.p, S32low_.p, ld, rhs_.p, flags_.p);
+    enqueue_schur(lambda);
     if (hook_S || hook_rhs) {
         GBA_HIP(hipGetLastError());
         GBA_HIP(hipStreamSynchronize(st_));
@@ -466,9 +535,7 @@ double Solver::time_kernel(int kernel, double lambda, int reps) {
                 launch_lin_cameras(st_, pb_, cams_[cur_].p, pts_[cur_].p, gram_.p);
                 break;
             case GBA_KERNEL_SCHUR:
-                launch_schur(st_, pb_, slist_.p, W64_.p, W32_.p, Vinv_.p, VB_.p, U_.p, bc_.p, lambda,
-                             mixed_ ? dscale_.p : nullptr, S64_.p, S32_.p, S32low_.p, ld, rhs_.p,
-                             flags_.p);
+                enqueue_schur(lambda);
                 break;
             case GBA_KERNEL_FACTOR:
                 launch_cholesky(st_, S64_.p, S32_.p, ld, nblk_, D64_.p, D32_.p, flags_.p);
@@ -501,8 +568,7 @@ double Solver::time_kernel(int kernel, double lambda, int reps) {
             launch_clear_flags(st_, flags_.p);
             launch_damp_invert_v(st_, pb_.P, VB_.p, lambda, Vinv_.p, flags_.p);
             if (mixed_) launch_diag_scale(st_, pb_.n, pb_.npad, U_.p, lambda, dscale_.p);
-            launch_schur(st_, pb_, slist_.p, W64_.p, W32_.p, Vinv_.p, VB_.p, U_.p, bc_.p, lambda,
-                         mixed_ ? dscale_.p : nullptr, S64_.p, S32_.p, S32low_.p, ld, rhs_.p, flags_.p);
+            enqueue_schur(lambda);
             GBA_HIP(hipEventRecord(a, st_));
             enqueue();
             GBA_HIP(hipEventRecord(b, st_));

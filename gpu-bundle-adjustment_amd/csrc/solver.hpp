@@ -8,6 +8,7 @@
 #include "device_common.hpp"
 #include "gba_c.h"
 #include "gba_internal.hpp"
+#include "launchers.hpp"
 
 namespace gba_amd {
 
@@ -88,6 +89,7 @@ private:
     void enqueue_trial_linear_algebra(double lambda, std::vector<double>* hook_S,
                                       std::vector<double>* hook_rhs, TrialOut& out);
     void sync_status();
+    void enqueue_schur(double lambda);
 
     gba_solver_config cfg_;
     bool mixed_;
@@ -98,7 +100,11 @@ private:
     double pair_count_ = 0.0;
     int cur_ = 0;  // index of the current parameter buffers
 
-    DevBuf<int> scam_, spt_, perm_, pt_off_, cam_list_, slist_, cam_off_, cam_chunk_off_;
+    DevBuf<int> scam_, spt_, perm_, pt_off_, cam_list_, cam_off_, cam_chunk_off_;
+    SchurShape sshape_;
+    DevBuf<SchurTask> stasks_;
+    DevBuf<int2> spairs_;
+    int num_stasks_ = 0;
     DevBuf<double2> spx_;
     DevBuf<int2> units_;
     DevBuf<int4> chunks_;
