@@ -64,61 +64,129 @@ __device__ __forceinline__ bool any_fail(const TrialFlags* f) {
     return f->nonspd != 0 || f->chol_fail != 0;
 }
 
-// Factor + invert the diagonal tile k.  D[k] receives L_kk (row-major, upper
-// zero), DinvT[k] receives (L_kk^-1)^T (row c holds column c of the inverse).
-// One wavefront: lane r owns row r in registers; at step j the pivot comes from
-// lane j (v_readlane) and the scaled column j is broadcast through LDS.
+// Cholesky factor and inverse of the symmetric tile in a (lower triangle
+// read), 256 threads.  On return a holds L (lower; the upper part is stale) and
+// b holds L^-1 (lower; the upper part is scratch).  Blocked by 16 columns:
+// wavefront 0 factors each 16-column panel with the rows in its lanes (pivot
+// from v_readlane, the scaled column broadcast through LDS), all threads apply
+// the rank-16 update to the trailing triangle; the inverse is built from the
+// four 16x16 diagonal inverses (forward substitution, one column per lane) and
+// then by block distance: X_rc = -Linv_rr (sum_{c <= m < r} L_rm Linv_mc).
+// Returns false on a pivot that is not > 0.
 template <typename T>
-__global__ __launch_bounds__(64) void k_potrf_tile(const T* __restrict__ S, int ld, int k,
+__device__ __forceinline__ bool tile_potrf_inv(T (*a)[kTile + 2], T (*b)[kTile + 2]) {
+    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    __shared__ int s_bad;
+    __shared__ T s_col[kTile];
+    if (tid == 0) s_bad = 0;
+    __syncthreads();
+    for (int p = 0; p < 4; ++p) {
+        const int c0 = 16 * p;
+        if (w == 0) {
+            T v[16];
+#pragma unroll
+            for (int m = 0; m < 16; ++m) v[m] = a[lane][c0 + m];
+            bool bad = false;
+#pragma unroll
+            for (int jj = 0; jj < 16; ++jj) {
+                const T d = readlane(v[jj], c0 + jj);
+                bad = bad || !(d > T(0));
+                const T piv = sqrt(d);
+                const T rinv = T(1) / piv;
+                const T l = (lane == c0 + jj) ? piv : v[jj] * rinv;
+                v[jj] = l;
+                // the column goes through LDS (in order within the wavefront)
+                s_col[lane] = l;
+                __builtin_amdgcn_wave_barrier();
+#pragma unroll
+                for (int m = jj + 1; m < 16; ++m) v[m] -= l * s_col[c0 + m];
+                __builtin_amdgcn_wave_barrier();
+            }
+            if (lane >= c0) {
+#pragma unroll
+                for (int m = 0; m < 16; ++m) a[lane][c0 + m] = v[m];
+            }
+            if (bad && lane == 0) s_bad = 1;
+        }
+        __syncthreads();
+        const int n2 = kTile - c0 - 16;
+        for (int q = tid; q < n2 * n2; q += 256) {
+            const int r = c0 + 16 + q / n2, c = c0 + 16 + q % n2;
+            if (c > r) continue;
+            T acc = a[r][c];
+#pragma unroll
+            for (int m = 0; m < 16; ++m) acc -= a[r][c0 + m] * a[c][c0 + m];
+            a[r][c] = acc;
+        }
+        __syncthreads();
+    }
+    // diagonal 16x16 inverses: wave w, lane l < 16 -> column 16 w + l
+    // (the column is kept in LDS, not registers: rows above the diagonal come
+    // out as exact zeros)
+    if (lane < 16) {
+        const int o = 16 * w;
+#pragma nounroll
+        for (int i = 0; i < 16; ++i) {
+            T sacc = (i == lane) ? T(1) : T(0);
+            for (int m = 0; m < i; ++m) sacc -= a[o + i][o + m] * b[o + m][o + lane];
+            b[o + i][o + lane] = sacc / a[o + i][o + i];
+        }
+    }
+    __syncthreads();
+    for (int d = 1; d < 4; ++d) {
+        const int nb = 4 - d;  // blocks (c + d, c), c < nb
+        // Y_rc = sum_{c <= m < r} L_rm Linv_mc, parked transposed in the upper triangle
+        for (int q = tid; q < nb * 256; q += 256) {
+            const int c = q >> 8, rr = (q >> 4) & 15, cc = q & 15;
+            const int r = c + d;
+            const int R = 16 * r + rr, Cc = 16 * c + cc;
+            T acc = T(0);
+            for (int m = 16 * c; m < 16 * r; ++m) acc += a[R][m] * b[m][Cc];
+            b[Cc][R] = acc;
+        }
+        __syncthreads();
+        for (int q = tid; q < nb * 256; q += 256) {
+            const int c = q >> 8, rr = (q >> 4) & 15, cc = q & 15;
+            const int r = c + d;
+            const int R = 16 * r + rr, Cc = 16 * c + cc;
+            T acc = T(0);
+            for (int m = 16 * r; m <= R; ++m) acc += b[R][m] * b[Cc][m];
+            b[R][Cc] = -acc;
+        }
+        __syncthreads();
+    }
+    return s_bad == 0;
+}
+
+// Writes the factored diagonal tile k: D[k] = L_kk (row-major, upper zero),
+// DinvT[k] = (L_kk^-1)^T; raises chol_fail on a bad pivot.
+template <typename T>
+__device__ __forceinline__ void store_diag_tile(const T (*a)[kTile + 2], const T (*b)[kTile + 2], bool ok, int k,
+                                T* __restrict__ D, T* __restrict__ DinvT, TrialFlags* flags) {
+    if (!ok && threadIdx.x == 0) atomicOr(&flags->chol_fail, 1);
+    T* d = D + static_cast<size_t>(k) * kTile * kTile;
+    T* it = DinvT + static_cast<size_t>(k) * kTile * kTile;
+    for (int q = threadIdx.x; q < kTile * kTile; q += 256) {
+        const int r = q >> 6, c = q & 63;
+        d[q] = (c <= r) ? a[r][c] : T(0);
+        it[q] = (r <= c) ? b[c][r] : T(0);
+    }
+}
+
+// Factor + invert diagonal tile k of S (the first panel; later diagonal tiles
+// are factored by block 0 of the trailing update that finishes them).
+template <typename T>
+__global__ __launch_bounds__(256) void k_potrf_tile(const T* __restrict__ S, int ld, int k,
                                                    T* __restrict__ D, T* __restrict__ DinvT,
                                                    TrialFlags* flags) {
-    __shared__ T col[kTile];
-    __shared__ T L[kTile][kTile + 1];
+    __shared__ T a[kTile][kTile + 2];
+    __shared__ T b[kTile][kTile + 2];
     if (any_fail(flags)) return;
-    const int lane = threadIdx.x;
-    bool bad = false;
-    {
-        const T* row = S + static_cast<size_t>(k * kTile + lane) * ld + k * kTile;
-        T a[kTile];
-#pragma unroll
-        for (int c = 0; c < kTile; ++c) a[c] = row[c];
-#pragma unroll
-        for (int j = 0; j < kTile; ++j) {
-            const T d = readlane(a[j], j);
-            bad = bad || !(d > T(0));
-            const T piv = sqrt(d);
-            const T lj = (lane == j) ? piv : a[j] / piv;
-            a[j] = lj;
-            col[lane] = lj;
-            __syncthreads();
-#pragma unroll
-            for (int m = j + 1; m < kTile; ++m) a[m] -= lj * col[m];
-            __syncthreads();
-        }
-        T* drow = D + static_cast<size_t>(k) * kTile * kTile + lane * kTile;
-#pragma unroll
-        for (int c = 0; c < kTile; ++c) {
-            const T v = (c <= lane) ? a[c] : T(0);
-            L[lane][c] = v;
-            drow[c] = v;
-        }
-    }
-    if (bad && lane == 0) atomicOr(&flags->chol_fail, 1);
+    const T* Akk = S + static_cast<size_t>(k * kTile) * ld + k * kTile;
+    for (int q = threadIdx.x; q < kTile * kTile; q += 256) a[q >> 6][q & 63] = Akk[static_cast<size_t>(q >> 6) * ld + (q & 63)];
     __syncthreads();
-    // column `lane` of L^-1: right-looking forward substitution on e_lane
-    T x[kTile];
-#pragma unroll
-    for (int i = 0; i < kTile; ++i) x[i] = (i == lane) ? T(1) : T(0);
-#pragma unroll
-    for (int i = 0; i < kTile; ++i) {
-        x[i] /= L[i][i];
-        const T xi = x[i];
-#pragma unroll
-        for (int m = i + 1; m < kTile; ++m) x[m] -= L[m][i] * xi;
-    }
-    T* irow = DinvT + static_cast<size_t>(k) * kTile * kTile + lane * kTile;
-#pragma unroll
-    for (int i = 0; i < kTile; ++i) irow[i] = (i >= lane) ? x[i] : T(0);
+    const bool ok = tile_potrf_inv<T>(a, b);
+    store_diag_tile<T>(a, b, ok, k, D, DinvT, flags);
 }
 
 // L_ik = A_ik (L_kk^-1)^T for tile rows i > k: an MFMA GEMM per tile.
@@ -167,9 +235,13 @@ __global__ __launch_bounds__(256) void k_trsm_tiles(T* __restrict__ S, int ld, i
             }
 }
 
+// Trailing update A_ij -= L_ik L_jk^T for k < j <= i.  Block 0 owns tile
+// (k+1, k+1), which this update completes: it then factors and inverts it
+// (lookahead), so the next panel needs no separate factor launch.
 template <typename T>
-__global__ __launch_bounds__(256) void k_chol_update(T* __restrict__ S, int ld, int k,
-                                                     const TrialFlags* flags) {
+__global__ __launch_bounds__(256, 2) void k_chol_update(T* __restrict__ S, int ld, int k,
+                                                     T* __restrict__ D, T* __restrict__ DinvT,
+                                                     TrialFlags* flags) {
     __shared__ T Ai[kTile][kTile + 2];
     __shared__ T Aj[kTile][kTile + 2];
     if (any_fail(flags)) return;
@@ -206,6 +278,23 @@ __global__ __launch_bounds__(256) void k_chol_update(T* __restrict__ S, int ld, 
         acc[1][1] = mfma16(a1, b1, acc[1][1]);
     }
     T* Aij = S + static_cast<size_t>(i * kTile) * ld + j * kTile;
+    if (t == 0) {
+        // lookahead: finish tile (k+1, k+1) in LDS and factor it
+        __syncthreads();  // every wave is done reading Ai / Aj
+#pragma unroll
+        for (int r = 0; r < 2; ++r)
+#pragma unroll
+            for (int c = 0; c < 2; ++c)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int row = wr + 16 * r + acc_row<T>(lk, q), col = wc + 16 * c + li;
+                    Ai[row][col] = Aij[static_cast<size_t>(row) * ld + col] - acc[r][c][q];
+                }
+        __syncthreads();
+        const bool ok = tile_potrf_inv<T>(Ai, Aj);
+        store_diag_tile<T>(Ai, Aj, ok, k + 1, D, DinvT, flags);
+        return;
+    }
 #pragma unroll
     for (int r = 0; r < 2; ++r)
 #pragma unroll
@@ -459,40 +548,27 @@ __global__ void k_vec_scale(int n, int npad, const double* __restrict__ in,
 
 }  // namespace
 
-void launch_cholesky_panel(hipStream_t st, double* S64, float* S32, int ld, int nblk, int k,
-                           double* D64, float* D32, TrialFlags* flags) {
-    const size_t tile = static_cast<size_t>(kTile) * kTile;
-    if (S64) {
-        hipLaunchKernelGGL(k_potrf_tile<double>, dim3(1), dim3(64), 0, st, S64, ld, k, D64,
-                           D64 + static_cast<size_t>(nblk) * tile, flags);
-        if (k + 1 < nblk)
-            hipLaunchKernelGGL(k_trsm_tiles<double>, dim3(nblk - k - 1), dim3(256), 0, st, S64, ld, k,
-                               D64 + static_cast<size_t>(nblk) * tile, flags);
-    } else {
-        hipLaunchKernelGGL(k_potrf_tile<float>, dim3(1), dim3(64), 0, st, S32, ld, k, D32,
-                           D32 + static_cast<size_t>(nblk) * tile, flags);
-        if (k + 1 < nblk)
-            hipLaunchKernelGGL(k_trsm_tiles<float>, dim3(nblk - k - 1), dim3(256), 0, st, S32, ld, k,
-                               D32 + static_cast<size_t>(nblk) * tile, flags);
-    }
-}
-
-void launch_cholesky_update(hipStream_t st, double* S64, float* S32, int ld, int nblk, int k,
-                            TrialFlags* flags) {
-    const int m = nblk - k - 1;
-    if (m <= 0) return;
-    const int tiles = m * (m + 1) / 2;
-    if (S64)
-        hipLaunchKernelGGL(k_chol_update<double>, dim3(tiles), dim3(256), 0, st, S64, ld, k, flags);
-    else
-        hipLaunchKernelGGL(k_chol_update<float>, dim3(tiles), dim3(256), 0, st, S32, ld, k, flags);
-}
-
 void launch_cholesky(hipStream_t st, double* S64, float* S32, int ld, int nblk, double* D64,
                      float* D32, TrialFlags* flags) {
-    for (int k = 0; k < nblk; ++k) {
-        launch_cholesky_panel(st, S64, S32, ld, nblk, k, D64, D32, flags);
-        launch_cholesky_update(st, S64, S32, ld, nblk, k, flags);
+    const size_t tile = static_cast<size_t>(kTile) * kTile;
+    if (S64) {
+        double* Dinv = D64 + static_cast<size_t>(nblk) * tile;
+        hipLaunchKernelGGL(k_potrf_tile<double>, dim3(1), dim3(256), 0, st, S64, ld, 0, D64, Dinv, flags);
+        for (int k = 0; k + 1 < nblk; ++k) {
+            const int m = nblk - k - 1;
+            hipLaunchKernelGGL(k_trsm_tiles<double>, dim3(m), dim3(256), 0, st, S64, ld, k, Dinv, flags);
+            hipLaunchKernelGGL(k_chol_update<double>, dim3(m * (m + 1) / 2), dim3(256), 0, st, S64, ld, k,
+                               D64, Dinv, flags);
+        }
+    } else {
+        float* Dinv = D32 + static_cast<size_t>(nblk) * tile;
+        hipLaunchKernelGGL(k_potrf_tile<float>, dim3(1), dim3(256), 0, st, S32, ld, 0, D32, Dinv, flags);
+        for (int k = 0; k + 1 < nblk; ++k) {
+            const int m = nblk - k - 1;
+            hipLaunchKernelGGL(k_trsm_tiles<float>, dim3(m), dim3(256), 0, st, S32, ld, k, Dinv, flags);
+            hipLaunchKernelGGL(k_chol_update<float>, dim3(m * (m + 1) / 2), dim3(256), 0, st, S32, ld, k,
+                               D32, Dinv, flags);
+        }
     }
 }
 

@@ -51,10 +51,6 @@ void launch_diag_scale(hipStream_t st, int n, int npad, const double* U, double 
 // inverses transposed; the off-diagonal tiles of L replace S's.
 void launch_cholesky(hipStream_t st, double* S64, float* S32, int ld, int nblk, double* D64,
                      float* D32, TrialFlags* flags);
-void launch_cholesky_panel(hipStream_t st, double* S64, float* S32, int ld, int nblk, int k,
-                           double* D64, float* D32, TrialFlags* flags);
-void launch_cholesky_update(hipStream_t st, double* S64, float* S32, int ld, int nblk, int k,
-                            TrialFlags* flags);
 // Solves L L^T x = b (b, x length nblk*kTile; b and x may alias).  sync holds
 // 2*nblk + 2 ints of scratch (done flags and tickets).
 void launch_chol_solve(hipStream_t st, const double* L64, const float* L32, int ld, int nblk,
