@@ -50,6 +50,35 @@ __device__ __forceinline__ v4d mfma16(double a, double b, v4d c) {
 __device__ __forceinline__ v4f mfma16(float a, float b, v4f c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
+// 1/sqrt(d) from the hardware estimate refined by Newton steps
+// r <- r (3 - d r^2) / 2: two steps for FP64, one for FP32 (the estimate
+// carries about 23 bits).  Used for the pivots of the tile factor: the pivot
+// is d r and the column scale is r, with no division in the chain.
+__device__ __forceinline__ double rsqrt_refined(double d) {
+    double r = __builtin_amdgcn_rsq(d);
+    r = r * (1.5 - 0.5 * d * r * r);
+    r = r * (1.5 - 0.5 * d * r * r);
+    return r;
+}
+__device__ __forceinline__ float rsqrt_refined(float d) {
+    float r = __builtin_amdgcn_rsqf(d);
+    r = r * (1.5f - 0.5f * d * r * r);
+    return r;
+}
+
+// 1/x from the hardware estimate refined by Newton steps r <- r (2 - x r).
+__device__ __forceinline__ double recip_refined(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    r = r * (2.0 - x * r);
+    r = r * (2.0 - x * r);
+    return r;
+}
+__device__ __forceinline__ float recip_refined(float x) {
+    float r = __builtin_amdgcn_rcpf(x);
+    r = r * (2.0f - x * r);
+    return r;
+}
+
 template <typename T> struct Acc4;
 template <> struct Acc4<double> { typedef v4d type; };
 template <> struct Acc4<float> { typedef v4f type; };
@@ -66,18 +95,23 @@ __device__ __forceinline__ bool any_fail(const TrialFlags* f) {
 
 // Cholesky factor and inverse of the symmetric tile in a (lower triangle
 // read), 256 threads.  On return a holds L (lower; the upper part is stale) and
-// b holds L^-1 (lower; the upper part is scratch).  Blocked by 16 columns:
-// wavefront 0 factors each 16-column panel with the rows in its lanes (pivot
-// from v_readlane, the scaled column broadcast through LDS), all threads apply
-// the rank-16 update to the trailing triangle; the inverse is built from the
-// four 16x16 diagonal inverses (forward substitution, one column per lane) and
-// then by block distance: X_rc = -Linv_rr (sum_{c <= m < r} L_rm Linv_mc).
+// b holds L^-1 (lower, zero above the diagonal inside the 16x16 diagonal
+// blocks; the other upper blocks are scratch).  Blocked by 16 columns:
+//   panel  : wavefront 0, one row per lane, 16 pivot steps; the pivot and the
+//            entries of the scaled column are broadcast with v_readlane, so a
+//            step is a chain of register operations (no LDS round trip);
+//   update : the rank-16 update of the trailing triangle in 16x16 MFMA blocks;
+//   inverse: each wavefront inverts one 16x16 diagonal block by Gauss-Jordan
+//            on its rows (lane = row, pivot rows broadcast with v_readlane),
+//            then by block distance d = 1..3 with MFMA:
+//            Y = sum_{c <= m < r} L_rm Linv_mc, Linv_rc = -Linv_rr Y.
 // Returns false on a pivot that is not > 0.
 template <typename T>
 __device__ __forceinline__ bool tile_potrf_inv(T (*a)[kTile + 2], T (*b)[kTile + 2]) {
+    typedef typename Acc4<T>::type acc_t;
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    const int li = lane & 15, lk = lane >> 4;
     __shared__ int s_bad;
-    __shared__ T s_col[kTile];
     if (tid == 0) s_bad = 0;
     __syncthreads();
     for (int p = 0; p < 4; ++p) {
@@ -91,16 +125,11 @@ __device__ __forceinline__ bool tile_potrf_inv(T (*a)[kTile + 2], T (*b)[kTile +
             for (int jj = 0; jj < 16; ++jj) {
                 const T d = readlane(v[jj], c0 + jj);
                 bad = bad || !(d > T(0));
-                const T piv = sqrt(d);
-                const T rinv = T(1) / piv;
-                const T l = (lane == c0 + jj) ? piv : v[jj] * rinv;
+                const T rinv = rsqrt_refined(d);
+                const T l = (lane == c0 + jj) ? d * rinv : v[jj] * rinv;
                 v[jj] = l;
-                // the column goes through LDS (in order within the wavefront)
-                s_col[lane] = l;
-                __builtin_amdgcn_wave_barrier();
 #pragma unroll
-                for (int m = jj + 1; m < 16; ++m) v[m] -= l * s_col[c0 + m];
-                __builtin_amdgcn_wave_barrier();
+                for (int m = jj + 1; m < 16; ++m) v[m] -= l * readlane(l, c0 + m);
             }
             if (lane >= c0) {
 #pragma unroll
@@ -109,49 +138,71 @@ __device__ __forceinline__ bool tile_potrf_inv(T (*a)[kTile + 2], T (*b)[kTile +
             if (bad && lane == 0) s_bad = 1;
         }
         __syncthreads();
-        const int n2 = kTile - c0 - 16;
-        for (int q = tid; q < n2 * n2; q += 256) {
-            const int r = c0 + 16 + q / n2, c = c0 + 16 + q % n2;
-            if (c > r) continue;
-            T acc = a[r][c];
+        // trailing blocks (rb, cb), cb <= rb, of the rows/columns after the panel
+        const int nb = 3 - p;
+        for (int bi = w; bi < nb * (nb + 1) / 2; bi += 4) {
+            int rb = 0;
+            while ((rb + 1) * (rb + 2) / 2 <= bi) ++rb;
+            const int cb = bi - rb * (rb + 1) / 2;
+            const int R0 = c0 + 16 + 16 * rb, C0 = c0 + 16 + 16 * cb;
+            acc_t acc = acc_t{0, 0, 0, 0};
 #pragma unroll
-            for (int m = 0; m < 16; ++m) acc -= a[r][c0 + m] * a[c][c0 + m];
-            a[r][c] = acc;
+            for (int kk = 0; kk < 16; kk += 4)
+                acc = mfma16(a[R0 + li][c0 + kk + lk], a[C0 + li][c0 + kk + lk], acc);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) a[R0 + acc_row<T>(lk, q)][C0 + li] -= acc[q];
         }
         __syncthreads();
     }
-    // diagonal 16x16 inverses: wave w, lane l < 16 -> column 16 w + l
-    // (the column is kept in LDS, not registers: rows above the diagonal come
-    // out as exact zeros)
-    if (lane < 16) {
-        const int o = 16 * w;
-#pragma nounroll
-        for (int i = 0; i < 16; ++i) {
-            T sacc = (i == lane) ? T(1) : T(0);
-            for (int m = 0; m < i; ++m) sacc -= a[o + i][o + m] * b[o + m][o + lane];
-            b[o + i][o + lane] = sacc / a[o + i][o + i];
+    {
+        // Gauss-Jordan on the rows of diagonal block w: lane i (mod 16) holds
+        // row i of L and of the growing inverse
+        const int o = 16 * w, i = li;
+        T lrow[16], x[16];
+#pragma unroll
+        for (int m = 0; m < 16; ++m) {
+            lrow[m] = a[o + i][o + m];
+            x[m] = (i == m) ? T(1) : T(0);
+        }
+#pragma unroll
+        for (int m = 0; m < 16; ++m) {
+            const T rm = recip_refined(readlane(lrow[m], m));
+            if (i == m) {
+#pragma unroll
+                for (int c = 0; c <= m; ++c) x[c] *= rm;
+            }
+#pragma unroll
+            for (int c = 0; c <= m; ++c) {
+                const T xm = readlane(x[c], m);
+                if (i > m) x[c] -= lrow[m] * xm;
+            }
+        }
+        if (lane < 16) {
+#pragma unroll
+            for (int c = 0; c < 16; ++c) b[o + i][o + c] = (c <= i) ? x[c] : T(0);
         }
     }
     __syncthreads();
     for (int d = 1; d < 4; ++d) {
-        const int nb = 4 - d;  // blocks (c + d, c), c < nb
-        // Y_rc = sum_{c <= m < r} L_rm Linv_mc, parked transposed in the upper triangle
-        for (int q = tid; q < nb * 256; q += 256) {
-            const int c = q >> 8, rr = (q >> 4) & 15, cc = q & 15;
-            const int r = c + d;
-            const int R = 16 * r + rr, Cc = 16 * c + cc;
-            T acc = T(0);
-            for (int m = 16 * c; m < 16 * r; ++m) acc += a[R][m] * b[m][Cc];
-            b[Cc][R] = acc;
-        }
-        __syncthreads();
-        for (int q = tid; q < nb * 256; q += 256) {
-            const int c = q >> 8, rr = (q >> 4) & 15, cc = q & 15;
-            const int r = c + d;
-            const int R = 16 * r + rr, Cc = 16 * c + cc;
-            T acc = T(0);
-            for (int m = 16 * r; m <= R; ++m) acc += b[R][m] * b[Cc][m];
-            b[R][Cc] = -acc;
+        if (w < 4 - d) {
+            const int c = w, r = c + d;
+            const int R0 = 16 * r, C0 = 16 * c;
+            acc_t y = acc_t{0, 0, 0, 0};
+            for (int mb = c; mb < r; ++mb) {
+#pragma unroll
+                for (int kk = 0; kk < 16; kk += 4)
+                    y = mfma16(a[R0 + li][16 * mb + kk + lk], b[16 * mb + kk + lk][C0 + li], y);
+            }
+            // park Y transposed in the upper block (c, r) of b
+#pragma unroll
+            for (int q = 0; q < 4; ++q) b[C0 + li][R0 + acc_row<T>(lk, q)] = y[q];
+            __builtin_amdgcn_wave_barrier();
+            acc_t xr = acc_t{0, 0, 0, 0};
+#pragma unroll
+            for (int kk = 0; kk < 16; kk += 4)
+                xr = mfma16(b[R0 + li][R0 + kk + lk], b[C0 + li][R0 + kk + lk], xr);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) b[R0 + acc_row<T>(lk, q)][C0 + li] = -xr[q];
         }
         __syncthreads();
     }
@@ -189,21 +240,34 @@ __global__ __launch_bounds__(256) void k_potrf_tile(const T* __restrict__ S, int
     store_diag_tile<T>(a, b, ok, k, D, DinvT, flags);
 }
 
-// L_ik = A_ik (L_kk^-1)^T for tile rows i > k: an MFMA GEMM per tile.
+// L_ik = A_ik (L_kk^-1)^T for tile rows i > k: an MFMA GEMM per tile, A_ik
+// staged in LDS, the B operand read straight from DinvT (row kk of DinvT is
+// column kk of L_kk^-1, so a 16-lane group reads 128 contiguous bytes).  The
+// result is written twice: L_ik over the lower tile (the triangular solves
+// read it) and L_ik^T into the unused upper tile (k, i), from which the
+// trailing update reads both MFMA operands with coalesced loads.
 template <typename T>
 __global__ __launch_bounds__(256) void k_trsm_tiles(T* __restrict__ S, int ld, int k,
                                                     const T* __restrict__ DinvT,
                                                     const TrialFlags* flags) {
-    __shared__ T Ai[kTile][kTile + 2];
-    __shared__ T Bi[kTile][kTile + 2];  // Bi[c][m] = (L_kk^-1)[c][m]
+    __shared__ T Ai[kTile][kTile + 1];
     if (any_fail(flags)) return;
     const int i = k + 1 + blockIdx.x;
     T* Aik = S + static_cast<size_t>(i * kTile) * ld + k * kTile;
+    T* LTki = S + static_cast<size_t>(k * kTile) * ld + i * kTile;
     const T* inv = DinvT + static_cast<size_t>(k) * kTile * kTile;
-    for (int q = threadIdx.x; q < kTile * kTile; q += 256) {
-        const int r = q >> 6, c = q & 63;
-        Ai[r][c] = Aik[static_cast<size_t>(r) * ld + c];
-        Bi[c][r] = inv[q];  // inv[r][c] = Linv[c][r]
+    {
+        T tmp[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int q = threadIdx.x + 256 * u;
+            tmp[u] = Aik[static_cast<size_t>(q >> 6) * ld + (q & 63)];
+        }
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int q = threadIdx.x + 256 * u;
+            Ai[q >> 6][q & 63] = tmp[u];
+        }
     }
     __syncthreads();
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -214,34 +278,48 @@ __global__ __launch_bounds__(256) void k_trsm_tiles(T* __restrict__ S, int ld, i
     for (int r = 0; r < 2; ++r)
 #pragma unroll
         for (int c = 0; c < 2; ++c) acc[r][c] = typename Acc4<T>::type{0, 0, 0, 0};
-#pragma unroll 4
+    T bv[16][2];
+#pragma unroll
+    for (int kk = 0; kk < kTile; kk += 4) {
+        bv[kk >> 2][0] = inv[(kk + lk) * kTile + wc + li];
+        bv[kk >> 2][1] = inv[(kk + lk) * kTile + wc + 16 + li];
+    }
+#pragma unroll
     for (int kk = 0; kk < kTile; kk += 4) {
         const T a0 = Ai[wr + li][kk + lk], a1 = Ai[wr + 16 + li][kk + lk];
-        const T b0 = Bi[wc + li][kk + lk], b1 = Bi[wc + 16 + li][kk + lk];
+        const T b0 = bv[kk >> 2][0], b1 = bv[kk >> 2][1];
         acc[0][0] = mfma16(a0, b0, acc[0][0]);
         acc[0][1] = mfma16(a0, b1, acc[0][1]);
         acc[1][0] = mfma16(a1, b0, acc[1][0]);
         acc[1][1] = mfma16(a1, b1, acc[1][1]);
     }
-    __syncthreads();  // every wave has read Ai before the tile is overwritten
+    __syncthreads();  // every wave has read Ai before it is overwritten
 #pragma unroll
     for (int r = 0; r < 2; ++r)
 #pragma unroll
         for (int c = 0; c < 2; ++c)
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int row = wr + 16 * r + acc_row<T>(lk, q), col = wc + 16 * c + li;
-                Aik[static_cast<size_t>(row) * ld + col] = acc[r][c][q];
-            }
+            for (int q = 0; q < 4; ++q) Ai[wr + 16 * r + acc_row<T>(lk, q)][wc + 16 * c + li] = acc[r][c][q];
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+        const int q = threadIdx.x + 256 * u;
+        const int r = q >> 6, c = q & 63;
+        Aik[static_cast<size_t>(r) * ld + c] = Ai[r][c];
+        LTki[static_cast<size_t>(r) * ld + c] = Ai[c][r];
+    }
 }
 
-// Trailing update A_ij -= L_ik L_jk^T for k < j <= i.  Block 0 owns tile
-// (k+1, k+1), which this update completes: it then factors and inverts it
-// (lookahead), so the next panel needs no separate factor launch.
+// Trailing update A_ij -= L_ik L_jk^T for k < j <= i, one workgroup per tile,
+// four wavefronts of 2x2 MFMA 16x16 blocks; both operands come straight from
+// the transposed panel tiles L_ik^T, L_jk^T (upper tiles (k, i), (k, j)).
+// Block 0 owns tile (k+1, k+1), which this update completes: it then factors
+// and inverts it (lookahead), so the next panel needs no separate factor
+// launch.
 template <typename T>
 __global__ __launch_bounds__(256, 2) void k_chol_update(T* __restrict__ S, int ld, int k,
-                                                     T* __restrict__ D, T* __restrict__ DinvT,
-                                                     TrialFlags* flags) {
+                                                        T* __restrict__ D, T* __restrict__ DinvT,
+                                                        TrialFlags* flags) {
     __shared__ T Ai[kTile][kTile + 2];
     __shared__ T Aj[kTile][kTile + 2];
     if (any_fail(flags)) return;
@@ -252,14 +330,8 @@ __global__ __launch_bounds__(256, 2) void k_chol_update(T* __restrict__ S, int l
     while ((ii + 1) * (ii + 2) / 2 <= t) ++ii;
     const int jj = t - ii * (ii + 1) / 2;
     const int i = k + 1 + ii, j = k + 1 + jj;
-    const T* Lik = S + static_cast<size_t>(i * kTile) * ld + k * kTile;
-    const T* Ljk = S + static_cast<size_t>(j * kTile) * ld + k * kTile;
-    for (int q = threadIdx.x; q < kTile * kTile; q += 256) {
-        const int r = q >> 6, c = q & 63;
-        Ai[r][c] = Lik[static_cast<size_t>(r) * ld + c];
-        Aj[r][c] = Ljk[static_cast<size_t>(r) * ld + c];
-    }
-    __syncthreads();
+    const T* LTi = S + static_cast<size_t>(k * kTile) * ld + i * kTile;
+    const T* LTj = S + static_cast<size_t>(k * kTile) * ld + j * kTile;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
     const int li = lane & 15, lk = lane >> 4;
@@ -268,10 +340,11 @@ __global__ __launch_bounds__(256, 2) void k_chol_update(T* __restrict__ S, int l
     for (int r = 0; r < 2; ++r)
 #pragma unroll
         for (int c = 0; c < 2; ++c) acc[r][c] = typename Acc4<T>::type{0, 0, 0, 0};
-#pragma unroll 4
+#pragma unroll
     for (int kk = 0; kk < kTile; kk += 4) {
-        const T a0 = Ai[wr + li][kk + lk], a1 = Ai[wr + 16 + li][kk + lk];
-        const T b0 = Aj[wc + li][kk + lk], b1 = Aj[wc + 16 + li][kk + lk];
+        const size_t ro = static_cast<size_t>(kk + lk) * ld;
+        const T a0 = LTi[ro + wr + li], a1 = LTi[ro + wr + 16 + li];
+        const T b0 = LTj[ro + wc + li], b1 = LTj[ro + wc + 16 + li];
         acc[0][0] = mfma16(a0, b0, acc[0][0]);
         acc[0][1] = mfma16(a0, b1, acc[0][1]);
         acc[1][0] = mfma16(a1, b0, acc[1][0]);
@@ -280,7 +353,6 @@ __global__ __launch_bounds__(256, 2) void k_chol_update(T* __restrict__ S, int l
     T* Aij = S + static_cast<size_t>(i * kTile) * ld + j * kTile;
     if (t == 0) {
         // lookahead: finish tile (k+1, k+1) in LDS and factor it
-        __syncthreads();  // every wave is done reading Ai / Aj
 #pragma unroll
         for (int r = 0; r < 2; ++r)
 #pragma unroll
