@@ -21,6 +21,7 @@ SOURCES = [
     "k_blocks.hip",
     "k_schur.hip",
     "k_dense.hip",
+    "k_trinv.hip",
     "k_backsub.hip",
     "solver.cpp",
     "capi.cpp",

@@ -193,7 +193,7 @@ int gba_dense_spd_solve(int n, int precision, const double* A, double* b, int* o
         hipStream_t st;
         GBA_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
         DevBuf<double> S64, D64, rhs, y, x, work;
-        DevBuf<float> S32, S32low, D32;
+        DevBuf<float> S32, S32low, D32, X32, T32;
         DevBuf<int> sync;
         DevBuf<TrialFlags> flags;
         DevBuf<PassStatus> status;
@@ -233,11 +233,14 @@ int gba_dense_spd_solve(int n, int precision, const double* A, double* b, int* o
             GBA_HIP(hipHostMalloc(&hs, sizeof(PassStatus), hipHostMallocDefault));
             GBA_HIP(hipMemcpyAsync(S32.p, hf.data(), sizeof(float) * np * np, hipMemcpyHostToDevice, st));
             GBA_HIP(hipMemcpyAsync(S32low.p, hf.data(), sizeof(float) * np * np, hipMemcpyHostToDevice, st));
+            X32.alloc(np * np);
+            T32.alloc(np * np);
             launch_cholesky(st, nullptr, S32.p, npad, nblk, nullptr, D32.p, flags.p);
+            launch_trinv(st, S32.p, D32.p, npad, nblk, X32.p, T32.p, flags.p);
             double rel = 0.0;
             try {
-                iters = pcg_solve(st, S32low.p, S32.p, D32.p, npad, npad, rhs.p, x.p, work.p, sync.p,
-                                  flags.p, status.p, hs, 1e-12, 50, &rel);
+                iters = pcg_solve(st, S32low.p, X32.p, npad, npad, rhs.p, x.p, work.p, flags.p, status.p,
+                                  hs, 1e-12, 50, &rel);
             } catch (...) {
                 (void)hipHostFree(hs);
                 throw;

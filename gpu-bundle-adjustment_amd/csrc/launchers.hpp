@@ -73,6 +73,17 @@ void launch_vec_axpy(hipStream_t st, int n, double* x, const double* d);
 void launch_vec_scale(hipStream_t st, int n, int npad, const double* in, const double* s,
                       double* out);
 
+// k_trinv.hip
+// X = L^-1 of the FP32 factor (L's off-diagonal tiles in L32, diagonal
+// inverses in D32 as laid out by launch_cholesky): X's lower tiles get X_ij,
+// its upper tiles X_ij^T; T32 is an npad x npad scratch matrix.
+void launch_trinv(hipStream_t st, const float* L32, const float* D32, int ld, int nblk, float* X32,
+                  float* T32, const TrialFlags* flags);
+// z = X^T (X r) = (L L^T)^-1 r; work holds precond_work(npad) doubles.
+size_t precond_work(int npad);
+void launch_precond_apply(hipStream_t st, const float* X32, int ld, int npad, const double* r,
+                          double* z, double* work);
+
 // k_backsub.hip
 void launch_trial_cams(hipStream_t st, int C, const double* cams, const double* dc, double* tcams);
 void launch_backsub(hipStream_t st, const DevProblem& pb, const double* W64, const float* W32,

@@ -109,21 +109,20 @@ void build_schur_tasks(int C, int cb, const std::vector<int>& cam_off,
 }  // namespace
 
 size_t pcg_work(int npad) {
-    return 5 * static_cast<size_t>(npad) + symv_work(npad) + 2 * static_cast<size_t>(residual_partials(npad));
+    return 5 * static_cast<size_t>(npad) + std::max(symv_work(npad), precond_work(npad)) +
+           2 * static_cast<size_t>(residual_partials(npad));
 }
 
-int pcg_solve(hipStream_t st, const float* A, const float* L32, const float* D32, int ld, int npad,
-              const double* b, double* x, double* work, int* sync, TrialFlags* flags,
-              PassStatus* status, PassStatus* h_status, double tol, int max_it, double* rel) {
-    const int nblk = npad / kTile;
+int pcg_solve(hipStream_t st, const float* A, const float* X32, int ld, int npad, const double* b,
+              double* x, double* work, TrialFlags* flags, PassStatus* status, PassStatus* h_status,
+              double tol, int max_it, double* rel) {
     const size_t np = static_cast<size_t>(npad);
     double* r = work;
     double* z = work + np;
     double* p = work + 2 * np;
     double* q = work + 3 * np;
-    double* y = work + 4 * np;
     double* sw = work + 5 * np;
-    double* blk = sw + symv_work(npad);
+    double* blk = sw + std::max(symv_work(npad), precond_work(npad));
     auto sync_status = [&]() {
         GBA_HIP(hipGetLastError());
         GBA_HIP(hipMemcpyAsync(h_status, status, sizeof(PassStatus), hipMemcpyDeviceToHost, st));
@@ -131,7 +130,7 @@ int pcg_solve(hipStream_t st, const float* A, const float* L32, const float* D32
     };
     launch_fill_double(st, x, np, 0.0);
     launch_vec_copy(st, npad, b, r);
-    launch_chol_solve(st, nullptr, L32, ld, nblk, nullptr, D32, r, y, z, sync, flags);
+    launch_precond_apply(st, X32, ld, npad, r, z, sw);
     launch_vec_copy(st, npad, z, p);
     launch_dot2(st, npad, r, z, b, b, blk, status, flags);
     sync_status();
@@ -154,13 +153,13 @@ int pcg_solve(hipStream_t st, const float* A, const float* L32, const float* D32
         if (!(pq > 0.0) || !(rz > 0.0)) break;
         const double alpha = rz / pq;
         launch_pcg_update(st, npad, alpha, x, p, r, q);
-        launch_chol_solve(st, nullptr, L32, ld, nblk, nullptr, D32, r, y, z, sync, flags);
+        launch_precond_apply(st, X32, ld, npad, r, z, sw);
         launch_dot2(st, npad, r, z, r, r, blk, status, flags);
         sync_status();
         ++it;
         const double rz_new = h_status->dot[0];
         rr = h_status->dot[1];
-        if (h_status->flags.solve_err || !std::isfinite(rr)) break;
+        if (!std::isfinite(rr)) break;
         if (rr <= tol * tol * bb) break;
         launch_pcg_direction(st, npad, rz_new / rz, z, p);
         rz = rz_new;
@@ -307,6 +306,8 @@ Solver::Solver(const gba_problem_view& prob, const gba_solver_config& cfg) : cfg
     if (mixed_) {
         S32_.alloc(np * np);
         S32low_.alloc(np * np);
+        X32_.alloc(np * np);
+        T32_.alloc(np * np);
         D32_.alloc(2 * static_cast<size_t>(nblk_) * kTile * kTile);
     } else {
         S64_.alloc(np * np);
@@ -429,10 +430,11 @@ void Solver::enqueue_trial_linear_algebra(double lambda, std::vector<double>* ho
     // mixed: FP32 factor of D S D; the camera system is solved in FP64 by CG
     // preconditioned with that factor, against the FP32-assembled D S D
     launch_cholesky(st_, nullptr, S32_.p, ld, nblk_, nullptr, D32_.p, flags_.p);
+    launch_trinv(st_, S32_.p, D32_.p, ld, nblk_, X32_.p, T32_.p, flags_.p);
     const double tol = cfg_.refine_tol > 0 ? cfg_.refine_tol : 1e-10;
     const int max_it = cfg_.refine_max_iters > 0 ? cfg_.refine_max_iters : 12;
-    out.refine_iters = pcg_solve(st_, S32low_.p, S32_.p, D32_.p, ld, pb_.npad, rhs_.p, x_.p, pcgw_.p,
-                                 sync_.p, flags_.p, status_.p, h_status_, tol, max_it, &out.refine_rel);
+    out.refine_iters = pcg_solve(st_, S32low_.p, X32_.p, ld, pb_.npad, rhs_.p, x_.p, pcgw_.p, flags_.p,
+                                 status_.p, h_status_, tol, max_it, &out.refine_rel);
     launch_vec_scale(st_, pb_.n, pb_.npad, x_.p, dscale_.p, dc_.p);
 }
 
@@ -538,11 +540,17 @@ double Solver::time_kernel(int kernel, double lambda, int reps) {
                 enqueue_schur(lambda);
                 break;
             case GBA_KERNEL_FACTOR:
+                // mixed: factor + explicit inverse (what a trial runs before PCG)
                 launch_cholesky(st_, S64_.p, S32_.p, ld, nblk_, D64_.p, D32_.p, flags_.p);
+                if (mixed_) launch_trinv(st_, S32_.p, D32_.p, ld, nblk_, X32_.p, T32_.p, flags_.p);
                 break;
             case GBA_KERNEL_SOLVE:
-                launch_chol_solve(st_, S64_.p, S32_.p, ld, nblk_, D64_.p, D32_.p, rhs_.p, y_.p, x_.p,
-                                  sync_.p, flags_.p);
+                // FP64: both triangular solves; mixed: one preconditioner application
+                if (mixed_)
+                    launch_precond_apply(st_, X32_.p, ld, pb_.npad, rhs_.p, x_.p, pcgw_.p + 5 * static_cast<size_t>(pb_.npad));
+                else
+                    launch_chol_solve(st_, S64_.p, nullptr, ld, nblk_, D64_.p, nullptr, rhs_.p, y_.p, x_.p,
+                                      sync_.p, flags_.p);
                 break;
             case GBA_KERNEL_BACKSUB:
                 launch_backsub(st_, pb_, W64_.p, W32_.p, Vinv_.p, VB_.p, dc_.p, pts_[cur_].p,

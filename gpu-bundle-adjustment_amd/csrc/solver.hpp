@@ -36,15 +36,15 @@ struct DevBuf {
 };
 
 // FP64 preconditioned conjugate gradients on A x = b, A symmetric given by the
-// lower triangle of an FP32 matrix, preconditioned by its FP32 Cholesky factor
-// (L32 off-diagonal tiles, D32 diagonal / inverse tiles).  Vectors have npad
-// entries.  Stops at |r| <= tol |b| or after max_it steps; returns the steps
-// taken and the final |r| / |b| in *rel.  work holds pcg_work(npad) doubles,
-// sync 2 nblk + 2 ints.  Synchronises the stream twice per step.
+// lower triangle of an FP32 matrix, preconditioned by (L L^T)^-1 applied
+// through the explicit FP32 inverse X = L^-1 (launch_trinv layout).  Vectors
+// have npad entries.  Stops at |r| <= tol |b| or after max_it steps; returns
+// the steps taken and the final |r| / |b| in *rel.  work holds pcg_work(npad)
+// doubles.  Synchronises the stream twice per step.
 size_t pcg_work(int npad);
-int pcg_solve(hipStream_t st, const float* A, const float* L32, const float* D32, int ld, int npad,
-              const double* b, double* x, double* work, int* sync, TrialFlags* flags,
-              PassStatus* status, PassStatus* h_status, double tol, int max_it, double* rel);
+int pcg_solve(hipStream_t st, const float* A, const float* X32, int ld, int npad, const double* b,
+              double* x, double* work, TrialFlags* flags, PassStatus* status, PassStatus* h_status,
+              double tol, int max_it, double* rel);
 
 // Result of one enqueued-and-synchronised LM trial.
 struct TrialOut {
@@ -115,7 +115,7 @@ private:
     DevBuf<double> unit_sse_, unit_den_, gram_, blk_sse_;
     DevBuf<unsigned> unit_anom_, blk_anom_;
     DevBuf<double> S64_, D64_;
-    DevBuf<float> S32_, S32low_, D32_;
+    DevBuf<float> S32_, S32low_, D32_, X32_, T32_;
     DevBuf<double> rhs_, y_, x_, dc_, dscale_, pcgw_;
     DevBuf<int> sync_;
     DevBuf<TrialFlags> flags_;

@@ -121,8 +121,8 @@ enum gba_kernel_id {
     GBA_KERNEL_EDGE_POINT = 0,  /* linearisation, point pass (residual, J, W, V, bp) */
     GBA_KERNEL_EDGE_CAMERA = 1, /* linearisation, camera pass (U, bc) */
     GBA_KERNEL_SCHUR = 2,       /* reduced camera system assembly */
-    GBA_KERNEL_FACTOR = 3,      /* dense Cholesky of S (all its launches) */
-    GBA_KERNEL_SOLVE = 4,       /* triangular solves */
+    GBA_KERNEL_FACTOR = 3,      /* dense Cholesky of S (all its launches); mixed: + explicit L^-1 */
+    GBA_KERNEL_SOLVE = 4,       /* FP64: both triangular solves; mixed: one preconditioner apply */
     GBA_KERNEL_BACKSUB = 5,     /* point back-substitution + trial cost */
     GBA_KERNEL_COST = 6         /* residual-only cost pass */
 };
