@@ -135,17 +135,24 @@ def main():
     edge_bytes = O * (16 + 8 + w_bytes) + P * (24 + 72) + 72 * C
     edge_gbs = edge_bytes / (km["edge_point"] * 1e-3) / 1e9
     factor_tflops = (n ** 3 / 3.0) / (km["factor"] * 1e-3) / 1e12
-    if dominant == "factor":
-        roof = {"kernel": "dense Cholesky of S (FP32 MFMA)", "bound": "mfma",
-                "achieved": round(factor_tflops, 3), "peak": PEAK["fp32_tflops"], "unit": "TFLOP/s",
-                "frac": round(factor_tflops / PEAK["fp32_tflops"], 4), "traffic": None,
-                "basis": f"n^3/3 flops, n = 9C = {n}"}
-    else:
-        # the remaining candidates are HBM / scatter bound; report the edge kernel's
-        # bytes basis when it dominates, else the dominant kernel's time only
-        roof = {"kernel": dominant, "bound": "hbm", "achieved": round(edge_gbs, 1),
-                "peak": PEAK["hbm_gbs"], "unit": "GB/s", "frac": round(edge_gbs / PEAK["hbm_gbs"], 4),
-                "traffic": None, "basis": "edge point pass algorithmic bytes"}
+    # Roofline of the Schur assembly (the single kernel with the largest share
+    # of the mixed solve at this size): algorithmic bytes per launch
+    # O * w_W + 72 P + n^2/2 * w_S, divided by its live HIP-event duration;
+    # traffic = HBM bytes per launch from the committed rocprofv3 FETCH_SIZE /
+    # WRITE_SIZE passes (scaled by the copy-kernel calibration), if present.
+    schur_bytes = O * w_bytes + 72 * P + (n * n // 2) * 4
+    schur_gbs = schur_bytes / (km["schur"] * 1e-3) / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "k_schur_C356_traffic.json")
+    if os.path.exists(tpath) and args.config == 3:
+        with open(tpath) as fh:
+            t = json.load(fh)
+        traffic = round((t["fetch_kib"] * t["fetch_scale"] + t["write_kib"] * t["write_scale"]) * 1024)
+    roof = {"kernel": "k_schur (Schur assembly, mixed form)", "bound": "hbm",
+            "achieved": round(schur_gbs, 1), "peak": PEAK["hbm_gbs"], "unit": "GB/s",
+            "frac": round(schur_gbs / PEAK["hbm_gbs"], 4), "traffic": traffic,
+            "basis": f"O*108 + 72P + n^2/2*4 = {schur_bytes} B per launch over {km['schur']:.4f} ms",
+            "largest_share": dominant}
 
     out = {
         "metric": "time-to-final-cost (rel err<=1e-6 vs FP64), C=356; mixed vs FP64 GPU",
@@ -178,6 +185,7 @@ def main():
         "kernels_ms": {k: {kk: round(vv, 4) for kk, vv in v.items()} for k, v in kern.items()},
         "edge_kernel": {"bytes": edge_bytes, "gbs": round(edge_gbs, 1),
                         "frac": round(edge_gbs / PEAK["hbm_gbs"], 4)},
+        "kernel_share_ms": {k: round(v, 3) for k, v in share.items()},
         "factor": {"flops": n ** 3 / 3.0, "tflops_fp32": round(factor_tflops, 3),
                    "tflops_fp64": round((n ** 3 / 3.0) / (kern["fp64"]["factor"] * 1e-3) / 1e12, 3)},
         "schur_pairs": st["pair_count"],
