@@ -74,6 +74,14 @@ _PROTOS = {
                                               ctypes.c_int, _dp]),
     "gba_solver_stats": (ctypes.c_int, [ctypes.c_void_p, _dp, _ip, _ip]),
     "gba_solver_destroy": (None, [ctypes.c_void_p]),
+    "gba_batch_create": (ctypes.c_int, [ctypes.POINTER(ProblemView), ctypes.c_int,
+                                        ctypes.POINTER(ctypes.c_void_p)]),
+    "gba_batch_solve": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(LMOptionsC),
+                                       ctypes.POINTER(LMResultC), _ip]),
+    "gba_batch_params": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp, _dp]),
+    "gba_batch_destroy": (None, [ctypes.c_void_p]),
+    "gba_solve_bal_batched": (ctypes.c_int, [ctypes.POINTER(ProblemView), ctypes.c_int,
+                                             ctypes.POINTER(LMOptionsC), ctypes.POINTER(LMResultC)]),
     "gba_linearize": (ctypes.c_int, [ctypes.POINTER(ProblemView), ctypes.c_int, _dp, _dp, _dp, _dp,
                                      _dp, _dp, ctypes.POINTER(ctypes.c_ulonglong)]),
     "gba_schur_step": (ctypes.c_int, [ctypes.POINTER(ProblemView), ctypes.c_int, ctypes.c_double, _dp,
@@ -288,6 +296,57 @@ class Solver:
             self.close()
         except Exception:
             pass
+
+
+class Batch:
+    """Batched solver handle (many independent problems, FP64, one launch per
+    LM stage for the whole batch): upload once, solve repeatedly."""
+
+    def __init__(self, probs):
+        self._probs = list(probs)
+        n = len(self._probs)
+        self._views = (ProblemView * n)(*[p.view() for p in self._probs])
+        self._h = ctypes.c_void_p()
+        _check(lib().gba_batch_create(self._views, n, ctypes.byref(self._h)))
+        self.rounds = 0
+
+    def solve(self, opts=None):
+        opts = opts or LMOptions()
+        n = len(self._probs)
+        res = (LMResultC * n)()
+        rounds = ctypes.c_int()
+        _check(lib().gba_batch_solve(self._h, ctypes.byref(opts.c()), res, ctypes.byref(rounds)))
+        self.rounds = rounds.value
+        return [LMResult.from_c(r) for r in res]
+
+    def params(self, i):
+        p = self._probs[i]
+        cams = np.zeros((p.num_cameras, 9))
+        pts = np.zeros((p.num_points, 3))
+        _check(lib().gba_batch_params(self._h, i, _ptr(cams), _ptr(pts)))
+        return cams, pts
+
+    def close(self):
+        if self._h:
+            lib().gba_batch_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def solve_bal_batched(probs, opts=None):
+    """Batched one-shot solve; updates each problem's cameras / points in place."""
+    opts = opts or LMOptions()
+    probs = list(probs)
+    n = len(probs)
+    views = (ProblemView * n)(*[p.view() for p in probs])
+    res = (LMResultC * n)()
+    _check(lib().gba_solve_bal_batched(views, n, ctypes.byref(opts.c()), res))
+    return [LMResult.from_c(r) for r in res]
 
 
 def linearize(prob, precision=PRECISION_FP64):

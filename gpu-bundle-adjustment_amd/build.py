@@ -23,7 +23,9 @@ SOURCES = [
     "k_dense.hip",
     "k_trinv.hip",
     "k_backsub.hip",
+    "k_batch.hip",
     "solver.cpp",
+    "batch.cpp",
     "capi.cpp",
     "cpp_api.cpp",
     "host_model.cpp",

@@ -1,0 +1,337 @@
+// batch.cpp -- batched LM solver (see batch.hpp).  The per-problem control
+// follows Solver::solve, i.e. the reference's src/lm.cu:455-714, problem by
+// problem; only the device work is shared.
+#include "batch.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+#include "launchers.hpp"
+
+namespace gba_amd {
+
+namespace {
+
+double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+template <typename T>
+void upload(DevBuf<T>& b, const std::vector<T>& v, hipStream_t st) {
+    b.alloc(std::max<size_t>(1, v.size()));
+    if (!v.empty()) GBA_HIP(hipMemcpyAsync(b.p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, st));
+}
+
+// Largest padded order the one-workgroup-per-problem factor handles.
+constexpr int kBatchMaxNpad = 1024;
+
+}  // namespace
+
+BatchSolver::BatchSolver(const gba_problem_view* probs, int count) : B_(count) {
+    if (count <= 0) throw Error("batch: no problems");
+    std::vector<int> scam, spt, pt_off(1, 0), cam_list, cam_off(1, 0), cam_chunk_off, cam_prob, pt_prob;
+    std::vector<double2> spx;
+    std::vector<int2> units, spairs;
+    std::vector<int4> chunks;
+    std::vector<SchurTask> stasks;
+    int Cmax = 1;
+    for (int b = 0; b < count; ++b) Cmax = std::max(Cmax, probs[b].num_cameras);
+    sshape_ = schur_shape(Cmax, false);
+    long long C0 = 0, P0 = 0, O0 = 0, soff = 0, doff = 0;
+    for (int b = 0; b < count; ++b) {
+        const gba_problem_view& pr = probs[b];
+        const int C = pr.num_cameras, P = pr.num_points, O = pr.num_observations;
+        if (C <= 0 || P <= 0 || O <= 0) throw Error("batch: problem sizes must be positive");
+        for (int e = 0; e < O; ++e)
+            if (pr.obs_camera[e] < 0 || pr.obs_camera[e] >= C || pr.obs_point[e] < 0 || pr.obs_point[e] >= P)
+                throw Error("batch: observation index out of range");
+        const int n = 9 * C, npad = (n + kTile - 1) / kTile * kTile;
+        if (npad > kBatchMaxNpad) throw Error("batch: a problem has more than 113 cameras");
+        if (C0 + C > (1LL << 30) || P0 + P > (1LL << 30) || O0 + O > (1LL << 30))
+            throw Error("batch: too large");
+        HostLayout L;
+        build_layout(pr, L);
+        std::vector<SchurTask> t;
+        std::vector<int2> pr_pairs;
+        build_schur_tasks(C, sshape_.cb, L.cam_off, L.cam_list, L.pt_off, L.scam, L.spt, t, pr_pairs);
+        BatchDesc d;
+        d.c0 = static_cast<int>(C0);
+        d.c1 = static_cast<int>(C0 + C);
+        d.u0 = static_cast<int>(units.size());
+        d.u1 = d.u0 + static_cast<int>(L.units.size());
+        d.soff = soff;
+        d.doff = doff;
+        d.n = n;
+        d.npad = npad;
+        hdesc_.push_back(d);
+        const long long pair0 = static_cast<long long>(spairs.size());
+        for (const int2& q : pr_pairs) spairs.push_back(make_int2(q.x + static_cast<int>(O0), q.y + static_cast<int>(O0)));
+        for (SchurTask x : t) {
+            x.a += static_cast<int>(C0);
+            x.lo += static_cast<int>(C0);
+            x.hi += static_cast<int>(C0);
+            x.a0 = static_cast<int>(C0);
+            x.begin += pair0;
+            x.end += pair0;
+            x.soff = soff;
+            x.ld = npad;
+            stasks.push_back(x);
+        }
+        for (int k = 0; k < O; ++k) {
+            scam.push_back(L.scam[k] + static_cast<int>(C0));
+            spt.push_back(L.spt[k] + static_cast<int>(P0));
+            spx.push_back(L.spx[k]);
+        }
+        for (int p = 0; p < P; ++p) pt_off.push_back(L.pt_off[p + 1] + static_cast<int>(O0));
+        for (const int2& u : L.units) units.push_back(make_int2(u.x + static_cast<int>(O0), u.y + static_cast<int>(O0)));
+        for (int k = 0; k < O; ++k) cam_list.push_back(L.cam_list[k] + static_cast<int>(O0));
+        for (int c = 0; c < C; ++c) cam_off.push_back(L.cam_off[c + 1] + static_cast<int>(O0));
+        const int ch0 = static_cast<int>(chunks.size());
+        for (int c = 0; c < C; ++c) cam_chunk_off.push_back(L.cam_chunk_off[c] + ch0);
+        for (const int4& ch : L.chunks)
+            chunks.push_back(make_int4(ch.x + static_cast<int>(C0), ch.y + static_cast<int>(O0),
+                                       ch.z + static_cast<int>(O0), 0));
+        for (int c = 0; c < C; ++c) cam_prob.push_back(b);
+        for (int p = 0; p < P; ++p) pt_prob.push_back(b);
+        init_cams_.insert(init_cams_.end(), pr.cameras, pr.cameras + 9 * static_cast<size_t>(C));
+        init_pts_.insert(init_pts_.end(), pr.points, pr.points + 3 * static_cast<size_t>(P));
+        pt0_.push_back(P0);
+        C0 += C;
+        P0 += P;
+        O0 += O;
+        soff += static_cast<long long>(npad) * npad;
+        doff += 2LL * (npad / kTile) * kTile * kTile;
+    }
+    cam_chunk_off.push_back(static_cast<int>(chunks.size()));
+    pt0_.push_back(P0);
+    // longest Schur tasks first across the whole batch
+    std::stable_sort(stasks.begin(), stasks.end(), [](const SchurTask& x, const SchurTask& y) {
+        return x.end - x.begin > y.end - y.begin;
+    });
+    num_stasks_ = static_cast<int>(stasks.size());
+
+    GBA_HIP(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
+    upload(scam_, scam, st_);
+    upload(spt_, spt, st_);
+    upload(spx_, spx, st_);
+    upload(pt_off_, pt_off, st_);
+    upload(units_, units, st_);
+    upload(cam_list_, cam_list, st_);
+    upload(cam_off_, cam_off, st_);
+    upload(chunks_, chunks, st_);
+    upload(cam_chunk_off_, cam_chunk_off, st_);
+    upload(cam_prob_, cam_prob, st_);
+    upload(pt_prob_, pt_prob, st_);
+    upload(spairs_, spairs, st_);
+    upload(stasks_, stasks, st_);
+    upload(desc_, hdesc_, st_);
+    const int C = static_cast<int>(C0), P = static_cast<int>(P0), O = static_cast<int>(O0);
+    pb_.C = C;
+    pb_.P = P;
+    pb_.O = O;
+    pb_.n = 9 * C;
+    pb_.npad = 0;  // per problem (BatchDesc)
+    pb_.num_units = static_cast<int>(units.size());
+    pb_.num_chunks = static_cast<int>(chunks.size());
+    pb_.scam = scam_.p;
+    pb_.spt = spt_.p;
+    pb_.spx = spx_.p;
+    pb_.perm = nullptr;
+    pb_.pt_off = pt_off_.p;
+    pb_.units = units_.p;
+    pb_.cam_list = cam_list_.p;
+    pb_.cam_off = cam_off_.p;
+    pb_.chunks = chunks_.p;
+    pb_.cam_chunk_off = cam_chunk_off_.p;
+    pb_.cam_prob = cam_prob_.p;
+    pb_.pt_prob = pt_prob_.p;
+    cams_.alloc(9 * static_cast<size_t>(C));
+    tcams_.alloc(9 * static_cast<size_t>(C));
+    pts_.alloc(3 * static_cast<size_t>(P));
+    tpts_.alloc(3 * static_cast<size_t>(P));
+    W_.alloc(27 * static_cast<size_t>(O));
+    VB_.alloc(9 * static_cast<size_t>(P));
+    Vinv_.alloc(6 * static_cast<size_t>(P));
+    U_.alloc(81 * static_cast<size_t>(C));
+    bc_.alloc(9 * static_cast<size_t>(C));
+    rhs_.alloc(9 * static_cast<size_t>(C));
+    dc_.alloc(9 * static_cast<size_t>(C));
+    S_.alloc(static_cast<size_t>(soff));
+    D_.alloc(static_cast<size_t>(doff));
+    const size_t nu = std::max<size_t>(1, units.size());
+    unit_sse_.alloc(nu);
+    unit_den_.alloc(nu);
+    unit_anom_.alloc(nu);
+    gram_.alloc(55 * std::max<size_t>(1, chunks.size()));
+    lam_.alloc(count);
+    active_.alloc(count);
+    accept_.alloc(count);
+    flags_.alloc(count);
+    lin_status_.alloc(count);
+    trial_status_.alloc(count);
+    pb_.lam = lam_.p;
+    GBA_HIP(hipHostMalloc(&h_status_, 2 * sizeof(PassStatus) * count, hipHostMallocDefault));
+    GBA_HIP(hipHostMalloc(&h_lam_, sizeof(double) * count, hipHostMallocDefault));
+    GBA_HIP(hipHostMalloc(&h_mask_, sizeof(int) * count, hipHostMallocDefault));
+    // S: zero, then the identity on each problem's padding
+    launch_fill_double(st_, S_.p, static_cast<size_t>(soff), 0.0);
+    for (const BatchDesc& d : hdesc_)
+        if (d.npad > d.n) launch_identity_pad(st_, S_.p + d.soff, nullptr, d.npad, d.n, d.npad);
+    GBA_HIP(hipGetLastError());
+    GBA_HIP(hipStreamSynchronize(st_));
+}
+
+BatchSolver::~BatchSolver() {
+    if (h_status_) (void)hipHostFree(h_status_);
+    if (h_lam_) (void)hipHostFree(h_lam_);
+    if (h_mask_) (void)hipHostFree(h_mask_);
+    if (st_) (void)hipStreamDestroy(st_);
+}
+
+void BatchSolver::upload_mask(const std::vector<int>& h, int* d) {
+    // pinned staging, then a stream-ordered copy; the caller synchronises
+    // before the staging buffer is rewritten
+    std::memcpy(h_mask_, h.data(), sizeof(int) * B_);
+    GBA_HIP(hipMemcpyAsync(d, h_mask_, sizeof(int) * B_, hipMemcpyHostToDevice, st_));
+    GBA_HIP(hipStreamSynchronize(st_));
+}
+
+void BatchSolver::solve(const gba_lm_options& opts, gba_lm_result* res) {
+    const double t0 = now_s();
+    GBA_HIP(hipMemcpyAsync(cams_.p, init_cams_.data(), sizeof(double) * init_cams_.size(),
+                           hipMemcpyHostToDevice, st_));
+    GBA_HIP(hipMemcpyAsync(pts_.p, init_pts_.data(), sizeof(double) * init_pts_.size(),
+                           hipMemcpyHostToDevice, st_));
+    const int B = B_;
+    std::vector<double> sse(B, 0.0), lambda(B, opts.initial_lambda), nu(B, 2.0);
+    std::vector<unsigned long long> counter(B, 0);
+    std::vector<int> need_lin(B, 1), done(B, 0), inner(B, 0), converged(B, 0), failed(B, 0);
+    std::vector<int> lin_mask(B), trial_mask(B), acc_mask(B);
+    for (int b = 0; b < B; ++b) {
+        res[b] = gba_lm_result{};
+    }
+    rounds_ = 0;
+    DevProblem pl = pb_;
+    pl.active = active_.p;
+    for (;;) {
+        int nlin = 0, ntrial = 0;
+        for (int b = 0; b < B; ++b) {
+            lin_mask[b] = (!done[b] && need_lin[b]) ? 1 : 0;
+            nlin += lin_mask[b];
+        }
+        if (nlin > 0) {
+            upload_mask(lin_mask, active_.p);
+            launch_lin_points(st_, pl, cams_.p, pts_.p, W_.p, nullptr, VB_.p, unit_sse_.p, unit_anom_.p);
+            launch_lin_cameras(st_, pl, cams_.p, pts_.p, gram_.p);
+            launch_lin_reduce(st_, pl, gram_.p, unit_sse_.p, unit_anom_.p, U_.p, bc_.p, lin_status_.p);
+            launch_batch_status(st_, B, desc_.p, 0, unit_sse_.p, unit_den_.p, unit_anom_.p, dc_.p, U_.p,
+                                bc_.p, lam_.p, flags_.p, active_.p, lin_status_.p);
+            GBA_HIP(hipMemcpyAsync(h_status_, lin_status_.p, sizeof(PassStatus) * B, hipMemcpyDeviceToHost, st_));
+            GBA_HIP(hipGetLastError());
+            GBA_HIP(hipStreamSynchronize(st_));
+            for (int b = 0; b < B; ++b) {
+                if (!lin_mask[b]) continue;
+                counter[b] += h_status_[b].anomalies;
+                if (res[b].trials == 0 && res[b].iterations == 0) {
+                    sse[b] = h_status_[b].sse;  // initial linearisation: the starting cost
+                    res[b].initial_cost = sse[b] / 2.0;
+                }
+                need_lin[b] = 0;
+                // the reference's outer loop condition, evaluated after the
+                // linearisation that ends an accepted iteration
+                if (res[b].iterations >= opts.max_iterations) done[b] = 1;
+            }
+        }
+        for (int b = 0; b < B; ++b) {
+            trial_mask[b] = done[b] ? 0 : 1;
+            ntrial += trial_mask[b];
+        }
+        if (ntrial == 0) break;
+        ++rounds_;
+        for (int b = 0; b < B; ++b) h_lam_[b] = lambda[b];
+        GBA_HIP(hipMemcpyAsync(lam_.p, h_lam_, sizeof(double) * B, hipMemcpyHostToDevice, st_));
+        upload_mask(trial_mask, active_.p);
+        launch_fill_int(st_, reinterpret_cast<int*>(flags_.p), 4 * B, 0);
+        launch_damp_invert_v_batched(st_, pb_.P, VB_.p, pt_prob_.p, lam_.p, active_.p, Vinv_.p, flags_.p);
+        launch_schur(st_, sshape_, pl, stasks_.p, num_stasks_, spairs_.p, W_.p, nullptr, Vinv_.p, VB_.p,
+                     U_.p, bc_.p, 0.0, nullptr, S_.p, nullptr, nullptr, 0, rhs_.p, flags_.p);
+        launch_batch_chol_solve(st_, B, desc_.p, S_.p, D_.p, rhs_.p, dc_.p, flags_.p, active_.p);
+        launch_trial_cams(st_, pb_.C, cams_.p, dc_.p, tcams_.p);
+        launch_backsub(st_, pl, W_.p, nullptr, Vinv_.p, VB_.p, dc_.p, pts_.p, tcams_.p, tpts_.p, 0.0,
+                       unit_sse_.p, unit_den_.p, unit_anom_.p, flags_.p);
+        launch_batch_status(st_, B, desc_.p, 1, unit_sse_.p, unit_den_.p, unit_anom_.p, dc_.p, U_.p, bc_.p,
+                            lam_.p, flags_.p, active_.p, trial_status_.p);
+        PassStatus* hs = h_status_ + B;
+        GBA_HIP(hipMemcpyAsync(hs, trial_status_.p, sizeof(PassStatus) * B, hipMemcpyDeviceToHost, st_));
+        GBA_HIP(hipGetLastError());
+        GBA_HIP(hipStreamSynchronize(st_));
+        int naccept = 0;
+        for (int b = 0; b < B; ++b) {
+            acc_mask[b] = 0;
+            if (!trial_mask[b]) continue;
+            const PassStatus& t = hs[b];
+            res[b].trials++;
+            bool accept = false;
+            if (!t.flags.nonspd && !t.flags.chol_fail && t.denom > 0.0) {
+                const unsigned long long before = counter[b];
+                counter[b] += t.anomalies;
+                if (counter[b] == before && std::isfinite(t.sse)) {
+                    const double rho = (sse[b] - t.sse) / (0.5 * t.denom);
+                    if (rho > 0.0 && t.sse < sse[b]) {
+                        accept = true;
+                        const double rel = (sse[b] - t.sse) / std::max(sse[b], 1.0);
+                        sse[b] = t.sse;
+                        lambda[b] *= std::max(1.0 / 3.0, 1.0 - std::pow(2.0 * rho - 1.0, 3.0));
+                        res[b].iterations++;
+                        acc_mask[b] = 1;
+                        ++naccept;
+                        inner[b] = 0;
+                        nu[b] = 2.0;
+                        if (rel < opts.rel_decrease_tol) {
+                            converged[b] = 1;
+                            done[b] = 1;
+                        } else {
+                            need_lin[b] = 1;
+                        }
+                    }
+                }
+            }
+            if (!accept) {
+                lambda[b] *= nu[b];
+                nu[b] *= 2.0;
+                if (++inner[b] >= opts.max_inner_trials) {
+                    failed[b] = 1;
+                    done[b] = 1;
+                }
+            }
+        }
+        if (naccept > 0) {
+            upload_mask(acc_mask, accept_.p);
+            launch_batch_accept(st_, pb_.C, pb_.P, cam_prob_.p, pt_prob_.p, accept_.p, tcams_.p, tpts_.p,
+                                cams_.p, pts_.p);
+        }
+    }
+    GBA_HIP(hipStreamSynchronize(st_));
+    const double t1 = now_s();
+    for (int b = 0; b < B; ++b) {
+        res[b].final_cost = sse[b] / 2.0;
+        res[b].anomalies = counter[b];
+        res[b].termination = converged[b] ? GBA_CONVERGENCE : (failed[b] ? GBA_FAILURE : GBA_MAX_ITERATIONS);
+        res[b].solve_time_s = t1 - t0;
+        res[b].total_time_s = t1 - t0;
+    }
+}
+
+void BatchSolver::download_params(int i, double* cams, double* pts) {
+    if (i < 0 || i >= B_) throw Error("batch: problem index out of range");
+    const BatchDesc& d = hdesc_[i];
+    GBA_HIP(hipStreamSynchronize(st_));
+    GBA_HIP(hipMemcpy(cams, cams_.p + 9 * static_cast<size_t>(d.c0), sizeof(double) * 9 * (d.c1 - d.c0),
+                      hipMemcpyDeviceToHost));
+    GBA_HIP(hipMemcpy(pts, pts_.p + 3 * static_cast<size_t>(pt0_[i]),
+                      sizeof(double) * 3 * static_cast<size_t>(pt0_[i + 1] - pt0_[i]), hipMemcpyDeviceToHost));
+}
+
+}  // namespace gba_amd

@@ -1,0 +1,66 @@
+// batch.hpp -- batched LM solver: many independent problems advanced together.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "solver.hpp"
+
+namespace gba_amd {
+
+// Solves a batch of independent bundle-adjustment problems, each with the
+// reference LM semantics (src/lm.cu:455-714, the same options for all), in
+// plain FP64.  All problems live in one concatenated device layout (cameras,
+// points and sorted observations side by side); every stage of an LM round is
+// one launch for the whole batch, masked per problem, so the launch and
+// synchronisation cost of a round is shared by all problems.  A round
+// re-linearises the problems whose last trial was accepted and evaluates one
+// trial for every problem still running; the host then applies each
+// problem's accept / reject / lambda / nu rules.  One stream synchronisation
+// per round.
+class BatchSolver {
+public:
+    BatchSolver(const gba_problem_view* probs, int count);
+    ~BatchSolver();
+    BatchSolver(const BatchSolver&) = delete;
+    BatchSolver& operator=(const BatchSolver&) = delete;
+
+    // Solves every problem from its initial parameters; res[count].
+    void solve(const gba_lm_options& opts, gba_lm_result* res);
+    void download_params(int i, double* cams, double* pts);
+    int count() const { return B_; }
+    int rounds() const { return rounds_; }
+
+private:
+    void upload_mask(const std::vector<int>& h, int* d);
+
+    int B_ = 0;
+    int rounds_ = 0;
+    DevProblem pb_{};
+    hipStream_t st_ = nullptr;
+    std::vector<BatchDesc> hdesc_;
+    std::vector<long long> pt0_;  // first point of each problem, and the total
+    std::vector<double> init_cams_, init_pts_;
+    SchurShape sshape_;
+    int num_stasks_ = 0;
+
+    DevBuf<int> scam_, spt_, pt_off_, cam_list_, cam_off_, cam_chunk_off_, cam_prob_, pt_prob_;
+    DevBuf<double2> spx_;
+    DevBuf<int2> units_, spairs_;
+    DevBuf<int4> chunks_;
+    DevBuf<SchurTask> stasks_;
+    DevBuf<BatchDesc> desc_;
+    DevBuf<double> cams_, pts_, tcams_, tpts_;
+    DevBuf<double> W_, VB_, Vinv_, U_, bc_, rhs_, dc_, S_, D_;
+    DevBuf<double> unit_sse_, unit_den_, gram_;
+    DevBuf<unsigned> unit_anom_;
+    DevBuf<double> lam_;
+    DevBuf<int> active_, accept_;
+    DevBuf<TrialFlags> flags_;
+    DevBuf<PassStatus> lin_status_, trial_status_;
+    PassStatus* h_status_ = nullptr;  // 2 B entries: linearisation, trial
+    double* h_lam_ = nullptr;
+    int* h_mask_ = nullptr;           // staging for the per-problem switches
+};
+
+}  // namespace gba_amd

@@ -15,6 +15,7 @@
 #include "launchers.hpp"
 #include "snavely_model.hpp"
 #include "solver.hpp"
+#include "batch.hpp"
 
 namespace gba_amd {
 const char* last_error();
@@ -136,6 +137,61 @@ int gba_solver_stats(gba_solver* s, double* pair_count, int* n, int* n_padded) {
 }
 
 void gba_solver_destroy(gba_solver* s) { delete s; }
+
+struct gba_batch {
+    std::unique_ptr<BatchSolver> s;
+};
+
+int gba_batch_create(const gba_problem_view* probs, int count, gba_batch** out) {
+    return guarded([&] {
+        *out = nullptr;
+        if (!probs || count <= 0) return set_error("gba_batch_create: no problems");
+        auto h = std::make_unique<gba_batch>();
+        h->s = std::make_unique<BatchSolver>(probs, count);
+        *out = h.release();
+        return 0;
+    });
+}
+
+int gba_batch_solve(gba_batch* b, const gba_lm_options* opts, gba_lm_result* results, int* rounds) {
+    return guarded([&] {
+        gba_lm_options o;
+        gba_default_options(&o);
+        if (opts) o = *opts;
+        b->s->solve(o, results);
+        if (rounds) *rounds = b->s->rounds();
+        return 0;
+    });
+}
+
+int gba_batch_params(gba_batch* b, int index, double* cameras, double* points) {
+    return guarded([&] {
+        b->s->download_params(index, cameras, points);
+        return 0;
+    });
+}
+
+void gba_batch_destroy(gba_batch* b) { delete b; }
+
+int gba_solve_bal_batched(gba_problem_view* probs, int count, const gba_lm_options* opts,
+                          gba_lm_result* results) {
+    return guarded([&] {
+        const double t0 = now_s();
+        if (!probs || count <= 0) return set_error("gba_solve_bal_batched: no problems");
+        BatchSolver bs(probs, count);
+        gba_lm_options o;
+        gba_default_options(&o);
+        if (opts) o = *opts;
+        bs.solve(o, results);
+        for (int i = 0; i < count; ++i) bs.download_params(i, probs[i].cameras, probs[i].points);
+        const double t1 = now_s();
+        for (int i = 0; i < count; ++i) {
+            results[i].setup_time_s = (t1 - t0) - results[i].solve_time_s;
+            results[i].total_time_s = t1 - t0;
+        }
+        return 0;
+    });
+}
 
 int gba_linearize(const gba_problem_view* prob, int precision, double* U, double* V, double* W,
                   double* bc, double* bp, double* sse, unsigned long long* anomalies) {

@@ -36,14 +36,42 @@ struct DevProblem {
     const int* cam_off;
     const int4* chunks;
     const int* cam_chunk_off;
+    // Batched solves (independent problems side by side in one layout; all
+    // null for a single problem): problem of each camera and point, lambda
+    // per problem, and a per-problem switch selecting the problems a pass
+    // runs for.  TrialFlags and PassStatus are then per problem as well.
+    const int* cam_prob;
+    const int* pt_prob;
+    const double* lam;
+    const int* active;
 };
+
+__device__ __forceinline__ int prob_of_pt(const DevProblem& pb, int p) { return pb.pt_prob ? pb.pt_prob[p] : 0; }
+__device__ __forceinline__ int prob_of_cam(const DevProblem& pb, int c) { return pb.cam_prob ? pb.cam_prob[c] : 0; }
+__device__ __forceinline__ bool prob_off(const DevProblem& pb, int q) { return pb.active && !pb.active[q]; }
+__device__ __forceinline__ double lam_of(const DevProblem& pb, int q, double lambda) {
+    return pb.lam ? pb.lam[q] : lambda;
+}
 
 // Schur assembly task: blocks S[a][b], lo <= b < hi <= a + 1, from the
 // observation pairs [begin, end) of the pair list (int2: e, f in sorted edge
-// indices, same point, camera(e) = a, lo <= camera(f) < hi).
+// indices, same point, camera(e) = a, lo <= camera(f) < hi).  Block (a, b) is
+// written at rows 9 (a - a0), columns 9 (b - a0) of the problem's S.
 struct SchurTask {
-    int a, lo, hi, pad;
+    int a, lo, hi, a0;     // a0: first camera of the task's problem
     long long begin, end;
+    long long soff;        // offset of the problem's S (leading dimension ld)
+    int ld, pad;
+};
+
+// One problem of a batch: its camera and work-unit ranges in the
+// concatenated layout, and its dense S (npad x npad, row-major) and diagonal
+// tile store (2 npad/64 tiles) offsets.
+struct BatchDesc {
+    int c0, c1;
+    int u0, u1;
+    long long soff, doff;
+    int n, npad;
 };
 
 // Per-trial device flags, cleared by a fill kernel at the start of a trial.

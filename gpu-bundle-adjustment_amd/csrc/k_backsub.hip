@@ -65,11 +65,13 @@ __global__ __launch_bounds__(256) void k_backsub(DevProblem pb, const WT* __rest
                                                  double* __restrict__ unit_den,
                                                  unsigned* __restrict__ unit_anom,
                                                  const TrialFlags* flags) {
-    if (trial_skipped(flags)) return;
     const int lane = lane_id();
     const int u = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (u >= pb.num_units) return;
     const int2 rg = pb.units[u];
+    const int q = prob_of_pt(pb, pb.spt[rg.x]);
+    if (prob_off(pb, q) || trial_skipped(flags + q)) return;
+    lambda = lam_of(pb, q, lambda);
     double sse = 0.0, den = 0.0;
     unsigned anom = 0;
     if (rg.y - rg.x <= kWave) {

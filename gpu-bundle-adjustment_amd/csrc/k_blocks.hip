@@ -31,9 +31,17 @@ __global__ void k_fill_double(double* p, size_t n, double v) {
 // V' = V + lambda diag'(V) per point, inverted (lower-6 packed in and out).
 __global__ __launch_bounds__(256) void k_damp_invert_v(int P, const double* __restrict__ VB,
                                                        double lambda, double* __restrict__ Vinv6,
-                                                       TrialFlags* flags) {
+                                                       TrialFlags* flags, const int* __restrict__ pt_prob,
+                                                       const double* __restrict__ lam,
+                                                       const int* __restrict__ active) {
     const int p = blockIdx.x * 256 + threadIdx.x;
     if (p >= P) return;
+    int q = 0;
+    if (pt_prob) {  // batched problems: this point's problem
+        q = pt_prob[p];
+        if (active && !active[q]) return;
+        lambda = lam[q];
+    }
     const double* v = VB + 9 * static_cast<size_t>(p);
     const double a[6] = {v[0] + lambda * damp_diag(v[0]), v[1], v[2] + lambda * damp_diag(v[2]),
                          v[3], v[4], v[5] + lambda * damp_diag(v[5])};
@@ -43,7 +51,7 @@ __global__ __launch_bounds__(256) void k_damp_invert_v(int P, const double* __re
 #pragma unroll
         for (int i = 0; i < 6; ++i) o[i] = inv[i];
     } else {
-        atomicAdd(&flags->nonspd, 1);
+        atomicAdd(&flags[q].nonspd, 1);
     }
 }
 
@@ -149,7 +157,16 @@ void launch_damp_invert_v(hipStream_t st, int P, const double* VB, double lambda
                           TrialFlags* flags) {
     if (P > 0)
         hipLaunchKernelGGL(k_damp_invert_v, dim3((P + 255) / 256), dim3(256), 0, st, P, VB, lambda,
-                           Vinv6, flags);
+                           Vinv6, flags, static_cast<const int*>(nullptr),
+                           static_cast<const double*>(nullptr), static_cast<const int*>(nullptr));
+}
+
+void launch_damp_invert_v_batched(hipStream_t st, int P, const double* VB, const int* pt_prob,
+                                  const double* lam, const int* active, double* Vinv6,
+                                  TrialFlags* flags) {
+    if (P > 0)
+        hipLaunchKernelGGL(k_damp_invert_v, dim3((P + 255) / 256), dim3(256), 0, st, P, VB, 0.0, Vinv6,
+                           flags, pt_prob, lam, active);
 }
 
 void launch_batch_chol9(hipStream_t st, const double* A, const double* b, double* x, int* ok, int n) {

@@ -317,6 +317,193 @@ __global__ __launch_bounds__(256, 2) void k_chol_update(T* __restrict__ S, int l
             }
 }
 
+// ---- batched small systems: one workgroup per problem ----------------------
+
+// 256 threads: out[r] = sum_c M[r][c] v[c] for a 64x64 row-major tile (lane
+// row r, wavefront w sums columns 16w..16w+15; fixed-order combination).
+__device__ __forceinline__ void tile_matvec(const double* __restrict__ M, int ld,
+                                            const double* v, double* part, double* out,
+                                            bool accumulate) {
+    const int r = threadIdx.x & 63, g = threadIdx.x >> 6;
+    const double* row = M + static_cast<size_t>(r) * ld + 16 * g;
+    double s = 0.0;
+#pragma unroll
+    for (int c = 0; c < 16; ++c) s += row[c] * v[16 * g + c];
+    part[g * 64 + r] = s;
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        const double t = (part[r] + part[64 + r]) + (part[128 + r] + part[192 + r]);
+        out[r] = accumulate ? out[r] + t : t;
+    }
+    __syncthreads();
+}
+
+// Dense factor and solve of every active problem's S (FP64), one workgroup of
+// 256 threads per problem, tile by tile: diagonal tile factored and inverted
+// in LDS (tile_potrf_inv), TRSM of the tiles below as MFMA products with the
+// stored inverse (L_ik written below and L_ik^T above the diagonal), trailing
+// update from the transposed tiles, then forward and backward substitution
+// with the diagonal inverses.  dc[9 c0 + r] receives the solution; a bad
+// pivot sets flags[problem].chol_fail.  npad <= 1024.
+__global__ __launch_bounds__(256) void k_batch_chol_solve(const BatchDesc* __restrict__ desc,
+                                                          double* __restrict__ S,
+                                                          double* __restrict__ D,
+                                                          const double* __restrict__ rhs,
+                                                          double* __restrict__ dc, TrialFlags* flags,
+                                                          const int* __restrict__ active) {
+    typedef Acc4<double>::type acc_t;
+    __shared__ double sa[kTile][kTile + 2];
+    __shared__ double sb[kTile][kTile + 2];
+    __shared__ double vec[1024];
+    __shared__ double part[256];
+    __shared__ double tmp[kTile];
+    const int b = blockIdx.x;
+    if (!active[b] || flags[b].nonspd) return;
+    const BatchDesc d = desc[b];
+    double* A = S + d.soff;
+    const int ld = d.npad, nblk = d.npad / kTile;
+    double* Lkk = D + d.doff;
+    double* DinvT = Lkk + static_cast<size_t>(nblk) * kTile * kTile;
+    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
+    const int li = lane & 15, lk = lane >> 4;
+    for (int k = 0; k < nblk; ++k) {
+        const double* Akk = A + static_cast<size_t>(k * kTile) * ld + k * kTile;
+        for (int q = tid; q < kTile * kTile; q += 256) sa[q >> 6][q & 63] = Akk[static_cast<size_t>(q >> 6) * ld + (q & 63)];
+        __syncthreads();
+        const bool ok = tile_potrf_inv<double>(sa, sb);
+        store_diag_tile<double>(sa, sb, ok, k, Lkk, DinvT, flags + b);
+        __syncthreads();
+        if (!ok) return;
+        const double* inv = DinvT + static_cast<size_t>(k) * kTile * kTile;
+        for (int i = k + 1; i < nblk; ++i) {
+            double* Aik = A + static_cast<size_t>(i * kTile) * ld + k * kTile;
+            double* LTki = A + static_cast<size_t>(k * kTile) * ld + i * kTile;
+            for (int q = tid; q < kTile * kTile; q += 256) sa[q >> 6][q & 63] = Aik[static_cast<size_t>(q >> 6) * ld + (q & 63)];
+            __syncthreads();
+            acc_t acc[2][2];
+#pragma unroll
+            for (int r = 0; r < 2; ++r)
+#pragma unroll
+                for (int c = 0; c < 2; ++c) acc[r][c] = acc_t{0, 0, 0, 0};
+#pragma unroll 4
+            for (int kk = 0; kk < kTile; kk += 4) {
+                const double a0 = sa[wr + li][kk + lk], a1 = sa[wr + 16 + li][kk + lk];
+                const double b0 = inv[(kk + lk) * kTile + wc + li], b1 = inv[(kk + lk) * kTile + wc + 16 + li];
+                acc[0][0] = mfma16(a0, b0, acc[0][0]);
+                acc[0][1] = mfma16(a0, b1, acc[0][1]);
+                acc[1][0] = mfma16(a1, b0, acc[1][0]);
+                acc[1][1] = mfma16(a1, b1, acc[1][1]);
+            }
+            __syncthreads();
+#pragma unroll
+            for (int r = 0; r < 2; ++r)
+#pragma unroll
+                for (int c = 0; c < 2; ++c)
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) sa[wr + 16 * r + acc_row<double>(lk, q)][wc + 16 * c + li] = acc[r][c][q];
+            __syncthreads();
+            for (int q = tid; q < kTile * kTile; q += 256) {
+                const int r = q >> 6, c = q & 63;
+                Aik[static_cast<size_t>(r) * ld + c] = sa[r][c];
+                LTki[static_cast<size_t>(r) * ld + c] = sa[c][r];
+            }
+            __syncthreads();
+        }
+        for (int i = k + 1; i < nblk; ++i)
+            for (int j = k + 1; j <= i; ++j) {
+                const double* LTi = A + static_cast<size_t>(k * kTile) * ld + i * kTile;
+                const double* LTj = A + static_cast<size_t>(k * kTile) * ld + j * kTile;
+                acc_t acc[2][2];
+#pragma unroll
+                for (int r = 0; r < 2; ++r)
+#pragma unroll
+                    for (int c = 0; c < 2; ++c) acc[r][c] = acc_t{0, 0, 0, 0};
+#pragma unroll 4
+                for (int kk = 0; kk < kTile; kk += 4) {
+                    const size_t ro = static_cast<size_t>(kk + lk) * ld;
+                    acc[0][0] = mfma16(LTi[ro + wr + li], LTj[ro + wc + li], acc[0][0]);
+                    acc[0][1] = mfma16(LTi[ro + wr + li], LTj[ro + wc + 16 + li], acc[0][1]);
+                    acc[1][0] = mfma16(LTi[ro + wr + 16 + li], LTj[ro + wc + li], acc[1][0]);
+                    acc[1][1] = mfma16(LTi[ro + wr + 16 + li], LTj[ro + wc + 16 + li], acc[1][1]);
+                }
+                double* Aij = A + static_cast<size_t>(i * kTile) * ld + j * kTile;
+#pragma unroll
+                for (int r = 0; r < 2; ++r)
+#pragma unroll
+                    for (int c = 0; c < 2; ++c)
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) {
+                            const int row = wr + 16 * r + acc_row<double>(lk, q), col = wc + 16 * c + li;
+                            Aij[static_cast<size_t>(row) * ld + col] -= acc[r][c][q];
+                        }
+            }
+        __syncthreads();
+    }
+    // forward: y_i = Linv_ii (b_i - sum_{j<i} L_ij y_j); vec holds y
+    const double* bb = rhs + 9 * static_cast<size_t>(d.c0);
+    for (int i = 0; i < nblk; ++i) {
+        if (tid < kTile) {
+            const int r = i * kTile + tid;
+            tmp[tid] = r < d.n ? bb[r] : 0.0;
+        }
+        __syncthreads();
+        for (int j = 0; j < i; ++j) {
+            tile_matvec(A + static_cast<size_t>(i * kTile) * ld + j * kTile, ld, vec + j * kTile, part,
+                        vec + i * kTile, j > 0);
+        }
+        if (tid < kTile) {
+            const double acc = i > 0 ? vec[i * kTile + tid] : 0.0;
+            tmp[tid] -= acc;
+        }
+        __syncthreads();
+        // Linv s: lane r sums DinvT[c][r] s[c] over its quarter of c
+        {
+            const double* it = DinvT + static_cast<size_t>(i) * kTile * kTile;
+            const int r = tid & 63, g = tid >> 6;
+            double s = 0.0;
+#pragma unroll
+            for (int c = 0; c < 16; ++c) s += it[(16 * g + c) * kTile + r] * tmp[16 * g + c];
+            part[g * 64 + r] = s;
+            __syncthreads();
+            if (tid < kTile) vec[i * kTile + tid] = (part[tid] + part[64 + tid]) + (part[128 + tid] + part[192 + tid]);
+            __syncthreads();
+        }
+    }
+    // backward: x_i = Linv_ii^T (y_i - sum_{j>i} L_ji^T x_j), in place over vec
+    for (int i = nblk - 1; i >= 0; --i) {
+        if (tid < kTile) tmp[tid] = vec[i * kTile + tid];
+        __syncthreads();
+        if (i + 1 < nblk) {
+            // L_ji^T is stored in the upper tile (i, j)
+            double* accv = part;  // reuse: accumulate into a separate 64-vector
+            for (int j = i + 1; j < nblk; ++j) {
+                const int r = tid & 63, g = tid >> 6;
+                const double* row = A + static_cast<size_t>(i * kTile + r) * ld + j * kTile + 16 * g;
+                double s = 0.0;
+#pragma unroll
+                for (int c = 0; c < 16; ++c) s += row[c] * vec[j * kTile + 16 * g + c];
+                accv[g * 64 + r] = s;
+                __syncthreads();
+                if (tid < kTile) tmp[tid] -= (accv[tid] + accv[64 + tid]) + (accv[128 + tid] + accv[192 + tid]);
+                __syncthreads();
+            }
+        }
+        {
+            const double* it = DinvT + static_cast<size_t>(i) * kTile * kTile;
+            const int r = tid & 63, g = tid >> 6;
+            double s = 0.0;
+#pragma unroll
+            for (int c = 0; c < 16; ++c) s += it[r * kTile + 16 * g + c] * tmp[16 * g + c];
+            part[g * 64 + r] = s;
+            __syncthreads();
+            if (tid < kTile) vec[i * kTile + tid] = (part[tid] + part[64 + tid]) + (part[128 + tid] + part[192 + tid]);
+            __syncthreads();
+        }
+    }
+    for (int r = tid; r < d.n; r += 256) dc[9 * static_cast<size_t>(d.c0) + r] = vec[r];
+}
+
 // Forward solve L y = b.  Workgroup with ticket t owns tile row i = t.
 // DinvT[i] holds (L_ii^-1)^T.
 template <typename T>
@@ -581,6 +768,13 @@ void launch_cholesky(hipStream_t st, double* S64, float* S32, int ld, int nblk, 
                                D32, Dinv, flags);
         }
     }
+}
+
+void launch_batch_chol_solve(hipStream_t st, int count, const BatchDesc* desc, double* S, double* D,
+                             const double* rhs, double* dc, TrialFlags* flags, const int* active) {
+    if (count > 0)
+        hipLaunchKernelGGL(k_batch_chol_solve, dim3(count), dim3(256), 0, st, desc, S, D, rhs, dc, flags,
+                           active);
 }
 
 void launch_chol_solve(hipStream_t st, const double* L64, const float* L32, int ld, int nblk,

@@ -56,6 +56,7 @@ __global__ __launch_bounds__(256) void k_lin_points(DevProblem pb, const double*
     const int u = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (u >= pb.num_units) return;
     const int2 rg = pb.units[u];
+    if (prob_off(pb, prob_of_pt(pb, pb.spt[rg.x]))) return;
     double sse = 0.0;
     unsigned anom = 0;
     if (rg.y - rg.x <= kWave) {
@@ -131,6 +132,7 @@ __global__ __launch_bounds__(256) void k_lin_cameras(DevProblem pb, const double
     const int chunk = blockIdx.x * 4 + w;
     if (chunk >= pb.num_chunks) return;
     const int4 ch = pb.chunks[chunk];
+    if (prob_off(pb, prob_of_cam(pb, ch.x))) return;
     const double* cam = cams + 9 * ch.x;
     double* R = rows[w];
     v4d acc = {0.0, 0.0, 0.0, 0.0};
@@ -216,7 +218,7 @@ __global__ __launch_bounds__(256) void k_lin_reduce(DevProblem pb, const double*
         return;
     }
     const int c = blockIdx.x;
-    if (tid >= 55) return;
+    if (tid >= 55 || prob_off(pb, prob_of_cam(pb, c))) return;
     double s = 0.0;
     for (int k = pb.cam_chunk_off[c]; k < pb.cam_chunk_off[c + 1]; ++k)
         s += gram_part[55 * static_cast<size_t>(k) + tid];

@@ -37,9 +37,14 @@ __global__ __launch_bounds__(NT) void k_schur(
     extern __shared__ __align__(16) unsigned char smem[];
     AT* acc = reinterpret_cast<AT*>(smem);
     __shared__ double s_rhs[NT / 64][9];
-    if (flags->nonspd) return;
     const SchurTask task = tasks[blockIdx.x];
     const int a = task.a, lo = task.lo, hi = task.hi;
+    const int q = prob_of_cam(pb, a);
+    if (prob_off(pb, q) || flags[q].nonspd) return;
+    lambda = lam_of(pb, q, lambda);
+    Sfac += task.soff;
+    if (Slow) Slow += task.soff;
+    ld = task.ld;
     const bool own_rhs = (hi == a + 1);
     const int tid = threadIdx.x;
     const int nb = hi - lo;
@@ -115,8 +120,8 @@ __global__ __launch_bounds__(NT) void k_schur(
             v += U[81 * static_cast<size_t>(a) + 9 * i + j];
             if (i == j) v += lambda * damp_diag(U[81 * static_cast<size_t>(a) + 10 * i]);
         }
-        const int row = 9 * a + i, col = 9 * b + j;
-        if (dscale) v *= dscale[row] * dscale[col];
+        if (dscale) v *= dscale[9 * a + i] * dscale[9 * b + j];
+        const int row = 9 * (a - task.a0) + i, col = 9 * (b - task.a0) + j;
         Sfac[static_cast<size_t>(row) * ld + col] = static_cast<ST>(v);
         if (Slow) Slow[static_cast<size_t>(row) * ld + col] = static_cast<ST>(v);
     }

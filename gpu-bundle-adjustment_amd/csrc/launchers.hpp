@@ -25,6 +25,10 @@ void launch_fill_int(hipStream_t st, int* p, int n, int v);
 void launch_fill_double(hipStream_t st, double* p, size_t n, double v);
 void launch_damp_invert_v(hipStream_t st, int P, const double* VB, double lambda, double* Vinv6,
                           TrialFlags* flags);
+// Batched problems: lambda and the active switch per problem, flags[problem].
+void launch_damp_invert_v_batched(hipStream_t st, int P, const double* VB, const int* pt_prob,
+                                  const double* lam, const int* active, double* Vinv6,
+                                  TrialFlags* flags);
 void launch_batch_chol9(hipStream_t st, const double* A, const double* b, double* x, int* ok, int n);
 void launch_batch_inverse3(hipStream_t st, const double* A, double* Ainv, int* ok, int n);
 void launch_identity_pad(hipStream_t st, double* S64, float* S32, int ld, int n, int npad);
@@ -51,6 +55,10 @@ void launch_diag_scale(hipStream_t st, int n, int npad, const double* U, double 
 // inverses transposed; the off-diagonal tiles of L replace S's.
 void launch_cholesky(hipStream_t st, double* S64, float* S32, int ld, int nblk, double* D64,
                      float* D32, TrialFlags* flags);
+// Batched problems (FP64): factor and solve every active problem's S in place,
+// one workgroup per problem (desc[count]); dc = S^-1 rhs per problem.
+void launch_batch_chol_solve(hipStream_t st, int count, const BatchDesc* desc, double* S, double* D,
+                             const double* rhs, double* dc, TrialFlags* flags, const int* active);
 // Solves L L^T x = b (b, x length nblk*kTile; b and x may alias).  sync holds
 // 2*nblk + 2 ints of scratch (done flags and tickets).
 void launch_chol_solve(hipStream_t st, const double* L64, const float* L32, int ld, int nblk,
@@ -83,6 +91,17 @@ void launch_trinv(hipStream_t st, const float* L32, const float* D32, int ld, in
 size_t precond_work(int npad);
 void launch_precond_apply(hipStream_t st, const float* X32, int ld, int npad, const double* r,
                           double* z, double* work);
+
+// k_batch.hip
+// Per-problem status of a pass (trial = 0: linearisation SSE and anomalies;
+// trial = 1: trial SSE, gain denominator, anomalies, flags).
+void launch_batch_status(hipStream_t st, int count, const BatchDesc* desc, int trial,
+                         const double* unit_sse, const double* unit_den, const unsigned* unit_anom,
+                         const double* dc, const double* U, const double* bc, const double* lam,
+                         const TrialFlags* flags, const int* active, PassStatus* status);
+void launch_batch_accept(hipStream_t st, int nc, int np, const int* cam_prob, const int* pt_prob,
+                         const int* accept, const double* tcams, const double* tpts, double* cams,
+                         double* pts);
 
 // k_backsub.hip
 void launch_trial_cams(hipStream_t st, int C, const double* cams, const double* dc, double* tcams);

@@ -55,6 +55,8 @@ void parallel_for(int n, F f) {
     for (auto& x : th) x.join();
 }
 
+}  // namespace
+
 // Task and pair lists of the Schur assembly for chunks of cb camera columns:
 // task (a, c) holds the pairs (e, f) with camera(e) = a, f an observation of
 // the same point by a camera b <= a in [c cb, (c + 1) cb), grouped by e
@@ -99,14 +101,12 @@ void build_schur_tasks(int C, int cb, const std::vector<int>& cam_off,
     for (int a = 0; a < C; ++a)
         for (int c = 0; c <= a / cb; ++c) {
             const int t = toff[a] + c;
-            tasks[t] = SchurTask{a, c * cb, std::min((c + 1) * cb, a + 1), 0, begin[t], begin[t + 1]};
+            tasks[t] = SchurTask{a, c * cb, std::min((c + 1) * cb, a + 1), 0, begin[t], begin[t + 1], 0, 0, 0};
         }
     std::stable_sort(tasks.begin(), tasks.end(), [](const SchurTask& x, const SchurTask& y) {
         return x.end - x.begin > y.end - y.begin;
     });
 }
-
-}  // namespace
 
 size_t pcg_work(int npad) {
     return 5 * static_cast<size_t>(npad) + std::max(symv_work(npad), precond_work(npad)) +
@@ -168,6 +168,80 @@ int pcg_solve(hipStream_t st, const float* A, const float* X32, int ld, int npad
     return it;
 }
 
+void build_layout(const gba_problem_view& prob, HostLayout& L) {
+    const int C = prob.num_cameras, P = prob.num_points, O = prob.num_observations;
+    // sort key (point, camera, caller index): counting sort by camera, then a
+    // stable counting sort by point, so each point's observations are
+    // contiguous and ordered by camera
+    std::vector<int> by_cam(static_cast<size_t>(O));
+    {
+        std::vector<int> coff(static_cast<size_t>(C) + 1, 0);
+        for (int e = 0; e < O; ++e) coff[prob.obs_camera[e] + 1]++;
+        for (int c = 0; c < C; ++c) coff[c + 1] += coff[c];
+        for (int e = 0; e < O; ++e) by_cam[coff[prob.obs_camera[e]]++] = e;
+    }
+    L.pt_off.assign(static_cast<size_t>(P) + 1, 0);
+    for (int e = 0; e < O; ++e) L.pt_off[prob.obs_point[e] + 1]++;
+    for (int p = 0; p < P; ++p) L.pt_off[p + 1] += L.pt_off[p];
+    L.perm.resize(static_cast<size_t>(O));
+    L.scam.resize(static_cast<size_t>(O));
+    L.spt.resize(static_cast<size_t>(O));
+    L.spx.resize(static_cast<size_t>(O));
+    {
+        std::vector<int> fill(L.pt_off.begin(), L.pt_off.end() - 1);
+        for (int idx = 0; idx < O; ++idx) {
+            const int e = by_cam[idx];
+            const int k = fill[prob.obs_point[e]]++;
+            L.perm[k] = e;
+            L.scam[k] = prob.obs_camera[e];
+            L.spt[k] = prob.obs_point[e];
+            L.spx[k] = make_double2(prob.obs_pixel[2 * static_cast<size_t>(e)],
+                                    prob.obs_pixel[2 * static_cast<size_t>(e) + 1]);
+        }
+    }
+    L.units.clear();
+    L.pair_count = 0.0;
+    {
+        int b = -1, e_end = -1;
+        for (int p = 0; p < P; ++p) {
+            const int k0 = L.pt_off[p], k1 = L.pt_off[p + 1], k = k1 - k0;
+            if (k == 0) continue;
+            L.pair_count += 0.5 * static_cast<double>(k) * (k + 1);
+            if (k > kWave) {
+                if (b >= 0) L.units.push_back(make_int2(b, e_end));
+                L.units.push_back(make_int2(k0, k1));
+                b = -1;
+            } else if (b < 0) {
+                b = k0;
+                e_end = k1;
+            } else if (k1 - b > kWave) {
+                L.units.push_back(make_int2(b, e_end));
+                b = k0;
+                e_end = k1;
+            } else {
+                e_end = k1;
+            }
+        }
+        if (b >= 0) L.units.push_back(make_int2(b, e_end));
+    }
+    L.cam_off.assign(static_cast<size_t>(C) + 1, 0);
+    L.cam_list.resize(static_cast<size_t>(O));
+    for (int k = 0; k < O; ++k) L.cam_off[L.scam[k] + 1]++;
+    for (int c = 0; c < C; ++c) L.cam_off[c + 1] += L.cam_off[c];
+    {
+        std::vector<int> fill(L.cam_off.begin(), L.cam_off.end() - 1);
+        for (int k = 0; k < O; ++k) L.cam_list[fill[L.scam[k]]++] = k;
+    }
+    L.chunks.clear();
+    L.cam_chunk_off.assign(static_cast<size_t>(C) + 1, 0);
+    for (int c = 0; c < C; ++c) {
+        L.cam_chunk_off[c] = static_cast<int>(L.chunks.size());
+        for (int b = L.cam_off[c]; b < L.cam_off[c + 1]; b += kCamChunk)
+            L.chunks.push_back(make_int4(c, b, std::min(b + kCamChunk, L.cam_off[c + 1]), 0));
+    }
+    L.cam_chunk_off[C] = static_cast<int>(L.chunks.size());
+}
+
 Solver::Solver(const gba_problem_view& prob, const gba_solver_config& cfg) : cfg_(cfg) {
     mixed_ = cfg.precision == GBA_PRECISION_MIXED;
     const int C = prob.num_cameras, P = prob.num_points, O = prob.num_observations;
@@ -187,77 +261,25 @@ Solver::Solver(const gba_problem_view& prob, const gba_solver_config& cfg) : cfg
     init_pts_.assign(prob.points, prob.points + 3 * static_cast<size_t>(P));
 
     // ---- host-side layout: point-major sort, work units, camera index ----
-    // sort key (point, camera, caller index): counting sort by camera, then a
-    // stable counting sort by point, so each point's observations are
-    // contiguous and ordered by camera
-    std::vector<int> by_cam(static_cast<size_t>(O));
-    {
-        std::vector<int> coff(static_cast<size_t>(C) + 1, 0);
-        for (int e = 0; e < O; ++e) coff[prob.obs_camera[e] + 1]++;
-        for (int c = 0; c < C; ++c) coff[c + 1] += coff[c];
-        for (int e = 0; e < O; ++e) by_cam[coff[prob.obs_camera[e]]++] = e;
-    }
-    std::vector<int> pt_off(static_cast<size_t>(P) + 1, 0);
-    for (int e = 0; e < O; ++e) pt_off[prob.obs_point[e] + 1]++;
-    for (int p = 0; p < P; ++p) pt_off[p + 1] += pt_off[p];
-    std::vector<int> perm(static_cast<size_t>(O)), scam(static_cast<size_t>(O)), spt(static_cast<size_t>(O));
-    std::vector<double2> spx(static_cast<size_t>(O));
-    {
-        std::vector<int> fill(pt_off.begin(), pt_off.end() - 1);
-        for (int idx = 0; idx < O; ++idx) {
-            const int e = by_cam[idx];
-            const int k = fill[prob.obs_point[e]]++;
-            perm[k] = e;
-            scam[k] = prob.obs_camera[e];
-            spt[k] = prob.obs_point[e];
-            spx[k] = make_double2(prob.obs_pixel[2 * static_cast<size_t>(e)],
-                                  prob.obs_pixel[2 * static_cast<size_t>(e) + 1]);
-        }
-    }
-    std::vector<int2> units;
-    {
-        int b = -1, e_end = -1;
-        for (int p = 0; p < P; ++p) {
-            const int k0 = pt_off[p], k1 = pt_off[p + 1], k = k1 - k0;
-            if (k == 0) continue;
-            pair_count_ += 0.5 * static_cast<double>(k) * (k + 1);
-            if (k > kWave) {
-                if (b >= 0) units.push_back(make_int2(b, e_end));
-                units.push_back(make_int2(k0, k1));
-                b = -1;
-            } else if (b < 0) {
-                b = k0;
-                e_end = k1;
-            } else if (k1 - b > kWave) {
-                units.push_back(make_int2(b, e_end));
-                b = k0;
-                e_end = k1;
-            } else {
-                e_end = k1;
-            }
-        }
-        if (b >= 0) units.push_back(make_int2(b, e_end));
-    }
-    std::vector<int> cam_off(static_cast<size_t>(C) + 1, 0), cam_list(static_cast<size_t>(O));
-    for (int k = 0; k < O; ++k) cam_off[scam[k] + 1]++;
-    for (int c = 0; c < C; ++c) cam_off[c + 1] += cam_off[c];
-    {
-        std::vector<int> fill(cam_off.begin(), cam_off.end() - 1);
-        for (int k = 0; k < O; ++k) cam_list[fill[scam[k]]++] = k;
-    }
+    HostLayout lay;
+    build_layout(prob, lay);
+    pair_count_ = lay.pair_count;
+    std::vector<int>& perm = lay.perm;
+    std::vector<int>& scam = lay.scam;
+    std::vector<int>& spt = lay.spt;
+    std::vector<double2>& spx = lay.spx;
+    std::vector<int>& pt_off = lay.pt_off;
+    std::vector<int2>& units = lay.units;
+    std::vector<int>& cam_off = lay.cam_off;
+    std::vector<int>& cam_list = lay.cam_list;
     sshape_ = schur_shape(C, mixed_);
     std::vector<SchurTask> stasks;
     std::vector<int2> spairs;
     build_schur_tasks(C, sshape_.cb, cam_off, cam_list, pt_off, scam, spt, stasks, spairs);
     num_stasks_ = static_cast<int>(stasks.size());
-    std::vector<int4> chunks;
-    std::vector<int> cam_chunk_off(static_cast<size_t>(C) + 1, 0);
-    for (int c = 0; c < C; ++c) {
-        cam_chunk_off[c] = static_cast<int>(chunks.size());
-        for (int b = cam_off[c]; b < cam_off[c + 1]; b += kCamChunk)
-            chunks.push_back(make_int4(c, b, std::min(b + kCamChunk, cam_off[c + 1]), 0));
-    }
-    cam_chunk_off[C] = static_cast<int>(chunks.size());
+    for (SchurTask& t : stasks) t.ld = pb_.npad;
+    std::vector<int4>& chunks = lay.chunks;
+    std::vector<int>& cam_chunk_off = lay.cam_chunk_off;
     pb_.num_units = static_cast<int>(units.size());
     pb_.num_chunks = static_cast<int>(chunks.size());
 

@@ -35,6 +35,23 @@ struct DevBuf {
     ~DevBuf() { release(); }
 };
 
+// Host-side layout of one problem: observations sorted by (point, camera),
+// point-pass work units, camera index and camera-pass chunks.
+struct HostLayout {
+    std::vector<int> perm, scam, spt, pt_off, cam_off, cam_list, cam_chunk_off;
+    std::vector<double2> spx;
+    std::vector<int2> units;
+    std::vector<int4> chunks;
+    double pair_count = 0.0;  // sum_p k_p (k_p + 1) / 2
+};
+void build_layout(const gba_problem_view& prob, HostLayout& L);
+// Schur task and pair lists for chunks of cb camera columns (task ld, soff
+// and a0 are left 0 for the caller to fill).
+void build_schur_tasks(int C, int cb, const std::vector<int>& cam_off,
+                       const std::vector<int>& cam_list, const std::vector<int>& pt_off,
+                       const std::vector<int>& scam, const std::vector<int>& spt,
+                       std::vector<SchurTask>& tasks, std::vector<int2>& pairs);
+
 // FP64 preconditioned conjugate gradients on A x = b, A symmetric given by the
 // lower triangle of an FP32 matrix, preconditioned by (L L^T)^-1 applied
 // through the explicit FP32 inverse X = L^-1 (launch_trinv layout).  Vectors

@@ -132,6 +132,24 @@ int gba_solver_time_kernel(gba_solver* s, int kernel, double lambda, int reps, d
 int gba_solver_stats(gba_solver* s, double* pair_count, int* n, int* n_padded);
 void gba_solver_destroy(gba_solver* s);
 
+/* ---- batched solve (no reference counterpart) --------------------------- */
+
+/* Many independent problems solved together (at most 113 cameras each), each
+ * with the LM semantics of gba_solve_bal and the same options, plain FP64;
+ * every stage of an LM round is one launch for the whole batch.  For callers
+ * that solve many small reconstructions (the reference solves them one call
+ * at a time).  Handle form: create uploads and sorts once; solve restarts from
+ * the uploaded initial parameters; results[count]; *rounds (optional)
+ * receives the number of LM rounds the batch took. */
+typedef struct gba_batch gba_batch;
+int gba_batch_create(const gba_problem_view* probs, int count, gba_batch** out);
+int gba_batch_solve(gba_batch* b, const gba_lm_options* opts, gba_lm_result* results, int* rounds);
+int gba_batch_params(gba_batch* b, int index, double* cameras, double* points);
+void gba_batch_destroy(gba_batch* b);
+/* One-shot form: writes each problem's last accepted parameters back. */
+int gba_solve_bal_batched(gba_problem_view* probs, int count, const gba_lm_options* opts,
+                          gba_lm_result* results);
+
 /* ---- pieces of the path, host arrays in / out (parity tests) ----------- */
 
 /* Linearisation at the given parameters (reference src/lm.cu:205-252 and

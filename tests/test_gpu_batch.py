@@ -1,0 +1,76 @@
+"""Batched solver (many independent problems per launch) against each
+problem's own sequential FP64 solve through the C ABI.  The batch factors each
+S with a different kernel (one workgroup per problem), so the steps agree to
+rounding; the tolerances are the LM convergence tolerance on the final cost
+(1e-6 relative) and at most one iteration of difference."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def small(gba, C, P, O, seed):
+    s = gba.synth_spec(1)
+    s.num_cameras, s.num_points, s.num_observations = C, P, O
+    s.kmin, s.kmax = 2, min(C, 16)
+    return gba.generate(spec=s, seed=seed)
+
+
+def check_pair(rb, rs):
+    assert rb.termination == rs.termination
+    assert abs(rb.initial_cost - rs.initial_cost) <= 1e-12 * rs.initial_cost
+    assert abs(rb.final_cost - rs.final_cost) <= 1e-6 * rs.final_cost
+    assert abs(rb.iterations - rs.iterations) <= 1
+    assert rb.anomalies == rs.anomalies
+
+
+def test_batch_matches_sequential_config5(gba):
+    probs = [gba.generate(5, seed) for seed in range(1, 7)]
+    seq = [gba.Solver(p).solve() for p in probs]
+    b = gba.Batch(probs)
+    res = b.solve()
+    assert b.rounds >= max(r.trials for r in res)
+    for rb, rs in zip(res, seq):
+        check_pair(rb, rs)
+        assert rb.termination == "CONVERGENCE"
+    # the same handle solves again from the uploaded initial parameters
+    res2 = b.solve()
+    for r1, r2 in zip(res, res2):
+        assert abs(r1.final_cost - r2.final_cost) <= 1e-9 * r1.final_cost
+
+
+def test_batch_mixed_sizes_and_params(gba):
+    probs = [small(gba, 8, 800, 3600, 11), small(gba, 64, 3000, 13500, 12), small(gba, 23, 1500, 6500, 13)]
+    seq_solvers = [gba.Solver(p) for p in probs]
+    seq = [s.solve() for s in seq_solvers]
+    b = gba.Batch(probs)
+    res = b.solve()
+    for i, (rb, rs) in enumerate(zip(res, seq)):
+        check_pair(rb, rs)
+        cams_b, pts_b = b.params(i)
+        cams_s, pts_s = seq_solvers[i].params()
+        assert np.max(np.abs(cams_b - cams_s)) <= 1e-3 * max(1.0, np.max(np.abs(cams_s)))
+        assert np.max(np.abs(pts_b - pts_s)) <= 1e-3 * max(1.0, np.max(np.abs(pts_s)))
+
+
+def test_batch_max_iterations_and_anomalies(gba):
+    p_nan = gba.generate(2, 1)
+    p_nan.obs_pixel[10, 1] = np.nan
+    probs = [p_nan, gba.generate(5, 3)]
+    opts = gba.LMOptions(max_iterations=3)
+    seq = [gba.solve_bal_gpu(p.copy(), opts) for p in probs]
+    res = gba.Batch(probs).solve(opts)
+    for rb, rs in zip(res, seq):
+        check_pair(rb, rs)
+    assert res[0].anomalies > 0
+    assert res[1].termination == "MAX_ITERATIONS" and res[1].iterations == 3
+
+
+def test_batch_of_one_and_one_shot(gba):
+    p = gba.generate(5, 9)
+    rs = gba.Solver(p).solve()
+    q = p.copy()
+    (rb,) = gba.solve_bal_batched([q])
+    check_pair(rb, rs)
+    # the one-shot form writes the accepted parameters back
+    assert not np.allclose(q.cameras, p.cameras)
