@@ -55,7 +55,7 @@ BatchSolver::BatchSolver(const gba_problem_view* probs, int count) : B_(count) {
         build_layout(pr, L);
         std::vector<SchurTask> t;
         std::vector<int2> pr_pairs;
-        build_schur_tasks(C, sshape_.cb, L.cam_off, L.cam_list, L.pt_off, L.scam, L.spt, t, pr_pairs);
+        build_schur_tasks(C, sshape_.cb, L.cam_off, L.cam_list, L.pt_off, L.scam, L.spt, t, pr_pairs, false);
         BatchDesc d;
         d.c0 = static_cast<int>(C0);
         d.c1 = static_cast<int>(C0 + C);
@@ -255,8 +255,10 @@ void BatchSolver::solve(const gba_lm_options& opts, gba_lm_result* res) {
         upload_mask(trial_mask, active_.p);
         launch_fill_int(st_, reinterpret_cast<int*>(flags_.p), 4 * B, 0);
         launch_damp_invert_v_batched(st_, pb_.P, VB_.p, pt_prob_.p, lam_.p, active_.p, Vinv_.p, flags_.p);
-        launch_schur(st_, sshape_, pl, stasks_.p, num_stasks_, spairs_.p, W_.p, nullptr, Vinv_.p, VB_.p,
-                     U_.p, bc_.p, 0.0, nullptr, S_.p, nullptr, nullptr, 0, rhs_.p, flags_.p);
+        launch_schur_diag(st_, pl, desc_.p, W_.p, Vinv_.p, VB_.p, U_.p, bc_.p, 0.0, S_.p, 0, rhs_.p, flags_.p);
+        if (num_stasks_ > 0)
+            launch_schur(st_, sshape_, pl, stasks_.p, num_stasks_, spairs_.p, W_.p, nullptr, Vinv_.p, VB_.p,
+                         U_.p, bc_.p, 0.0, nullptr, S_.p, nullptr, nullptr, 0, rhs_.p, flags_.p);
         launch_batch_chol_solve(st_, B, desc_.p, S_.p, D_.p, rhs_.p, dc_.p, flags_.p, active_.p);
         launch_trial_cams(st_, pb_.C, cams_.p, dc_.p, tcams_.p);
         launch_backsub(st_, pl, W_.p, nullptr, Vinv_.p, VB_.p, dc_.p, pts_.p, tcams_.p, tpts_.p, 0.0,

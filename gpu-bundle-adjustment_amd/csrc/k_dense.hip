@@ -345,7 +345,7 @@ __device__ __forceinline__ void tile_matvec(const double* __restrict__ M, int ld
 // update from the transposed tiles, then forward and backward substitution
 // with the diagonal inverses.  dc[9 c0 + r] receives the solution; a bad
 // pivot sets flags[problem].chol_fail.  npad <= 1024.
-__global__ __launch_bounds__(256) void k_batch_chol_solve(const BatchDesc* __restrict__ desc,
+__global__ __launch_bounds__(256, 2) void k_batch_chol_solve(const BatchDesc* __restrict__ desc,
                                                           double* __restrict__ S,
                                                           double* __restrict__ D,
                                                           const double* __restrict__ rhs,
@@ -365,7 +365,6 @@ __global__ __launch_bounds__(256) void k_batch_chol_solve(const BatchDesc* __res
     double* Lkk = D + d.doff;
     double* DinvT = Lkk + static_cast<size_t>(nblk) * kTile * kTile;
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
-    const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
     const int li = lane & 15, lk = lane >> 4;
     for (int k = 0; k < nblk; ++k) {
         const double* Akk = A + static_cast<size_t>(k * kTile) * ld + k * kTile;
@@ -376,68 +375,84 @@ __global__ __launch_bounds__(256) void k_batch_chol_solve(const BatchDesc* __res
         __syncthreads();
         if (!ok) return;
         const double* inv = DinvT + static_cast<size_t>(k) * kTile * kTile;
-        for (int i = k + 1; i < nblk; ++i) {
+        // TRSM: wavefront w takes the half tiles (32 rows) h = w, w+4, ... of the
+        // tiles below: L_ik = A_ik Linv^T (2x4 MFMA blocks), written below the
+        // diagonal and, transposed, into the upper tile (k, i)
+        for (int h = w; h < 2 * (nblk - k - 1); h += 4) {
+            const int i = k + 1 + (h >> 1), r0 = 32 * (h & 1);
             double* Aik = A + static_cast<size_t>(i * kTile) * ld + k * kTile;
             double* LTki = A + static_cast<size_t>(k * kTile) * ld + i * kTile;
-            for (int q = tid; q < kTile * kTile; q += 256) sa[q >> 6][q & 63] = Aik[static_cast<size_t>(q >> 6) * ld + (q & 63)];
-            __syncthreads();
-            acc_t acc[2][2];
+            acc_t acc[2][4];
 #pragma unroll
             for (int r = 0; r < 2; ++r)
 #pragma unroll
-                for (int c = 0; c < 2; ++c) acc[r][c] = acc_t{0, 0, 0, 0};
-#pragma unroll 4
+                for (int c = 0; c < 4; ++c) acc[r][c] = acc_t{0, 0, 0, 0};
+#pragma unroll 2
             for (int kk = 0; kk < kTile; kk += 4) {
-                const double a0 = sa[wr + li][kk + lk], a1 = sa[wr + 16 + li][kk + lk];
-                const double b0 = inv[(kk + lk) * kTile + wc + li], b1 = inv[(kk + lk) * kTile + wc + 16 + li];
-                acc[0][0] = mfma16(a0, b0, acc[0][0]);
-                acc[0][1] = mfma16(a0, b1, acc[0][1]);
-                acc[1][0] = mfma16(a1, b0, acc[1][0]);
-                acc[1][1] = mfma16(a1, b1, acc[1][1]);
-            }
-            __syncthreads();
+                double av[2], bv[4];
 #pragma unroll
-            for (int r = 0; r < 2; ++r)
+                for (int r = 0; r < 2; ++r) av[r] = Aik[static_cast<size_t>(r0 + 16 * r + li) * ld + kk + lk];
 #pragma unroll
-                for (int c = 0; c < 2; ++c)
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) sa[wr + 16 * r + acc_row<double>(lk, q)][wc + 16 * c + li] = acc[r][c][q];
-            __syncthreads();
-            for (int q = tid; q < kTile * kTile; q += 256) {
-                const int r = q >> 6, c = q & 63;
-                Aik[static_cast<size_t>(r) * ld + c] = sa[r][c];
-                LTki[static_cast<size_t>(r) * ld + c] = sa[c][r];
-            }
-            __syncthreads();
-        }
-        for (int i = k + 1; i < nblk; ++i)
-            for (int j = k + 1; j <= i; ++j) {
-                const double* LTi = A + static_cast<size_t>(k * kTile) * ld + i * kTile;
-                const double* LTj = A + static_cast<size_t>(k * kTile) * ld + j * kTile;
-                acc_t acc[2][2];
+                for (int c = 0; c < 4; ++c) bv[c] = inv[(kk + lk) * kTile + 16 * c + li];
 #pragma unroll
                 for (int r = 0; r < 2; ++r)
 #pragma unroll
-                    for (int c = 0; c < 2; ++c) acc[r][c] = acc_t{0, 0, 0, 0};
-#pragma unroll 4
+                    for (int c = 0; c < 4; ++c) acc[r][c] = mfma16(av[r], bv[c], acc[r][c]);
+            }
+#pragma unroll
+            for (int r = 0; r < 2; ++r)
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const int row = r0 + 16 * r + acc_row<double>(lk, q), col = 16 * c + li;
+                        Aik[static_cast<size_t>(row) * ld + col] = acc[r][c][q];
+                        LTki[static_cast<size_t>(col) * ld + row] = acc[r][c][q];
+                    }
+        }
+        __syncthreads();
+        // trailing update: wavefront w takes every fourth half tile of the
+        // tiles (i, j), k < j <= i; operands from the transposed tiles
+        {
+            const int m = nblk - k - 1;
+            for (int h = w; h < m * (m + 1); h += 4) {
+                const int t = h >> 1, r0 = 32 * (h & 1);
+                int ii = 0;
+                while ((ii + 1) * (ii + 2) / 2 <= t) ++ii;
+                const int jj = t - ii * (ii + 1) / 2;
+                const int i = k + 1 + ii, j = k + 1 + jj;
+                const double* LTi = A + static_cast<size_t>(k * kTile) * ld + i * kTile;
+                const double* LTj = A + static_cast<size_t>(k * kTile) * ld + j * kTile;
+                acc_t acc[2][4];
+#pragma unroll
+                for (int r = 0; r < 2; ++r)
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) acc[r][c] = acc_t{0, 0, 0, 0};
+#pragma unroll 2
                 for (int kk = 0; kk < kTile; kk += 4) {
                     const size_t ro = static_cast<size_t>(kk + lk) * ld;
-                    acc[0][0] = mfma16(LTi[ro + wr + li], LTj[ro + wc + li], acc[0][0]);
-                    acc[0][1] = mfma16(LTi[ro + wr + li], LTj[ro + wc + 16 + li], acc[0][1]);
-                    acc[1][0] = mfma16(LTi[ro + wr + 16 + li], LTj[ro + wc + li], acc[1][0]);
-                    acc[1][1] = mfma16(LTi[ro + wr + 16 + li], LTj[ro + wc + 16 + li], acc[1][1]);
+                    double av[2], bv[4];
+#pragma unroll
+                    for (int r = 0; r < 2; ++r) av[r] = LTi[ro + r0 + 16 * r + li];
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) bv[c] = LTj[ro + 16 * c + li];
+#pragma unroll
+                    for (int r = 0; r < 2; ++r)
+#pragma unroll
+                        for (int c = 0; c < 4; ++c) acc[r][c] = mfma16(av[r], bv[c], acc[r][c]);
                 }
                 double* Aij = A + static_cast<size_t>(i * kTile) * ld + j * kTile;
 #pragma unroll
                 for (int r = 0; r < 2; ++r)
 #pragma unroll
-                    for (int c = 0; c < 2; ++c)
+                    for (int c = 0; c < 4; ++c)
 #pragma unroll
                         for (int q = 0; q < 4; ++q) {
-                            const int row = wr + 16 * r + acc_row<double>(lk, q), col = wc + 16 * c + li;
+                            const int row = r0 + 16 * r + acc_row<double>(lk, q), col = 16 * c + li;
                             Aij[static_cast<size_t>(row) * ld + col] -= acc[r][c][q];
                         }
             }
+        }
         __syncthreads();
     }
     // forward: y_i = Linv_ii (b_i - sum_{j<i} L_ij y_j); vec holds y

@@ -184,15 +184,16 @@ __global__ __launch_bounds__(256) void k_lin_cameras(DevProblem pb, const double
     }
 }
 
-// Per camera: fixed-order sum of its chunk partials -> U (full 9x9), bc.
-// Workgroup 0 additionally reduces the unit SSE / anomaly partials.
+// Per camera (one wavefront, lanes 0..54): fixed-order sum of its chunk
+// partials -> U (full 9x9), bc.  The last workgroup reduces the unit SSE /
+// anomaly partials.
 __global__ __launch_bounds__(256) void k_lin_reduce(DevProblem pb, const double* __restrict__ gram_part,
                                                     const double* __restrict__ unit_sse,
                                                     const unsigned* __restrict__ unit_anom,
                                                     double* __restrict__ U, double* __restrict__ bc,
                                                     PassStatus* status) {
     const int tid = threadIdx.x;
-    if (blockIdx.x == static_cast<unsigned>(pb.C)) {
+    if (blockIdx.x == static_cast<unsigned>((pb.C + 3) / 4)) {
         __shared__ double s_sse[256];
         __shared__ unsigned s_an[256];
         double s = 0.0;
@@ -217,15 +218,16 @@ __global__ __launch_bounds__(256) void k_lin_reduce(DevProblem pb, const double*
         }
         return;
     }
-    const int c = blockIdx.x;
-    if (tid >= 55 || prob_off(pb, prob_of_cam(pb, c))) return;
+    const int c = blockIdx.x * 4 + (tid >> 6);
+    const int lane = tid & 63;
+    if (c >= pb.C || lane >= 55 || prob_off(pb, prob_of_cam(pb, c))) return;
     double s = 0.0;
     for (int k = pb.cam_chunk_off[c]; k < pb.cam_chunk_off[c + 1]; ++k)
-        s += gram_part[55 * static_cast<size_t>(k) + tid];
+        s += gram_part[55 * static_cast<size_t>(k) + lane];
     // packed index tid -> (i, j), j <= i
     int i = 0;
-    while ((i + 1) * (i + 2) / 2 <= tid) ++i;
-    const int j = tid - i * (i + 1) / 2;
+    while ((i + 1) * (i + 2) / 2 <= lane) ++i;
+    const int j = lane - i * (i + 1) / 2;
     if (i < 9) {
         U[81 * static_cast<size_t>(c) + 9 * i + j] = s;
         U[81 * static_cast<size_t>(c) + 9 * j + i] = s;
@@ -320,7 +322,7 @@ void launch_lin_cameras(hipStream_t st, const DevProblem& pb, const double* cams
 void launch_lin_reduce(hipStream_t st, const DevProblem& pb, const double* gram_part,
                        const double* unit_sse, const unsigned* unit_anom, double* U, double* bc,
                        PassStatus* status) {
-    hipLaunchKernelGGL(k_lin_reduce, dim3(pb.C + 1), dim3(256), 0, st, pb, gram_part, unit_sse,
+    hipLaunchKernelGGL(k_lin_reduce, dim3((pb.C + 3) / 4 + 1), dim3(256), 0, st, pb, gram_part, unit_sse,
                        unit_anom, U, bc, status);
 }
 

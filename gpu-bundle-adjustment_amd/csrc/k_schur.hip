@@ -127,6 +127,87 @@ __global__ __launch_bounds__(NT) void k_schur(
     }
 }
 
+// Diagonal block a and rhs segment a, one workgroup per camera (used when the
+// pair list leaves out the pairs e == f, as the batched solver does: for
+// problems with few observations per point those pairs are a large share of
+// the list and all land on one LDS block):
+//   S_aa = U_aa + lambda diag'(U_aa) - sum_{e in a} W_e V'^-1 W_e^T,
+//   rhs_a = bc_a - sum_{e in a} W_e V'^-1 bp.
+// Each thread sums its strided share of camera a's observations in FP64
+// registers (45 lower entries + 9 rhs entries), then a fixed-order reduction
+// (wavefront shuffles, then the four wavefronts in order): no atomics.
+// desc (batched problems) gives the problem's S offset, order and first
+// camera; without it S has leading dimension ld and starts at camera 0.
+template <typename WT>
+__global__ __launch_bounds__(256) void k_schur_diag(
+    DevProblem pb, const BatchDesc* __restrict__ desc, const WT* __restrict__ W,
+    const double* __restrict__ Vinv6, const double* __restrict__ VB, const double* __restrict__ U,
+    const double* __restrict__ bc, double lambda, double* __restrict__ S, int ld,
+    double* __restrict__ rhs, const TrialFlags* flags) {
+    __shared__ double s_red[4][54];
+    const int a = blockIdx.x, tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    const int q = prob_of_cam(pb, a);
+    if (prob_off(pb, q) || flags[q].nonspd) return;
+    lambda = lam_of(pb, q, lambda);
+    int a0 = 0;
+    if (desc) {
+        S += desc[q].soff;
+        ld = desc[q].npad;
+        a0 = desc[q].c0;
+    }
+    double acc[45], racc[9];
+#pragma unroll
+    for (int i = 0; i < 45; ++i) acc[i] = 0.0;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) racc[i] = 0.0;
+    for (int idx = pb.cam_off[a] + tid; idx < pb.cam_off[a + 1]; idx += 256) {
+        const int e = pb.cam_list[idx];
+        const int p = pb.spt[e];
+        const double* vi = Vinv6 + 6 * static_cast<size_t>(p);
+        const double v00 = vi[0], v10 = vi[1], v11 = vi[2], v20 = vi[3], v21 = vi[4], v22 = vi[5];
+        const double* bpp = VB + 9 * static_cast<size_t>(p) + 6;
+        const double vb0 = v00 * bpp[0] + v10 * bpp[1] + v20 * bpp[2];
+        const double vb1 = v10 * bpp[0] + v11 * bpp[1] + v21 * bpp[2];
+        const double vb2 = v20 * bpp[0] + v21 * bpp[1] + v22 * bpp[2];
+        const WT* We = W + 27 * static_cast<size_t>(e);
+        double wv[27];
+#pragma unroll
+        for (int k = 0; k < 27; ++k) wv[k] = static_cast<double>(We[k]);
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            const double t0 = wv[3 * i] * v00 + wv[3 * i + 1] * v10 + wv[3 * i + 2] * v20;
+            const double t1 = wv[3 * i] * v10 + wv[3 * i + 1] * v11 + wv[3 * i + 2] * v21;
+            const double t2 = wv[3 * i] * v20 + wv[3 * i + 1] * v21 + wv[3 * i + 2] * v22;
+#pragma unroll
+            for (int j = 0; j <= i; ++j)
+                acc[i * (i + 1) / 2 + j] += t0 * wv[3 * j] + t1 * wv[3 * j + 1] + t2 * wv[3 * j + 2];
+            racc[i] += wv[3 * i] * vb0 + wv[3 * i + 1] * vb1 + wv[3 * i + 2] * vb2;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 54; ++i) {
+        const double t = wave_sum(i < 45 ? acc[i] : racc[i - 45]);
+        if (lane == 0) s_red[w][i] = t;
+    }
+    __syncthreads();
+    if (tid < 54) {
+        const double sum = ((s_red[0][tid] + s_red[1][tid]) + s_red[2][tid]) + s_red[3][tid];
+        if (tid >= 45) {
+            const int row = 9 * a + (tid - 45);
+            rhs[row] = bc[row] - sum;
+        } else {
+            int i = 0;
+            while ((i + 1) * (i + 2) / 2 <= tid) ++i;
+            const int j = tid - i * (i + 1) / 2;
+            double v = U[81 * static_cast<size_t>(a) + 9 * i + j] - sum;
+            if (i == j) v += lambda * damp_diag(U[81 * static_cast<size_t>(a) + 10 * i]);
+            const int row = 9 * (a - a0) + i, col = 9 * (a - a0) + j;
+            S[static_cast<size_t>(row) * ld + col] = v;
+            S[static_cast<size_t>(col) * ld + row] = v;
+        }
+    }
+}
+
 // D_i = (U_ii + lambda diag'(U_ii))^-1/2 for i < n, 1 for the padding.
 __global__ void k_diag_scale(int n, int npad, const double* __restrict__ U, double lambda,
                              double* __restrict__ d) {
@@ -201,6 +282,14 @@ void launch_schur(hipStream_t st, const SchurShape& sh, const DevProblem& pb,
                                                     bc, lambda, dscale, S64, static_cast<double*>(nullptr),
                                                     ld, rhs, flags);
     }
+}
+
+void launch_schur_diag(hipStream_t st, const DevProblem& pb, const BatchDesc* desc, const double* W64,
+                       const double* Vinv6, const double* VB, const double* U, const double* bc,
+                       double lambda, double* S64, int ld, double* rhs, const TrialFlags* flags) {
+    if (pb.C > 0)
+        hipLaunchKernelGGL(k_schur_diag<double>, dim3(pb.C), dim3(256), 0, st, pb, desc, W64, Vinv6, VB, U,
+                           bc, lambda, S64, ld, rhs, flags);
 }
 
 void launch_diag_scale(hipStream_t st, int n, int npad, const double* U, double lambda,

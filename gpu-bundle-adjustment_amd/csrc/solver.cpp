@@ -66,9 +66,11 @@ void parallel_for(int n, F f) {
 void build_schur_tasks(int C, int cb, const std::vector<int>& cam_off,
                        const std::vector<int>& cam_list, const std::vector<int>& pt_off,
                        const std::vector<int>& scam, const std::vector<int>& spt,
-                       std::vector<SchurTask>& tasks, std::vector<int2>& pairs) {
+                       std::vector<SchurTask>& tasks, std::vector<int2>& pairs, bool with_diag) {
+    // without the diagonal: blocks b < a only (b <= bmax = a - 1)
+    const int dg = with_diag ? 1 : 0;
     std::vector<int> toff(static_cast<size_t>(C) + 1, 0);
-    for (int a = 0; a < C; ++a) toff[a + 1] = toff[a] + a / cb + 1;
+    for (int a = 0; a < C; ++a) toff[a + 1] = toff[a] + (a + dg + cb - 1) / cb;
     const int ntask = toff[C];
     std::vector<long long> cnt(static_cast<size_t>(ntask), 0);
     parallel_for(C, [&](int a) {
@@ -77,7 +79,7 @@ void build_schur_tasks(int C, int cb, const std::vector<int>& cam_off,
             const int p = spt[cam_list[i]];
             for (int f = pt_off[p]; f < pt_off[p + 1]; ++f) {
                 const int b = scam[f];
-                if (b > a) break;
+                if (b > a - 1 + dg) break;
                 ++c[b / cb];
             }
         }
@@ -92,16 +94,16 @@ void build_schur_tasks(int C, int cb, const std::vector<int>& cam_off,
             const int p = spt[e];
             for (int f = pt_off[p]; f < pt_off[p + 1]; ++f) {
                 const int b = scam[f];
-                if (b > a) break;
+                if (b > a - 1 + dg) break;
                 pairs[static_cast<size_t>(cur[b / cb]++)] = make_int2(e, f);
             }
         }
     });
     tasks.resize(static_cast<size_t>(ntask));
     for (int a = 0; a < C; ++a)
-        for (int c = 0; c <= a / cb; ++c) {
+        for (int c = 0; c < toff[a + 1] - toff[a]; ++c) {
             const int t = toff[a] + c;
-            tasks[t] = SchurTask{a, c * cb, std::min((c + 1) * cb, a + 1), 0, begin[t], begin[t + 1], 0, 0, 0};
+            tasks[t] = SchurTask{a, c * cb, std::min((c + 1) * cb, a + dg), 0, begin[t], begin[t + 1], 0, 0, 0};
         }
     std::stable_sort(tasks.begin(), tasks.end(), [](const SchurTask& x, const SchurTask& y) {
         return x.end - x.begin > y.end - y.begin;
@@ -275,7 +277,7 @@ Solver::Solver(const gba_problem_view& prob, const gba_solver_config& cfg) : cfg
     sshape_ = schur_shape(C, mixed_);
     std::vector<SchurTask> stasks;
     std::vector<int2> spairs;
-    build_schur_tasks(C, sshape_.cb, cam_off, cam_list, pt_off, scam, spt, stasks, spairs);
+    build_schur_tasks(C, sshape_.cb, cam_off, cam_list, pt_off, scam, spt, stasks, spairs, true);
     num_stasks_ = static_cast<int>(stasks.size());
     for (SchurTask& t : stasks) t.ld = pb_.npad;
     std::vector<int4>& chunks = lay.chunks;

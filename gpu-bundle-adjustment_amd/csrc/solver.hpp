@@ -46,11 +46,13 @@ struct HostLayout {
 };
 void build_layout(const gba_problem_view& prob, HostLayout& L);
 // Schur task and pair lists for chunks of cb camera columns (task ld, soff
-// and a0 are left 0 for the caller to fill).
+// and a0 are left 0 for the caller to fill); with_diag = false leaves out the
+// diagonal blocks and their pairs e == f (they then come from
+// launch_schur_diag, which also forms the rhs).
 void build_schur_tasks(int C, int cb, const std::vector<int>& cam_off,
                        const std::vector<int>& cam_list, const std::vector<int>& pt_off,
                        const std::vector<int>& scam, const std::vector<int>& spt,
-                       std::vector<SchurTask>& tasks, std::vector<int2>& pairs);
+                       std::vector<SchurTask>& tasks, std::vector<int2>& pairs, bool with_diag);
 
 // FP64 preconditioned conjugate gradients on A x = b, A symmetric given by the
 // lower triangle of an FP32 matrix, preconditioned by (L L^T)^-1 applied
