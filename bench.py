@@ -95,6 +95,50 @@ def run_batch(g, args):
     print(json.dumps(out))
 
 
+def run_fp16(g, args):
+    """Preset 4 (C=1,936, P=525,000, O=3.4M): the mixed-precision solve with
+    FP16 pixel storage (per-camera FP32 scale), timed over --steps solves with
+    the observations resident, against the same solve with FP64 pixels and the
+    plain FP64 solve, whose final cost is the reference for the reported
+    final-cost error."""
+    import torch
+    prob = g.generate(4, args.seed)
+    opts = g.LMOptions(initial_lambda=args.lambda0)
+    runs = {}
+    for name, prec, obs in (("mixed_fp16_px", g.PRECISION_MIXED, g.OBS_FP16),
+                            ("mixed_fp64_px", g.PRECISION_MIXED, g.OBS_FP64),
+                            ("fp64", g.PRECISION_FP64, g.OBS_FP64)):
+        s = g.Solver(prob, precision=prec, obs_storage=obs)
+        steps = args.steps if name == "mixed_fp16_px" else 1
+        for _ in range(args.warmup if name == "mixed_fp16_px" else 0):
+            s.solve(opts)
+        torch.cuda.synchronize()
+        times = []
+        for _ in range(steps):
+            r = s.solve(opts)
+            times.append(r.solve_time_s)
+        runs[name] = (r, statistics.median(times), len(times))
+        print(f"[fp16] {name}: {r.termination} cost {r.final_cost:.9e} trials {r.trials} "
+              f"solve {statistics.median(times):.3f} s", file=sys.stderr, flush=True)
+        s.close()
+        torch.cuda.synchronize()
+    c64 = runs["fp64"][0].final_cost
+    r16, t16, n16 = runs["mixed_fp16_px"]
+    out = {
+        "metric": "LM solve time, preset 4, mixed + FP16 pixel storage (s)", "value": round(t16, 4),
+        "unit": "s", "n_gpus": 1, "steps": n16, "warmup": args.warmup, "ms_per_step": round(1e3 * t16, 2),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f32/f64 (fp16 pixels)",
+        "data": "synthetic (seeded Snavely generator, seed %d)" % args.seed,
+        "config": {"workload": f"LM solve C={prob.num_cameras} P={prob.num_points} O={prob.num_observations} "
+                               "(preset 4)"},
+        "final_cost_rel_err_vs_fp64": {k: (v[0].final_cost - c64) / c64 for k, v in runs.items()},
+        "solve_s": {k: round(v[1], 4) for k, v in runs.items()},
+        "trials": {k: v[0].trials for k, v in runs.items()},
+        "terminations": {k: v[0].termination for k, v in runs.items()},
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -106,10 +150,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-iterations", type=int, default=2)
     ap.add_argument("--kernel-reps", type=int, default=10)
-    ap.add_argument("--mode", choices=["solve", "kernels", "batch"], default="solve",
+    ap.add_argument("--mode", choices=["solve", "kernels", "batch", "fp16"], default="solve",
                     help="kernels: only the per-kernel timing loops plus a 512 MiB device copy "
                          "(short run for rocprofv3 counter passes); batch: the batched solver on "
-                         "--batch problems of preset 5 (problems/s) against sequential FP64 solves")
+                         "--batch problems of preset 5 (problems/s) against sequential FP64 solves; "
+                         "fp16: preset 4 with FP16 pixel storage, final-cost error vs FP64")
     ap.add_argument("--batch", type=int, default=512, help="problems in --mode batch")
     ap.add_argument("--seq-sample", type=int, default=32,
                     help="problems solved one by one for the sequential rate in --mode batch")
@@ -124,6 +169,9 @@ def main():
     g = load_pkg()
     if args.mode == "batch":
         run_batch(g, args)
+        return
+    if args.mode == "fp16":
+        run_fp16(g, args)
         return
     prob = g.generate(args.config, args.seed)
     C, P, O = prob.num_cameras, prob.num_points, prob.num_observations

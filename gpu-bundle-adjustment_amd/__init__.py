@@ -20,6 +20,7 @@ _ip = ctypes.POINTER(ctypes.c_int)
 CONVERGENCE, MAX_ITERATIONS, FAILURE = 0, 1, 2
 TERMINATION = {CONVERGENCE: "CONVERGENCE", MAX_ITERATIONS: "MAX_ITERATIONS", FAILURE: "FAILURE"}
 PRECISION_FP64, PRECISION_MIXED = 0, 1
+OBS_FP64, OBS_FP16 = 0, 1  # device storage of the observed pixels
 TRACE_STRIDE = 6
 TRIAL_STATUS = {0: "accept", 1: "reject", 2: "nonspd_v", 3: "chol_fail", 4: "denom", 5: "nonfinite"}
 KERNELS = {"edge_point": 0, "edge_camera": 1, "schur": 2, "factor": 3, "solve": 4, "backsub": 5,
@@ -49,7 +50,8 @@ class LMResultC(ctypes.Structure):
 
 class SolverConfigC(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("refine_tol", ctypes.c_double),
-                ("refine_max_iters", ctypes.c_int), ("trace", _dp), ("trace_capacity", ctypes.c_int)]
+                ("refine_max_iters", ctypes.c_int), ("trace", _dp), ("trace_capacity", ctypes.c_int),
+                ("obs_storage", ctypes.c_int)]
 
 
 class SynthSpecC(ctypes.Structure):
@@ -86,6 +88,7 @@ _PROTOS = {
                                      _dp, _dp, ctypes.POINTER(ctypes.c_ulonglong)]),
     "gba_schur_step": (ctypes.c_int, [ctypes.POINTER(ProblemView), ctypes.c_int, ctypes.c_double, _dp,
                                       _dp, _dp, _ip, _dp, _dp]),
+    "gba_encode_obs_fp16": (ctypes.c_int, [ctypes.POINTER(ProblemView), ctypes.c_void_p, ctypes.c_void_p]),
     "gba_cost": (ctypes.c_int, [ctypes.POINTER(ProblemView), _dp, ctypes.POINTER(ctypes.c_ulonglong)]),
     "gba_dense_spd_solve": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _dp, _dp, _ip, _ip]),
     "gba_batch_chol9_factor_solve": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
@@ -220,10 +223,11 @@ class LMResult:
                         r.refine_iterations, r.max_refine_residual, trace)
 
 
-def _config(precision, trace_capacity=0, refine_tol=1e-10, refine_max_iters=12):
+def _config(precision, trace_capacity=0, refine_tol=1e-10, refine_max_iters=12, obs_storage=OBS_FP64):
     c = SolverConfigC()
     lib().gba_default_config(ctypes.byref(c))
     c.precision = precision
+    c.obs_storage = obs_storage
     c.refine_tol = refine_tol
     c.refine_max_iters = refine_max_iters
     trace = None
@@ -241,10 +245,10 @@ def _trim_trace(trace):
     return trace[keep]
 
 
-def solve_bal_gpu(prob, opts=None, precision=PRECISION_FP64, trace_capacity=1024):
+def solve_bal_gpu(prob, opts=None, precision=PRECISION_FP64, trace_capacity=1024, obs_storage=OBS_FP64):
     """gba::solve_bal_gpu: updates prob.cameras / prob.points in place."""
     opts = opts or LMOptions()
-    cfg, trace = _config(precision, trace_capacity)
+    cfg, trace = _config(precision, trace_capacity, obs_storage=obs_storage)
     res = LMResultC()
     v = prob.view()
     _check(lib().gba_solve_bal(ctypes.byref(v), ctypes.byref(opts.c()), ctypes.byref(cfg),
@@ -255,8 +259,8 @@ def solve_bal_gpu(prob, opts=None, precision=PRECISION_FP64, trace_capacity=1024
 class Solver:
     """Device-resident solver handle: upload once, solve repeatedly."""
 
-    def __init__(self, prob, precision=PRECISION_FP64, trace_capacity=1024):
-        self._cfg, self._trace = _config(precision, trace_capacity)
+    def __init__(self, prob, precision=PRECISION_FP64, trace_capacity=1024, obs_storage=OBS_FP64):
+        self._cfg, self._trace = _config(precision, trace_capacity, obs_storage=obs_storage)
         self._h = ctypes.c_void_p()
         self._prob = prob
         v = prob.view()
@@ -374,6 +378,16 @@ def schur_step(prob, lam, precision=PRECISION_FP64, want_S=False):
                                 ctypes.byref(st), _ptr(S) if want_S else None,
                                 _ptr(rhs) if want_S else None))
     return dict(dc=dc, dp=dp, denom=den.value, status=TRIAL_STATUS[st.value], S=S, rhs=rhs)
+
+
+def encode_obs_fp16(prob):
+    """FP16 pixel encoding used by obs_storage=OBS_FP16: (scale float32[C],
+    px16 float16[O, 2]); the decoded pixels are px16 * scale[camera]."""
+    scale = np.zeros(prob.num_cameras, dtype=np.float32)
+    px16 = np.zeros((prob.num_observations, 2), dtype=np.float16)
+    v = prob.view()
+    _check(lib().gba_encode_obs_fp16(ctypes.byref(v), scale.ctypes.data, px16.ctypes.data))
+    return scale, px16
 
 
 def cost(prob):

@@ -26,6 +26,7 @@ SOURCES = [
     "k_batch.hip",
     "solver.cpp",
     "batch.cpp",
+    "obs_fp16.cpp",
     "capi.cpp",
     "cpp_api.cpp",
     "host_model.cpp",

@@ -225,7 +225,7 @@ void BatchSolver::solve(const gba_lm_options& opts, gba_lm_result* res) {
             upload_mask(lin_mask, active_.p);
             launch_lin_points(st_, pl, cams_.p, pts_.p, W_.p, nullptr, VB_.p, unit_sse_.p, unit_anom_.p);
             launch_lin_cameras(st_, pl, cams_.p, pts_.p, gram_.p);
-            launch_lin_reduce(st_, pl, gram_.p, unit_sse_.p, unit_anom_.p, U_.p, bc_.p, lin_status_.p);
+            launch_lin_reduce(st_, pl, gram_.p, unit_sse_.p, unit_anom_.p, U_.p, bc_.p, nullptr);
             launch_batch_status(st_, B, desc_.p, 0, unit_sse_.p, unit_den_.p, unit_anom_.p, dc_.p, U_.p,
                                 bc_.p, lam_.p, flags_.p, active_.p, lin_status_.p);
             GBA_HIP(hipMemcpyAsync(h_status_, lin_status_.p, sizeof(PassStatus) * B, hipMemcpyDeviceToHost, st_));
@@ -255,7 +255,8 @@ void BatchSolver::solve(const gba_lm_options& opts, gba_lm_result* res) {
         upload_mask(trial_mask, active_.p);
         launch_fill_int(st_, reinterpret_cast<int*>(flags_.p), 4 * B, 0);
         launch_damp_invert_v_batched(st_, pb_.P, VB_.p, pt_prob_.p, lam_.p, active_.p, Vinv_.p, flags_.p);
-        launch_schur_diag(st_, pl, desc_.p, W_.p, Vinv_.p, VB_.p, U_.p, bc_.p, 0.0, S_.p, 0, rhs_.p, flags_.p);
+        launch_schur_diag(st_, pl, desc_.p, W_.p, nullptr, Vinv_.p, VB_.p, U_.p, bc_.p, 0.0, nullptr, S_.p,
+                          nullptr, nullptr, 0, rhs_.p, flags_.p);
         if (num_stasks_ > 0)
             launch_schur(st_, sshape_, pl, stasks_.p, num_stasks_, spairs_.p, W_.p, nullptr, Vinv_.p, VB_.p,
                          U_.p, bc_.p, 0.0, nullptr, S_.p, nullptr, nullptr, 0, rhs_.p, flags_.p);

@@ -230,6 +230,19 @@ int gba_schur_step(const gba_problem_view* prob, int precision, double lambda, d
     });
 }
 
+int gba_encode_obs_fp16(const gba_problem_view* prob, float* cam_scale, unsigned short* px16) {
+    return guarded([&] {
+        if (!prob || !cam_scale || !px16) return set_error("gba_encode_obs_fp16: null argument");
+        if (prob->num_cameras < 0 || prob->num_observations < 0)
+            return set_error("gba_encode_obs_fp16: negative size");
+        for (int e = 0; e < prob->num_observations; ++e)
+            if (prob->obs_camera[e] < 0 || prob->obs_camera[e] >= prob->num_cameras)
+                return set_error("gba_encode_obs_fp16: camera index out of range");
+        encode_obs_fp16(*prob, cam_scale, reinterpret_cast<uint16_t*>(px16));
+        return 0;
+    });
+}
+
 int gba_cost(const gba_problem_view* prob, double* sse, unsigned long long* anomalies) {
     return guarded([&] {
         gba_solver_config c;

@@ -44,7 +44,26 @@ struct DevProblem {
     const int* pt_prob;
     const double* lam;
     const int* active;
+    // FP16 pixel storage (null for FP64): per sorted observation the two
+    // binary16 pixels packed in one word (x in the low half), and the FP32
+    // scale of each camera.
+    const unsigned* spx16;
+    const float* cam_scale;
 };
+
+// Observed pixel of sorted observation e (camera c) in FP64.  With FP16
+// storage the value is (double)half * (double)scale: an 11-bit times a 24-bit
+// significand, exact in FP64.
+__device__ __forceinline__ double2 obs_px(const DevProblem& pb, int e, int c) {
+    if (pb.spx16) {
+        const unsigned u = pb.spx16[e];
+        const double s = static_cast<double>(pb.cam_scale[c]);
+        const _Float16 hx = __builtin_bit_cast(_Float16, static_cast<unsigned short>(u & 0xffffu));
+        const _Float16 hy = __builtin_bit_cast(_Float16, static_cast<unsigned short>(u >> 16));
+        return make_double2(static_cast<double>(hx) * s, static_cast<double>(hy) * s);
+    }
+    return pb.spx[e];
+}
 
 __device__ __forceinline__ int prob_of_pt(const DevProblem& pb, int p) { return pb.pt_prob ? pb.pt_prob[p] : 0; }
 __device__ __forceinline__ int prob_of_cam(const DevProblem& pb, int c) { return pb.cam_prob ? pb.cam_prob[c] : 0; }

@@ -98,7 +98,7 @@ __global__ __launch_bounds__(256) void k_backsub(DevProblem pb, const WT* __rest
                 out[2] = Xt[2];
                 den = d;
             }
-            const double2 o = pb.spx[e];
+            const double2 o = obs_px(pb, e, c);
             double r[2];
             if (snavely_eval<false>(tcams + 9 * c, Xt, o.x, o.y, r, nullptr, nullptr))
                 sse = r[0] * r[0] + r[1] * r[1];
@@ -135,9 +135,10 @@ __global__ __launch_bounds__(256) void k_backsub(DevProblem pb, const WT* __rest
         for (int base = rg.x; base < rg.y; base += kWave) {
             const int e = base + lane;
             if (e < rg.y) {
-                const double2 o = pb.spx[e];
+                const int c = pb.scam[e];
+                const double2 o = obs_px(pb, e, c);
                 double r[2];
-                if (snavely_eval<false>(tcams + 9 * pb.scam[e], Xt, o.x, o.y, r, nullptr, nullptr))
+                if (snavely_eval<false>(tcams + 9 * c, Xt, o.x, o.y, r, nullptr, nullptr))
                     sse += r[0] * r[0] + r[1] * r[1];
                 else
                     anom += 1;

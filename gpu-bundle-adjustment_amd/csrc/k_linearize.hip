@@ -69,7 +69,7 @@ __global__ __launch_bounds__(256) void k_lin_points(DevProblem pb, const double*
         if (active) {
             key = pb.spt[e];
             const int c = pb.scam[e];
-            const double2 o = pb.spx[e];
+            const double2 o = obs_px(pb, e, c);
             double r[2], Jc[18], Jp[6];
             const bool ok = snavely_eval<true>(cams + 9 * c, pts + 3 * key, o.x, o.y, r, Jc, Jp);
             store_w(W + 27 * static_cast<size_t>(e), Jc, Jp, ok);
@@ -93,7 +93,7 @@ __global__ __launch_bounds__(256) void k_lin_points(DevProblem pb, const double*
             const int e = base + lane;
             if (e < rg.y) {
                 const int c = pb.scam[e];
-                const double2 o = pb.spx[e];
+                const double2 o = obs_px(pb, e, c);
                 double r[2], Jc[18], Jp[6], t[9];
                 const bool ok = snavely_eval<true>(cams + 9 * c, pts + 3 * p, o.x, o.y, r, Jc, Jp);
                 store_w(W + 27 * static_cast<size_t>(e), Jc, Jp, ok);
@@ -144,7 +144,7 @@ __global__ __launch_bounds__(256) void k_lin_cameras(DevProblem pb, const double
         for (int k = 0; k < 18; ++k) Jc[k] = 0.0;
         if (i < ch.z) {
             const int ed = pb.cam_list[i];
-            const double2 o = pb.spx[ed];
+            const double2 o = obs_px(pb, ed, ch.x);
             const int p = pb.spt[ed];
             const bool ok = snavely_eval<true>(cam, pts + 3 * p, o.x, o.y, r, Jc, Jp);
             if (!ok) {
@@ -249,7 +249,7 @@ __global__ __launch_bounds__(256) void k_cost(DevProblem pb, const double* __res
     unsigned an = 0;
     if (e < pb.O) {
         const int c = pb.scam[e], p = pb.spt[e];
-        const double2 o = pb.spx[e];
+        const double2 o = obs_px(pb, e, c);
         double r[2];
         if (snavely_eval<false>(cams + 9 * c, pts + 3 * p, o.x, o.y, r, nullptr, nullptr))
             sse = r[0] * r[0] + r[1] * r[1];
@@ -322,8 +322,12 @@ void launch_lin_cameras(hipStream_t st, const DevProblem& pb, const double* cams
 void launch_lin_reduce(hipStream_t st, const DevProblem& pb, const double* gram_part,
                        const double* unit_sse, const unsigned* unit_anom, double* U, double* bc,
                        PassStatus* status) {
-    hipLaunchKernelGGL(k_lin_reduce, dim3((pb.C + 3) / 4 + 1), dim3(256), 0, st, pb, gram_part, unit_sse,
-                       unit_anom, U, bc, status);
+    // the extra last workgroup sums the whole-problem SSE into *status; callers
+    // that sum per problem themselves pass status == nullptr
+    const int grid = (pb.C + 3) / 4 + (status ? 1 : 0);
+    if (grid > 0)
+        hipLaunchKernelGGL(k_lin_reduce, dim3(grid), dim3(256), 0, st, pb, gram_part, unit_sse, unit_anom, U,
+                           bc, status);
 }
 
 void launch_cost(hipStream_t st, const DevProblem& pb, const double* cams, const double* pts,

@@ -138,12 +138,13 @@ __global__ __launch_bounds__(NT) void k_schur(
 // (wavefront shuffles, then the four wavefronts in order): no atomics.
 // desc (batched problems) gives the problem's S offset, order and first
 // camera; without it S has leading dimension ld and starts at camera 0.
-template <typename WT>
+template <typename WT, typename ST>
 __global__ __launch_bounds__(256) void k_schur_diag(
     DevProblem pb, const BatchDesc* __restrict__ desc, const WT* __restrict__ W,
     const double* __restrict__ Vinv6, const double* __restrict__ VB, const double* __restrict__ U,
-    const double* __restrict__ bc, double lambda, double* __restrict__ S, int ld,
-    double* __restrict__ rhs, const TrialFlags* flags) {
+    const double* __restrict__ bc, double lambda, const double* __restrict__ dscale,
+    ST* __restrict__ S, ST* __restrict__ Slow, int ld, double* __restrict__ rhs,
+    const TrialFlags* flags) {
     __shared__ double s_red[4][54];
     const int a = blockIdx.x, tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
     const int q = prob_of_cam(pb, a);
@@ -194,16 +195,22 @@ __global__ __launch_bounds__(256) void k_schur_diag(
         const double sum = ((s_red[0][tid] + s_red[1][tid]) + s_red[2][tid]) + s_red[3][tid];
         if (tid >= 45) {
             const int row = 9 * a + (tid - 45);
-            rhs[row] = bc[row] - sum;
+            const double v = bc[row] - sum;
+            rhs[row] = dscale ? v * dscale[row] : v;
         } else {
             int i = 0;
             while ((i + 1) * (i + 2) / 2 <= tid) ++i;
             const int j = tid - i * (i + 1) / 2;
             double v = U[81 * static_cast<size_t>(a) + 9 * i + j] - sum;
             if (i == j) v += lambda * damp_diag(U[81 * static_cast<size_t>(a) + 10 * i]);
+            if (dscale) v *= dscale[9 * a + i] * dscale[9 * a + j];
             const int row = 9 * (a - a0) + i, col = 9 * (a - a0) + j;
-            S[static_cast<size_t>(row) * ld + col] = v;
-            S[static_cast<size_t>(col) * ld + row] = v;
+            S[static_cast<size_t>(row) * ld + col] = static_cast<ST>(v);
+            S[static_cast<size_t>(col) * ld + row] = static_cast<ST>(v);
+            if (Slow) {
+                Slow[static_cast<size_t>(row) * ld + col] = static_cast<ST>(v);
+                Slow[static_cast<size_t>(col) * ld + row] = static_cast<ST>(v);
+            }
         }
     }
 }
@@ -285,11 +292,17 @@ void launch_schur(hipStream_t st, const SchurShape& sh, const DevProblem& pb,
 }
 
 void launch_schur_diag(hipStream_t st, const DevProblem& pb, const BatchDesc* desc, const double* W64,
-                       const double* Vinv6, const double* VB, const double* U, const double* bc,
-                       double lambda, double* S64, int ld, double* rhs, const TrialFlags* flags) {
-    if (pb.C > 0)
-        hipLaunchKernelGGL(k_schur_diag<double>, dim3(pb.C), dim3(256), 0, st, pb, desc, W64, Vinv6, VB, U,
-                           bc, lambda, S64, ld, rhs, flags);
+                       const float* W32, const double* Vinv6, const double* VB, const double* U,
+                       const double* bc, double lambda, const double* dscale, double* S64,
+                       float* S32fac, float* S32low, int ld, double* rhs, const TrialFlags* flags) {
+    if (pb.C <= 0) return;
+    if (W32)
+        hipLaunchKernelGGL((k_schur_diag<float, float>), dim3(pb.C), dim3(256), 0, st, pb, desc, W32, Vinv6,
+                           VB, U, bc, lambda, dscale, S32fac, S32low, ld, rhs, flags);
+    else
+        hipLaunchKernelGGL((k_schur_diag<double, double>), dim3(pb.C), dim3(256), 0, st, pb, desc, W64,
+                           Vinv6, VB, U, bc, lambda, dscale, S64, static_cast<double*>(nullptr), ld, rhs,
+                           flags);
 }
 
 void launch_diag_scale(hipStream_t st, int n, int npad, const double* U, double lambda,

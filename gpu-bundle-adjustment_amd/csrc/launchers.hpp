@@ -46,12 +46,13 @@ void launch_schur(hipStream_t st, const SchurShape& sh, const DevProblem& pb,
                   const float* W32, const double* Vinv6, const double* VB, const double* U,
                   const double* bc, double lambda, const double* dscale, double* S64, float* S32fac,
                   float* S32low, int ld, double* rhs, const TrialFlags* flags);
-// Diagonal blocks + rhs, one workgroup per camera (FP64; for task lists
-// built without the pairs e == f).  desc: per-problem S layout (batched) or
-// NULL (single problem, leading dimension ld).
+// Diagonal blocks (both triangles) + rhs, one workgroup per camera.  desc:
+// per-problem S layout (batched) or NULL (single problem, leading dimension
+// ld).  W32 selects the mixed form (D S D into the FP32 copies, rhs scaled).
 void launch_schur_diag(hipStream_t st, const DevProblem& pb, const BatchDesc* desc, const double* W64,
-                       const double* Vinv6, const double* VB, const double* U, const double* bc,
-                       double lambda, double* S64, int ld, double* rhs, const TrialFlags* flags);
+                       const float* W32, const double* Vinv6, const double* VB, const double* U,
+                       const double* bc, double lambda, const double* dscale, double* S64,
+                       float* S32fac, float* S32low, int ld, double* rhs, const TrialFlags* flags);
 void launch_diag_scale(hipStream_t st, int n, int npad, const double* U, double lambda,
                        double* dscale);
 

@@ -14,6 +14,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <thread>
 
@@ -277,7 +278,13 @@ Solver::Solver(const gba_problem_view& prob, const gba_solver_config& cfg) : cfg
     sshape_ = schur_shape(C, mixed_);
     std::vector<SchurTask> stasks;
     std::vector<int2> spairs;
-    build_schur_tasks(C, sshape_.cb, cam_off, cam_list, pt_off, scam, spt, stasks, spairs, true);
+    // diagonal blocks and rhs in their own per-camera kernel (GBA_SCHUR_DIAG=0:
+    // inside the task lists, as pairs e == f)
+    {
+        const char* env = std::getenv("GBA_SCHUR_DIAG");
+        sep_diag_ = !(env && env[0] == '0');
+    }
+    build_schur_tasks(C, sshape_.cb, cam_off, cam_list, pt_off, scam, spt, stasks, spairs, !sep_diag_);
     num_stasks_ = static_cast<int>(stasks.size());
     for (SchurTask& t : stasks) t.ld = pb_.npad;
     std::vector<int4>& chunks = lay.chunks;
@@ -291,7 +298,25 @@ Solver::Solver(const gba_problem_view& prob, const gba_solver_config& cfg) : cfg
     upload(spt_, spt, st_);
     upload(perm_, perm, st_);
     upload(pt_off_, pt_off, st_);
-    upload(spx_, spx, st_);
+    if (cfg.obs_storage == GBA_OBS_FP16) {
+        // only the binary16 pixels and the camera scales reach the device
+        std::vector<float> scale(static_cast<size_t>(C));
+        std::vector<uint16_t> h(2 * static_cast<size_t>(O));
+        encode_obs_fp16(prob, scale.data(), h.data());
+        std::vector<unsigned> packed(static_cast<size_t>(O));
+        for (int k = 0; k < O; ++k) {
+            const size_t e = static_cast<size_t>(perm[k]);
+            packed[k] = static_cast<unsigned>(h[2 * e]) | (static_cast<unsigned>(h[2 * e + 1]) << 16);
+        }
+        upload(spx16_, packed, st_);
+        upload(cam_scale_, scale, st_);
+        pb_.spx16 = spx16_.p;
+        pb_.cam_scale = cam_scale_.p;
+    } else if (cfg.obs_storage == GBA_OBS_FP64) {
+        upload(spx_, spx, st_);
+    } else {
+        throw Error("unknown observation storage");
+    }
     upload(units_, units, st_);
     upload(cam_list_, cam_list, st_);
     upload(stasks_, stasks, st_);
@@ -405,7 +430,11 @@ void Solver::cost(double* sse, unsigned long long* anomalies) {
 }
 
 void Solver::enqueue_schur(double lambda) {
-    launch_schur(st_, sshape_, pb_, stasks_.p, num_stasks_, spairs_.p, W64_.p, W32_.p, Vinv_.p,
+    if (sep_diag_)
+        launch_schur_diag(st_, pb_, nullptr, W64_.p, W32_.p, Vinv_.p, VB_.p, U_.p, bc_.p, lambda,
+                          mixed_ ? dscale_.p : nullptr, S64_.p, S32_.p, S32low_.p, pb_.npad, rhs_.p, flags_.p);
+    if (num_stasks_ > 0)
+        launch_schur(st_, sshape_, pb_, stasks_.p, num_stasks_, spairs_.p, W64_.p, W32_.p, Vinv_.p,
                  VB_.p, U_.p, bc_.p, lambda, mixed_ ? dscale_.p : nullptr, S64_.p, S32_.p,
                  S32low_.p, pb_.npad, rhs_.p, flags_.p);
 }

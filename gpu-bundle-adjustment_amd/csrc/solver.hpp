@@ -45,6 +45,12 @@ struct HostLayout {
     double pair_count = 0.0;  // sum_p k_p (k_p + 1) / 2
 };
 void build_layout(const gba_problem_view& prob, HostLayout& L);
+
+// IEEE binary16 bits of x, rounded to nearest even.
+uint16_t double_to_half_rne(double x);
+// FP16 pixel encoding with a per-camera FP32 scale, in the caller's
+// observation order (gba_encode_obs_fp16): cam_scale[C], px16[2 O].
+void encode_obs_fp16(const gba_problem_view& prob, float* cam_scale, uint16_t* px16);
 // Schur task and pair lists for chunks of cb camera columns (task ld, soff
 // and a0 are left 0 for the caller to fill); with_diag = false leaves out the
 // diagonal blocks and their pairs e == f (they then come from
@@ -112,6 +118,7 @@ private:
 
     gba_solver_config cfg_;
     bool mixed_;
+    bool sep_diag_ = true;  // diagonal Schur blocks + rhs from launch_schur_diag
     DevProblem pb_{};
     int nblk_ = 0;
     hipStream_t st_ = nullptr;
@@ -125,6 +132,8 @@ private:
     DevBuf<int2> spairs_;
     int num_stasks_ = 0;
     DevBuf<double2> spx_;
+    DevBuf<unsigned> spx16_;  // FP16 storage: packed pixel pair per sorted observation
+    DevBuf<float> cam_scale_;
     DevBuf<int2> units_;
     DevBuf<int4> chunks_;
     DevBuf<double> cams_[2], pts_[2];

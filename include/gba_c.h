@@ -76,6 +76,13 @@ enum gba_precision {
     GBA_PRECISION_MIXED = 1  /* FP32 W / S / MFMA factor, FP64 refinement, cost, gain ratio */
 };
 
+/* Device storage of the observed pixels. */
+enum gba_obs_storage {
+    GBA_OBS_FP64 = 0, /* two doubles per observation, as given */
+    GBA_OBS_FP16 = 1  /* two IEEE binary16 values per observation times a per-camera FP32
+                         scale (see gba_encode_obs_fp16); decoded exactly into FP64 */
+};
+
 /* Solver configuration beyond gba_lm_options (no reference counterpart). */
 typedef struct gba_solver_config {
     int precision;          /* enum gba_precision */
@@ -83,6 +90,7 @@ typedef struct gba_solver_config {
     int refine_max_iters;   /* mixed: refinement cap per trial (default 12) */
     double* trace;          /* optional, GBA_TRACE_STRIDE doubles per trial */
     int trace_capacity;     /* trials that fit in trace */
+    int obs_storage;        /* enum gba_obs_storage (default FP64) */
 } gba_solver_config;
 
 /* Trace record per LM trial: iteration, lambda used, trial cost, rho,
@@ -151,6 +159,16 @@ int gba_solve_bal_batched(gba_problem_view* probs, int count, const gba_lm_optio
                           gba_lm_result* results);
 
 /* ---- pieces of the path, host arrays in / out (parity tests) ----------- */
+
+/* The FP16 observation encoding the solver uses with GBA_OBS_FP16 (no
+ * reference counterpart; the reference studied FP16 pixel storage offline,
+ * docs/RUNGS.md:44,50).  Per camera c: m_c = the largest finite |x| or |y| of
+ * its observations, scale[c] = (float)(m_c / 32768), or 1 when m_c is 0 or
+ * there is none.  Per observation e (caller order): px16[2e + i] = binary16
+ * of pixel_i / scale[c], rounded to nearest even (a NaN stays NaN, an
+ * infinity stays infinite).  The decoded pixel is (double)px16 * scale[c],
+ * exact in FP64. */
+int gba_encode_obs_fp16(const gba_problem_view* prob, float* cam_scale, unsigned short* px16);
 
 /* Linearisation at the given parameters (reference src/lm.cu:205-252 and
  * accumulate() :398-422).  Outputs in the reference layout: U 81C and V 9P
