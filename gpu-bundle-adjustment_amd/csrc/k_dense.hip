@@ -33,7 +33,8 @@ namespace gba_amd {
 namespace {
 
 // Cholesky factor and inverse of the symmetric tile in a (lower triangle
-// read), 256 threads.  On return a holds L (lower; the upper part is stale) and
+// read), 256 threads or more (the trailing blocks are dealt to every
+// wavefront, the inverse stages use the first four).  On return a holds L (lower; the upper part is stale) and
 // b holds L^-1 (lower, zero above the diagonal inside the 16x16 diagonal
 // blocks; the other upper blocks are scratch).  Blocked by 16 columns:
 //   panel  : wavefront 0, one row per lane, 16 pivot steps; the pivot and the
@@ -79,7 +80,7 @@ __device__ __forceinline__ bool tile_potrf_inv(T (*a)[kTile + 2], T (*b)[kTile +
         __syncthreads();
         // trailing blocks (rb, cb), cb <= rb, of the rows/columns after the panel
         const int nb = 3 - p;
-        for (int bi = w; bi < nb * (nb + 1) / 2; bi += 4) {
+        for (int bi = w; bi < nb * (nb + 1) / 2; bi += static_cast<int>(blockDim.x >> 6)) {
             int rb = 0;
             while ((rb + 1) * (rb + 2) / 2 <= bi) ++rb;
             const int cb = bi - rb * (rb + 1) / 2;
@@ -93,7 +94,7 @@ __device__ __forceinline__ bool tile_potrf_inv(T (*a)[kTile + 2], T (*b)[kTile +
         }
         __syncthreads();
     }
-    {
+    if (w < 4) {
         // Gauss-Jordan on the rows of diagonal block w: lane i (mod 16) holds
         // row i of L and of the growing inverse
         const int o = 16 * w, i = li;
@@ -156,7 +157,7 @@ __device__ __forceinline__ void store_diag_tile(const T (*a)[kTile + 2], const T
     if (!ok && threadIdx.x == 0) atomicOr(&flags->chol_fail, 1);
     T* d = D + static_cast<size_t>(k) * kTile * kTile;
     T* it = DinvT + static_cast<size_t>(k) * kTile * kTile;
-    for (int q = threadIdx.x; q < kTile * kTile; q += 256) {
+    for (int q = threadIdx.x; q < kTile * kTile; q += blockDim.x) {
         const int r = q >> 6, c = q & 63;
         d[q] = (c <= r) ? a[r][c] : T(0);
         it[q] = (r <= c) ? b[c][r] : T(0);
@@ -319,33 +320,17 @@ __global__ __launch_bounds__(256, 2) void k_chol_update(T* __restrict__ S, int l
 
 // ---- batched small systems: one workgroup per problem ----------------------
 
-// 256 threads: out[r] = sum_c M[r][c] v[c] for a 64x64 row-major tile (lane
-// row r, wavefront w sums columns 16w..16w+15; fixed-order combination).
-__device__ __forceinline__ void tile_matvec(const double* __restrict__ M, int ld,
-                                            const double* v, double* part, double* out,
-                                            bool accumulate) {
-    const int r = threadIdx.x & 63, g = threadIdx.x >> 6;
-    const double* row = M + static_cast<size_t>(r) * ld + 16 * g;
-    double s = 0.0;
-#pragma unroll
-    for (int c = 0; c < 16; ++c) s += row[c] * v[16 * g + c];
-    part[g * 64 + r] = s;
-    __syncthreads();
-    if (threadIdx.x < 64) {
-        const double t = (part[r] + part[64 + r]) + (part[128 + r] + part[192 + r]);
-        out[r] = accumulate ? out[r] + t : t;
-    }
-    __syncthreads();
-}
-
 // Dense factor and solve of every active problem's S (FP64), one workgroup of
-// 256 threads per problem, tile by tile: diagonal tile factored and inverted
+// kBatchThreads threads (8 wavefronts) per problem, tile by tile: diagonal tile factored and inverted
 // in LDS (tile_potrf_inv), TRSM of the tiles below as MFMA products with the
 // stored inverse (L_ik written below and L_ik^T above the diagonal), trailing
 // update from the transposed tiles, then forward and backward substitution
 // with the diagonal inverses.  dc[9 c0 + r] receives the solution; a bad
 // pivot sets flags[problem].chol_fail.  npad <= 1024.
-__global__ __launch_bounds__(256, 2) void k_batch_chol_solve(const BatchDesc* __restrict__ desc,
+constexpr int kBatchThreads = 512;
+constexpr int kBatchWaves = kBatchThreads / 64;
+
+__global__ __launch_bounds__(kBatchThreads) void k_batch_chol_solve(const BatchDesc* __restrict__ desc,
                                                           double* __restrict__ S,
                                                           double* __restrict__ D,
                                                           const double* __restrict__ rhs,
@@ -355,7 +340,7 @@ __global__ __launch_bounds__(256, 2) void k_batch_chol_solve(const BatchDesc* __
     __shared__ double sa[kTile][kTile + 2];
     __shared__ double sb[kTile][kTile + 2];
     __shared__ double vec[1024];
-    __shared__ double part[256];
+    __shared__ double part[kBatchThreads];
     __shared__ double tmp[kTile];
     const int b = blockIdx.x;
     if (!active[b] || flags[b].nonspd) return;
@@ -368,17 +353,18 @@ __global__ __launch_bounds__(256, 2) void k_batch_chol_solve(const BatchDesc* __
     const int li = lane & 15, lk = lane >> 4;
     for (int k = 0; k < nblk; ++k) {
         const double* Akk = A + static_cast<size_t>(k * kTile) * ld + k * kTile;
-        for (int q = tid; q < kTile * kTile; q += 256) sa[q >> 6][q & 63] = Akk[static_cast<size_t>(q >> 6) * ld + (q & 63)];
+        for (int q = tid; q < kTile * kTile; q += kBatchThreads)
+            sa[q >> 6][q & 63] = Akk[static_cast<size_t>(q >> 6) * ld + (q & 63)];
         __syncthreads();
         const bool ok = tile_potrf_inv<double>(sa, sb);
         store_diag_tile<double>(sa, sb, ok, k, Lkk, DinvT, flags + b);
         __syncthreads();
         if (!ok) return;
         const double* inv = DinvT + static_cast<size_t>(k) * kTile * kTile;
-        // TRSM: wavefront w takes the half tiles (32 rows) h = w, w+4, ... of the
-        // tiles below: L_ik = A_ik Linv^T (2x4 MFMA blocks), written below the
-        // diagonal and, transposed, into the upper tile (k, i)
-        for (int h = w; h < 2 * (nblk - k - 1); h += 4) {
+        // TRSM: wavefront w takes every kBatchWaves-th half tile (32 rows) of
+        // the tiles below: L_ik = A_ik Linv^T (2x4 MFMA blocks), written below
+        // the diagonal and, transposed, into the upper tile (k, i)
+        for (int h = w; h < 2 * (nblk - k - 1); h += kBatchWaves) {
             const int i = k + 1 + (h >> 1), r0 = 32 * (h & 1);
             double* Aik = A + static_cast<size_t>(i * kTile) * ld + k * kTile;
             double* LTki = A + static_cast<size_t>(k * kTile) * ld + i * kTile;
@@ -411,11 +397,11 @@ __global__ __launch_bounds__(256, 2) void k_batch_chol_solve(const BatchDesc* __
                     }
         }
         __syncthreads();
-        // trailing update: wavefront w takes every fourth half tile of the
-        // tiles (i, j), k < j <= i; operands from the transposed tiles
+        // trailing update: wavefront w takes every kBatchWaves-th half tile of
+        // the tiles (i, j), k < j <= i; operands from the transposed tiles
         {
             const int m = nblk - k - 1;
-            for (int h = w; h < m * (m + 1); h += 4) {
+            for (int h = w; h < m * (m + 1); h += kBatchWaves) {
                 const int t = h >> 1, r0 = 32 * (h & 1);
                 int ii = 0;
                 while ((ii + 1) * (ii + 2) / 2 <= t) ++ii;
@@ -455,68 +441,67 @@ __global__ __launch_bounds__(256, 2) void k_batch_chol_solve(const BatchDesc* __
         }
         __syncthreads();
     }
+    // Triangular solves, one sweep per tile row: thread (r, g) = (lane, wave)
+    // sums row r over its eighth of every tile's columns, the eight partial
+    // sums are combined in a fixed order.
+    constexpr int kSlice = kTile / kBatchWaves;
+    const int r = lane, c0 = kSlice * w;
+    auto combine = [&](int row) {
+        double t = 0.0;
+#pragma unroll
+        for (int g = 0; g < kBatchWaves; ++g) t += part[g * kTile + row];
+        return t;
+    };
     // forward: y_i = Linv_ii (b_i - sum_{j<i} L_ij y_j); vec holds y
     const double* bb = rhs + 9 * static_cast<size_t>(d.c0);
     for (int i = 0; i < nblk; ++i) {
-        if (tid < kTile) {
-            const int r = i * kTile + tid;
-            tmp[tid] = r < d.n ? bb[r] : 0.0;
-        }
-        __syncthreads();
+        double s = 0.0;
+        const double* row = A + static_cast<size_t>(i * kTile + r) * ld + c0;
         for (int j = 0; j < i; ++j) {
-            tile_matvec(A + static_cast<size_t>(i * kTile) * ld + j * kTile, ld, vec + j * kTile, part,
-                        vec + i * kTile, j > 0);
+#pragma unroll
+            for (int c = 0; c < kSlice; ++c) s += row[j * kTile + c] * vec[j * kTile + c0 + c];
         }
+        part[w * kTile + r] = s;
+        __syncthreads();
         if (tid < kTile) {
-            const double acc = i > 0 ? vec[i * kTile + tid] : 0.0;
-            tmp[tid] -= acc;
+            const int gr = i * kTile + tid;
+            tmp[tid] = (gr < d.n ? bb[gr] : 0.0) - combine(tid);
         }
         __syncthreads();
-        // Linv s: lane r sums DinvT[c][r] s[c] over its quarter of c
-        {
-            const double* it = DinvT + static_cast<size_t>(i) * kTile * kTile;
-            const int r = tid & 63, g = tid >> 6;
-            double s = 0.0;
+        // Linv s: DinvT[c][r] = Linv[r][c]
+        const double* it = DinvT + static_cast<size_t>(i) * kTile * kTile;
+        s = 0.0;
 #pragma unroll
-            for (int c = 0; c < 16; ++c) s += it[(16 * g + c) * kTile + r] * tmp[16 * g + c];
-            part[g * 64 + r] = s;
-            __syncthreads();
-            if (tid < kTile) vec[i * kTile + tid] = (part[tid] + part[64 + tid]) + (part[128 + tid] + part[192 + tid]);
-            __syncthreads();
-        }
+        for (int c = 0; c < kSlice; ++c) s += it[(c0 + c) * kTile + r] * tmp[c0 + c];
+        part[w * kTile + r] = s;
+        __syncthreads();
+        if (tid < kTile) vec[i * kTile + tid] = combine(tid);
+        __syncthreads();
     }
-    // backward: x_i = Linv_ii^T (y_i - sum_{j>i} L_ji^T x_j), in place over vec
+    // backward: x_i = Linv_ii^T (y_i - sum_{j>i} L_ji^T x_j), in place over vec;
+    // L_ji^T is stored in the upper tile (i, j)
     for (int i = nblk - 1; i >= 0; --i) {
-        if (tid < kTile) tmp[tid] = vec[i * kTile + tid];
+        double s = 0.0;
+        const double* row = A + static_cast<size_t>(i * kTile + r) * ld + c0;
+        for (int j = i + 1; j < nblk; ++j) {
+#pragma unroll
+            for (int c = 0; c < kSlice; ++c) s += row[j * kTile + c] * vec[j * kTile + c0 + c];
+        }
+        part[w * kTile + r] = s;
         __syncthreads();
-        if (i + 1 < nblk) {
-            // L_ji^T is stored in the upper tile (i, j)
-            double* accv = part;  // reuse: accumulate into a separate 64-vector
-            for (int j = i + 1; j < nblk; ++j) {
-                const int r = tid & 63, g = tid >> 6;
-                const double* row = A + static_cast<size_t>(i * kTile + r) * ld + j * kTile + 16 * g;
-                double s = 0.0;
+        if (tid < kTile) tmp[tid] = vec[i * kTile + tid] - combine(tid);
+        __syncthreads();
+        // Linv^T s: DinvT[r][c] = Linv[c][r]
+        const double* it = DinvT + static_cast<size_t>(i) * kTile * kTile;
+        s = 0.0;
 #pragma unroll
-                for (int c = 0; c < 16; ++c) s += row[c] * vec[j * kTile + 16 * g + c];
-                accv[g * 64 + r] = s;
-                __syncthreads();
-                if (tid < kTile) tmp[tid] -= (accv[tid] + accv[64 + tid]) + (accv[128 + tid] + accv[192 + tid]);
-                __syncthreads();
-            }
-        }
-        {
-            const double* it = DinvT + static_cast<size_t>(i) * kTile * kTile;
-            const int r = tid & 63, g = tid >> 6;
-            double s = 0.0;
-#pragma unroll
-            for (int c = 0; c < 16; ++c) s += it[r * kTile + 16 * g + c] * tmp[16 * g + c];
-            part[g * 64 + r] = s;
-            __syncthreads();
-            if (tid < kTile) vec[i * kTile + tid] = (part[tid] + part[64 + tid]) + (part[128 + tid] + part[192 + tid]);
-            __syncthreads();
-        }
+        for (int c = 0; c < kSlice; ++c) s += it[r * kTile + c0 + c] * tmp[c0 + c];
+        part[w * kTile + r] = s;
+        __syncthreads();
+        if (tid < kTile) vec[i * kTile + tid] = combine(tid);
+        __syncthreads();
     }
-    for (int r = tid; r < d.n; r += 256) dc[9 * static_cast<size_t>(d.c0) + r] = vec[r];
+    for (int q = tid; q < d.n; q += kBatchThreads) dc[9 * static_cast<size_t>(d.c0) + q] = vec[q];
 }
 
 // Forward solve L y = b.  Workgroup with ticket t owns tile row i = t.
@@ -788,7 +773,7 @@ void launch_cholesky(hipStream_t st, double* S64, float* S32, int ld, int nblk, 
 void launch_batch_chol_solve(hipStream_t st, int count, const BatchDesc* desc, double* S, double* D,
                              const double* rhs, double* dc, TrialFlags* flags, const int* active) {
     if (count > 0)
-        hipLaunchKernelGGL(k_batch_chol_solve, dim3(count), dim3(256), 0, st, desc, S, D, rhs, dc, flags,
+        hipLaunchKernelGGL(k_batch_chol_solve, dim3(count), dim3(kBatchThreads), 0, st, desc, S, D, rhs, dc, flags,
                            active);
 }
 
