@@ -151,7 +151,7 @@ BatchSolver::BatchSolver(const gba_problem_view* probs, int count) : B_(count) {
     tcams_.alloc(9 * static_cast<size_t>(C));
     pts_.alloc(3 * static_cast<size_t>(P));
     tpts_.alloc(3 * static_cast<size_t>(P));
-    W_.alloc(27 * static_cast<size_t>(O));
+    W_.alloc(kWStride * static_cast<size_t>(O));
     VB_.alloc(9 * static_cast<size_t>(P));
     Vinv_.alloc(6 * static_cast<size_t>(P));
     U_.alloc(81 * static_cast<size_t>(C));

@@ -21,6 +21,10 @@ namespace gba_amd {
 constexpr int kWave = 64;
 constexpr int kCamChunk = 256;  // camera-pass edges per wave
 constexpr int kTile = 64;       // dense factor tile edge
+// W_e (9x3 row-major, 27 values) is stored with a stride of 28 values, so
+// every record starts on a 16-byte boundary and moves as 16-byte vectors
+// (14 for FP64, 7 for FP32); the 28th value is zero padding.
+constexpr int kWStride = 28;
 
 struct DevProblem {
     int C, P, O;
@@ -63,6 +67,64 @@ __device__ __forceinline__ double2 obs_px(const DevProblem& pb, int e, int c) {
         return make_double2(static_cast<double>(hx) * s, static_cast<double>(hy) * s);
     }
     return pb.spx[e];
+}
+
+// The 27 values of W record e, widened to FP64, read as 16-byte vectors.
+__device__ __forceinline__ void load_w(const double* __restrict__ W, size_t e, double (&w)[27]) {
+    const double2* v = reinterpret_cast<const double2*>(W + kWStride * e);
+#pragma unroll
+    for (int k = 0; k < 13; ++k) {
+        const double2 t = v[k];
+        w[2 * k] = t.x;
+        w[2 * k + 1] = t.y;
+    }
+    w[26] = v[13].x;
+}
+__device__ __forceinline__ void load_w(const float* __restrict__ W, size_t e, double (&w)[27]) {
+    const float4* v = reinterpret_cast<const float4*>(W + kWStride * e);
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+        const float4 t = v[k];
+        w[4 * k] = t.x;
+        w[4 * k + 1] = t.y;
+        w[4 * k + 2] = t.z;
+        if (4 * k + 3 < 27) w[4 * k + 3] = t.w;
+    }
+}
+// W record e in its stored type (28 values, the last one padding).
+__device__ __forceinline__ void load_w_raw(const double* __restrict__ W, size_t e, double (&w)[28]) {
+    const double2* v = reinterpret_cast<const double2*>(W + kWStride * e);
+#pragma unroll
+    for (int k = 0; k < 14; ++k) {
+        const double2 t = v[k];
+        w[2 * k] = t.x;
+        w[2 * k + 1] = t.y;
+    }
+}
+__device__ __forceinline__ void load_w_raw(const float* __restrict__ W, size_t e, float (&w)[28]) {
+    const float4* v = reinterpret_cast<const float4*>(W + kWStride * e);
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+        const float4 t = v[k];
+        w[4 * k] = t.x;
+        w[4 * k + 1] = t.y;
+        w[4 * k + 2] = t.z;
+        w[4 * k + 3] = t.w;
+    }
+}
+// Stores the 27 values (and the zero padding) of W record e as 16-byte vectors.
+__device__ __forceinline__ void store_w_rec(double* __restrict__ W, size_t e, const double (&w)[27]) {
+    double2* v = reinterpret_cast<double2*>(W + kWStride * e);
+#pragma unroll
+    for (int k = 0; k < 13; ++k) v[k] = make_double2(w[2 * k], w[2 * k + 1]);
+    v[13] = make_double2(w[26], 0.0);
+}
+__device__ __forceinline__ void store_w_rec(float* __restrict__ W, size_t e, const double (&w)[27]) {
+    float4* v = reinterpret_cast<float4*>(W + kWStride * e);
+#pragma unroll
+    for (int k = 0; k < 7; ++k)
+        v[k] = make_float4(static_cast<float>(w[4 * k]), static_cast<float>(w[4 * k + 1]),
+                           static_cast<float>(w[4 * k + 2]), 4 * k + 3 < 27 ? static_cast<float>(w[4 * k + 3]) : 0.0f);
 }
 
 __device__ __forceinline__ int prob_of_pt(const DevProblem& pb, int p) { return pb.pt_prob ? pb.pt_prob[p] : 0; }

@@ -28,14 +28,16 @@ __device__ __forceinline__ bool trial_skipped(const TrialFlags* f) {
 }
 
 template <typename WT>
-__device__ __forceinline__ void wt_dc(const WT* We, const double* dcc, double (&m)[3]) {
+__device__ __forceinline__ void wt_dc(const WT* W, int e, const double* dcc, double (&m)[3]) {
+    double w[27];
+    load_w(W, static_cast<size_t>(e), w);
     m[0] = m[1] = m[2] = 0.0;
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
         const double d = dcc[i];
-        m[0] += static_cast<double>(We[3 * i]) * d;
-        m[1] += static_cast<double>(We[3 * i + 1]) * d;
-        m[2] += static_cast<double>(We[3 * i + 2]) * d;
+        m[0] += w[3 * i] * d;
+        m[1] += w[3 * i + 1] * d;
+        m[2] += w[3 * i + 2] * d;
     }
 }
 
@@ -82,7 +84,7 @@ __global__ __launch_bounds__(256) void k_backsub(DevProblem pb, const WT* __rest
         if (active) {
             key = pb.spt[e];
             c = pb.scam[e];
-            wt_dc(W + 27 * static_cast<size_t>(e), dc + 9 * c, m);
+            wt_dc(W, e, dc + 9 * c, m);
         }
         seg_allreduce<3>(m, key);
         const int prev = __shfl_up(key, 1, kWave);
@@ -112,7 +114,7 @@ __global__ __launch_bounds__(256) void k_backsub(DevProblem pb, const WT* __rest
             const int e = base + lane;
             if (e < rg.y) {
                 double t[3];
-                wt_dc(W + 27 * static_cast<size_t>(e), dc + 9 * pb.scam[e], t);
+                wt_dc(W, e, dc + 9 * pb.scam[e], t);
                 m[0] += t[0];
                 m[1] += t[1];
                 m[2] += t[2];
@@ -172,7 +174,7 @@ __global__ __launch_bounds__(256) void k_point_step(DevProblem pb, const WT* __r
         const int e = rg.x + lane;
         if (e < rg.y) {
             key = pb.spt[e];
-            wt_dc(W + 27 * static_cast<size_t>(e), dc + 9 * pb.scam[e], m);
+            wt_dc(W, e, dc + 9 * pb.scam[e], m);
         }
         seg_allreduce<3>(m, key);
     } else {
@@ -181,7 +183,7 @@ __global__ __launch_bounds__(256) void k_point_step(DevProblem pb, const WT* __r
             const int e = base + lane;
             if (e < rg.y) {
                 double t[3];
-                wt_dc(W + 27 * static_cast<size_t>(e), dc + 9 * pb.scam[e], t);
+                wt_dc(W, e, dc + 9 * pb.scam[e], t);
                 m[0] += t[0];
                 m[1] += t[1];
                 m[2] += t[2];

@@ -20,12 +20,13 @@ namespace gba_amd {
 namespace {
 
 template <typename WT>
-__device__ __forceinline__ void store_w(WT* We, const double* Jc, const double* Jp, bool ok) {
+__device__ __forceinline__ void store_w(WT* W, size_t e, const double* Jc, const double* Jp, bool ok) {
+    double w[27];
 #pragma unroll
     for (int i = 0; i < 9; ++i)
 #pragma unroll
-        for (int j = 0; j < 3; ++j)
-            We[3 * i + j] = ok ? static_cast<WT>(Jc[i] * Jp[j] + Jc[9 + i] * Jp[3 + j]) : WT(0);
+        for (int j = 0; j < 3; ++j) w[3 * i + j] = ok ? Jc[i] * Jp[j] + Jc[9 + i] * Jp[3 + j] : 0.0;
+    store_w_rec(W, e, w);
 }
 
 __device__ __forceinline__ void point_terms(const double* r, const double* Jp, bool ok,
@@ -72,7 +73,7 @@ __global__ __launch_bounds__(256) void k_lin_points(DevProblem pb, const double*
             const double2 o = obs_px(pb, e, c);
             double r[2], Jc[18], Jp[6];
             const bool ok = snavely_eval<true>(cams + 9 * c, pts + 3 * key, o.x, o.y, r, Jc, Jp);
-            store_w(W + 27 * static_cast<size_t>(e), Jc, Jp, ok);
+            store_w(W, static_cast<size_t>(e), Jc, Jp, ok);
             point_terms(r, Jp, ok, acc);
             if (ok) sse = r[0] * r[0] + r[1] * r[1];
             else anom = 1;
@@ -96,7 +97,7 @@ __global__ __launch_bounds__(256) void k_lin_points(DevProblem pb, const double*
                 const double2 o = obs_px(pb, e, c);
                 double r[2], Jc[18], Jp[6], t[9];
                 const bool ok = snavely_eval<true>(cams + 9 * c, pts + 3 * p, o.x, o.y, r, Jc, Jp);
-                store_w(W + 27 * static_cast<size_t>(e), Jc, Jp, ok);
+                store_w(W, static_cast<size_t>(e), Jc, Jp, ok);
                 point_terms(r, Jp, ok, t);
 #pragma unroll
                 for (int i = 0; i < 9; ++i) acc[i] += t[i];

@@ -60,8 +60,24 @@ __global__ __launch_bounds__(NT) void k_schur(
         const int b = pb.scam[f];
         const double* vi = Vinv6 + 6 * static_cast<size_t>(p);
         const double v00 = vi[0], v10 = vi[1], v11 = vi[2], v20 = vi[3], v21 = vi[4], v22 = vi[5];
-        const WT* We = W + 27 * static_cast<size_t>(e);
-        const WT* Wf = W + 27 * static_cast<size_t>(f);
+        // Z = V'^-1 W_f^T, column j = V'^-1 (row j of W_f)
+        const AT a00 = static_cast<AT>(v00), a10 = static_cast<AT>(v10), a11 = static_cast<AT>(v11),
+                 a20 = static_cast<AT>(v20), a21 = static_cast<AT>(v21), a22 = static_cast<AT>(v22);
+        AT z0[9], z1[9], z2[9];
+        {
+            WT wf[28];
+            load_w_raw(W, static_cast<size_t>(f), wf);
+#pragma unroll
+            for (int j = 0; j < 9; ++j) {
+                const AT w0 = static_cast<AT>(wf[3 * j]), w1 = static_cast<AT>(wf[3 * j + 1]),
+                         w2 = static_cast<AT>(wf[3 * j + 2]);
+                z0[j] = a00 * w0 + a10 * w1 + a20 * w2;
+                z1[j] = a10 * w0 + a11 * w1 + a21 * w2;
+                z2[j] = a20 * w0 + a21 * w1 + a22 * w2;
+            }
+        }
+        // W_e rows are read where used: the lanes of an e group share them
+        const WT* we = W + kWStride * static_cast<size_t>(e);
         if (own_rhs && e == f) {
             // T_e bp = W_e (V'^-1 bp), in FP64
             const double* bpp = VB + 9 * static_cast<size_t>(p) + 6;
@@ -70,27 +86,14 @@ __global__ __launch_bounds__(NT) void k_schur(
             const double vb2 = v20 * bpp[0] + v21 * bpp[1] + v22 * bpp[2];
 #pragma unroll
             for (int i = 0; i < 9; ++i)
-                racc[i] += static_cast<double>(We[3 * i]) * vb0 +
-                           static_cast<double>(We[3 * i + 1]) * vb1 +
-                           static_cast<double>(We[3 * i + 2]) * vb2;
-        }
-        // Z = V'^-1 W_f^T, column j = V'^-1 (row j of W_f)
-        const AT a00 = static_cast<AT>(v00), a10 = static_cast<AT>(v10), a11 = static_cast<AT>(v11),
-                 a20 = static_cast<AT>(v20), a21 = static_cast<AT>(v21), a22 = static_cast<AT>(v22);
-        AT z0[9], z1[9], z2[9];
-#pragma unroll
-        for (int j = 0; j < 9; ++j) {
-            const AT w0 = static_cast<AT>(Wf[3 * j]), w1 = static_cast<AT>(Wf[3 * j + 1]),
-                     w2 = static_cast<AT>(Wf[3 * j + 2]);
-            z0[j] = a00 * w0 + a10 * w1 + a20 * w2;
-            z1[j] = a10 * w0 + a11 * w1 + a21 * w2;
-            z2[j] = a20 * w0 + a21 * w1 + a22 * w2;
+                racc[i] += static_cast<double>(we[3 * i]) * vb0 + static_cast<double>(we[3 * i + 1]) * vb1 +
+                           static_cast<double>(we[3 * i + 2]) * vb2;
         }
         AT* dst = acc + (b - lo) * 81;
 #pragma unroll
         for (int i = 0; i < 9; ++i) {
-            const AT x0 = static_cast<AT>(We[3 * i]), x1 = static_cast<AT>(We[3 * i + 1]),
-                     x2 = static_cast<AT>(We[3 * i + 2]);
+            const AT x0 = static_cast<AT>(we[3 * i]), x1 = static_cast<AT>(we[3 * i + 1]),
+                     x2 = static_cast<AT>(we[3 * i + 2]);
 #pragma unroll
             for (int j = 0; j < 9; ++j) atomicAdd(dst + 9 * i + j, -(x0 * z0[j] + x1 * z1[j] + x2 * z2[j]));
         }
@@ -170,10 +173,8 @@ __global__ __launch_bounds__(256) void k_schur_diag(
         const double vb0 = v00 * bpp[0] + v10 * bpp[1] + v20 * bpp[2];
         const double vb1 = v10 * bpp[0] + v11 * bpp[1] + v21 * bpp[2];
         const double vb2 = v20 * bpp[0] + v21 * bpp[1] + v22 * bpp[2];
-        const WT* We = W + 27 * static_cast<size_t>(e);
         double wv[27];
-#pragma unroll
-        for (int k = 0; k < 27; ++k) wv[k] = static_cast<double>(We[k]);
+        load_w(W, static_cast<size_t>(e), wv);
 #pragma unroll
         for (int i = 0; i < 9; ++i) {
             const double t0 = wv[3 * i] * v00 + wv[3 * i + 1] * v10 + wv[3 * i + 2] * v20;

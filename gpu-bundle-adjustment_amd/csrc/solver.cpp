@@ -338,8 +338,8 @@ Solver::Solver(const gba_problem_view& prob, const gba_solver_config& cfg) : cfg
         cams_[i].alloc(9 * static_cast<size_t>(C));
         pts_[i].alloc(3 * static_cast<size_t>(P));
     }
-    if (mixed_) W32_.alloc(27 * static_cast<size_t>(O));
-    else W64_.alloc(27 * static_cast<size_t>(O));
+    if (mixed_) W32_.alloc(kWStride * static_cast<size_t>(O));
+    else W64_.alloc(kWStride * static_cast<size_t>(O));
     VB_.alloc(9 * static_cast<size_t>(P));
     Vinv_.alloc(6 * static_cast<size_t>(P));
     U_.alloc(81 * static_cast<size_t>(C));
@@ -546,16 +546,17 @@ void Solver::download_linearization(double* U, double* V, double* W, double* bc,
         }
     }
     if (W) {
-        std::vector<double> hW(27 * O);
+        // device records are kWStride values apart (27 used)
+        std::vector<double> hW(kWStride * O);
         if (mixed_) {
-            std::vector<float> hw(27 * O);
+            std::vector<float> hw(kWStride * O);
             GBA_HIP(hipMemcpy(hw.data(), W32_.p, sizeof(float) * hw.size(), hipMemcpyDeviceToHost));
             for (size_t i = 0; i < hw.size(); ++i) hW[i] = hw[i];
         } else {
             GBA_HIP(hipMemcpy(hW.data(), W64_.p, sizeof(double) * hW.size(), hipMemcpyDeviceToHost));
         }
         for (size_t k = 0; k < O; ++k)
-            std::memcpy(W + 27 * static_cast<size_t>(hperm[k]), &hW[27 * k], 27 * sizeof(double));
+            std::memcpy(W + 27 * static_cast<size_t>(hperm[k]), &hW[kWStride * k], 27 * sizeof(double));
     }
 }
 
