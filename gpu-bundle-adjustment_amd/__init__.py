@@ -51,7 +51,7 @@ class LMResultC(ctypes.Structure):
 class SolverConfigC(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("refine_tol", ctypes.c_double),
                 ("refine_max_iters", ctypes.c_int), ("trace", _dp), ("trace_capacity", ctypes.c_int),
-                ("obs_storage", ctypes.c_int)]
+                ("obs_storage", ctypes.c_int), ("huber_delta", ctypes.c_double)]
 
 
 class SynthSpecC(ctypes.Structure):
@@ -89,6 +89,10 @@ _PROTOS = {
     "gba_schur_step": (ctypes.c_int, [ctypes.POINTER(ProblemView), ctypes.c_int, ctypes.c_double, _dp,
                                       _dp, _dp, _ip, _dp, _dp]),
     "gba_encode_obs_fp16": (ctypes.c_int, [ctypes.POINTER(ProblemView), ctypes.c_void_p, ctypes.c_void_p]),
+    "gba_linearize_cfg": (ctypes.c_int, [ctypes.POINTER(ProblemView), ctypes.POINTER(SolverConfigC), _dp, _dp,
+                                         _dp, _dp, _dp, _dp, ctypes.POINTER(ctypes.c_ulonglong)]),
+    "gba_cost_cfg": (ctypes.c_int, [ctypes.POINTER(ProblemView), ctypes.POINTER(SolverConfigC), _dp,
+                                    ctypes.POINTER(ctypes.c_ulonglong)]),
     "gba_cost": (ctypes.c_int, [ctypes.POINTER(ProblemView), _dp, ctypes.POINTER(ctypes.c_ulonglong)]),
     "gba_dense_spd_solve": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _dp, _dp, _ip, _ip]),
     "gba_batch_chol9_factor_solve": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
@@ -223,11 +227,13 @@ class LMResult:
                         r.refine_iterations, r.max_refine_residual, trace)
 
 
-def _config(precision, trace_capacity=0, refine_tol=1e-10, refine_max_iters=12, obs_storage=OBS_FP64):
+def _config(precision, trace_capacity=0, refine_tol=1e-10, refine_max_iters=12, obs_storage=OBS_FP64,
+            huber_delta=0.0):
     c = SolverConfigC()
     lib().gba_default_config(ctypes.byref(c))
     c.precision = precision
     c.obs_storage = obs_storage
+    c.huber_delta = huber_delta
     c.refine_tol = refine_tol
     c.refine_max_iters = refine_max_iters
     trace = None
@@ -245,10 +251,12 @@ def _trim_trace(trace):
     return trace[keep]
 
 
-def solve_bal_gpu(prob, opts=None, precision=PRECISION_FP64, trace_capacity=1024, obs_storage=OBS_FP64):
-    """gba::solve_bal_gpu: updates prob.cameras / prob.points in place."""
+def solve_bal_gpu(prob, opts=None, precision=PRECISION_FP64, trace_capacity=1024, obs_storage=OBS_FP64,
+                  huber_delta=0.0):
+    """gba::solve_bal_gpu: updates prob.cameras / prob.points in place.
+    huber_delta > 0 minimises sum rho(|r|^2) / 2 with the Huber loss."""
     opts = opts or LMOptions()
-    cfg, trace = _config(precision, trace_capacity, obs_storage=obs_storage)
+    cfg, trace = _config(precision, trace_capacity, obs_storage=obs_storage, huber_delta=huber_delta)
     res = LMResultC()
     v = prob.view()
     _check(lib().gba_solve_bal(ctypes.byref(v), ctypes.byref(opts.c()), ctypes.byref(cfg),
@@ -259,8 +267,10 @@ def solve_bal_gpu(prob, opts=None, precision=PRECISION_FP64, trace_capacity=1024
 class Solver:
     """Device-resident solver handle: upload once, solve repeatedly."""
 
-    def __init__(self, prob, precision=PRECISION_FP64, trace_capacity=1024, obs_storage=OBS_FP64):
-        self._cfg, self._trace = _config(precision, trace_capacity, obs_storage=obs_storage)
+    def __init__(self, prob, precision=PRECISION_FP64, trace_capacity=1024, obs_storage=OBS_FP64,
+                 huber_delta=0.0):
+        self._cfg, self._trace = _config(precision, trace_capacity, obs_storage=obs_storage,
+                                         huber_delta=huber_delta)
         self._h = ctypes.c_void_p()
         self._prob = prob
         v = prob.view()
@@ -353,16 +363,22 @@ def solve_bal_batched(probs, opts=None):
     return [LMResult.from_c(r) for r in res]
 
 
-def linearize(prob, precision=PRECISION_FP64):
+def linearize(prob, precision=PRECISION_FP64, huber_delta=0.0):
     """GPU linearisation in the reference layout: U (C,9,9 lower), V (P,3,3 lower),
-    W (O,9,3), bc (C,9), bp (P,3), sse, anomalies."""
+    W (O,9,3), bc (C,9), bp (P,3), sse, anomalies.  huber_delta > 0: IRLS-weighted
+    by the Huber loss, sse = sum rho(|r|^2)."""
     C, P, O = prob.num_cameras, prob.num_points, prob.num_observations
     U, V, W = np.zeros((C, 9, 9)), np.zeros((P, 3, 3)), np.zeros((O, 9, 3))
     bc, bp = np.zeros((C, 9)), np.zeros((P, 3))
     sse, an = ctypes.c_double(), ctypes.c_ulonglong()
     v = prob.view()
-    _check(lib().gba_linearize(ctypes.byref(v), precision, _ptr(U), _ptr(V), _ptr(W), _ptr(bc), _ptr(bp),
-                               ctypes.byref(sse), ctypes.byref(an)))
+    if huber_delta:
+        cfg, _ = _config(precision, huber_delta=huber_delta)
+        _check(lib().gba_linearize_cfg(ctypes.byref(v), ctypes.byref(cfg), _ptr(U), _ptr(V), _ptr(W), _ptr(bc),
+                                       _ptr(bp), ctypes.byref(sse), ctypes.byref(an)))
+    else:
+        _check(lib().gba_linearize(ctypes.byref(v), precision, _ptr(U), _ptr(V), _ptr(W), _ptr(bc), _ptr(bp),
+                                   ctypes.byref(sse), ctypes.byref(an)))
     return dict(U=U, V=V, W=W, bc=bc, bp=bp, sse=sse.value, anomalies=an.value)
 
 
@@ -390,10 +406,16 @@ def encode_obs_fp16(prob):
     return scale, px16
 
 
-def cost(prob):
+def cost(prob, huber_delta=0.0):
+    """(sse, anomalies) at the problem's parameters; with huber_delta > 0 the
+    first value is sum rho(|r|^2)."""
     sse, an = ctypes.c_double(), ctypes.c_ulonglong()
     v = prob.view()
-    _check(lib().gba_cost(ctypes.byref(v), ctypes.byref(sse), ctypes.byref(an)))
+    if huber_delta:
+        cfg, _ = _config(PRECISION_FP64, huber_delta=huber_delta)
+        _check(lib().gba_cost_cfg(ctypes.byref(v), ctypes.byref(cfg), ctypes.byref(sse), ctypes.byref(an)))
+    else:
+        _check(lib().gba_cost(ctypes.byref(v), ctypes.byref(sse), ctypes.byref(an)))
     return sse.value, an.value
 
 

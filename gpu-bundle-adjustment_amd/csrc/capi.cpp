@@ -244,10 +244,23 @@ int gba_encode_obs_fp16(const gba_problem_view* prob, float* cam_scale, unsigned
 }
 
 int gba_cost(const gba_problem_view* prob, double* sse, unsigned long long* anomalies) {
+    return gba_cost_cfg(prob, nullptr, sse, anomalies);
+}
+
+int gba_linearize_cfg(const gba_problem_view* prob, const gba_solver_config* cfg, double* U, double* V,
+                      double* W, double* bc, double* bp, double* sse, unsigned long long* anomalies) {
     return guarded([&] {
-        gba_solver_config c;
-        gba_default_config(&c);
-        Solver s(*prob, c);
+        Solver s(*prob, config_or_default(cfg));
+        s.linearize(sse, anomalies);
+        s.download_linearization(U, V, W, bc, bp);
+        return 0;
+    });
+}
+
+int gba_cost_cfg(const gba_problem_view* prob, const gba_solver_config* cfg, double* sse,
+                 unsigned long long* anomalies) {
+    return guarded([&] {
+        Solver s(*prob, config_or_default(cfg));
         s.cost(sse, anomalies);
         return 0;
     });

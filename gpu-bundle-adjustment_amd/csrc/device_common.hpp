@@ -53,7 +53,36 @@ struct DevProblem {
     // scale of each camera.
     const unsigned* spx16;
     const float* cam_scale;
+    // Huber loss threshold on the residual norm (<= 0: plain least squares).
+    double huber;
 };
+
+// Huber loss on the squared residual norm s = |r|^2, as the reference's
+// Ceres harness applies HuberLoss(delta) (benchmarks/bench_ceres.cpp:120-131):
+// rho(s) = s for s <= delta^2, else 2 delta sqrt(s) - delta^2; the cost is
+// sum rho / 2.  The linearisation scales r and J by sqrt(rho'(s)) (IRLS);
+// the second-order correction of a robustified Gauss-Newton step vanishes
+// where rho'' <= 0, which holds for Huber everywhere.
+__device__ __forceinline__ double robust_rho(double s, double delta) {
+    return (delta > 0.0 && s > delta * delta) ? 2.0 * delta * sqrt(s) - delta * delta : s;
+}
+__device__ __forceinline__ double robust_sqrt_drho(double s, double delta) {
+    return (delta > 0.0 && s > delta * delta) ? sqrt(delta / sqrt(s)) : 1.0;
+}
+// Scales the residual and the Jacobian rows (Jc 2x9, Jp 2x3; either may be
+// null) of one observation for the loss; returns rho(|r|^2).
+__device__ __forceinline__ double robustify(double delta, double* r, double* Jc, double* Jp) {
+    const double s = r[0] * r[0] + r[1] * r[1];
+    if (!(delta > 0.0) || s <= delta * delta) return s;
+    const double w = robust_sqrt_drho(s, delta);
+    r[0] *= w;
+    r[1] *= w;
+    if (Jc)
+        for (int k = 0; k < 18; ++k) Jc[k] *= w;
+    if (Jp)
+        for (int k = 0; k < 6; ++k) Jp[k] *= w;
+    return robust_rho(s, delta);
+}
 
 // Observed pixel of sorted observation e (camera c) in FP64.  With FP16
 // storage the value is (double)half * (double)scale: an 11-bit times a 24-bit

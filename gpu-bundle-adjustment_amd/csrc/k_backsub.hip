@@ -103,7 +103,7 @@ __global__ __launch_bounds__(256) void k_backsub(DevProblem pb, const WT* __rest
             const double2 o = obs_px(pb, e, c);
             double r[2];
             if (snavely_eval<false>(tcams + 9 * c, Xt, o.x, o.y, r, nullptr, nullptr))
-                sse = r[0] * r[0] + r[1] * r[1];
+                sse = robust_rho(r[0] * r[0] + r[1] * r[1], pb.huber);
             else
                 anom = 1;
         }
@@ -141,7 +141,7 @@ __global__ __launch_bounds__(256) void k_backsub(DevProblem pb, const WT* __rest
                 const double2 o = obs_px(pb, e, c);
                 double r[2];
                 if (snavely_eval<false>(tcams + 9 * c, Xt, o.x, o.y, r, nullptr, nullptr))
-                    sse += r[0] * r[0] + r[1] * r[1];
+                    sse += robust_rho(r[0] * r[0] + r[1] * r[1], pb.huber);
                 else
                     anom += 1;
             }

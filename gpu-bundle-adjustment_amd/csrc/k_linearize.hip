@@ -73,9 +73,10 @@ __global__ __launch_bounds__(256) void k_lin_points(DevProblem pb, const double*
             const double2 o = obs_px(pb, e, c);
             double r[2], Jc[18], Jp[6];
             const bool ok = snavely_eval<true>(cams + 9 * c, pts + 3 * key, o.x, o.y, r, Jc, Jp);
+            const double rho = ok ? robustify(pb.huber, r, Jc, Jp) : 0.0;
             store_w(W, static_cast<size_t>(e), Jc, Jp, ok);
             point_terms(r, Jp, ok, acc);
-            if (ok) sse = r[0] * r[0] + r[1] * r[1];
+            if (ok) sse = rho;
             else anom = 1;
         }
         seg_suffix_sum<9>(acc, key);
@@ -97,11 +98,12 @@ __global__ __launch_bounds__(256) void k_lin_points(DevProblem pb, const double*
                 const double2 o = obs_px(pb, e, c);
                 double r[2], Jc[18], Jp[6], t[9];
                 const bool ok = snavely_eval<true>(cams + 9 * c, pts + 3 * p, o.x, o.y, r, Jc, Jp);
+                const double rho = ok ? robustify(pb.huber, r, Jc, Jp) : 0.0;
                 store_w(W, static_cast<size_t>(e), Jc, Jp, ok);
                 point_terms(r, Jp, ok, t);
 #pragma unroll
                 for (int i = 0; i < 9; ++i) acc[i] += t[i];
-                if (ok) sse += r[0] * r[0] + r[1] * r[1];
+                if (ok) sse += rho;
                 else anom += 1;
             }
         }
@@ -148,6 +150,7 @@ __global__ __launch_bounds__(256) void k_lin_cameras(DevProblem pb, const double
             const double2 o = obs_px(pb, ed, ch.x);
             const int p = pb.spt[ed];
             const bool ok = snavely_eval<true>(cam, pts + 3 * p, o.x, o.y, r, Jc, Jp);
+            if (ok) robustify(pb.huber, r, Jc, nullptr);
             if (!ok) {
                 r[0] = r[1] = 0.0;
 #pragma unroll
@@ -253,7 +256,7 @@ __global__ __launch_bounds__(256) void k_cost(DevProblem pb, const double* __res
         const double2 o = obs_px(pb, e, c);
         double r[2];
         if (snavely_eval<false>(cams + 9 * c, pts + 3 * p, o.x, o.y, r, nullptr, nullptr))
-            sse = r[0] * r[0] + r[1] * r[1];
+            sse = robust_rho(r[0] * r[0] + r[1] * r[1], pb.huber);
         else
             an = 1;
     }

@@ -254,6 +254,9 @@ Solver::Solver(const gba_problem_view& prob, const gba_solver_config& cfg) : cfg
         if (prob.obs_camera[e] < 0 || prob.obs_camera[e] >= C || prob.obs_point[e] < 0 ||
             prob.obs_point[e] >= P)
             throw Error("observation index out of range");
+    if (!(cfg.huber_delta >= 0.0) || !std::isfinite(cfg.huber_delta))
+        throw Error("huber_delta must be finite and >= 0");
+    pb_.huber = cfg.huber_delta;
     pb_.C = C;
     pb_.P = P;
     pb_.O = O;

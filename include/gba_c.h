@@ -91,6 +91,12 @@ typedef struct gba_solver_config {
     double* trace;          /* optional, GBA_TRACE_STRIDE doubles per trial */
     int trace_capacity;     /* trials that fit in trace */
     int obs_storage;        /* enum gba_obs_storage (default FP64) */
+    double huber_delta;     /* > 0: Huber loss on |r|^2 with this threshold, as the
+                               reference's Ceres harness applies HuberLoss(delta)
+                               (benchmarks/bench_ceres.cpp:120-141): cost = sum rho / 2,
+                               rho(s) = s (s <= delta^2) or 2 delta sqrt(s) - delta^2,
+                               IRLS-weighted linearisation.  0 (default): plain least
+                               squares, as solve_bal_gpu. */
 } gba_solver_config;
 
 /* Trace record per LM trial: iteration, lambda used, trial cost, rho,
@@ -188,6 +194,14 @@ int gba_schur_step(const gba_problem_view* prob, int precision, double lambda, d
 
 /* Residual-only SSE at the given parameters (reference src/lm.cu:255-281). */
 int gba_cost(const gba_problem_view* prob, double* sse, unsigned long long* anomalies);
+
+/* gba_linearize / gba_cost under a solver configuration (precision, pixel
+ * storage, loss); with a Huber loss *sse is sum rho(|r|^2) and U, V, W, bc,
+ * bp come from the IRLS-weighted residuals and Jacobians.  cfg may be NULL. */
+int gba_linearize_cfg(const gba_problem_view* prob, const gba_solver_config* cfg, double* U, double* V,
+                      double* W, double* bc, double* bp, double* sse, unsigned long long* anomalies);
+int gba_cost_cfg(const gba_problem_view* prob, const gba_solver_config* cfg, double* sse,
+                 unsigned long long* anomalies);
 
 /* Dense SPD factor + solve of an n x n column-major matrix on the GPU with
  * the solver's blocked MFMA Cholesky (replaces Eigen::LLT at

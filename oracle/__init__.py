@@ -29,7 +29,7 @@ TERMINATION = {0: "CONVERGENCE", 1: "MAX_ITERATIONS", 2: "FAILURE"}
 class Options(ctypes.Structure):
     _fields_ = [("max_iterations", ctypes.c_int), ("initial_lambda", ctypes.c_double),
                 ("rel_decrease_tol", ctypes.c_double), ("max_inner_trials", ctypes.c_int),
-                ("verbose", ctypes.c_int)]
+                ("verbose", ctypes.c_int), ("huber_delta", ctypes.c_double)]
 
 
 class Result(ctypes.Structure):
@@ -63,6 +63,10 @@ def lib():
         L.orc_linearize.argtypes = [ctypes.c_int] * 3 + [_dp, _dp, _ip, _ip, _dp] + [_dp] * 6 + [_ullp, ctypes.c_int]
         L.orc_cost.restype = ctypes.c_double
         L.orc_cost.argtypes = [ctypes.c_int, _dp, _dp, _ip, _ip, _dp, _ullp, ctypes.c_int]
+        L.orc_linearize_loss.argtypes = ([ctypes.c_int] * 3 + [_dp, _dp, _ip, _ip, _dp, ctypes.c_double] +
+                                         [_dp] * 6 + [_ullp, ctypes.c_int])
+        L.orc_cost_loss.restype = ctypes.c_double
+        L.orc_cost_loss.argtypes = [ctypes.c_int, _dp, _dp, _ip, _ip, _dp, ctypes.c_double, _ullp, ctypes.c_int]
         L.orc_chol9_factor.restype = ctypes.c_int
         L.orc_chol9_factor.argtypes = [_dp, _dp]
         L.orc_chol9_solve.argtypes = [_dp, _dp, _dp]
@@ -141,22 +145,23 @@ def total_squared_error(prob):
                                          _p(op, ctypes.c_int), _p(px))
 
 
-def linearize(prob, nthreads=8):
+def linearize(prob, nthreads=8, huber_delta=0.0):
     cams, pts, oc, op, px = _arrs(prob)
     C, P, O = cams.shape[0], pts.shape[0], len(oc)
     U, V, W = np.zeros((C, 9, 9)), np.zeros((P, 3, 3)), np.zeros((O, 9, 3))
     bc, bp = np.zeros((C, 9)), np.zeros((P, 3))
     sse, an = ctypes.c_double(), ctypes.c_ulonglong()
-    lib().orc_linearize(C, P, O, _p(cams), _p(pts), _p(oc, ctypes.c_int), _p(op, ctypes.c_int), _p(px),
-                        _p(U), _p(V), _p(W), _p(bc), _p(bp), ctypes.byref(sse), ctypes.byref(an), nthreads)
+    lib().orc_linearize_loss(C, P, O, _p(cams), _p(pts), _p(oc, ctypes.c_int), _p(op, ctypes.c_int), _p(px),
+                             huber_delta, _p(U), _p(V), _p(W), _p(bc), _p(bp), ctypes.byref(sse),
+                             ctypes.byref(an), nthreads)
     return dict(U=U, V=V, W=W, bc=bc, bp=bp, sse=sse.value, anomalies=an.value)
 
 
-def cost(prob, nthreads=8):
+def cost(prob, nthreads=8, huber_delta=0.0):
     cams, pts, oc, op, px = _arrs(prob)
     an = ctypes.c_ulonglong()
-    s = lib().orc_cost(len(oc), _p(cams), _p(pts), _p(oc, ctypes.c_int), _p(op, ctypes.c_int), _p(px),
-                       ctypes.byref(an), nthreads)
+    s = lib().orc_cost_loss(len(oc), _p(cams), _p(pts), _p(oc, ctypes.c_int), _p(op, ctypes.c_int), _p(px),
+                            huber_delta, ctypes.byref(an), nthreads)
     return s, an.value
 
 
@@ -217,10 +222,11 @@ def schur_step(prob, lin, lam, nthreads=8, want_S=False):
 
 
 def lm_solve(prob, max_iterations=50, initial_lambda=1e-3, rel_decrease_tol=1e-6,
-             max_inner_trials=16, verbose=False, nthreads=8, max_trace=4096):
+             max_inner_trials=16, verbose=False, nthreads=8, max_trace=4096, huber_delta=0.0):
     """Runs the restated reference LM; updates prob.cameras / prob.points in place."""
     cams, pts, oc, op, px = _arrs(prob)
-    opts = Options(max_iterations, initial_lambda, rel_decrease_tol, max_inner_trials, int(verbose))
+    opts = Options(max_iterations, initial_lambda, rel_decrease_tol, max_inner_trials, int(verbose),
+                   huber_delta)
     res = Result()
     trace = np.zeros((max_trace, 5))
     ntr = ctypes.c_int()

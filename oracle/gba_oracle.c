@@ -255,14 +255,31 @@ void orc_edge_residual_jacobian(const double* cam, const double* X, const double
 /* linearisation (src/lm.cu:205-252) and cost (:255-281)               */
 /* ------------------------------------------------------------------ */
 
+/* Huber loss on s = |r|^2, as the reference's Ceres harness applies
+ * HuberLoss(delta) (benchmarks/bench_ceres.cpp:120-131): rho(s) = s for
+ * s <= delta^2, else 2 delta sqrt(s) - delta^2; delta <= 0 means none.  The
+ * robustified Gauss-Newton model (Triggs, McLauchlan, Hartley, Fitzgibbon,
+ * "Bundle Adjustment - A Modern Synthesis", 2000, sec. 4.3) scales r and J by
+ * sqrt(rho'(s)) plus a rank-one curvature term that vanishes when
+ * rho''(s) <= 0; for Huber rho'' is 0 inside and negative outside, so the
+ * model is the plain IRLS scaling. */
+static double huber_rho(double s, double delta) {
+    return (delta > 0.0 && s > delta * delta) ? 2.0 * delta * sqrt(s) - delta * delta : s;
+}
+static double huber_sqrt_drho(double s, double delta) {
+    return (delta > 0.0 && s > delta * delta) ? sqrt(delta / sqrt(s)) : 1.0;
+}
+
 typedef struct {
     int C, P, O;
     const double *cams, *pts, *px;
     const int *oc, *op;
-    double* J;         /* 26 per edge: r(2) Jc(18) Jp(6) */
+    double* J;         /* 26 per edge: r(2) Jc(18) Jp(6), loss-weighted */
+    double* rho;       /* rho(|r|^2) per edge */
     unsigned char* ok; /* finite flag per edge */
     double *U, *V, *W, *bc, *bp;
     const int *coff, *cidx, *poff, *pidx;
+    double huber;
 } lin_ctx;
 
 static void lin_edges(void* vctx, int t, int nt) {
@@ -278,6 +295,12 @@ static void lin_edges(void* vctx, int t, int nt) {
         for (int j = 0; j < 18; ++j) fin = fin && isfinite(Jc[j]);
         for (int j = 0; j < 6; ++j) fin = fin && isfinite(Jp[j]);
         c->ok[k] = (unsigned char)fin;
+        if (fin) {
+            const double sq = r[0] * r[0] + r[1] * r[1];
+            const double w = huber_sqrt_drho(sq, c->huber);
+            c->rho[k] = huber_rho(sq, c->huber);
+            for (int j = 0; j < 26; ++j) Jk[j] *= w;
+        }
         double* We = c->W + 27 * k;
         if (!fin) { /* the reference leaves W of a skipped edge stale (lm.cu:222-225);
                        here it is zero, so the edge contributes nothing (lm.cu:19-21) */
@@ -332,8 +355,17 @@ static void lin_pts(void* vctx, int t, int nt) {
 void orc_linearize(int C, int P, int O, const double* cams, const double* pts, const int* oc,
                    const int* op, const double* px, double* U, double* V, double* W, double* bc,
                    double* bp, double* sse, unsigned long long* anomalies, int nthreads) {
+    orc_linearize_loss(C, P, O, cams, pts, oc, op, px, 0.0, U, V, W, bc, bp, sse, anomalies, nthreads);
+}
+
+void orc_linearize_loss(int C, int P, int O, const double* cams, const double* pts, const int* oc,
+                        const int* op, const double* px, double huber_delta, double* U, double* V,
+                        double* W, double* bc, double* bp, double* sse, unsigned long long* anomalies,
+                        int nthreads) {
     if (nthreads < 1) nthreads = 1;
     lin_ctx c;
+    c.huber = huber_delta;
+    c.rho = (double*)malloc(sizeof(double) * (size_t)(O > 0 ? O : 1));
     c.C = C;
     c.P = P;
     c.O = O;
@@ -370,11 +402,11 @@ void orc_linearize(int C, int P, int O, const double* cams, const double* pts, c
             ++an;
             continue;
         }
-        const double* r = c.J + 26 * k;
-        s += r[0] * r[0] + r[1] * r[1];
+        s += c.rho[k];
     }
     *sse = s;
     *anomalies = an;
+    free(c.rho);
     free(c.J);
     free(c.ok);
     free(coff);
@@ -387,7 +419,8 @@ typedef struct {
     int O;
     const double *cams, *pts, *px;
     const int *oc, *op;
-    double* r2; /* per-edge squared residual, NaN marks anomaly */
+    double* r2; /* per-edge rho(squared residual), NaN marks anomaly */
+    double huber;
 } cost_ctx;
 
 static void cost_edges(void* vctx, int t, int nt) {
@@ -397,13 +430,19 @@ static void cost_edges(void* vctx, int t, int nt) {
     for (long k = b; k < e; ++k) {
         double r[2];
         orc_edge_residual(c->cams + 9 * (size_t)c->oc[k], c->pts + 3 * (size_t)c->op[k], c->px + 2 * k, r);
-        c->r2[k] = (isfinite(r[0]) && isfinite(r[1])) ? r[0] * r[0] + r[1] * r[1] : NAN;
+        c->r2[k] = (isfinite(r[0]) && isfinite(r[1])) ? huber_rho(r[0] * r[0] + r[1] * r[1], c->huber) : NAN;
     }
 }
 
 double orc_cost(int O, const double* cams, const double* pts, const int* oc, const int* op,
                 const double* px, unsigned long long* anomalies, int nthreads) {
-    cost_ctx c = {O, cams, pts, px, oc, op, (double*)malloc(sizeof(double) * (size_t)(O > 0 ? O : 1))};
+    return orc_cost_loss(O, cams, pts, oc, op, px, 0.0, anomalies, nthreads);
+}
+
+double orc_cost_loss(int O, const double* cams, const double* pts, const int* oc, const int* op,
+                     const double* px, double huber_delta, unsigned long long* anomalies, int nthreads) {
+    cost_ctx c = {O, cams, pts, px, oc, op, (double*)malloc(sizeof(double) * (size_t)(O > 0 ? O : 1)),
+                  huber_delta};
     parallel_run(nthreads < 1 ? 1 : nthreads, cost_edges, &c);
     double s = 0.0;
     unsigned long long an = 0;
@@ -864,7 +903,7 @@ int orc_lm_solve(int C, int P, int O, double* cams, double* pts, const int* oc, 
     unsigned long long counter0 = 0, an = 0;
     double sse = 0.0, ta;
     ta = now_s();
-    orc_linearize(C, P, O, cams, pts, oc, op, px, U, V, W, bc, bp, &sse, &an, hw);
+    orc_linearize_loss(C, P, O, cams, pts, oc, op, px, opts->huber_delta, U, V, W, bc, bp, &sse, &an, hw);
     res->t_linearize += now_s() - ta;
     counter0 += an;
     res->initial_cost = sse / 2.0;
@@ -892,7 +931,7 @@ int orc_lm_solve(int C, int P, int O, double* cams, double* pts, const int* oc, 
                 for (size_t i = 0; i < 9 * sC; ++i) tc[i] = cams[i] + dc[i];
                 for (size_t i = 0; i < 3 * sP; ++i) tp[i] = pts[i] + dp[i];
                 unsigned long long tan_ = 0;
-                double tsse = orc_cost(O, tc, tp, oc, op, px, &tan_, hw);
+                double tsse = orc_cost_loss(O, tc, tp, oc, op, px, opts->huber_delta, &tan_, hw);
                 counter0 += tan_;
                 res->t_trial_cost += now_s() - ta;
                 tcost = tsse / 2.0;
@@ -943,7 +982,8 @@ int orc_lm_solve(int C, int P, int O, double* cams, double* pts, const int* oc, 
          * current cost stays the trial cost (lm.cu:682), not this SSE */
         double lin_sse;
         ta = now_s();
-        orc_linearize(C, P, O, cams, pts, oc, op, px, U, V, W, bc, bp, &lin_sse, &an, hw);
+        orc_linearize_loss(C, P, O, cams, pts, oc, op, px, opts->huber_delta, U, V, W, bc, bp, &lin_sse, &an,
+                           hw);
         res->t_linearize += now_s() - ta;
         counter0 += an;
     }

@@ -40,6 +40,16 @@ void orc_linearize(int C, int P, int O, const double* cams, const double* pts, c
 /* src/lm.cu:255-281: SSE of the residuals, non-finite residuals skipped and counted. */
 double orc_cost(int O, const double* cams, const double* pts, const int* obs_cam, const int* obs_pt,
                 const double* obs_px, unsigned long long* anomalies, int nthreads);
+/* The same two passes with the Huber loss of the reference's Ceres harness
+ * (benchmarks/bench_ceres.cpp:120-141; huber_delta <= 0: none): *sse and the
+ * cost are sum rho(|r|^2); the linearisation uses sqrt(rho') r and
+ * sqrt(rho') J. */
+void orc_linearize_loss(int C, int P, int O, const double* cams, const double* pts, const int* obs_cam,
+                        const int* obs_pt, const double* obs_px, double huber_delta, double* U, double* V,
+                        double* W, double* bc, double* bp, double* sse, unsigned long long* anomalies,
+                        int nthreads);
+double orc_cost_loss(int O, const double* cams, const double* pts, const int* obs_cam, const int* obs_pt,
+                     const double* obs_px, double huber_delta, unsigned long long* anomalies, int nthreads);
 
 /* ---- batched blocks (include/gba/schur.cuh) ------------------------- */
 int orc_chol9_factor(const double* A81, double* L81);                   /* :23-38 */
@@ -79,6 +89,8 @@ typedef struct {
     double rel_decrease_tol; /* :16 */
     int max_inner_trials;    /* :17 */
     int verbose;             /* :18 */
+    double huber_delta;      /* > 0: Huber loss (no counterpart in LMOptions; the
+                                Ceres harness's --loss huber --delta) */
 } orc_options;
 
 enum { ORC_CONVERGENCE = 0, ORC_MAX_ITERATIONS = 1, ORC_FAILURE = 2 };
