@@ -139,6 +139,60 @@ def run_fp16(g, args):
     print(json.dumps(out))
 
 
+def run_bal(g, args):
+    """The reference's bench_bal harness (benchmarks/bench_bal.cpp:63-139) on a
+    BAL file: the reader-vs-solver initial-cost check (exit 2 on a mismatch)
+    and the anomaly check (exit 3), then --pairs interleaved FP64 / mixed
+    solves (the reference times its Ceres / GPU columns in interleaved pairs,
+    docs/RUNGS.md:136-142), each printed as one JSON object in bench_bal's
+    schema plus best / median / worst seconds.  Every solve starts from the
+    file's parameters; total_time_s includes upload and sort, as the
+    reference's does."""
+    import torch
+    base = g.load_bal(args.file)
+    delta = args.delta if args.loss == "huber" else 0.0
+    if args.loss == "huber":
+        own = 0.5 * g.cost(base, huber_delta=delta)[0]
+    else:
+        own = g.total_squared_error(base) / 2.0
+    device = torch.cuda.get_device_name(0)
+    opts = g.LMOptions(max_iterations=args.iterations, initial_lambda=args.lambda0)
+    precs = {"fp64": g.PRECISION_FP64, "mixed": g.PRECISION_MIXED}
+    runs = {k: [] for k in precs}
+    for _ in range(args.pairs):
+        for name, prec in precs.items():
+            p = base.copy()
+            r = g.solve_bal_gpu(p, opts, prec, huber_delta=delta)
+            if abs(r.initial_cost - own) > 1e-9 * max(1.0, abs(own)):
+                print(f"MISMATCH: reader cost {own:.17g} vs solver initial cost {r.initial_cost:.17g}",
+                      file=sys.stderr)
+                sys.exit(2)
+            if r.anomalies != 0:
+                print(f"device anomaly counter is {r.anomalies} -- refusing to report", file=sys.stderr)
+                sys.exit(3)
+            runs[name].append(r)
+    out = []
+    for name, rs in runs.items():
+        r = rs[-1]
+        ts = sorted(x.total_time_s for x in rs)
+        out.append({
+            "file": args.file, "solver": f"gba-amd-lm-{name}", "loss": args.loss,
+            "num_cameras": base.num_cameras, "num_points": base.num_points,
+            "num_observations": base.num_observations,
+            "initial_cost": r.initial_cost, "final_cost": r.final_cost, "iterations": r.iterations,
+            "total_time_s": round(r.total_time_s, 6),
+            "per_iteration_s": round(r.total_time_s / r.iterations, 6) if r.iterations else 0.0,
+            "termination": r.termination, "device": device,
+            "pairs": len(rs), "best_s": round(ts[0], 6), "median_s": round(statistics.median(ts), 6),
+            "worst_s": round(ts[-1], 6),
+        })
+    text = "\n".join(json.dumps(o) for o in out)
+    print(text)
+    if args.json:
+        with open(args.json, "w") as fh:
+            fh.write(text + "\n")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -146,22 +200,32 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", type=int, default=3)
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--lambda0", type=float, default=1e-4)
+    ap.add_argument("--lambda0", type=float, default=None,
+                    help="initial LM damping (default 1e-4; 1e-3 in --mode bal, as bench_bal)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-iterations", type=int, default=2)
     ap.add_argument("--kernel-reps", type=int, default=10)
-    ap.add_argument("--mode", choices=["solve", "kernels", "batch", "fp16"], default="solve",
+    ap.add_argument("--file", help="BAL problem for --mode bal")
+    ap.add_argument("--iterations", type=int, default=50, help="--mode bal: max LM iterations")
+    ap.add_argument("--json", help="--mode bal: also write the JSON objects here")
+    ap.add_argument("--loss", choices=["none", "huber"], default="none", help="--mode bal")
+    ap.add_argument("--delta", type=float, default=1.0, help="--mode bal: Huber threshold")
+    ap.add_argument("--pairs", type=int, default=3, help="--mode bal: interleaved FP64 / mixed pairs")
+    ap.add_argument("--mode", choices=["solve", "kernels", "batch", "fp16", "bal"], default="solve",
                     help="kernels: only the per-kernel timing loops plus a 512 MiB device copy "
                          "(short run for rocprofv3 counter passes); batch: the batched solver on "
                          "--batch problems of preset 5 (problems/s) against sequential FP64 solves; "
-                         "fp16: preset 4 with FP16 pixel storage, final-cost error vs FP64")
+                         "fp16: preset 4 with FP16 pixel storage, final-cost error vs FP64; "
+                         "bal: the reference's bench_bal harness on --file (interleaved FP64 / mixed pairs)")
     ap.add_argument("--batch", type=int, default=512, help="problems in --mode batch")
     ap.add_argument("--seq-sample", type=int, default=32,
                     help="problems solved one by one for the sequential rate in --mode batch")
     args = ap.parse_args()
+    if args.lambda0 is None:
+        args.lambda0 = 1e-3 if args.mode == "bal" else 1e-4
     if args.gpus != 1:
-        print(json.dumps({"error": "this solver runs one problem on one GPU; multi-GPU sharding is "
-                                   "design-only (see DESIGN.md), run with --gpus 1"}))
+        print(json.dumps({"error": "this solver runs on one GPU; a multi-GPU split is not built, "
+                                   "run with --gpus 1"}))
         sys.exit(2)
 
     import torch
@@ -172,6 +236,11 @@ def main():
         return
     if args.mode == "fp16":
         run_fp16(g, args)
+        return
+    if args.mode == "bal":
+        if not args.file:
+            ap.error("--mode bal needs --file")
+        run_bal(g, args)
         return
     prob = g.generate(args.config, args.seed)
     C, P, O = prob.num_cameras, prob.num_points, prob.num_observations
