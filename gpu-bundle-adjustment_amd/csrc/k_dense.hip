@@ -34,8 +34,8 @@ namespace {
 
 // Cholesky factor and inverse of the symmetric tile in a (lower triangle
 // read), 256 threads or more (the trailing blocks are dealt to every
-// wavefront, the inverse stages use the first four).  On return a holds L (lower; the upper part is stale) and
-// b holds L^-1 (lower, zero above the diagonal inside the 16x16 diagonal
+// wavefront, the inverse stages use the first four).  On return a holds L
+// (lower; the upper part is stale) and b holds L^-1 (lower, zero above the diagonal inside the 16x16 diagonal
 // blocks; the other upper blocks are scratch).  Blocked by 16 columns:
 //   panel  : wavefront 0, one row per lane, 16 pivot steps; the pivot and the
 //            entries of the scaled column are broadcast with v_readlane, so a
@@ -329,6 +329,9 @@ __global__ __launch_bounds__(256, 2) void k_chol_update(T* __restrict__ S, int l
 // pivot sets flags[problem].chol_fail.  npad <= 1024.
 constexpr int kBatchThreads = 512;
 constexpr int kBatchWaves = kBatchThreads / 64;
+// work unit of the TRSM / update: a strip of 16 kRB rows of one 64x64 tile
+constexpr int kRB = kBatchWaves >= 16 ? 1 : 2;
+constexpr int kStrips = 4 / kRB;
 
 __global__ __launch_bounds__(kBatchThreads) void k_batch_chol_solve(const BatchDesc* __restrict__ desc,
                                                           double* __restrict__ S,
@@ -361,32 +364,32 @@ __global__ __launch_bounds__(kBatchThreads) void k_batch_chol_solve(const BatchD
         __syncthreads();
         if (!ok) return;
         const double* inv = DinvT + static_cast<size_t>(k) * kTile * kTile;
-        // TRSM: wavefront w takes every kBatchWaves-th half tile (32 rows) of
-        // the tiles below: L_ik = A_ik Linv^T (2x4 MFMA blocks), written below
-        // the diagonal and, transposed, into the upper tile (k, i)
-        for (int h = w; h < 2 * (nblk - k - 1); h += kBatchWaves) {
-            const int i = k + 1 + (h >> 1), r0 = 32 * (h & 1);
+        // TRSM: wavefront w takes every kBatchWaves-th row strip (16 kRB rows)
+        // of the tiles below: L_ik = A_ik Linv^T (kRB x 4 MFMA blocks), written
+        // below the diagonal and, transposed, into the upper tile (k, i)
+        for (int h = w; h < kStrips * (nblk - k - 1); h += kBatchWaves) {
+            const int i = k + 1 + h / kStrips, r0 = 16 * kRB * (h % kStrips);
             double* Aik = A + static_cast<size_t>(i * kTile) * ld + k * kTile;
             double* LTki = A + static_cast<size_t>(k * kTile) * ld + i * kTile;
-            acc_t acc[2][4];
+            acc_t acc[kRB][4];
 #pragma unroll
-            for (int r = 0; r < 2; ++r)
+            for (int r = 0; r < kRB; ++r)
 #pragma unroll
                 for (int c = 0; c < 4; ++c) acc[r][c] = acc_t{0, 0, 0, 0};
 #pragma unroll 2
             for (int kk = 0; kk < kTile; kk += 4) {
-                double av[2], bv[4];
+                double av[kRB], bv[4];
 #pragma unroll
-                for (int r = 0; r < 2; ++r) av[r] = Aik[static_cast<size_t>(r0 + 16 * r + li) * ld + kk + lk];
+                for (int r = 0; r < kRB; ++r) av[r] = Aik[static_cast<size_t>(r0 + 16 * r + li) * ld + kk + lk];
 #pragma unroll
                 for (int c = 0; c < 4; ++c) bv[c] = inv[(kk + lk) * kTile + 16 * c + li];
 #pragma unroll
-                for (int r = 0; r < 2; ++r)
+                for (int r = 0; r < kRB; ++r)
 #pragma unroll
                     for (int c = 0; c < 4; ++c) acc[r][c] = mfma16(av[r], bv[c], acc[r][c]);
             }
 #pragma unroll
-            for (int r = 0; r < 2; ++r)
+            for (int r = 0; r < kRB; ++r)
 #pragma unroll
                 for (int c = 0; c < 4; ++c)
 #pragma unroll
@@ -397,39 +400,39 @@ __global__ __launch_bounds__(kBatchThreads) void k_batch_chol_solve(const BatchD
                     }
         }
         __syncthreads();
-        // trailing update: wavefront w takes every kBatchWaves-th half tile of
+        // trailing update: wavefront w takes every kBatchWaves-th row strip of
         // the tiles (i, j), k < j <= i; operands from the transposed tiles
         {
             const int m = nblk - k - 1;
-            for (int h = w; h < m * (m + 1); h += kBatchWaves) {
-                const int t = h >> 1, r0 = 32 * (h & 1);
+            for (int h = w; h < kStrips * (m * (m + 1) / 2); h += kBatchWaves) {
+                const int t = h / kStrips, r0 = 16 * kRB * (h % kStrips);
                 int ii = 0;
                 while ((ii + 1) * (ii + 2) / 2 <= t) ++ii;
                 const int jj = t - ii * (ii + 1) / 2;
                 const int i = k + 1 + ii, j = k + 1 + jj;
                 const double* LTi = A + static_cast<size_t>(k * kTile) * ld + i * kTile;
                 const double* LTj = A + static_cast<size_t>(k * kTile) * ld + j * kTile;
-                acc_t acc[2][4];
+                acc_t acc[kRB][4];
 #pragma unroll
-                for (int r = 0; r < 2; ++r)
+                for (int r = 0; r < kRB; ++r)
 #pragma unroll
                     for (int c = 0; c < 4; ++c) acc[r][c] = acc_t{0, 0, 0, 0};
 #pragma unroll 2
                 for (int kk = 0; kk < kTile; kk += 4) {
                     const size_t ro = static_cast<size_t>(kk + lk) * ld;
-                    double av[2], bv[4];
+                    double av[kRB], bv[4];
 #pragma unroll
-                    for (int r = 0; r < 2; ++r) av[r] = LTi[ro + r0 + 16 * r + li];
+                    for (int r = 0; r < kRB; ++r) av[r] = LTi[ro + r0 + 16 * r + li];
 #pragma unroll
                     for (int c = 0; c < 4; ++c) bv[c] = LTj[ro + 16 * c + li];
 #pragma unroll
-                    for (int r = 0; r < 2; ++r)
+                    for (int r = 0; r < kRB; ++r)
 #pragma unroll
                         for (int c = 0; c < 4; ++c) acc[r][c] = mfma16(av[r], bv[c], acc[r][c]);
                 }
                 double* Aij = A + static_cast<size_t>(i * kTile) * ld + j * kTile;
 #pragma unroll
-                for (int r = 0; r < 2; ++r)
+                for (int r = 0; r < kRB; ++r)
 #pragma unroll
                     for (int c = 0; c < 4; ++c)
 #pragma unroll
@@ -457,6 +460,7 @@ __global__ __launch_bounds__(kBatchThreads) void k_batch_chol_solve(const BatchD
     for (int i = 0; i < nblk; ++i) {
         double s = 0.0;
         const double* row = A + static_cast<size_t>(i * kTile + r) * ld + c0;
+#pragma unroll 4
         for (int j = 0; j < i; ++j) {
 #pragma unroll
             for (int c = 0; c < kSlice; ++c) s += row[j * kTile + c] * vec[j * kTile + c0 + c];
@@ -483,6 +487,7 @@ __global__ __launch_bounds__(kBatchThreads) void k_batch_chol_solve(const BatchD
     for (int i = nblk - 1; i >= 0; --i) {
         double s = 0.0;
         const double* row = A + static_cast<size_t>(i * kTile + r) * ld + c0;
+#pragma unroll 4
         for (int j = i + 1; j < nblk; ++j) {
 #pragma unroll
             for (int c = 0; c < kSlice; ++c) s += row[j * kTile + c] * vec[j * kTile + c0 + c];
