@@ -106,22 +106,31 @@ __global__ __launch_bounds__(256) void k_trinv_diag(const T* __restrict__ DinvT,
 }
 
 // T_ij = sum_{j <= k < s+h} L_ik X_kj for the merges of level h.
+// kStage k steps share one staging round (their A tiles in LDS at once),
+// which divides the barriers and multiplies the loads in flight of these
+// latency-bound chains.
+constexpr int kStage = 2;
 template <typename T>
 __global__ __launch_bounds__(256) void k_trinv_lx(const T* __restrict__ L, const T* __restrict__ X,
                                                   T* __restrict__ Tb, int ld, int nblk, int h,
                                                   const TrialFlags* flags) {
-    __shared__ T sA[kTile][kTile + 1];
+    __shared__ T sA[kStage][kTile][kTile + 1];
     if (flags->nonspd || flags->chol_fail) return;
     int s, i, j;
     merge_tile(h, s, i, j);
     if (i >= nblk) return;
     typename Acc4<T>::type acc[2][2];
     acc_zero<T>(acc);
-    for (int k = j; k < s + h; ++k) {
+    for (int k = j; k < s + h; k += kStage) {
+        const int n = min(kStage, s + h - k);
         __syncthreads();
-        stage_tile<T>(sA, L + static_cast<size_t>(i * kTile) * ld + k * kTile, ld, false);
+#pragma unroll
+        for (int u = 0; u < kStage; ++u)
+            if (u < n) stage_tile<T>(sA[u], L + static_cast<size_t>(i * kTile) * ld + (k + u) * kTile, ld, false);
         __syncthreads();
-        tile_mfma_acc<T>(acc, sA, X + static_cast<size_t>(k * kTile) * ld + j * kTile, ld, k == j);
+        for (int u = 0; u < n; ++u)
+            tile_mfma_acc<T>(acc, sA[u], X + static_cast<size_t>((k + u) * kTile) * ld + j * kTile, ld,
+                             k + u == j);
     }
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
@@ -142,18 +151,23 @@ __global__ __launch_bounds__(256) void k_trinv_lx(const T* __restrict__ L, const
 template <typename T>
 __global__ __launch_bounds__(256) void k_trinv_xt(T* __restrict__ X, const T* __restrict__ Tb,
                                                   int ld, int nblk, int h, const TrialFlags* flags) {
-    __shared__ T sA[kTile][kTile + 1];
+    __shared__ T sA[kStage][kTile][kTile + 1];
     if (flags->nonspd || flags->chol_fail) return;
     int s, i, j;
     merge_tile(h, s, i, j);
     if (i >= nblk) return;
     typename Acc4<T>::type acc[2][2];
     acc_zero<T>(acc);
-    for (int k = s + h; k <= i; ++k) {
+    for (int k = s + h; k <= i; k += kStage) {
+        const int n = min(kStage, i + 1 - k);
         __syncthreads();
-        stage_tile<T>(sA, X + static_cast<size_t>(i * kTile) * ld + k * kTile, ld, k == i);
+#pragma unroll
+        for (int u = 0; u < kStage; ++u)
+            if (u < n)
+                stage_tile<T>(sA[u], X + static_cast<size_t>(i * kTile) * ld + (k + u) * kTile, ld, k + u == i);
         __syncthreads();
-        tile_mfma_acc<T>(acc, sA, Tb + static_cast<size_t>(k * kTile) * ld + j * kTile, ld, false);
+        for (int u = 0; u < n; ++u)
+            tile_mfma_acc<T>(acc, sA[u], Tb + static_cast<size_t>((k + u) * kTile) * ld + j * kTile, ld, false);
     }
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
@@ -164,7 +178,7 @@ __global__ __launch_bounds__(256) void k_trinv_xt(T* __restrict__ X, const T* __
 #pragma unroll
         for (int c = 0; c < 2; ++c)
 #pragma unroll
-            for (int q = 0; q < 4; ++q) sA[wr + 16 * r + acc_row<T>(lk, q)][wc + 16 * c + li] = -acc[r][c][q];
+            for (int q = 0; q < 4; ++q) sA[0][wr + 16 * r + acc_row<T>(lk, q)][wc + 16 * c + li] = -acc[r][c][q];
     __syncthreads();
     T* low = X + static_cast<size_t>(i * kTile) * ld + j * kTile;
     T* up = X + static_cast<size_t>(j * kTile) * ld + i * kTile;
@@ -172,8 +186,8 @@ __global__ __launch_bounds__(256) void k_trinv_xt(T* __restrict__ X, const T* __
     for (int u = 0; u < 16; ++u) {
         const int q = threadIdx.x + 256 * u;
         const int r = q >> 6, c = q & 63;
-        low[static_cast<size_t>(r) * ld + c] = sA[r][c];
-        up[static_cast<size_t>(r) * ld + c] = sA[c][r];
+        low[static_cast<size_t>(r) * ld + c] = sA[0][r][c];
+        up[static_cast<size_t>(r) * ld + c] = sA[0][c][r];
     }
 }
 
