@@ -199,6 +199,7 @@ struct PassStatus {
     double dot[2];     // refinement dot products (mixed precision)
     unsigned long long anomalies;  // non-finite edges in this pass
     TrialFlags flags;
+    int pcg_stop;      // refinement: the last step found p.q or r.z not > 0
 };
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }

@@ -654,15 +654,21 @@ __global__ __launch_bounds__(256) void k_symv_tiles(const float* __restrict__ A,
     }
 }
 
-// y_b = sum_{j <= b} rp[(b, j)] + sum_{i > b} cp[(i, b)].
-__global__ __launch_bounds__(64) void k_symv_reduce(int nblk, const double* __restrict__ rp,
-                                                    const double* __restrict__ cp,
-                                                    double* __restrict__ y) {
-    const int b = blockIdx.x, lane = threadIdx.x;
+// y_b = sum_{j <= b} rp[(b, j)] + sum_{i > b} cp[(i, b)]: wavefront g sums
+// the partials with index = g mod 4, the four sums are combined in order.
+__global__ __launch_bounds__(256) void k_symv_reduce(int nblk, const double* __restrict__ rp,
+                                                     const double* __restrict__ cp,
+                                                     double* __restrict__ y) {
+    __shared__ double s_part[4][kTile];
+    const int b = blockIdx.x, lane = threadIdx.x & 63, g = threadIdx.x >> 6;
     double s = 0.0;
-    for (int j = 0; j <= b; ++j) s += rp[static_cast<size_t>(b * (b + 1) / 2 + j) * kTile + lane];
-    for (int i = b; i < nblk; ++i) s += cp[static_cast<size_t>(i * (i + 1) / 2 + b) * kTile + lane];
-    y[b * kTile + lane] = s;
+#pragma unroll 4
+    for (int j = g; j <= b; j += 4) s += rp[static_cast<size_t>(b * (b + 1) / 2 + j) * kTile + lane];
+#pragma unroll 4
+    for (int i = b + g; i < nblk; i += 4) s += cp[static_cast<size_t>(i * (i + 1) / 2 + b) * kTile + lane];
+    s_part[g][lane] = s;
+    __syncthreads();
+    if (g == 0) y[b * kTile + lane] = (s_part[0][lane] + s_part[1][lane]) + (s_part[2][lane] + s_part[3][lane]);
 }
 
 // Per-workgroup partials of a.b and c.d (FP64).
@@ -690,8 +696,16 @@ __global__ __launch_bounds__(256) void k_dot2(int n, const double* __restrict__ 
     }
 }
 
-__global__ __launch_bounds__(256) void k_dot2_reduce(int nb, const double* __restrict__ blk,
-                                                     PassStatus* status, const TrialFlags* flags) {
+// Refinement scalars kept on the device, so a PCG step needs one host
+// synchronisation (for the convergence test) instead of two:
+// sc[0] = r.z, sc[1] = alpha, sc[2] = beta, sc[3] = stop (p.q or r.z not > 0).
+//   mode 0: s0 = r.z, s1 = b.b of the start; status gets both.
+//   mode 1: s0 = p.q -> alpha = r.z / p.q (0 and stop when either is not > 0).
+//   mode 2: s0 = r.z (new), s1 = r.r; status gets both and the stop flag;
+//           unless stopped, beta = r.z (new) / r.z and r.z is replaced.
+__global__ __launch_bounds__(256) void k_pcg_reduce(int nb, const double* __restrict__ blk, int mode,
+                                                    double* __restrict__ sc, PassStatus* status,
+                                                    const TrialFlags* flags) {
     __shared__ double s0[256], s1[256];
     const int tid = threadIdx.x;
     double u = 0.0, v = 0.0;
@@ -709,26 +723,43 @@ __global__ __launch_bounds__(256) void k_dot2_reduce(int nb, const double* __res
         }
         __syncthreads();
     }
-    if (tid == 0) {
-        status->dot[0] = s0[0];
-        status->dot[1] = s1[0];
-        status->flags = *flags;
+    if (tid != 0) return;
+    const double a = s0[0], c = s1[0];
+    if (mode == 0) {
+        sc[0] = a;
+        sc[3] = 0.0;
+    } else if (mode == 1) {
+        const bool bad = !(a > 0.0) || !(sc[0] > 0.0);
+        sc[1] = bad ? 0.0 : sc[0] / a;
+        if (bad) sc[3] = 1.0;
+        return;
+    } else if (sc[3] == 0.0) {
+        sc[2] = a / sc[0];
+        sc[0] = a;
     }
+    status->dot[0] = a;
+    status->dot[1] = c;
+    status->flags = *flags;
+    status->pcg_stop = sc[3] != 0.0 ? 1 : 0;
 }
 
-// x += alpha p ; r -= alpha q
-__global__ void k_pcg_update(int n, double alpha, double* __restrict__ x, const double* __restrict__ p,
-                             double* __restrict__ r, const double* __restrict__ q) {
+// x += alpha p ; r -= alpha q  (alpha = sc[1])
+__global__ void k_pcg_update(int n, const double* __restrict__ sc, double* __restrict__ x,
+                             const double* __restrict__ p, double* __restrict__ r,
+                             const double* __restrict__ q) {
     const int i = blockIdx.x * 256 + threadIdx.x;
+    const double alpha = sc[1];
     if (i < n) {
         x[i] += alpha * p[i];
         r[i] -= alpha * q[i];
     }
 }
 
-// p = z + beta p
-__global__ void k_pcg_direction(int n, double beta, const double* __restrict__ z, double* __restrict__ p) {
+// p = z + beta p  (beta = sc[2])
+__global__ void k_pcg_direction(int n, const double* __restrict__ sc, const double* __restrict__ z,
+                                double* __restrict__ p) {
     const int i = blockIdx.x * 256 + threadIdx.x;
+    const double beta = sc[2];
     if (i < n) p[i] = z[i] + beta * p[i];
 }
 
@@ -817,23 +848,24 @@ void launch_symv(hipStream_t st, const float* A, int ld, int npad, const double*
     double* rp = work;
     double* cp = work + static_cast<size_t>(tiles) * kTile;
     hipLaunchKernelGGL(k_symv_tiles, dim3(tiles), dim3(256), 0, st, A, ld, x, rp, cp);
-    hipLaunchKernelGGL(k_symv_reduce, dim3(nblk), dim3(64), 0, st, nblk, rp, cp, y);
+    hipLaunchKernelGGL(k_symv_reduce, dim3(nblk), dim3(256), 0, st, nblk, rp, cp, y);
 }
 
-void launch_dot2(hipStream_t st, int n, const double* a, const double* b, const double* c,
-                 const double* d, double* blk, PassStatus* status, const TrialFlags* flags) {
+void launch_pcg_dot(hipStream_t st, int n, const double* a, const double* b, const double* c,
+                    const double* d, double* blk, int mode, double* sc, PassStatus* status,
+                    const TrialFlags* flags) {
     const int grid = residual_partials(n);
     hipLaunchKernelGGL(k_dot2, dim3(grid), dim3(256), 0, st, n, a, b, c, d, blk);
-    hipLaunchKernelGGL(k_dot2_reduce, dim3(1), dim3(256), 0, st, grid, blk, status, flags);
+    hipLaunchKernelGGL(k_pcg_reduce, dim3(1), dim3(256), 0, st, grid, blk, mode, sc, status, flags);
 }
 
-void launch_pcg_update(hipStream_t st, int n, double alpha, double* x, const double* p, double* r,
+void launch_pcg_update(hipStream_t st, int n, const double* sc, double* x, const double* p, double* r,
                        const double* q) {
-    hipLaunchKernelGGL(k_pcg_update, dim3((n + 255) / 256), dim3(256), 0, st, n, alpha, x, p, r, q);
+    hipLaunchKernelGGL(k_pcg_update, dim3((n + 255) / 256), dim3(256), 0, st, n, sc, x, p, r, q);
 }
 
-void launch_pcg_direction(hipStream_t st, int n, double beta, const double* z, double* p) {
-    hipLaunchKernelGGL(k_pcg_direction, dim3((n + 255) / 256), dim3(256), 0, st, n, beta, z, p);
+void launch_pcg_direction(hipStream_t st, int n, const double* sc, const double* z, double* p) {
+    hipLaunchKernelGGL(k_pcg_direction, dim3((n + 255) / 256), dim3(256), 0, st, n, sc, z, p);
 }
 
 void launch_vec_copy(hipStream_t st, int n, const double* a, double* b) {

@@ -227,18 +227,25 @@ __global__ __launch_bounds__(256) void k_tile_colmv(const T* __restrict__ X, int
             (s_part[0][threadIdx.x] + s_part[1][threadIdx.x]) + (s_part[2][threadIdx.x] + s_part[3][threadIdx.x]);
 }
 
-// out_b = sum of the partials of output block b in a fixed order.
-__global__ __launch_bounds__(64) void k_tile_colmv_reduce(int nblk, int mode,
-                                                          const double* __restrict__ part,
-                                                          double* __restrict__ out) {
-    const int b = blockIdx.x, lane = threadIdx.x;
+// out_b = sum of the partials of output block b in a fixed order: wavefront
+// g sums the partials with index = g mod 4, the four sums are combined in
+// order.
+__global__ __launch_bounds__(256) void k_tile_colmv_reduce(int nblk, int mode,
+                                                           const double* __restrict__ part,
+                                                           double* __restrict__ out) {
+    __shared__ double s_part[4][kTile];
+    const int b = blockIdx.x, lane = threadIdx.x & 63, g = threadIdx.x >> 6;
     double s = 0.0;
     if (mode == 0) {
-        for (int j = 0; j <= b; ++j) s += part[static_cast<size_t>(b * (b + 1) / 2 + j) * kTile + lane];
+#pragma unroll 4
+        for (int j = g; j <= b; j += 4) s += part[static_cast<size_t>(b * (b + 1) / 2 + j) * kTile + lane];
     } else {
-        for (int i = b; i < nblk; ++i) s += part[static_cast<size_t>(i * (i + 1) / 2 + b) * kTile + lane];
+#pragma unroll 4
+        for (int i = b + g; i < nblk; i += 4) s += part[static_cast<size_t>(i * (i + 1) / 2 + b) * kTile + lane];
     }
-    out[b * kTile + lane] = s;
+    s_part[g][lane] = s;
+    __syncthreads();
+    if (g == 0) out[b * kTile + lane] = (s_part[0][lane] + s_part[1][lane]) + (s_part[2][lane] + s_part[3][lane]);
 }
 
 }  // namespace
@@ -268,9 +275,9 @@ void launch_precond_apply(hipStream_t st, const float* X32, int ld, int npad, co
     double* part = work;
     double* y = work + static_cast<size_t>(tiles) * kTile;
     hipLaunchKernelGGL(k_tile_colmv<float>, dim3(tiles), dim3(256), 0, st, X32, ld, 0, r, part);
-    hipLaunchKernelGGL(k_tile_colmv_reduce, dim3(nblk), dim3(64), 0, st, nblk, 0, part, y);
+    hipLaunchKernelGGL(k_tile_colmv_reduce, dim3(nblk), dim3(256), 0, st, nblk, 0, part, y);
     hipLaunchKernelGGL(k_tile_colmv<float>, dim3(tiles), dim3(256), 0, st, X32, ld, 1, y, part);
-    hipLaunchKernelGGL(k_tile_colmv_reduce, dim3(nblk), dim3(64), 0, st, nblk, 1, part, z);
+    hipLaunchKernelGGL(k_tile_colmv_reduce, dim3(nblk), dim3(256), 0, st, nblk, 1, part, z);
 }
 
 }  // namespace gba_amd

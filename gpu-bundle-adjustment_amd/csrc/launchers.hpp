@@ -77,12 +77,15 @@ size_t symv_work(int npad);
 void launch_symv(hipStream_t st, const float* A, int ld, int npad, const double* x, double* y,
                  double* work);
 int residual_partials(int npad);
-// status->dot[0] = a.b, status->dot[1] = c.d; blk holds 2 residual_partials(n) doubles.
-void launch_dot2(hipStream_t st, int n, const double* a, const double* b, const double* c,
-                 const double* d, double* blk, PassStatus* status, const TrialFlags* flags);
-void launch_pcg_update(hipStream_t st, int n, double alpha, double* x, const double* p, double* r,
+// blk holds 2 residual_partials(n) doubles (launch_pcg_dot).
+// Refinement dot products with device-side scalars sc[4] (see k_pcg_reduce):
+// mode 0 start (r.z, b.b), 1 alpha from p.q, 2 beta from the new r.z and r.r.
+void launch_pcg_dot(hipStream_t st, int n, const double* a, const double* b, const double* c,
+                    const double* d, double* blk, int mode, double* sc, PassStatus* status,
+                    const TrialFlags* flags);
+void launch_pcg_update(hipStream_t st, int n, const double* sc, double* x, const double* p, double* r,
                        const double* q);
-void launch_pcg_direction(hipStream_t st, int n, double beta, const double* z, double* p);
+void launch_pcg_direction(hipStream_t st, int n, const double* sc, const double* z, double* p);
 void launch_vec_copy(hipStream_t st, int n, const double* a, double* b);
 void launch_vec_axpy(hipStream_t st, int n, double* x, const double* d);
 void launch_vec_scale(hipStream_t st, int n, int npad, const double* in, const double* s,

@@ -124,6 +124,7 @@ int pcg_solve(hipStream_t st, const float* A, const float* X32, int ld, int npad
     double* z = work + np;
     double* p = work + 2 * np;
     double* q = work + 3 * np;
+    double* sc = work + 4 * np;  // device scalars (k_pcg_reduce)
     double* sw = work + 5 * np;
     double* blk = sw + std::max(symv_work(npad), precond_work(npad));
     auto sync_status = [&]() {
@@ -135,12 +136,11 @@ int pcg_solve(hipStream_t st, const float* A, const float* X32, int ld, int npad
     launch_vec_copy(st, npad, b, r);
     launch_precond_apply(st, X32, ld, npad, r, z, sw);
     launch_vec_copy(st, npad, z, p);
-    launch_dot2(st, npad, r, z, b, b, blk, status, flags);
+    launch_pcg_dot(st, npad, r, z, b, b, blk, 0, sc, status, flags);
     sync_status();
     *rel = 1.0;
     const TrialFlags f = h_status->flags;
     if (f.nonspd || f.chol_fail || f.solve_err) return 0;
-    double rz = h_status->dot[0];
     const double bb = h_status->dot[1];
     if (!(bb > 0.0)) {
         *rel = 0.0;
@@ -148,24 +148,22 @@ int pcg_solve(hipStream_t st, const float* A, const float* X32, int ld, int npad
     }
     double rr = bb;
     int it = 0;
+    // alpha and beta stay on the device; one synchronisation per step reads
+    // r.r for the stopping test.  A step whose p.q or r.z is not > 0 applies
+    // alpha = 0 (x and r unchanged) and ends the loop uncounted.
     while (it < max_it) {
         launch_symv(st, A, ld, npad, p, q, sw);
-        launch_dot2(st, npad, p, q, p, q, blk, status, flags);
-        sync_status();
-        const double pq = h_status->dot[0];
-        if (!(pq > 0.0) || !(rz > 0.0)) break;
-        const double alpha = rz / pq;
-        launch_pcg_update(st, npad, alpha, x, p, r, q);
+        launch_pcg_dot(st, npad, p, q, p, q, blk, 1, sc, status, flags);
+        launch_pcg_update(st, npad, sc, x, p, r, q);
         launch_precond_apply(st, X32, ld, npad, r, z, sw);
-        launch_dot2(st, npad, r, z, r, r, blk, status, flags);
+        launch_pcg_dot(st, npad, r, z, r, r, blk, 2, sc, status, flags);
         sync_status();
+        if (h_status->pcg_stop) break;
         ++it;
-        const double rz_new = h_status->dot[0];
         rr = h_status->dot[1];
         if (!std::isfinite(rr)) break;
         if (rr <= tol * tol * bb) break;
-        launch_pcg_direction(st, npad, rz_new / rz, z, p);
-        rz = rz_new;
+        launch_pcg_direction(st, npad, sc, z, p);
     }
     *rel = std::sqrt(rr / bb);
     return it;
