@@ -205,26 +205,29 @@ __global__ __launch_bounds__(256) void k_point_step(DevProblem pb, const WT* __r
     }
 }
 
-__global__ __launch_bounds__(256) void k_trial_finalize(DevProblem pb, const double* __restrict__ unit_sse,
+constexpr int kFinThreads = 1024;
+
+__global__ __launch_bounds__(kFinThreads) void k_trial_finalize(DevProblem pb, const double* __restrict__ unit_sse,
                                                         const double* __restrict__ unit_den,
                                                         const unsigned* __restrict__ unit_anom,
                                                         const double* __restrict__ dc,
                                                         const double* __restrict__ U,
                                                         const double* __restrict__ bc, double lambda,
                                                         PassStatus* status, const TrialFlags* flags) {
-    __shared__ double s_a[256], s_b[256], s_c[256];
-    __shared__ unsigned s_n[256];
+    __shared__ double s_a[kFinThreads], s_b[kFinThreads], s_c[kFinThreads];
+    __shared__ unsigned s_n[kFinThreads];
     const int tid = threadIdx.x;
     if (tid == 0) status->flags = *flags;
     if (trial_skipped(flags)) return;
     double sse = 0.0, den = 0.0, cden = 0.0;
     unsigned an = 0;
-    for (int u = tid; u < pb.num_units; u += 256) {
+#pragma unroll 4
+    for (int u = tid; u < pb.num_units; u += kFinThreads) {
         sse += unit_sse[u];
         den += unit_den[u];
         an += unit_anom[u];
     }
-    for (int i = tid; i < pb.n; i += 256) {
+    for (int i = tid; i < pb.n; i += kFinThreads) {
         const double ud = U[81 * static_cast<size_t>(i / 9) + 10 * (i % 9)];
         const double d = dc[i];
         cden += d * (lambda * damp_diag(ud) * d + bc[i]);
@@ -234,7 +237,7 @@ __global__ __launch_bounds__(256) void k_trial_finalize(DevProblem pb, const dou
     s_c[tid] = cden;
     s_n[tid] = an;
     __syncthreads();
-    for (int h = 128; h > 0; h >>= 1) {
+    for (int h = kFinThreads / 2; h > 0; h >>= 1) {
         if (tid < h) {
             s_a[tid] += s_a[tid + h];
             s_b[tid] += s_b[tid + h];
@@ -275,7 +278,7 @@ void launch_trial_finalize(hipStream_t st, const DevProblem& pb, const double* u
                            const double* unit_den, const unsigned* unit_anom, const double* dc,
                            const double* U, const double* bc, double lambda, PassStatus* status,
                            const TrialFlags* flags) {
-    hipLaunchKernelGGL(k_trial_finalize, dim3(1), dim3(256), 0, st, pb, unit_sse, unit_den, unit_anom,
+    hipLaunchKernelGGL(k_trial_finalize, dim3(1), dim3(kFinThreads), 0, st, pb, unit_sse, unit_den, unit_anom,
                        dc, U, bc, lambda, status, flags);
 }
 

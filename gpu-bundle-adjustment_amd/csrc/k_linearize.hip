@@ -202,6 +202,7 @@ __global__ __launch_bounds__(256) void k_lin_reduce(DevProblem pb, const double*
         __shared__ unsigned s_an[256];
         double s = 0.0;
         unsigned a = 0;
+#pragma unroll 8
         for (int u = tid; u < pb.num_units; u += 256) {
             s += unit_sse[u];
             a += unit_anom[u];
