@@ -30,33 +30,28 @@ __device__ __forceinline__ v4f mfma16(float a, float b, v4f c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 // 1/sqrt(d) from the hardware estimate refined by Newton steps
-// r <- r (3 - d r^2) / 2: two steps for FP64, one for FP32 (the estimate
-// carries about 23 bits).  Used for the pivots of the tile factor: the pivot
-// is d r and the column scale is r, with no division in the chain.
+// r <- r (3 - d r^2) / 2: two steps for FP64; none for FP32, whose estimate
+// (v_rsq_f32) is within about one ulp -- the FP32 factor only preconditions
+// an FP64 refinement, and each step removed is four dependent operations on
+// the pivot chain.  Used for the pivots of the tile factor: the pivot is d r
+// and the column scale is r, with no division in the chain.
 __device__ __forceinline__ double rsqrt_refined(double d) {
     double r = __builtin_amdgcn_rsq(d);
     r = r * (1.5 - 0.5 * d * r * r);
     r = r * (1.5 - 0.5 * d * r * r);
     return r;
 }
-__device__ __forceinline__ float rsqrt_refined(float d) {
-    float r = __builtin_amdgcn_rsqf(d);
-    r = r * (1.5f - 0.5f * d * r * r);
-    return r;
-}
+__device__ __forceinline__ float rsqrt_refined(float d) { return __builtin_amdgcn_rsqf(d); }
 
-// 1/x from the hardware estimate refined by Newton steps r <- r (2 - x r).
+// 1/x from the hardware estimate refined by Newton steps r <- r (2 - x r)
+// (FP64: two; FP32: none, v_rcp_f32 is within about one ulp).
 __device__ __forceinline__ double recip_refined(double x) {
     double r = __builtin_amdgcn_rcp(x);
     r = r * (2.0 - x * r);
     r = r * (2.0 - x * r);
     return r;
 }
-__device__ __forceinline__ float recip_refined(float x) {
-    float r = __builtin_amdgcn_rcpf(x);
-    r = r * (2.0f - x * r);
-    return r;
-}
+__device__ __forceinline__ float recip_refined(float x) { return __builtin_amdgcn_rcpf(x); }
 
 template <typename T> struct Acc4;
 template <> struct Acc4<double> { typedef v4d type; };
