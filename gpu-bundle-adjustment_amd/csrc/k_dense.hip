@@ -41,9 +41,9 @@ namespace {
 //            entries of the scaled column are broadcast with v_readlane, so a
 //            step is a chain of register operations (no LDS round trip);
 //   update : the rank-16 update of the trailing triangle in 16x16 MFMA blocks;
-//   inverse: each wavefront inverts one 16x16 diagonal block by Gauss-Jordan
-//            on its rows (lane = row, pivot rows broadcast with v_readlane),
-//            then by block distance d = 1..3 with MFMA:
+//   inverse: each wavefront inverts one 16x16 diagonal block by forward
+//            substitution (lane = column of the inverse, L entries read from
+//            LDS as broadcasts), then by block distance d = 1..3 with MFMA:
 //            Y = sum_{c <= m < r} L_rm Linv_mc, Linv_rc = -Linv_rr Y.
 // Returns false on a pivot that is not > 0.
 template <typename T>
@@ -95,31 +95,25 @@ __device__ __forceinline__ bool tile_potrf_inv(T (*a)[kTile + 2], T (*b)[kTile +
         __syncthreads();
     }
     if (w < 4) {
-        // Gauss-Jordan on the rows of diagonal block w: lane i (mod 16) holds
-        // row i of L and of the growing inverse
-        const int o = 16 * w, i = li;
-        T lrow[16], x[16];
+        // inverse of diagonal block w by forward substitution, one lane per
+        // column c of the inverse (L x = e_c; x_i = 0 for i < c), the entries
+        // of L read from LDS as broadcasts; the lane groups of the wavefront
+        // compute the same columns and the first writes them
+        const int o = 16 * w, c = li;
+        T rd[16];
 #pragma unroll
-        for (int m = 0; m < 16; ++m) {
-            lrow[m] = a[o + i][o + m];
-            x[m] = (i == m) ? T(1) : T(0);
+        for (int i = 0; i < 16; ++i) rd[i] = recip_refined(a[o + i][o + i]);
+        T x[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            T s = (i == c) ? T(1) : T(0);
+#pragma unroll
+            for (int j = 0; j < i; ++j) s -= a[o + i][o + j] * x[j];
+            x[i] = s * rd[i];
         }
+        if (lk == 0) {
 #pragma unroll
-        for (int m = 0; m < 16; ++m) {
-            const T rm = recip_refined(readlane(lrow[m], m));
-            if (i == m) {
-#pragma unroll
-                for (int c = 0; c <= m; ++c) x[c] *= rm;
-            }
-#pragma unroll
-            for (int c = 0; c <= m; ++c) {
-                const T xm = readlane(x[c], m);
-                if (i > m) x[c] -= lrow[m] * xm;
-            }
-        }
-        if (lane < 16) {
-#pragma unroll
-            for (int c = 0; c < 16; ++c) b[o + i][o + c] = (c <= i) ? x[c] : T(0);
+            for (int i = 0; i < 16; ++i) b[o + i][o + c] = (i >= c) ? x[i] : T(0);
         }
     }
     __syncthreads();
