@@ -37,9 +37,10 @@ namespace {
 // wavefront, the inverse stages use the first four).  On return a holds L
 // (lower; the upper part is stale) and b holds L^-1 (lower, zero above the diagonal inside the 16x16 diagonal
 // blocks; the other upper blocks are scratch).  Blocked by 16 columns:
-//   panel  : wavefront 0, one row per lane, 16 pivot steps; the pivot and the
-//            entries of the scaled column are broadcast with v_readlane, so a
-//            step is a chain of register operations (no LDS round trip);
+//   panel  : wavefront 0, one row per lane, 16 pivot steps; the pivot is
+//            broadcast with v_readlane, the scaled column with v_readlane
+//            (FP32) or one LDS store and broadcast reads (FP64, where each
+//            entry would take two v_readlane);
 //   update : the rank-16 update of the trailing triangle in 16x16 MFMA blocks;
 //   inverse: each wavefront inverts one 16x16 diagonal block by forward
 //            substitution (lane = column of the inverse, L entries read from
@@ -52,6 +53,7 @@ __device__ __forceinline__ bool tile_potrf_inv(T (*a)[kTile + 2], T (*b)[kTile +
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
     const int li = lane & 15, lk = lane >> 4;
     __shared__ int s_bad;
+    __shared__ T s_col[16];
     if (tid == 0) s_bad = 0;
     __syncthreads();
     for (int p = 0; p < 4; ++p) {
@@ -68,8 +70,24 @@ __device__ __forceinline__ bool tile_potrf_inv(T (*a)[kTile + 2], T (*b)[kTile +
                 const T rinv = rsqrt_refined(d);
                 const T l = (lane == c0 + jj) ? d * rinv : v[jj] * rinv;
                 v[jj] = l;
+                if constexpr (sizeof(T) == 8) {
+                    // FP64: column jj of the diagonal block to every lane
+                    // through LDS (one store, broadcast reads; LDS operations
+                    // of a wavefront complete in order) -- fewer instructions
+                    // than two v_readlane per entry
+                    if (lane > c0 + jj && lane < c0 + 16) s_col[lane - c0] = l;
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
-                for (int m = jj + 1; m < 16; ++m) v[m] -= l * readlane(l, c0 + m);
+                    for (int m = jj + 1; m < 16; ++m) v[m] -= l * s_col[m];
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                } else {
+#pragma unroll
+                    for (int m = jj + 1; m < 16; ++m) v[m] -= l * readlane(l, c0 + m);
+                }
             }
             if (lane >= c0) {
 #pragma unroll
