@@ -6,7 +6,9 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
+#include <string>
 
 #include "launchers.hpp"
 
@@ -127,6 +129,43 @@ BatchSolver::BatchSolver(const gba_problem_view* probs, int count) : B_(count) {
     upload(spairs_, spairs, st_);
     upload(stasks_, stasks, st_);
     upload(desc_, hdesc_, st_);
+    {
+        // work tables of the tile-parallel factor; at each step the lookahead
+        // tiles (update tile 0, which factors the next diagonal tile) of all
+        // problems come first
+        const char* env = std::getenv("GBA_BATCH_FACTOR");
+        tile_factor_ = !(env && std::string(env) == "workgroup");
+        int nsteps = 0;
+        for (const BatchDesc& d : hdesc_) nsteps = std::max(nsteps, d.npad / kTile);
+        std::vector<int2> potrf, trsm, upd;
+        for (int q = 0; q < B_; ++q) potrf.push_back(make_int2(q, 0));
+        ftrsm_off_.assign(1, 0);
+        fupd_off_.assign(1, 0);
+        for (int k = 0; k + 1 < nsteps; ++k) {
+            for (int q = 0; q < B_; ++q) {
+                const int m = hdesc_[q].npad / kTile - k - 1;
+                for (int i = 0; i < m; ++i) trsm.push_back(make_int2(q, i));
+                if (m > 0) upd.push_back(make_int2(q, 0));
+            }
+            for (int q = 0; q < B_; ++q) {
+                const int m = hdesc_[q].npad / kTile - k - 1;  // negative once q is factored
+                if (m <= 0) continue;
+                for (int t = 1; t < m * (m + 1) / 2; ++t) upd.push_back(make_int2(q, t));
+            }
+            ftrsm_off_.push_back(static_cast<int>(trsm.size()));
+            fupd_off_.push_back(static_cast<int>(upd.size()));
+        }
+        upload(fpotrf_, potrf, st_);
+        upload(ftrsm_, trsm, st_);
+        upload(fupd_, upd, st_);
+        ftab_.potrf0 = fpotrf_.p;
+        ftab_.trsm = ftrsm_.p;
+        ftab_.upd = fupd_.p;
+        ftab_.trsm_off = ftrsm_off_.data();
+        ftab_.upd_off = fupd_off_.data();
+        ftab_.nsteps = nsteps;
+        GBA_HIP(hipStreamSynchronize(st_));  // the tables above are released here
+    }
     const int C = static_cast<int>(C0), P = static_cast<int>(P0), O = static_cast<int>(O0);
     pb_.C = C;
     pb_.P = P;
@@ -260,7 +299,8 @@ void BatchSolver::solve(const gba_lm_options& opts, gba_lm_result* res) {
         if (num_stasks_ > 0)
             launch_schur(st_, sshape_, pl, stasks_.p, num_stasks_, spairs_.p, W_.p, nullptr, Vinv_.p, VB_.p,
                          U_.p, bc_.p, 0.0, nullptr, S_.p, nullptr, nullptr, 0, rhs_.p, flags_.p);
-        launch_batch_chol_solve(st_, B, desc_.p, S_.p, D_.p, rhs_.p, dc_.p, flags_.p, active_.p);
+        if (tile_factor_) launch_batch_cholesky(st_, B, ftab_, desc_.p, S_.p, D_.p, flags_.p, active_.p);
+        launch_batch_chol_solve(st_, B, desc_.p, S_.p, D_.p, rhs_.p, dc_.p, flags_.p, active_.p, !tile_factor_);
         launch_trial_cams(st_, pb_.C, cams_.p, dc_.p, tcams_.p);
         launch_backsub(st_, pl, W_.p, nullptr, Vinv_.p, VB_.p, dc_.p, pts_.p, tcams_.p, tpts_.p, 0.0,
                        unit_sse_.p, unit_den_.p, unit_anom_.p, flags_.p);

@@ -58,6 +58,11 @@ private:
     DevBuf<int> active_, accept_;
     DevBuf<TrialFlags> flags_;
     DevBuf<PassStatus> lin_status_, trial_status_;
+    // tile-parallel factor (launch_batch_cholesky) and its work tables
+    bool tile_factor_ = true;
+    DevBuf<int2> fpotrf_, ftrsm_, fupd_;
+    std::vector<int> ftrsm_off_, fupd_off_;
+    BatchFactorTables ftab_;
     PassStatus* h_status_ = nullptr;  // 2 B entries: linearisation, trial
     double* h_lam_ = nullptr;
     int* h_mask_ = nullptr;           // staging for the per-problem switches

@@ -176,14 +176,45 @@ __device__ __forceinline__ void store_diag_tile(const T (*a)[kTile + 2], const T
     }
 }
 
+// Batched use of the per-tile factor kernels: with tab set, workgroup w works
+// on problem tab[w].x (its S at desc.soff with leading dimension desc.npad,
+// its diagonal tiles and their inverses at desc.doff, its own flags) and
+// takes tab[w].y as its tile index; problems switched off by active skip.
+struct TileBatch {
+    const int2* tab;
+    const BatchDesc* desc;
+    const int* active;
+};
+
+// Resolves the workgroup's problem; false when it has nothing to do.
+template <typename T>
+__device__ __forceinline__ bool tile_batch_bind(const TileBatch& tb, T* __restrict__& S, int& ld,
+                                                T* __restrict__& D, T* __restrict__& DinvT, TrialFlags*& flags,
+                                                int& unit) {
+    unit = blockIdx.x;
+    if (!tb.tab) return true;
+    const int2 e = tb.tab[blockIdx.x];
+    if (!tb.active[e.x]) return false;
+    const BatchDesc d = tb.desc[e.x];
+    S += d.soff;
+    ld = d.npad;
+    D += d.doff;
+    DinvT = D + static_cast<size_t>(d.npad / kTile) * kTile * kTile;
+    flags += e.x;
+    unit = e.y;
+    return true;
+}
+
 // Factor + invert diagonal tile k of S (the first panel; later diagonal tiles
 // are factored by block 0 of the trailing update that finishes them).
 template <typename T>
-__global__ __launch_bounds__(256) void k_potrf_tile(const T* __restrict__ S, int ld, int k,
+__global__ __launch_bounds__(256) void k_potrf_tile(T* __restrict__ S, int ld, int k,
                                                    T* __restrict__ D, T* __restrict__ DinvT,
-                                                   TrialFlags* flags) {
+                                                   TrialFlags* flags, TileBatch tb) {
     __shared__ T a[kTile][kTile + 2];
     __shared__ T b[kTile][kTile + 2];
+    int unit;
+    if (!tile_batch_bind(tb, S, ld, D, DinvT, flags, unit)) return;
     if (any_fail(flags)) return;
     const T* Akk = S + static_cast<size_t>(k * kTile) * ld + k * kTile;
     for (int q = threadIdx.x; q < kTile * kTile; q += 256) a[q >> 6][q & 63] = Akk[static_cast<size_t>(q >> 6) * ld + (q & 63)];
@@ -199,12 +230,13 @@ __global__ __launch_bounds__(256) void k_potrf_tile(const T* __restrict__ S, int
 // read it) and L_ik^T into the unused upper tile (k, i), from which the
 // trailing update reads both MFMA operands with coalesced loads.
 template <typename T>
-__global__ __launch_bounds__(256) void k_trsm_tiles(T* __restrict__ S, int ld, int k,
-                                                    const T* __restrict__ DinvT,
-                                                    const TrialFlags* flags) {
+__global__ __launch_bounds__(256) void k_trsm_tiles(T* __restrict__ S, int ld, int k, T* __restrict__ D,
+                                                    T* __restrict__ DinvT, TrialFlags* flags, TileBatch tb) {
     __shared__ T Ai[kTile][kTile + 1];
+    int unit;
+    if (!tile_batch_bind(tb, S, ld, D, DinvT, flags, unit)) return;
     if (any_fail(flags)) return;
-    const int i = k + 1 + blockIdx.x;
+    const int i = k + 1 + unit;
     T* Aik = S + static_cast<size_t>(i * kTile) * ld + k * kTile;
     T* LTki = S + static_cast<size_t>(k * kTile) * ld + i * kTile;
     const T* inv = DinvT + static_cast<size_t>(k) * kTile * kTile;
@@ -271,12 +303,13 @@ __global__ __launch_bounds__(256) void k_trsm_tiles(T* __restrict__ S, int ld, i
 template <typename T>
 __global__ __launch_bounds__(256, 2) void k_chol_update(T* __restrict__ S, int ld, int k,
                                                         T* __restrict__ D, T* __restrict__ DinvT,
-                                                        TrialFlags* flags) {
+                                                        TrialFlags* flags, TileBatch tb) {
     __shared__ T Ai[kTile][kTile + 2];
     __shared__ T Aj[kTile][kTile + 2];
+    int t;
+    if (!tile_batch_bind(tb, S, ld, D, DinvT, flags, t)) return;
     if (any_fail(flags)) return;
     // tile index -> (ii, jj), jj <= ii, over the trailing lower triangle
-    const int t = blockIdx.x;
     int ii = static_cast<int>((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
     while (ii * (ii + 1) / 2 > t) --ii;
     while ((ii + 1) * (ii + 2) / 2 <= t) ++ii;
@@ -345,6 +378,9 @@ constexpr int kBatchWaves = kBatchThreads / 64;
 constexpr int kRB = kBatchWaves >= 16 ? 1 : 2;
 constexpr int kStrips = 4 / kRB;
 
+// kFactor = false: S already holds the factor (launch_batch_cholesky), only
+// the substitutions run.
+template <bool kFactor>
 __global__ __launch_bounds__(kBatchThreads) void k_batch_chol_solve(const BatchDesc* __restrict__ desc,
                                                           double* __restrict__ S,
                                                           double* __restrict__ D,
@@ -358,7 +394,7 @@ __global__ __launch_bounds__(kBatchThreads) void k_batch_chol_solve(const BatchD
     __shared__ double part[kBatchThreads];
     __shared__ double tmp[kTile];
     const int b = blockIdx.x;
-    if (!active[b] || flags[b].nonspd) return;
+    if (!active[b] || (kFactor ? flags[b].nonspd != 0 : any_fail(flags + b))) return;
     const BatchDesc d = desc[b];
     double* A = S + d.soff;
     const int ld = d.npad, nblk = d.npad / kTile;
@@ -366,7 +402,7 @@ __global__ __launch_bounds__(kBatchThreads) void k_batch_chol_solve(const BatchD
     double* DinvT = Lkk + static_cast<size_t>(nblk) * kTile * kTile;
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
     const int li = lane & 15, lk = lane >> 4;
-    for (int k = 0; k < nblk; ++k) {
+    for (int k = 0; kFactor && k < nblk; ++k) {
         const double* Akk = A + static_cast<size_t>(k * kTile) * ld + k * kTile;
         for (int q = tid; q < kTile * kTile; q += kBatchThreads)
             sa[q >> 6][q & 63] = Akk[static_cast<size_t>(q >> 6) * ld + (q & 63)];
@@ -797,32 +833,56 @@ __global__ void k_vec_scale(int n, int npad, const double* __restrict__ in,
 void launch_cholesky(hipStream_t st, double* S64, float* S32, int ld, int nblk, double* D64,
                      float* D32, TrialFlags* flags) {
     const size_t tile = static_cast<size_t>(kTile) * kTile;
+    const TileBatch none{nullptr, nullptr, nullptr};
     if (S64) {
         double* Dinv = D64 + static_cast<size_t>(nblk) * tile;
-        hipLaunchKernelGGL(k_potrf_tile<double>, dim3(1), dim3(256), 0, st, S64, ld, 0, D64, Dinv, flags);
+        hipLaunchKernelGGL(k_potrf_tile<double>, dim3(1), dim3(256), 0, st, S64, ld, 0, D64, Dinv, flags, none);
         for (int k = 0; k + 1 < nblk; ++k) {
             const int m = nblk - k - 1;
-            hipLaunchKernelGGL(k_trsm_tiles<double>, dim3(m), dim3(256), 0, st, S64, ld, k, Dinv, flags);
+            hipLaunchKernelGGL(k_trsm_tiles<double>, dim3(m), dim3(256), 0, st, S64, ld, k, D64, Dinv, flags, none);
             hipLaunchKernelGGL(k_chol_update<double>, dim3(m * (m + 1) / 2), dim3(256), 0, st, S64, ld, k,
-                               D64, Dinv, flags);
+                               D64, Dinv, flags, none);
         }
     } else {
         float* Dinv = D32 + static_cast<size_t>(nblk) * tile;
-        hipLaunchKernelGGL(k_potrf_tile<float>, dim3(1), dim3(256), 0, st, S32, ld, 0, D32, Dinv, flags);
+        hipLaunchKernelGGL(k_potrf_tile<float>, dim3(1), dim3(256), 0, st, S32, ld, 0, D32, Dinv, flags, none);
         for (int k = 0; k + 1 < nblk; ++k) {
             const int m = nblk - k - 1;
-            hipLaunchKernelGGL(k_trsm_tiles<float>, dim3(m), dim3(256), 0, st, S32, ld, k, Dinv, flags);
+            hipLaunchKernelGGL(k_trsm_tiles<float>, dim3(m), dim3(256), 0, st, S32, ld, k, D32, Dinv, flags, none);
             hipLaunchKernelGGL(k_chol_update<float>, dim3(m * (m + 1) / 2), dim3(256), 0, st, S32, ld, k,
-                               D32, Dinv, flags);
+                               D32, Dinv, flags, none);
+        }
+    }
+}
+
+void launch_batch_cholesky(hipStream_t st, int count, const BatchFactorTables& t, const BatchDesc* desc,
+                           double* S, double* D, TrialFlags* flags, const int* active) {
+    if (count <= 0) return;
+    TileBatch tb{t.potrf0, desc, active};
+    hipLaunchKernelGGL(k_potrf_tile<double>, dim3(count), dim3(256), 0, st, S, 0, 0, D, D, flags, tb);
+    for (int k = 0; k + 1 < t.nsteps; ++k) {
+        const int ntr = t.trsm_off[k + 1] - t.trsm_off[k], nup = t.upd_off[k + 1] - t.upd_off[k];
+        if (ntr > 0) {
+            tb.tab = t.trsm + t.trsm_off[k];
+            hipLaunchKernelGGL(k_trsm_tiles<double>, dim3(ntr), dim3(256), 0, st, S, 0, k, D, D, flags, tb);
+        }
+        if (nup > 0) {
+            tb.tab = t.upd + t.upd_off[k];
+            hipLaunchKernelGGL(k_chol_update<double>, dim3(nup), dim3(256), 0, st, S, 0, k, D, D, flags, tb);
         }
     }
 }
 
 void launch_batch_chol_solve(hipStream_t st, int count, const BatchDesc* desc, double* S, double* D,
-                             const double* rhs, double* dc, TrialFlags* flags, const int* active) {
-    if (count > 0)
-        hipLaunchKernelGGL(k_batch_chol_solve, dim3(count), dim3(kBatchThreads), 0, st, desc, S, D, rhs, dc, flags,
-                           active);
+                             const double* rhs, double* dc, TrialFlags* flags, const int* active,
+                             bool factor) {
+    if (count <= 0) return;
+    if (factor)
+        hipLaunchKernelGGL(k_batch_chol_solve<true>, dim3(count), dim3(kBatchThreads), 0, st, desc, S, D, rhs, dc,
+                           flags, active);
+    else
+        hipLaunchKernelGGL(k_batch_chol_solve<false>, dim3(count), dim3(kBatchThreads), 0, st, desc, S, D, rhs,
+                           dc, flags, active);
 }
 
 void launch_chol_solve(hipStream_t st, const double* L64, const float* L32, int ld, int nblk,

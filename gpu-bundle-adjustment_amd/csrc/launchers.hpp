@@ -62,10 +62,33 @@ void launch_diag_scale(hipStream_t st, int n, int npad, const double* U, double 
 // inverses transposed; the off-diagonal tiles of L replace S's.
 void launch_cholesky(hipStream_t st, double* S64, float* S32, int ld, int nblk, double* D64,
                      float* D32, TrialFlags* flags);
-// Batched problems (FP64): factor and solve every active problem's S in place,
-// one workgroup per problem (desc[count]); dc = S^-1 rhs per problem.
+// Batched problems (FP64): solve every active problem's S in place, one
+// workgroup per problem (desc[count]); dc = S^-1 rhs per problem.  factor =
+// true factors S in the same workgroup first; false expects the factor from
+// launch_batch_cholesky.
 void launch_batch_chol_solve(hipStream_t st, int count, const BatchDesc* desc, double* S, double* D,
-                             const double* rhs, double* dc, TrialFlags* flags, const int* active);
+                             const double* rhs, double* dc, TrialFlags* flags, const int* active,
+                             bool factor);
+// Work tables of the tile-parallel batched factor (device arrays of
+// (problem, tile) pairs, host offsets): potrf0[count] factors every problem's
+// first diagonal tile; step k runs trsm[trsm_off[k] .. trsm_off[k+1]) (tile
+// rows below the diagonal, as k_trsm_tiles numbers them) and
+// upd[upd_off[k] .. upd_off[k+1]) (trailing tiles, as k_chol_update numbers
+// them, tile 0 first) for every problem with more than k+1 tile columns.
+struct BatchFactorTables {
+    const int2* potrf0 = nullptr;
+    const int2* trsm = nullptr;
+    const int2* upd = nullptr;
+    const int* trsm_off = nullptr;  // host, nsteps entries
+    const int* upd_off = nullptr;   // host, nsteps entries
+    int nsteps = 0;                 // largest tile-column count
+};
+// Factors every active problem's S (FP64, desc layout) in place with the
+// single-problem tile kernels, one launch per stage and column for the whole
+// batch: D receives each problem's diagonal tiles and their inverses as
+// launch_cholesky lays them out; a bad pivot sets flags[problem].chol_fail.
+void launch_batch_cholesky(hipStream_t st, int count, const BatchFactorTables& t, const BatchDesc* desc,
+                           double* S, double* D, TrialFlags* flags, const int* active);
 // Solves L L^T x = b (b, x length nblk*kTile; b and x may alias).  sync holds
 // 2*nblk + 2 ints of scratch (done flags and tickets).
 void launch_chol_solve(hipStream_t st, const double* L64, const float* L32, int ld, int nblk,

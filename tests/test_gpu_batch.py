@@ -1,8 +1,10 @@
 """Batched solver (many independent problems per launch) against each
-problem's own sequential FP64 solve through the C ABI.  The batch factors each
-S with a different kernel (one workgroup per problem), so the steps agree to
-rounding; the tolerances are the LM convergence tolerance on the final cost
-(1e-6 relative) and at most one iteration of difference."""
+problem's own sequential FP64 solve through the C ABI.  The batch solves each
+factored S with a different kernel (one workgroup per problem), so the steps
+agree to rounding; the tolerances are the LM convergence tolerance on the
+final cost (1e-6 relative) and at most one iteration of difference.  Both
+batched factors are covered: the tile-parallel one (default) and the
+one-workgroup-per-problem one (GBA_BATCH_FACTOR=workgroup)."""
 import numpy as np
 import pytest
 
@@ -39,7 +41,9 @@ def test_batch_matches_sequential_config5(gba):
         assert abs(r1.final_cost - r2.final_cost) <= 1e-9 * r1.final_cost
 
 
-def test_batch_mixed_sizes_and_params(gba):
+@pytest.mark.parametrize("factor", ["tiles", "workgroup"])
+def test_batch_mixed_sizes_and_params(gba, monkeypatch, factor):
+    monkeypatch.setenv("GBA_BATCH_FACTOR", factor)
     probs = [small(gba, 8, 800, 3600, 11), small(gba, 64, 3000, 13500, 12), small(gba, 23, 1500, 6500, 13)]
     seq_solvers = [gba.Solver(p) for p in probs]
     seq = [s.solve() for s in seq_solvers]
