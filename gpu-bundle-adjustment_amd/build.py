@@ -21,7 +21,7 @@ SOURCES = [
     "k_blocks.hip",
     "k_schur.hip",
     "k_dense.hip",
-    "k_trinv.hip",
+    "k_precond.hip",
     "k_backsub.hip",
     "k_batch.hip",
     "solver.cpp",

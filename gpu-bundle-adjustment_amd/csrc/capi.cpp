@@ -317,8 +317,7 @@ int gba_dense_spd_solve(int n, int precision, const double* A, double* b, int* o
             GBA_HIP(hipMemcpyAsync(S32low.p, hf.data(), sizeof(float) * np * np, hipMemcpyHostToDevice, st));
             X32.alloc(np * np);
             T32.alloc(np * np);
-            launch_cholesky(st, nullptr, S32.p, npad, nblk, nullptr, D32.p, flags.p);
-            launch_trinv(st, S32.p, D32.p, npad, nblk, X32.p, T32.p, flags.p);
+            launch_cholesky(st, nullptr, S32.p, npad, nblk, nullptr, D32.p, flags.p, X32.p, T32.p);
             double rel = 0.0;
             try {
                 iters = pcg_solve(st, S32low.p, X32.p, npad, npad, rhs.p, x.p, work.p, flags.p, status.p,

@@ -14,7 +14,9 @@
 //   trsm  : L_ik = A_ik L_kk^-T for every tile below, an MFMA GEMM per tile;
 //   update: A_ij -= L_ik L_jk^T for all trailing lower tiles, one workgroup per
 //           tile, four wavefronts of 2x2 MFMA 16x16 blocks
-//           (v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32), operands from LDS.
+//           (v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32), operands from LDS;
+//           in the FP32 form, extra workgroups of update k form row k of the
+//           explicit inverse X = L^-1 (the mixed form's preconditioner).
 // A pivot that is not > 0 sets TrialFlags::chol_fail (the LM then escalates
 // lambda, as the reference does on LLT failure).
 //
@@ -223,6 +225,178 @@ __global__ __launch_bounds__(256) void k_potrf_tile(T* __restrict__ S, int ld, i
     store_diag_tile<T>(a, b, ok, k, D, DinvT, flags);
 }
 
+// ---- explicit inverse X = L^-1 formed alongside the factor ----------------
+// Row k of X needs row k of L (finished by TRSM k-1) and Linv_kk (factored by
+// the lookahead of update launch k-1), so it is formed by extra workgroups of
+// update launch k, which otherwise waits on its block 0:
+//   X_kk = Linv_kk,   X_kj = -Linv_kk (R_kj + L_{k,k-1} X_{k-1,j})   (j < k),
+//   R_kj = sum_{j <= m <= k-2} L_km X_mj,
+// with R accumulated one term per launch: launch k also forms
+// R_ij += L_{i,k-1} X_{k-1,j} for i > k, j < k (stored, not added, for the
+// first term m = j = k-1).  Within a launch these roles read only row k-1 of
+// X, row k of R and L tiles that no workgroup of the launch writes.  X layout:
+// lower tile (i, j) X_ij, upper tile (j, i) X_ij^T, diagonal tile X_ii below
+// and X_ii^T above its diagonal.  R lives in a separate n_pad^2 buffer.
+
+// Roles of launch k: row k of X (k + 1 tiles), then the R tiles.
+__host__ __device__ constexpr int inv_roles(int k, int nblk) { return (k + 1) + (nblk - 1 - k) * k; }
+
+// acc = A B for two 64x64 tiles staged in LDS.
+template <typename T>
+__device__ __forceinline__ void acc_lds_lds(typename Acc4<T>::type (&acc)[2][2], const T (*sA)[kTile + 2],
+                                            const T (*sB)[kTile + 2]) {
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
+    const int li = lane & 15, lk = lane >> 4;
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int c = 0; c < 2; ++c) acc[r][c] = typename Acc4<T>::type{0, 0, 0, 0};
+#pragma unroll 4
+    for (int kk = 0; kk < kTile; kk += 4) {
+        const T a0 = sA[wr + li][kk + lk], a1 = sA[wr + 16 + li][kk + lk];
+        const T b0 = sB[kk + lk][wc + li], b1 = sB[kk + lk][wc + 16 + li];
+        acc[0][0] = mfma16(a0, b0, acc[0][0]);
+        acc[0][1] = mfma16(a0, b1, acc[0][1]);
+        acc[1][0] = mfma16(a1, b0, acc[1][0]);
+        acc[1][1] = mfma16(a1, b1, acc[1][1]);
+    }
+}
+
+// Role r (0 <= r < inv_roles(k, nblk)) of launch k, 256 threads; sa, sb are
+// two LDS tiles of the calling kernel.
+template <typename T>
+__device__ void inv_role(int r, int k, int nblk, const T* __restrict__ S, int ld, const T* __restrict__ DinvT,
+                         T* __restrict__ X, T* __restrict__ R, T (*sa)[kTile + 2], T (*sb)[kTile + 2]) {
+    typedef typename Acc4<T>::type acc_t;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
+    const int li = lane & 15, lk = lane >> 4;
+    const size_t tile = static_cast<size_t>(kTile) * kTile;
+    if (r == k) {
+        // X_kk from DinvT[k] = (Linv_kk)^T
+        const T* it = DinvT + static_cast<size_t>(k) * tile;
+        T* xt = X + static_cast<size_t>(k * kTile) * ld + k * kTile;
+#pragma unroll 4
+        for (int q = threadIdx.x; q < kTile * kTile; q += 256) {
+            const int rr = q >> 6, c = q & 63;
+            xt[static_cast<size_t>(rr) * ld + c] = it[(rr >= c) ? c * kTile + rr : rr * kTile + c];
+        }
+        return;
+    }
+    int i, j;
+    if (r < k) {
+        i = k;
+        j = r;
+    } else {
+        const int b = r - (k + 1);
+        i = k + 1 + b / k;
+        j = b % k;
+    }
+    // stage A = L_{i,k-1} and B = X_{k-1,j} (lower part only for a diagonal
+    // tile of X), all loads in flight at once
+    {
+        const T* A = S + static_cast<size_t>(i * kTile) * ld + (k - 1) * kTile;
+        const T* B = X + static_cast<size_t>((k - 1) * kTile) * ld + j * kTile;
+        const bool bl = (j == k - 1);
+#pragma unroll
+        for (int h = 0; h < 16; h += 8) {
+            T ta[8], tb[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int q = threadIdx.x + 256 * (h + u);
+                ta[u] = A[static_cast<size_t>(q >> 6) * ld + (q & 63)];
+                tb[u] = B[static_cast<size_t>(q >> 6) * ld + (q & 63)];
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int q = threadIdx.x + 256 * (h + u);
+                const int rr = q >> 6, c = q & 63;
+                sa[rr][c] = ta[u];
+                sb[rr][c] = (bl && c > rr) ? T(0) : tb[u];
+            }
+        }
+    }
+    __syncthreads();
+    acc_t acc[2][2];
+    acc_lds_lds<T>(acc, sa, sb);
+    T* Rij = R + static_cast<size_t>(i * kTile) * ld + j * kTile;
+    if (j < k - 1) {
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int c = 0; c < 2; ++c)
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    acc[a][c][q] += Rij[static_cast<size_t>(wr + 16 * a + acc_row<T>(lk, q)) * ld + wc + 16 * c + li];
+    }
+    if (i > k) {
+        // R role: R_ij = acc
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int c = 0; c < 2; ++c)
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    Rij[static_cast<size_t>(wr + 16 * a + acc_row<T>(lk, q)) * ld + wc + 16 * c + li] = acc[a][c][q];
+        return;
+    }
+    // X row role: X_kj = -Linv_kk Z with Z = acc; Linv_kk (rows of DinvT[k]
+    // transposed) into sa, Z into sb
+    __syncthreads();  // every wave is done reading sa and sb
+    {
+        const T* it = DinvT + static_cast<size_t>(k) * tile;  // it[c][rr] = Linv_kk[rr][c]
+#pragma unroll
+        for (int h = 0; h < 16; h += 8) {
+            T tl[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) tl[u] = it[threadIdx.x + 256 * (h + u)];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int q = threadIdx.x + 256 * (h + u);
+                sa[q & 63][q >> 6] = tl[u];
+            }
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) sb[wr + 16 * a + acc_row<T>(lk, q)][wc + 16 * c + li] = acc[a][c][q];
+    __syncthreads();
+    acc_lds_lds<T>(acc, sa, sb);
+    __syncthreads();  // every wave is done reading sa
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) sa[wr + 16 * a + acc_row<T>(lk, q)][wc + 16 * c + li] = -acc[a][c][q];
+    __syncthreads();
+    T* low = X + static_cast<size_t>(k * kTile) * ld + j * kTile;
+    T* up = X + static_cast<size_t>(j * kTile) * ld + k * kTile;
+#pragma unroll 4
+    for (int u = 0; u < 16; ++u) {
+        const int q = threadIdx.x + 256 * u;
+        const int rr = q >> 6, c = q & 63;
+        low[static_cast<size_t>(rr) * ld + c] = sa[rr][c];
+        up[static_cast<size_t>(rr) * ld + c] = sa[c][rr];
+    }
+}
+
+// Row nblk-1 of X, after the last trailing update (its Linv comes from that
+// update's lookahead).
+template <typename T>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_inv_last_row(const T* __restrict__ S, int ld, int nblk,
+                                                         const T* __restrict__ DinvT, T* __restrict__ X,
+                                                         T* __restrict__ R, const TrialFlags* flags) {
+    __shared__ T sa[kTile][kTile + 2];
+    __shared__ T sb[kTile][kTile + 2];
+    if (any_fail(flags)) return;
+    inv_role<T>(blockIdx.x, nblk - 1, nblk, S, ld, DinvT, X, R, sa, sb);
+}
+
 // L_ik = A_ik (L_kk^-1)^T for tile rows i > k: an MFMA GEMM per tile, A_ik
 // staged in LDS, the B operand read straight from DinvT (row kk of DinvT is
 // column kk of L_kk^-1, so a 16-lane group reads 128 contiguous bytes).  The
@@ -299,16 +473,24 @@ __global__ __launch_bounds__(256) void k_trsm_tiles(T* __restrict__ S, int ld, i
 // the transposed panel tiles L_ik^T, L_jk^T (upper tiles (k, i), (k, j)).
 // Block 0 owns tile (k+1, k+1), which this update completes: it then factors
 // and inverts it (lookahead), so the next panel needs no separate factor
-// launch.
+// launch.  With X set (FP32), the workgroups from nup on form row k of
+// X = L^-1 (inv_role).
 template <typename T>
-__global__ __launch_bounds__(256, 2) void k_chol_update(T* __restrict__ S, int ld, int k,
-                                                        T* __restrict__ D, T* __restrict__ DinvT,
-                                                        TrialFlags* flags, TileBatch tb) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 2))) void k_chol_update(
+    T* __restrict__ S, int ld, int k, T* __restrict__ D, T* __restrict__ DinvT, TrialFlags* flags, TileBatch tb,
+    int nup, int nblk, T* __restrict__ X, T* __restrict__ R) {
     __shared__ T Ai[kTile][kTile + 2];
     __shared__ T Aj[kTile][kTile + 2];
     int t;
     if (!tile_batch_bind(tb, S, ld, D, DinvT, flags, t)) return;
     if (any_fail(flags)) return;
+    if constexpr (sizeof(T) == 4) {
+        if (X && t >= nup) {
+            // workgroups past the update tiles form row k of X = L^-1 (see inv_role)
+            inv_role<T>(t - nup, k, nblk, S, ld, DinvT, X, R, Ai, Aj);
+            return;
+        }
+    }
     // tile index -> (ii, jj), jj <= ii, over the trailing lower triangle
     int ii = static_cast<int>((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
     while (ii * (ii + 1) / 2 > t) --ii;
@@ -831,27 +1013,33 @@ __global__ void k_vec_scale(int n, int npad, const double* __restrict__ in,
 }  // namespace
 
 void launch_cholesky(hipStream_t st, double* S64, float* S32, int ld, int nblk, double* D64,
-                     float* D32, TrialFlags* flags) {
+                     float* D32, TrialFlags* flags, float* X32, float* R32) {
     const size_t tile = static_cast<size_t>(kTile) * kTile;
     const TileBatch none{nullptr, nullptr, nullptr};
     if (S64) {
         double* Dinv = D64 + static_cast<size_t>(nblk) * tile;
+        double* const nd = nullptr;
         hipLaunchKernelGGL(k_potrf_tile<double>, dim3(1), dim3(256), 0, st, S64, ld, 0, D64, Dinv, flags, none);
         for (int k = 0; k + 1 < nblk; ++k) {
-            const int m = nblk - k - 1;
+            const int m = nblk - k - 1, nup = m * (m + 1) / 2;
             hipLaunchKernelGGL(k_trsm_tiles<double>, dim3(m), dim3(256), 0, st, S64, ld, k, D64, Dinv, flags, none);
-            hipLaunchKernelGGL(k_chol_update<double>, dim3(m * (m + 1) / 2), dim3(256), 0, st, S64, ld, k,
-                               D64, Dinv, flags, none);
+            hipLaunchKernelGGL(k_chol_update<double>, dim3(nup), dim3(256), 0, st, S64, ld, k, D64, Dinv, flags,
+                               none, nup, nblk, nd, nd);
         }
     } else {
         float* Dinv = D32 + static_cast<size_t>(nblk) * tile;
         hipLaunchKernelGGL(k_potrf_tile<float>, dim3(1), dim3(256), 0, st, S32, ld, 0, D32, Dinv, flags, none);
         for (int k = 0; k + 1 < nblk; ++k) {
-            const int m = nblk - k - 1;
+            const int m = nblk - k - 1, nup = m * (m + 1) / 2;
+            const int grid = nup + (X32 ? inv_roles(k, nblk) : 0);
             hipLaunchKernelGGL(k_trsm_tiles<float>, dim3(m), dim3(256), 0, st, S32, ld, k, D32, Dinv, flags, none);
-            hipLaunchKernelGGL(k_chol_update<float>, dim3(m * (m + 1) / 2), dim3(256), 0, st, S32, ld, k,
-                               D32, Dinv, flags, none);
+            hipLaunchKernelGGL(k_chol_update<float>, dim3(grid), dim3(256), 0, st, S32, ld, k, D32, Dinv, flags,
+                               none, nup, nblk, X32, R32);
         }
+        if (X32)
+            hipLaunchKernelGGL(k_inv_last_row<float>, dim3(inv_roles(nblk - 1, nblk)), dim3(256), 0, st,
+                               static_cast<const float*>(S32), ld, nblk, static_cast<const float*>(Dinv), X32, R32,
+                               static_cast<const TrialFlags*>(flags));
     }
 }
 
@@ -868,7 +1056,8 @@ void launch_batch_cholesky(hipStream_t st, int count, const BatchFactorTables& t
         }
         if (nup > 0) {
             tb.tab = t.upd + t.upd_off[k];
-            hipLaunchKernelGGL(k_chol_update<double>, dim3(nup), dim3(256), 0, st, S, 0, k, D, D, flags, tb);
+            hipLaunchKernelGGL(k_chol_update<double>, dim3(nup), dim3(256), 0, st, S, 0, k, D, D, flags, tb, nup, 0,
+                               static_cast<double*>(nullptr), static_cast<double*>(nullptr));
         }
     }
 }

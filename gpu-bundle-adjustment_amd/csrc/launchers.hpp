@@ -59,9 +59,12 @@ void launch_diag_scale(hipStream_t st, int n, int npad, const double* U, double 
 // k_dense.hip
 // Factors in place; D holds 2*nblk tiles of 64x64: tiles [0, nblk) receive the
 // diagonal tiles of L (row-major, upper part zero), tiles [nblk, 2 nblk) their
-// inverses transposed; the off-diagonal tiles of L replace S's.
+// inverses transposed; the off-diagonal tiles of L replace S's.  FP32 with X32
+// set: X32 also receives X = L^-1, formed inside the factor's launches (lower
+// tile (i, j) X_ij, upper tile (j, i) X_ij^T, diagonal tile X_ii below and
+// X_ii^T above its diagonal; R32: n_pad^2 floats of scratch).
 void launch_cholesky(hipStream_t st, double* S64, float* S32, int ld, int nblk, double* D64,
-                     float* D32, TrialFlags* flags);
+                     float* D32, TrialFlags* flags, float* X32 = nullptr, float* R32 = nullptr);
 // Batched problems (FP64): solve every active problem's S in place, one
 // workgroup per problem (desc[count]); dc = S^-1 rhs per problem.  factor =
 // true factors S in the same workgroup first; false expects the factor from
@@ -114,12 +117,7 @@ void launch_vec_axpy(hipStream_t st, int n, double* x, const double* d);
 void launch_vec_scale(hipStream_t st, int n, int npad, const double* in, const double* s,
                       double* out);
 
-// k_trinv.hip
-// X = L^-1 of the FP32 factor (L's off-diagonal tiles in L32, diagonal
-// inverses in D32 as laid out by launch_cholesky): X's lower tiles get X_ij,
-// its upper tiles X_ij^T; T32 is an npad x npad scratch matrix.
-void launch_trinv(hipStream_t st, const float* L32, const float* D32, int ld, int nblk, float* X32,
-                  float* T32, const TrialFlags* flags);
+// k_precond.hip
 // z = X^T (X r) = (L L^T)^-1 r; work holds precond_work(npad) doubles.
 size_t precond_work(int npad);
 void launch_precond_apply(hipStream_t st, const float* X32, int ld, int npad, const double* r,

@@ -483,8 +483,7 @@ void Solver::enqueue_trial_linear_algebra(double lambda, std::vector<double>* ho
     }
     // mixed: FP32 factor of D S D; the camera system is solved in FP64 by CG
     // preconditioned with that factor, against the FP32-assembled D S D
-    launch_cholesky(st_, nullptr, S32_.p, ld, nblk_, nullptr, D32_.p, flags_.p);
-    launch_trinv(st_, S32_.p, D32_.p, ld, nblk_, X32_.p, T32_.p, flags_.p);
+    launch_cholesky(st_, nullptr, S32_.p, ld, nblk_, nullptr, D32_.p, flags_.p, X32_.p, T32_.p);
     const double tol = cfg_.refine_tol > 0 ? cfg_.refine_tol : 1e-10;
     const int max_it = cfg_.refine_max_iters > 0 ? cfg_.refine_max_iters : 12;
     out.refine_iters = pcg_solve(st_, S32low_.p, X32_.p, ld, pb_.npad, rhs_.p, x_.p, pcgw_.p, flags_.p,
@@ -596,8 +595,10 @@ double Solver::time_kernel(int kernel, double lambda, int reps) {
                 break;
             case GBA_KERNEL_FACTOR:
                 // mixed: factor + explicit inverse (what a trial runs before PCG)
-                launch_cholesky(st_, S64_.p, S32_.p, ld, nblk_, D64_.p, D32_.p, flags_.p);
-                if (mixed_) launch_trinv(st_, S32_.p, D32_.p, ld, nblk_, X32_.p, T32_.p, flags_.p);
+                if (mixed_)
+                    launch_cholesky(st_, nullptr, S32_.p, ld, nblk_, nullptr, D32_.p, flags_.p, X32_.p, T32_.p);
+                else
+                    launch_cholesky(st_, S64_.p, nullptr, ld, nblk_, D64_.p, nullptr, flags_.p);
                 break;
             case GBA_KERNEL_SOLVE:
                 // FP64: both triangular solves; mixed: one preconditioner application

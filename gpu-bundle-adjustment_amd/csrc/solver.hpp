@@ -62,7 +62,7 @@ void build_schur_tasks(int C, int cb, const std::vector<int>& cam_off,
 
 // FP64 preconditioned conjugate gradients on A x = b, A symmetric given by the
 // lower triangle of an FP32 matrix, preconditioned by (L L^T)^-1 applied
-// through the explicit FP32 inverse X = L^-1 (launch_trinv layout).  Vectors
+// through the explicit FP32 inverse X = L^-1 (launch_cholesky with X32).  Vectors
 // have npad entries.  Stops at |r| <= tol |b| or after max_it steps; returns
 // the steps taken and the final |r| / |b| in *rel.  work holds pcg_work(npad)
 // doubles.  Synchronises the stream twice per step.

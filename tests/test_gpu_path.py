@@ -104,7 +104,9 @@ def test_dense_spd_solve_fp64(gba, n):
     assert arel(x, np.linalg.solve(A, b)) < 1e-10
 
 
-@pytest.mark.parametrize("n", [64, 441, 1000])
+# 1, 4, 7, 16 and 32 tile columns: the explicit inverse X = L^-1 that the
+# refinement applies is formed inside the factor's launches, row by row
+@pytest.mark.parametrize("n", [64, 200, 441, 1000, 2000])
 def test_dense_spd_solve_mixed_refines_to_fp64(gba, n):
     rng = np.random.default_rng(n + 1)
     M = rng.uniform(-1, 1, (n, n))
