@@ -105,7 +105,8 @@ def test_dense_spd_solve_fp64(gba, n):
 
 
 # 1, 4, 7, 16 and 32 tile columns: the explicit inverse X = L^-1 that the
-# refinement applies is formed inside the factor's launches, row by row
+# refinement applies is formed inside the factor's launches, row by row, and
+# the last 32 columns take their TRSM inside the trailing update
 @pytest.mark.parametrize("n", [64, 200, 441, 1000, 2000])
 def test_dense_spd_solve_mixed_refines_to_fp64(gba, n):
     rng = np.random.default_rng(n + 1)
