@@ -47,6 +47,12 @@ int guarded(F&& f) {
     }
 }
 
+// Argument checks of the entry points: a null pointer where one is required
+// is an error return, not a crash.
+void need(bool ok, const char* what) {
+    if (!ok) throw Error(what);
+}
+
 gba_solver_config config_or_default(const gba_solver_config* cfg) {
     gba_solver_config c;
     gba_default_config(&c);
@@ -76,6 +82,7 @@ void gba_default_config(gba_solver_config* c) {
 int gba_solve_bal(gba_problem_view* prob, const gba_lm_options* opts, const gba_solver_config* cfg,
                   gba_lm_result* res) {
     return guarded([&] {
+        need(prob && res, "gba_solve_bal: null problem or result");
         const double t0 = now_s();
         gba_lm_options o;
         gba_default_options(&o);
@@ -92,6 +99,7 @@ int gba_solve_bal(gba_problem_view* prob, const gba_lm_options* opts, const gba_
 
 int gba_solver_create(const gba_problem_view* prob, const gba_solver_config* cfg, gba_solver** out) {
     return guarded([&] {
+        need(prob && out, "gba_solver_create: null problem or handle");
         *out = nullptr;
         auto h = std::make_unique<gba_solver>();
         h->s = std::make_unique<Solver>(*prob, config_or_default(cfg));
@@ -102,6 +110,7 @@ int gba_solver_create(const gba_problem_view* prob, const gba_solver_config* cfg
 
 int gba_solver_solve(gba_solver* s, const gba_lm_options* opts, gba_lm_result* res) {
     return guarded([&] {
+        need(s && res, "gba_solver_solve: null handle or result");
         const double t0 = now_s();
         gba_lm_options o;
         gba_default_options(&o);
@@ -115,6 +124,7 @@ int gba_solver_solve(gba_solver* s, const gba_lm_options* opts, gba_lm_result* r
 
 int gba_solver_params(gba_solver* s, double* cameras, double* points) {
     return guarded([&] {
+        need(s && cameras && points, "gba_solver_params: null argument");
         s->s->download_params(cameras, points);
         return 0;
     });
@@ -122,6 +132,7 @@ int gba_solver_params(gba_solver* s, double* cameras, double* points) {
 
 int gba_solver_time_kernel(gba_solver* s, int kernel, double lambda, int reps, double* avg_ms) {
     return guarded([&] {
+        need(s && avg_ms, "gba_solver_time_kernel: null argument");
         *avg_ms = s->s->time_kernel(kernel, lambda, reps < 1 ? 1 : reps);
         return 0;
     });
@@ -129,6 +140,7 @@ int gba_solver_time_kernel(gba_solver* s, int kernel, double lambda, int reps, d
 
 int gba_solver_stats(gba_solver* s, double* pair_count, int* n, int* n_padded) {
     return guarded([&] {
+        need(s, "gba_solver_stats: null handle");
         if (pair_count) *pair_count = s->s->pair_count();
         if (n) *n = s->s->n();
         if (n_padded) *n_padded = s->s->npad();
@@ -144,6 +156,7 @@ struct gba_batch {
 
 int gba_batch_create(const gba_problem_view* probs, int count, gba_batch** out) {
     return guarded([&] {
+        need(out, "gba_batch_create: null handle");
         *out = nullptr;
         if (!probs || count <= 0) return set_error("gba_batch_create: no problems");
         auto h = std::make_unique<gba_batch>();
@@ -155,6 +168,7 @@ int gba_batch_create(const gba_problem_view* probs, int count, gba_batch** out) 
 
 int gba_batch_solve(gba_batch* b, const gba_lm_options* opts, gba_lm_result* results, int* rounds) {
     return guarded([&] {
+        need(b && results, "gba_batch_solve: null handle or results");
         gba_lm_options o;
         gba_default_options(&o);
         if (opts) o = *opts;
@@ -166,6 +180,7 @@ int gba_batch_solve(gba_batch* b, const gba_lm_options* opts, gba_lm_result* res
 
 int gba_batch_params(gba_batch* b, int index, double* cameras, double* points) {
     return guarded([&] {
+        need(b && cameras && points, "gba_batch_params: null argument");
         b->s->download_params(index, cameras, points);
         return 0;
     });
@@ -178,6 +193,7 @@ int gba_solve_bal_batched(gba_problem_view* probs, int count, const gba_lm_optio
     return guarded([&] {
         const double t0 = now_s();
         if (!probs || count <= 0) return set_error("gba_solve_bal_batched: no problems");
+        need(results, "gba_solve_bal_batched: null results");
         BatchSolver bs(probs, count);
         gba_lm_options o;
         gba_default_options(&o);
@@ -196,6 +212,7 @@ int gba_solve_bal_batched(gba_problem_view* probs, int count, const gba_lm_optio
 int gba_linearize(const gba_problem_view* prob, int precision, double* U, double* V, double* W,
                   double* bc, double* bp, double* sse, unsigned long long* anomalies) {
     return guarded([&] {
+        need(prob && sse && anomalies, "gba_linearize: null argument");
         gba_solver_config c;
         gba_default_config(&c);
         c.precision = precision;
@@ -209,6 +226,7 @@ int gba_linearize(const gba_problem_view* prob, int precision, double* U, double
 int gba_schur_step(const gba_problem_view* prob, int precision, double lambda, double* dc, double* dp,
                    double* denom, int* status, double* S_out, double* rhs_out) {
     return guarded([&] {
+        need(prob && dc && dp && denom && status, "gba_schur_step: null argument");
         gba_solver_config c;
         gba_default_config(&c);
         c.precision = precision;
@@ -250,6 +268,7 @@ int gba_cost(const gba_problem_view* prob, double* sse, unsigned long long* anom
 int gba_linearize_cfg(const gba_problem_view* prob, const gba_solver_config* cfg, double* U, double* V,
                       double* W, double* bc, double* bp, double* sse, unsigned long long* anomalies) {
     return guarded([&] {
+        need(prob && sse && anomalies, "gba_linearize_cfg: null argument");
         Solver s(*prob, config_or_default(cfg));
         s.linearize(sse, anomalies);
         s.download_linearization(U, V, W, bc, bp);
@@ -260,6 +279,7 @@ int gba_linearize_cfg(const gba_problem_view* prob, const gba_solver_config* cfg
 int gba_cost_cfg(const gba_problem_view* prob, const gba_solver_config* cfg, double* sse,
                  unsigned long long* anomalies) {
     return guarded([&] {
+        need(prob && sse && anomalies, "gba_cost: null argument");
         Solver s(*prob, config_or_default(cfg));
         s.cost(sse, anomalies);
         return 0;
@@ -269,16 +289,27 @@ int gba_cost_cfg(const gba_problem_view* prob, const gba_solver_config* cfg, dou
 int gba_dense_spd_solve(int n, int precision, const double* A, double* b, int* ok, int* refine_iters) {
     return guarded([&] {
         if (n <= 0) return set_error("gba_dense_spd_solve: n must be positive");
+        need(A && b && ok, "gba_dense_spd_solve: null argument");
         const int npad = (n + kTile - 1) / kTile * kTile, nblk = npad / kTile;
         const size_t np = static_cast<size_t>(npad);
         const bool mixed = precision == GBA_PRECISION_MIXED;
-        hipStream_t st;
-        GBA_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
         DevBuf<double> S64, D64, rhs, y, x, work;
         DevBuf<float> S32, S32low, D32, X32, T32;
         DevBuf<int> sync;
         DevBuf<TrialFlags> flags;
         DevBuf<PassStatus> status;
+        // declared after the buffers: on an error it drains and goes first
+        struct StreamGuard {
+            hipStream_t s = nullptr;
+            ~StreamGuard() {
+                if (s) {
+                    (void)hipStreamSynchronize(s);
+                    (void)hipStreamDestroy(s);
+                }
+            }
+        } guard;
+        GBA_HIP(hipStreamCreateWithFlags(&guard.s, hipStreamNonBlocking));
+        hipStream_t st = guard.s;
         // host copy: row-major with identity padding; A is column-major
         std::vector<double> h(np * np, 0.0);
         for (size_t rr = 0; rr < np; ++rr)
@@ -334,7 +365,6 @@ int gba_dense_spd_solve(int n, int precision, const double* A, double* b, int* o
         std::vector<double> hx(np);
         GBA_HIP(hipMemcpyAsync(hx.data(), x.p, sizeof(double) * np, hipMemcpyDeviceToHost, st));
         GBA_HIP(hipStreamSynchronize(st));
-        GBA_HIP(hipStreamDestroy(st));
         if (hf.solve_err) return set_error("triangular solve timed out");
         *ok = hf.chol_fail ? 0 : 1;
         if (*ok) std::memcpy(b, hx.data(), sizeof(double) * n);

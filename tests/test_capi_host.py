@@ -125,3 +125,25 @@ def test_launch_bounds_generator_matches_reference_recipe(gba):
     assert prob.num_observations == 2000
     assert list(prob.obs_camera[:4]) == [0, 1, 1, 2]
     assert np.all(prob.obs_point[::2] == np.arange(1000))
+
+
+def test_entry_points_reject_null_arguments(gba):
+    # checked before any device call: an error return and a message, no crash
+    lib = gba.lib()
+    calls = [
+        ("gba_solve_bal", (None, None, None, None)),
+        ("gba_solver_create", (None, None, None)),
+        ("gba_solver_solve", (None, None, None)),
+        ("gba_solver_params", (None, None, None)),
+        ("gba_batch_create", (None, 0, None)),
+        ("gba_batch_solve", (None, None, None, None)),
+        ("gba_solve_bal_batched", (None, 0, None, None)),
+        ("gba_linearize", (None, 0, None, None, None, None, None, None, None)),
+        ("gba_cost", (None, None, None)),
+        ("gba_dense_spd_solve", (4, 0, None, None, None, None)),
+        ("gba_encode_obs_fp16", (None, None, None)),
+    ]
+    for name, args in calls:
+        assert getattr(lib, name)(*args) != 0, name
+        msg = lib.gba_last_error().decode()
+        assert name.replace("_bal_batched", "") in msg or "null" in msg or "no problems" in msg, (name, msg)
