@@ -941,32 +941,58 @@ __global__ __launch_bounds__(256) void k_trsv_bwd(const T* __restrict__ L, int l
 __global__ __launch_bounds__(256) void k_symv_tiles(const float* __restrict__ A, int ld,
                                                     const double* __restrict__ x,
                                                     double* __restrict__ rp, double* __restrict__ cp) {
+    __shared__ float s_a[kTile][kTile + 1];
+    __shared__ double s_xi[kTile], s_xj[kTile];
     __shared__ double s_col[4][kTile];
-    __shared__ double s_row[kTile];
-    const int t = blockIdx.x;
+    const int t = blockIdx.x, tid = threadIdx.x;
     int i = static_cast<int>((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
     while (i * (i + 1) / 2 > t) --i;
     while ((i + 1) * (i + 2) / 2 <= t) ++i;
     const int j = t - i * (i + 1) / 2;
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const bool diag = (i == j);
-    const double xj = x[j * kTile + lane];
-    double cacc = 0.0;
-    for (int rr = 0; rr < 16; ++rr) {
-        const int r = 16 * w + rr;
-        const double v = static_cast<double>(A[static_cast<size_t>(i * kTile + r) * ld + j * kTile + lane]);
-        const double xi = x[i * kTile + r];
-        const double rs = wave_sum((!diag || lane <= r) ? v * xj : 0.0);
-        if (lane == 0) s_row[r] = rs;
-        cacc += (!diag || lane < r) ? v * xi : 0.0;
+    {
+        // the tile into LDS with coalesced loads, all in flight at once
+        const float* src = A + static_cast<size_t>(i * kTile) * ld + j * kTile;
+        float tmp[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int q = tid + 256 * u;
+            tmp[u] = src[static_cast<size_t>(q >> 6) * ld + (q & 63)];
+        }
+        if (tid < kTile) s_xj[tid] = x[j * kTile + tid];
+        else if (tid < 2 * kTile) s_xi[tid - kTile] = x[i * kTile + tid - kTile];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int q = tid + 256 * u;
+            s_a[q >> 6][q & 63] = tmp[u];
+        }
     }
-    s_col[w][lane] = cacc;
     __syncthreads();
-    if (threadIdx.x < kTile) {
-        rp[static_cast<size_t>(t) * kTile + threadIdx.x] = s_row[threadIdx.x];
-        cp[static_cast<size_t>(t) * kTile + threadIdx.x] =
-            (s_col[0][threadIdx.x] + s_col[1][threadIdx.x]) + (s_col[2][threadIdx.x] + s_col[3][threadIdx.x]);
+    // row partials: four threads per row, sixteen columns each, combined by
+    // two lane exchanges (the four are adjacent lanes)
+    {
+        const int r = tid >> 2, c0 = 16 * (tid & 3);
+        double rs = 0.0;
+#pragma unroll
+        for (int c = c0; c < c0 + 16; ++c)
+            rs += (!diag || c <= r) ? static_cast<double>(s_a[r][c]) * s_xj[c] : 0.0;
+        rs += __shfl_xor(rs, 1, kWave);
+        rs += __shfl_xor(rs, 2, kWave);
+        if ((tid & 3) == 0) rp[static_cast<size_t>(t) * kTile + r] = rs;
     }
+    // column partials (strictly lower part of a diagonal tile): one thread per
+    // column and sixteen rows, the four row groups combined in order
+    {
+        const int c = tid & 63, r0 = 16 * (tid >> 6);
+        double cs = 0.0;
+#pragma unroll
+        for (int r = r0; r < r0 + 16; ++r)
+            cs += (!diag || c < r) ? static_cast<double>(s_a[r][c]) * s_xi[r] : 0.0;
+        s_col[tid >> 6][c] = cs;
+    }
+    __syncthreads();
+    if (tid < kTile)
+        cp[static_cast<size_t>(t) * kTile + tid] = (s_col[0][tid] + s_col[1][tid]) + (s_col[2][tid] + s_col[3][tid]);
 }
 
 // y_b = sum_{j <= b} rp[(b, j)] + sum_{i > b} cp[(i, b)]: wavefront g sums
