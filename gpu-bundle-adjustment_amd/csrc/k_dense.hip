@@ -426,11 +426,16 @@ __global__ __launch_bounds__(256) void k_trsm_tiles(T* __restrict__ S, int ld, i
     __shared__ T Ai[kTile][kTile + 1];
     int unit;
     if (!tile_batch_bind(tb, S, ld, D, DinvT, flags, unit)) return;
-    if (any_fail(flags)) return;
     const int i = k + 1 + unit;
     T* Aik = S + static_cast<size_t>(i * kTile) * ld + k * kTile;
     T* LTki = S + static_cast<size_t>(k * kTile) * ld + i * kTile;
     const T* inv = DinvT + static_cast<size_t>(k) * kTile * kTile;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
+    const int li = lane & 15, lk = lane >> 4;
+    // all operand loads first (in bounds whatever the flags say), then the
+    // failure check, so the latencies overlap
+    T bv[16][2];
     {
         T tmp[16];
 #pragma unroll
@@ -439,26 +444,23 @@ __global__ __launch_bounds__(256) void k_trsm_tiles(T* __restrict__ S, int ld, i
             tmp[u] = Aik[static_cast<size_t>(q >> 6) * ld + (q & 63)];
         }
 #pragma unroll
+        for (int kk = 0; kk < kTile; kk += 4) {
+            bv[kk >> 2][0] = inv[(kk + lk) * kTile + wc + li];
+            bv[kk >> 2][1] = inv[(kk + lk) * kTile + wc + 16 + li];
+        }
+        if (any_fail(flags)) return;
+#pragma unroll
         for (int u = 0; u < 16; ++u) {
             const int q = threadIdx.x + 256 * u;
             Ai[q >> 6][q & 63] = tmp[u];
         }
     }
     __syncthreads();
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
-    const int li = lane & 15, lk = lane >> 4;
     typename Acc4<T>::type acc[2][2];
 #pragma unroll
     for (int r = 0; r < 2; ++r)
 #pragma unroll
         for (int c = 0; c < 2; ++c) acc[r][c] = typename Acc4<T>::type{0, 0, 0, 0};
-    T bv[16][2];
-#pragma unroll
-    for (int kk = 0; kk < kTile; kk += 4) {
-        bv[kk >> 2][0] = inv[(kk + lk) * kTile + wc + li];
-        bv[kk >> 2][1] = inv[(kk + lk) * kTile + wc + 16 + li];
-    }
 #pragma unroll
     for (int kk = 0; kk < kTile; kk += 4) {
         const T a0 = Ai[wr + li][kk + lk], a1 = Ai[wr + 16 + li][kk + lk];
@@ -504,14 +506,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
     __shared__ T Aj[kTile][kTile + 2];
     int t;
     if (!tile_batch_bind(tb, S, ld, D, DinvT, flags, t)) return;
-    if (any_fail(flags)) return;
     if constexpr (sizeof(T) == 4) {
         if (X && t >= nup) {
             // workgroups past the update tiles form row k of X = L^-1 (see inv_role)
+            if (any_fail(flags)) return;
             inv_role<T>(t - nup, k, nblk, S, ld, DinvT, X, R, Ai, Aj);
             return;
         }
     }
+    // the failure check below comes after this workgroup's first loads are
+    // issued (they are in bounds either way), so its latency overlaps them
     // tile index -> (ii, jj), jj <= ii, over the trailing lower triangle
     int ii = static_cast<int>((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
     while (ii * (ii + 1) / 2 > t) --ii;
@@ -520,29 +524,63 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
     const int i = k + 1 + ii, j = k + 1 + jj;
     const T* LTi = S + static_cast<size_t>(k * kTile) * ld + i * kTile;
     const T* LTj = S + static_cast<size_t>(k * kTile) * ld + j * kTile;
+    T* Aij = S + static_cast<size_t>(i * kTile) * ld + j * kTile;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
     const int li = lane & 15, lk = lane >> 4;
+    // block 0 reads its tile (k+1, k+1) up front: no load of this launch
+    // writes it, and its latency then overlaps the operand loads
+    T pd[16];
+    if (t == 0) {
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int q = threadIdx.x + 256 * u;
+            pd[u] = Aij[static_cast<size_t>(q >> 6) * ld + (q & 63)];
+        }
+    }
     typename Acc4<T>::type acc[2][2];
 #pragma unroll
     for (int r = 0; r < 2; ++r)
 #pragma unroll
         for (int c = 0; c < 2; ++c) acc[r][c] = typename Acc4<T>::type{0, 0, 0, 0};
     if (fused) {
-        // L_ik = A_ik Linv_kk^T and L_jk likewise, into Ai and Aj
+        // L_ik = A_ik Linv_kk^T and L_jk likewise, into Ai and Aj; every
+        // operand load is issued before the first barrier
         const T* inv = DinvT + static_cast<size_t>(k) * kTile * kTile;
         const bool two = (j != i);
-        tile_to_lds<T>(Ai, S + static_cast<size_t>(i * kTile) * ld + k * kTile, ld);
-        if (two) tile_to_lds<T>(Aj, S + static_cast<size_t>(j * kTile) * ld + k * kTile, ld);
+        T bv[16][2];
+#pragma unroll
+        for (int kk = 0; kk < kTile; kk += 4) {
+            bv[kk >> 2][0] = inv[(kk + lk) * kTile + wc + li];
+            bv[kk >> 2][1] = inv[(kk + lk) * kTile + wc + 16 + li];
+        }
+        {
+            const T* srci = S + static_cast<size_t>(i * kTile) * ld + k * kTile;
+            const T* srcj = S + static_cast<size_t>(j * kTile) * ld + k * kTile;
+            T ta[16], tb[16];
+#pragma unroll
+            for (int u = 0; u < 16; ++u) {
+                const int q = threadIdx.x + 256 * u;
+                ta[u] = srci[static_cast<size_t>(q >> 6) * ld + (q & 63)];
+                if (two) tb[u] = srcj[static_cast<size_t>(q >> 6) * ld + (q & 63)];
+            }
+            if (any_fail(flags)) return;
+#pragma unroll
+            for (int u = 0; u < 16; ++u) {
+                const int q = threadIdx.x + 256 * u;
+                Ai[q >> 6][q & 63] = ta[u];
+                if (two) Aj[q >> 6][q & 63] = tb[u];
+            }
+        }
         __syncthreads();
         typename Acc4<T>::type acc2[2][2];
 #pragma unroll
         for (int r = 0; r < 2; ++r)
 #pragma unroll
             for (int c = 0; c < 2; ++c) acc2[r][c] = typename Acc4<T>::type{0, 0, 0, 0};
-#pragma unroll 4
+#pragma unroll
         for (int kk = 0; kk < kTile; kk += 4) {
-            const T b0 = inv[(kk + lk) * kTile + wc + li], b1 = inv[(kk + lk) * kTile + wc + 16 + li];
+            const T b0 = bv[kk >> 2][0], b1 = bv[kk >> 2][1];
             const T a0 = Ai[wr + li][kk + lk], a1 = Ai[wr + 16 + li][kk + lk];
             acc[0][0] = mfma16(a0, b0, acc[0][0]);
             acc[0][1] = mfma16(a0, b1, acc[0][1]);
@@ -591,6 +629,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
         }
         __syncthreads();  // every wave is done with Ai / Aj before the lookahead reuses them
     } else {
+        if (any_fail(flags)) return;
 #pragma unroll
         for (int kk = 0; kk < kTile; kk += 4) {
             const size_t ro = static_cast<size_t>(kk + lk) * ld;
@@ -602,9 +641,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
             acc[1][1] = mfma16(a1, b1, acc[1][1]);
         }
     }
-    T* Aij = S + static_cast<size_t>(i * kTile) * ld + j * kTile;
     if (t == 0) {
-        // lookahead: finish tile (k+1, k+1) in LDS and factor it
+        // lookahead: finish tile (k+1, k+1) in LDS (the prefetched tile via
+        // Aj, minus the product) and factor it
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int q = threadIdx.x + 256 * u;
+            Aj[q >> 6][q & 63] = pd[u];
+        }
+        __syncthreads();
 #pragma unroll
         for (int r = 0; r < 2; ++r)
 #pragma unroll
@@ -612,7 +657,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
                     const int row = wr + 16 * r + acc_row<T>(lk, q), col = wc + 16 * c + li;
-                    Ai[row][col] = Aij[static_cast<size_t>(row) * ld + col] - acc[r][c][q];
+                    Ai[row][col] = Aj[row][col] - acc[r][c][q];
                 }
         __syncthreads();
         const bool ok = tile_potrf_inv<T>(Ai, Aj);
