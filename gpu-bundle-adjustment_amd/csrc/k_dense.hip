@@ -433,22 +433,30 @@ __global__ __launch_bounds__(256) void k_trsm_tiles(T* __restrict__ S, int ld, i
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
     const int li = lane & 15, lk = lane >> 4;
-    // all operand loads first (in bounds whatever the flags say), then the
-    // failure check, so the latencies overlap
+    // FP32: all operand loads first (in bounds whatever the flags say), then
+    // the failure check, so the latencies overlap; FP64 keeps the inverse
+    // fragments for after the barrier (the extra registers cost it more)
+    constexpr bool kEarly = sizeof(T) == 4;
     T bv[16][2];
+    auto load_inv = [&]() {
+#pragma unroll
+        for (int kk = 0; kk < kTile; kk += 4) {
+            bv[kk >> 2][0] = inv[(kk + lk) * kTile + wc + li];
+            bv[kk >> 2][1] = inv[(kk + lk) * kTile + wc + 16 + li];
+        }
+    };
     {
+        if (!kEarly && any_fail(flags)) return;
         T tmp[16];
 #pragma unroll
         for (int u = 0; u < 16; ++u) {
             const int q = threadIdx.x + 256 * u;
             tmp[u] = Aik[static_cast<size_t>(q >> 6) * ld + (q & 63)];
         }
-#pragma unroll
-        for (int kk = 0; kk < kTile; kk += 4) {
-            bv[kk >> 2][0] = inv[(kk + lk) * kTile + wc + li];
-            bv[kk >> 2][1] = inv[(kk + lk) * kTile + wc + 16 + li];
+        if constexpr (kEarly) {
+            load_inv();
+            if (any_fail(flags)) return;
         }
-        if (any_fail(flags)) return;
 #pragma unroll
         for (int u = 0; u < 16; ++u) {
             const int q = threadIdx.x + 256 * u;
@@ -456,6 +464,7 @@ __global__ __launch_bounds__(256) void k_trsm_tiles(T* __restrict__ S, int ld, i
         }
     }
     __syncthreads();
+    if constexpr (!kEarly) load_inv();
     typename Acc4<T>::type acc[2][2];
 #pragma unroll
     for (int r = 0; r < 2; ++r)
@@ -530,8 +539,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
     const int li = lane & 15, lk = lane >> 4;
     // block 0 reads its tile (k+1, k+1) up front: no load of this launch
     // writes it, and its latency then overlaps the operand loads
+    // (FP32 only: in FP64 the sixteen extra registers cost more than the
+    // overlap gains)
+    constexpr bool kPrefetch = sizeof(T) == 4;
     T pd[16];
-    if (t == 0) {
+    if (kPrefetch && t == 0) {
 #pragma unroll
         for (int u = 0; u < 16; ++u) {
             const int q = threadIdx.x + 256 * u;
@@ -642,14 +654,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
         }
     }
     if (t == 0) {
-        // lookahead: finish tile (k+1, k+1) in LDS (the prefetched tile via
-        // Aj, minus the product) and factor it
+        // lookahead: finish tile (k+1, k+1) in LDS (the tile, prefetched via
+        // Aj in FP32, minus the product) and factor it
+        if constexpr (kPrefetch) {
 #pragma unroll
-        for (int u = 0; u < 16; ++u) {
-            const int q = threadIdx.x + 256 * u;
-            Aj[q >> 6][q & 63] = pd[u];
+            for (int u = 0; u < 16; ++u) {
+                const int q = threadIdx.x + 256 * u;
+                Aj[q >> 6][q & 63] = pd[u];
+            }
+            __syncthreads();
         }
-        __syncthreads();
 #pragma unroll
         for (int r = 0; r < 2; ++r)
 #pragma unroll
@@ -657,7 +671,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
                     const int row = wr + 16 * r + acc_row<T>(lk, q), col = wc + 16 * c + li;
-                    Ai[row][col] = Aj[row][col] - acc[r][c][q];
+                    Ai[row][col] = (kPrefetch ? Aj[row][col] : Aij[static_cast<size_t>(row) * ld + col]) - acc[r][c][q];
                 }
         __syncthreads();
         const bool ok = tile_potrf_inv<T>(Ai, Aj);
