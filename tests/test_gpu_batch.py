@@ -78,3 +78,14 @@ def test_batch_of_one_and_one_shot(gba):
     check_pair(rb, rs)
     # the one-shot form writes the accepted parameters back
     assert not np.allclose(q.cameras, p.cameras)
+
+
+def test_batch_ragged_problem(gba):
+    # a camera without observations and points seen once, beside a regular problem
+    from test_gpu_path import ragged
+    probs = [ragged(gba), gba.generate(5, 4)]
+    seq = [gba.Solver(p).solve() for p in probs]
+    res = gba.Batch(probs).solve()
+    for rb, rs in zip(res, seq):
+        check_pair(rb, rs)
+        assert rb.termination == "CONVERGENCE"

@@ -17,6 +17,21 @@ def heavy(gba, seed=3):
     return gba.generate(spec=s, seed=seed)
 
 
+def ragged(gba):
+    # a camera without observations (zero U block: its S diagonal is only the
+    # damping floor lambda 1e-12) and 300 points seen once (rank-2 V blocks)
+    p = gba.generate(2, 1)
+    keep = np.ones(p.num_observations, bool)
+    seen = set()
+    for e, q in enumerate(p.obs_point):
+        if q < 300:
+            keep[e] = q not in seen
+            seen.add(q)
+    cams = np.vstack([p.cameras, p.cameras[:1] * 1.01])
+    return gba.BALProblem(cams, p.points.copy(), p.obs_camera[keep].copy(), p.obs_point[keep].copy(),
+                          p.obs_pixel[keep].copy())
+
+
 def arel(a, b):
     """max |a-b| relative to max |b| (per array)."""
     return float(np.max(np.abs(np.asarray(a) - np.asarray(b))) / max(1e-300, np.max(np.abs(b))))
@@ -171,6 +186,21 @@ def test_lm_mixed_reaches_fp64_cost(gba):
     assert r32.termination == "CONVERGENCE"
     assert abs(r32.final_cost - r64.final_cost) <= 1e-6 * r64.final_cost
     assert r32.max_refine_residual <= 1e-8
+
+
+def test_lm_ragged_problem_matches_oracle(gba, orc):
+    q = ragged(gba)
+    assert np.bincount(q.obs_camera, minlength=q.num_cameras)[-1] == 0
+    opts = dict(initial_lambda=1e-4)
+    r = gba.solve_bal_gpu(q.copy(), gba.LMOptions(**opts))
+    o = orc.lm_solve(q.copy(), **opts)
+    assert r.termination == o["termination"] == "CONVERGENCE"
+    assert r.iterations == o["iterations"] and r.anomalies == 0
+    assert abs(r.initial_cost - o["initial_cost"]) <= 1e-12 * o["initial_cost"]
+    assert abs(r.final_cost - o["final_cost"]) <= 1e-9 * o["final_cost"]
+    r32 = gba.solve_bal_gpu(q.copy(), gba.LMOptions(**opts), gba.PRECISION_MIXED)
+    assert r32.termination == "CONVERGENCE"
+    assert abs(r32.final_cost - r.final_cost) <= 1e-6 * r.final_cost
 
 
 def test_lm_failure_and_max_iterations_paths(gba):
