@@ -12,28 +12,46 @@
 // lane forms Z = V'^-1 W_f^T (3x9) and adds -W_e Z row by row.  Each block of
 // S is written by exactly one workgroup with plain stores; no global atomics.
 // The rhs term T_e bp comes from the pairs with e == f, in FP64.
-// FP64 path: everything double.  Mixed path: W is FP32 (widened; products and
-// LDS accumulation stay FP64), the result is written diagonally scaled
-// (D S D, D = (U_ii + lambda U_ii)^-1/2) into the factor copy and a second
-// lower copy used by the FP64 refinement residual.
+// FP64 path: everything double.  Mixed path: W is FP32 (widened; products in
+// FP64), the result is written diagonally scaled (D S D,
+// D = (U_ii + lambda U_ii)^-1/2) into the factor copy and a second lower copy
+// used by the FP64 refinement residual, and the LDS sums are 64-bit fixed
+// point (2^-46) of the scaled products, added with integer LDS atomics
+// (on gfx950 ds_add_u64 runs at 1.3x the rate of ds_add_f64, and the sums do
+// not depend on the order of the adds).  No partial sum can leave [-1, 1]: by
+// Cauchy-Schwarz in the V'^-1 inner product, sum_p |w_ei V'^-1 w_fj| is at
+// most sqrt(U_a,ii U_b,jj), since each point's W_p V'_p^-1 W_p^T is dominated
+// by its share of U, and D_i <= U_ii^-1/2.  A pair with a non-finite input
+// (which FP64 sums would carry into the factor as a failed pivot) raises
+// chol_fail instead.
 #include "device_common.hpp"
 #include "launchers.hpp"
 
 #include <algorithm>
 #include <cstdlib>
+#include <type_traits>
 
 namespace gba_amd {
 
 namespace {
 
-// WT: stored W type; AT: LDS accumulator (and product) type; ST: type of S.
+// WT: stored W type; AT: LDS accumulator type (int: fixed point of the
+// scaled products, which are then formed in FP64; otherwise also the product
+// type); ST: type of S.
+constexpr double kFix = 70368744177664.0;  // 2^46: fixed-point scale of the mixed sums
+// Round-to-nearest of |x| < 2^51 to an integer by one FP64 add: x + 1.5 2^52
+// keeps round(x) in the low mantissa bits, read back as a 64-bit integer.
+constexpr double kRoundMagic = 6755399441055744.0;  // 1.5 * 2^52
+constexpr long long kRoundBits = 0x4338000000000000LL;
 template <int NT, typename WT, typename AT, typename ST>
 __global__ __launch_bounds__(NT) void k_schur(
     DevProblem pb, const SchurTask* __restrict__ tasks, const int2* __restrict__ pairs,
     const WT* __restrict__ W, const double* __restrict__ Vinv6, const double* __restrict__ VB,
     const double* __restrict__ U, const double* __restrict__ bc, double lambda,
     const double* __restrict__ dscale, ST* __restrict__ Sfac, ST* __restrict__ Slow, int ld,
-    double* __restrict__ rhs, const TrialFlags* flags) {
+    double* __restrict__ rhs, TrialFlags* flags) {
+    constexpr bool kFixed = std::is_same<AT, long long>::value;
+    typedef typename std::conditional<kFixed, double, AT>::type PT;
     extern __shared__ __align__(16) unsigned char smem[];
     AT* acc = reinterpret_cast<AT*>(smem);
     __shared__ double s_rhs[NT / 64][9];
@@ -49,10 +67,14 @@ __global__ __launch_bounds__(NT) void k_schur(
     const int tid = threadIdx.x;
     const int nb = hi - lo;
     for (int t = tid; t < nb * 81; t += NT) acc[t] = AT(0);
+    PT da[9];  // fixed point: row scales of camera a
+#pragma unroll
+    for (int i = 0; i < 9; ++i) da[i] = kFixed ? static_cast<PT>(dscale[9 * a + i]) : PT(1);
     __syncthreads();
     double racc[9];
 #pragma unroll
     for (int i = 0; i < 9; ++i) racc[i] = 0.0;
+    bool bad = false;  // fixed point: a product out of range
     for (long long q = task.begin + tid; q < task.end; q += NT) {
         const int2 ef = pairs[q];
         const int e = ef.x, f = ef.y;
@@ -61,19 +83,26 @@ __global__ __launch_bounds__(NT) void k_schur(
         const double* vi = Vinv6 + 6 * static_cast<size_t>(p);
         const double v00 = vi[0], v10 = vi[1], v11 = vi[2], v20 = vi[3], v21 = vi[4], v22 = vi[5];
         // Z = V'^-1 W_f^T, column j = V'^-1 (row j of W_f)
-        const AT a00 = static_cast<AT>(v00), a10 = static_cast<AT>(v10), a11 = static_cast<AT>(v11),
-                 a20 = static_cast<AT>(v20), a21 = static_cast<AT>(v21), a22 = static_cast<AT>(v22);
-        AT z0[9], z1[9], z2[9];
+        const PT a00 = static_cast<PT>(v00), a10 = static_cast<PT>(v10), a11 = static_cast<PT>(v11),
+                 a20 = static_cast<PT>(v20), a21 = static_cast<PT>(v21), a22 = static_cast<PT>(v22);
+        PT z0[9], z1[9], z2[9];
         {
             WT wf[28];
             load_w_raw(W, static_cast<size_t>(f), wf);
 #pragma unroll
             for (int j = 0; j < 9; ++j) {
-                const AT w0 = static_cast<AT>(wf[3 * j]), w1 = static_cast<AT>(wf[3 * j + 1]),
-                         w2 = static_cast<AT>(wf[3 * j + 2]);
+                const PT w0 = static_cast<PT>(wf[3 * j]), w1 = static_cast<PT>(wf[3 * j + 1]),
+                         w2 = static_cast<PT>(wf[3 * j + 2]);
                 z0[j] = a00 * w0 + a10 * w1 + a20 * w2;
                 z1[j] = a10 * w0 + a11 * w1 + a21 * w2;
                 z2[j] = a20 * w0 + a21 * w1 + a22 * w2;
+                if constexpr (kFixed) {
+                    // column scale of camera b, with the fixed-point scale folded in
+                    const PT sb = static_cast<PT>(dscale[9 * b + j]) * kFix;
+                    z0[j] *= sb;
+                    z1[j] *= sb;
+                    z2[j] *= sb;
+                }
             }
         }
         // W_e rows are read where used: the lanes of an e group share them
@@ -90,14 +119,37 @@ __global__ __launch_bounds__(NT) void k_schur(
                            static_cast<double>(we[3 * i + 2]) * vb2;
         }
         AT* dst = acc + (b - lo) * 81;
+        if constexpr (kFixed) {
+            // the inputs of this pair are finite (a sum of magnitudes is);
+            // then no scaled product leaves the fixed-point range
+            PT mag = PT(0);
+#pragma unroll
+            for (int j = 0; j < 9; ++j) mag += fabs(z0[j]) + fabs(z1[j]) + fabs(z2[j]);
+#pragma unroll
+            for (int i = 0; i < 27; ++i) mag += fabs(static_cast<PT>(we[i]));
+            if (!(mag < 1e300)) {
+                bad = true;
+                continue;
+            }
+        }
 #pragma unroll
         for (int i = 0; i < 9; ++i) {
-            const AT x0 = static_cast<AT>(we[3 * i]), x1 = static_cast<AT>(we[3 * i + 1]),
-                     x2 = static_cast<AT>(we[3 * i + 2]);
+            const PT x0 = static_cast<PT>(we[3 * i]) * da[i], x1 = static_cast<PT>(we[3 * i + 1]) * da[i],
+                     x2 = static_cast<PT>(we[3 * i + 2]) * da[i];
 #pragma unroll
-            for (int j = 0; j < 9; ++j) atomicAdd(dst + 9 * i + j, -(x0 * z0[j] + x1 * z1[j] + x2 * z2[j]));
+            for (int j = 0; j < 9; ++j) {
+                const PT prod = -(x0 * z0[j] + x1 * z1[j] + x2 * z2[j]);
+                if constexpr (kFixed) {
+                    const long long v = __double_as_longlong(prod + kRoundMagic) - kRoundBits;
+                    atomicAdd(reinterpret_cast<unsigned long long*>(dst + 9 * i + j),
+                              static_cast<unsigned long long>(v));
+                } else {
+                    atomicAdd(dst + 9 * i + j, prod);
+                }
+            }
         }
     }
+    if (kFixed && bad) atomicOr(&flags[q].chol_fail, 1);
     if (own_rhs) {
         const int w = tid >> 6, lane = tid & 63;
 #pragma unroll
@@ -118,12 +170,18 @@ __global__ __launch_bounds__(NT) void k_schur(
         const int bb = t / 81, ij = t - 81 * bb;
         const int i = ij / 9, j = ij - 9 * i;
         const int b = lo + bb;
-        double v = static_cast<double>(acc[t]);
+        double v = static_cast<double>(acc[t]) * (kFixed ? 1.0 / kFix : 1.0);
+        double u = 0.0;
         if (b == a) {
-            v += U[81 * static_cast<size_t>(a) + 9 * i + j];
-            if (i == j) v += lambda * damp_diag(U[81 * static_cast<size_t>(a) + 10 * i]);
+            u = U[81 * static_cast<size_t>(a) + 9 * i + j];
+            if (i == j) u += lambda * damp_diag(U[81 * static_cast<size_t>(a) + 10 * i]);
         }
-        if (dscale) v *= dscale[9 * a + i] * dscale[9 * b + j];
+        if (kFixed)
+            v += u * dscale[9 * a + i] * dscale[9 * b + j];  // the sums are scaled already
+        else if (dscale)
+            v = (v + u) * dscale[9 * a + i] * dscale[9 * b + j];
+        else
+            v += u;
         const int row = 9 * (a - task.a0) + i, col = 9 * (b - task.a0) + j;
         Sfac[static_cast<size_t>(row) * ld + col] = static_cast<ST>(v);
         if (Slow) Slow[static_cast<size_t>(row) * ld + col] = static_cast<ST>(v);
@@ -235,7 +293,7 @@ void launch_one(hipStream_t st, const SchurShape& sh, const DevProblem& pb, cons
                 int num_tasks, const int2* pairs, const WT* W, const double* Vinv6,
                 const double* VB, const double* U, const double* bc, double lambda,
                 const double* dscale, ST* Sfac, ST* Slow, int ld, double* rhs,
-                const TrialFlags* flags) {
+                TrialFlags* flags) {
     const size_t lds = static_cast<size_t>(sh.cb) * 81 * sizeof(AT);
     if (lds > 64 * 1024 - 4096)
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_schur<NT, WT, AT, ST>),
@@ -253,18 +311,21 @@ SchurShape schur_shape(int C, bool /*w32*/) {
     if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
         v = 64 * 1024;
     (void)hipGetLastError();
-    // FP64 accumulator LDS per workgroup: 128 KB (one workgroup of 1024
-    // threads per CU) unless GBA_SCHUR_LDS_KB asks otherwise.  FP64 LDS adds
-    // are the accumulation in both forms: on gfx950 ds_add_f32 measured about
-    // eight times slower than ds_add_f64 for this access pattern.
+    // Accumulator LDS per workgroup: 128 KB (one workgroup of 1024 threads
+    // per CU) unless GBA_SCHUR_LDS_KB asks otherwise; FP64 sums in the FP64
+    // form, 64-bit fixed point in the mixed form (on gfx950, LDS atomics
+    // measured 7.5 lanes/clk/CU for ds_add_f64, 9.75 for ds_add_u64, 13.9
+    // for ds_add_u32 and 0.33 for ds_add_f32; 32-bit fixed point, 2^-30,
+    // was faster still but too coarse for the small entries).
     int bytes = 128 * 1024;
     if (const char* env = std::getenv("GBA_SCHUR_LDS_KB")) {
         const int kb = std::atoi(env);
         if (kb >= 8) bytes = kb * 1024;
     }
     bytes = std::min(bytes, v - 4096);
-    sh.cb = std::max(1, std::min(C, bytes / (81 * 8)));
-    sh.threads = (static_cast<size_t>(sh.cb) * 81 * 8 > 80 * 1024) ? 1024 : 512;
+    const int acc_bytes = 8;  // FP64 or 64-bit fixed-point LDS sums
+    sh.cb = std::max(1, std::min(C, bytes / (81 * acc_bytes)));
+    sh.threads = (static_cast<size_t>(sh.cb) * 81 * acc_bytes > 80 * 1024) ? 1024 : 512;
     return sh;
 }
 
@@ -272,14 +333,14 @@ void launch_schur(hipStream_t st, const SchurShape& sh, const DevProblem& pb,
                   const SchurTask* tasks, int num_tasks, const int2* pairs, const double* W64,
                   const float* W32, const double* Vinv6, const double* VB, const double* U,
                   const double* bc, double lambda, const double* dscale, double* S64, float* S32fac,
-                  float* S32low, int ld, double* rhs, const TrialFlags* flags) {
+                  float* S32low, int ld, double* rhs, TrialFlags* flags) {
     if (W32) {
         if (sh.threads == 1024)
-            launch_one<1024, float, double, float>(st, sh, pb, tasks, num_tasks, pairs, W32, Vinv6, VB,
-                                                   U, bc, lambda, dscale, S32fac, S32low, ld, rhs, flags);
+            launch_one<1024, float, long long, float>(st, sh, pb, tasks, num_tasks, pairs, W32, Vinv6, VB,
+                                                      U, bc, lambda, dscale, S32fac, S32low, ld, rhs, flags);
         else
-            launch_one<512, float, double, float>(st, sh, pb, tasks, num_tasks, pairs, W32, Vinv6, VB, U,
-                                                  bc, lambda, dscale, S32fac, S32low, ld, rhs, flags);
+            launch_one<512, float, long long, float>(st, sh, pb, tasks, num_tasks, pairs, W32, Vinv6, VB, U,
+                                                     bc, lambda, dscale, S32fac, S32low, ld, rhs, flags);
     } else {
         if (sh.threads == 1024)
             launch_one<1024, double, double, double>(st, sh, pb, tasks, num_tasks, pairs, W64, Vinv6, VB, U,

@@ -45,7 +45,7 @@ void launch_schur(hipStream_t st, const SchurShape& sh, const DevProblem& pb,
                   const SchurTask* tasks, int num_tasks, const int2* pairs, const double* W64,
                   const float* W32, const double* Vinv6, const double* VB, const double* U,
                   const double* bc, double lambda, const double* dscale, double* S64, float* S32fac,
-                  float* S32low, int ld, double* rhs, const TrialFlags* flags);
+                  float* S32low, int ld, double* rhs, TrialFlags* flags);
 // Diagonal blocks (both triangles) + rhs, one workgroup per camera.  desc:
 // per-problem S layout (batched) or NULL (single problem, leading dimension
 // ld).  W32 selects the mixed form (D S D into the FP32 copies, rhs scaled).
