@@ -156,3 +156,42 @@ def test_schur_step_equals_full_damped_normal_equations(gba, orc):
     nc = 9 * C
     S = A[:nc, :nc] - A[:nc, nc:] @ np.linalg.solve(A[nc:, nc:], A[nc:, :nc])
     assert rel(np.tril(out["S"]), np.tril(S)) < 1e-8
+
+
+@pytest.mark.parametrize("huber", [0.0, 3.0])
+def test_scaled_schur_sums_stay_in_fixed_point_range(gba, orc, huber):
+    # The mixed form sums the off-diagonal Schur blocks in 64-bit fixed point
+    # of the diagonally scaled products D_a W_e V'^-1 W_f^T D_b, relying on
+    # every partial sum lying in [-1, 1]: by Cauchy-Schwarz the sum of the
+    # magnitudes is at most sqrt(U_a,ii U_b,jj) D_a,i D_b,j <= 1.  Checked here
+    # on the oracle's linearisation (with gross outliers and Huber weights,
+    # and at a tiny damping), summing magnitudes, the worst case.
+    s = gba.synth_spec(1)
+    s.num_cameras, s.num_points, s.num_observations = 12, 500, 2500
+    s.kmin, s.kmax = 2, 12
+    p = gba.generate(spec=s, seed=9)
+    rng = np.random.default_rng(9)
+    k = rng.choice(p.num_observations, 60, replace=False)
+    p.obs_pixel[k] += rng.choice([-40.0, 40.0], size=(60, 2))
+    o = orc.linearize(p, huber_delta=huber)
+    U, V, W = o["U"], o["V"], o["W"]
+    for lam in (1e-4, 1e-12):
+        ud = np.array([np.diag(u) for u in U])
+        D = 1.0 / np.sqrt(ud + lam * np.maximum(ud, 1e-12))
+        Vf = V + np.transpose(V, (0, 2, 1)) * (1 - np.eye(3))
+        vd = np.einsum("pii->pi", Vf)
+        Vinv = np.linalg.inv(Vf + lam * np.maximum(vd, 1e-12)[:, :, None] * np.eye(3))
+        C = p.num_cameras
+        mag = np.zeros((C, C, 9, 9))
+        order = np.argsort(p.obs_point, kind="stable")
+        bounds = np.flatnonzero(np.diff(p.obs_point[order])) + 1
+        for es in np.split(order, bounds):
+            q = p.obs_point[es[0]]
+            for e in es:
+                for f in es:
+                    a, b = p.obs_camera[e], p.obs_camera[f]
+                    if b >= a:
+                        continue
+                    mag[a, b] += np.abs(W[e] @ Vinv[q] @ W[f].T) * D[a][:, None] * D[b][None, :]
+        assert mag.max() <= 1.0 + 1e-9, mag.max()
+        assert mag.max() > 0.05  # the check is not vacuous
