@@ -81,6 +81,7 @@ _PROTOS = {
     "gba_batch_solve": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(LMOptionsC),
                                        ctypes.POINTER(LMResultC), _ip]),
     "gba_batch_params": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp, _dp]),
+    "gba_batch_info": (ctypes.c_int, [ctypes.c_void_p, _ip, _dp, _ip]),
     "gba_batch_destroy": (None, [ctypes.c_void_p]),
     "gba_solve_bal_batched": (ctypes.c_int, [ctypes.POINTER(ProblemView), ctypes.c_int,
                                              ctypes.POINTER(LMOptionsC), ctypes.POINTER(LMResultC)]),
@@ -323,6 +324,16 @@ class Batch:
         self._h = ctypes.c_void_p()
         _check(lib().gba_batch_create(self._views, n, ctypes.byref(self._h)))
         self.rounds = 0
+
+    def info(self):
+        """The factor form the autotuner chose ("tiles": tile-parallel across
+        the batch, "workgroup": one workgroup per problem), its trial times in
+        ms for the two forms, and whether rounds replay captured HIP graphs."""
+        tf, gr = ctypes.c_int(), ctypes.c_int()
+        ms = (ctypes.c_double * 2)()
+        _check(lib().gba_batch_info(self._h, ctypes.byref(tf), ms, ctypes.byref(gr)))
+        return {"factor": "tiles" if tf.value else "workgroup",
+                "tune_ms": {"tiles": ms[0], "workgroup": ms[1]}, "graphs": bool(gr.value)}
 
     def solve(self, opts=None):
         opts = opts or LMOptions()

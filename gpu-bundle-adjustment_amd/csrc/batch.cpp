@@ -31,7 +31,7 @@ constexpr int kBatchMaxNpad = 1024;
 
 }  // namespace
 
-BatchSolver::BatchSolver(const gba_problem_view* probs, int count) : B_(count) {
+BatchSolver::BatchSolver(const gba_problem_view* probs, int count, bool autotune) : B_(count) {
     if (count <= 0) throw Error("batch: no problems");
     std::vector<int> scam, spt, pt_off(1, 0), cam_list, cam_off(1, 0), cam_chunk_off, cam_prob, pt_prob;
     std::vector<double2> spx;
@@ -135,6 +135,7 @@ BatchSolver::BatchSolver(const gba_problem_view* probs, int count) : B_(count) {
         // problems come first
         const char* env = std::getenv("GBA_BATCH_FACTOR");
         tile_factor_ = !(env && std::string(env) == "workgroup");
+        if (env && (std::string(env) == "workgroup" || std::string(env) == "tiles")) autotune = false;
         int nsteps = 0;
         for (const BatchDesc& d : hdesc_) nsteps = std::max(nsteps, d.npad / kTile);
         std::vector<int2> potrf, trsm, upd;
@@ -220,9 +221,13 @@ BatchSolver::BatchSolver(const gba_problem_view* probs, int count) : B_(count) {
         if (d.npad > d.n) launch_identity_pad(st_, S_.p + d.soff, nullptr, d.npad, d.n, d.npad);
     GBA_HIP(hipGetLastError());
     GBA_HIP(hipStreamSynchronize(st_));
+    if (const char* env = std::getenv("GBA_BATCH_GRAPH")) use_graphs_ = std::string(env) != "0";
+    if (autotune) autotune_factor();
 }
 
 BatchSolver::~BatchSolver() {
+    if (lin_exec_) (void)hipGraphExecDestroy(lin_exec_);
+    if (trial_exec_) (void)hipGraphExecDestroy(trial_exec_);
     if (h_status_) (void)hipHostFree(h_status_);
     if (h_lam_) (void)hipHostFree(h_lam_);
     if (h_mask_) (void)hipHostFree(h_mask_);
@@ -235,6 +240,99 @@ void BatchSolver::upload_mask(const std::vector<int>& h, int* d) {
     std::memcpy(h_mask_, h.data(), sizeof(int) * B_);
     GBA_HIP(hipMemcpyAsync(d, h_mask_, sizeof(int) * B_, hipMemcpyHostToDevice, st_));
     GBA_HIP(hipStreamSynchronize(st_));
+}
+
+void BatchSolver::enqueue_lin(const DevProblem& pl) {
+    launch_lin_points(st_, pl, cams_.p, pts_.p, W_.p, nullptr, VB_.p, unit_sse_.p, unit_anom_.p);
+    launch_lin_cameras(st_, pl, cams_.p, pts_.p, gram_.p);
+    launch_lin_reduce(st_, pl, gram_.p, unit_sse_.p, unit_anom_.p, U_.p, bc_.p, nullptr);
+    launch_batch_status(st_, B_, desc_.p, 0, unit_sse_.p, unit_den_.p, unit_anom_.p, dc_.p, U_.p, bc_.p,
+                        lam_.p, flags_.p, active_.p, lin_status_.p);
+}
+
+void BatchSolver::enqueue_trial(const DevProblem& pl, bool tile_factor) {
+    launch_fill_int(st_, reinterpret_cast<int*>(flags_.p), 4 * B_, 0);
+    launch_damp_invert_v_batched(st_, pb_.P, VB_.p, pt_prob_.p, lam_.p, active_.p, Vinv_.p, flags_.p);
+    launch_schur_diag(st_, pl, desc_.p, W_.p, nullptr, Vinv_.p, VB_.p, U_.p, bc_.p, 0.0, nullptr, S_.p,
+                      nullptr, nullptr, 0, rhs_.p, flags_.p);
+    if (num_stasks_ > 0)
+        launch_schur(st_, sshape_, pl, stasks_.p, num_stasks_, spairs_.p, W_.p, nullptr, Vinv_.p, VB_.p,
+                     U_.p, bc_.p, 0.0, nullptr, S_.p, nullptr, nullptr, 0, rhs_.p, flags_.p);
+    if (tile_factor) launch_batch_cholesky(st_, B_, ftab_, desc_.p, S_.p, D_.p, flags_.p, active_.p);
+    launch_batch_chol_solve(st_, B_, desc_.p, S_.p, D_.p, rhs_.p, dc_.p, flags_.p, active_.p, !tile_factor);
+    launch_trial_cams(st_, pb_.C, cams_.p, dc_.p, tcams_.p);
+    launch_backsub(st_, pl, W_.p, nullptr, Vinv_.p, VB_.p, dc_.p, pts_.p, tcams_.p, tpts_.p, 0.0,
+                   unit_sse_.p, unit_den_.p, unit_anom_.p, flags_.p);
+    launch_batch_status(st_, B_, desc_.p, 1, unit_sse_.p, unit_den_.p, unit_anom_.p, dc_.p, U_.p, bc_.p,
+                        lam_.p, flags_.p, active_.p, trial_status_.p);
+}
+
+template <class F>
+void BatchSolver::run_sequence(hipGraphExec_t& exec, F&& enqueue) {
+    if (!use_graphs_) {
+        enqueue();
+        return;
+    }
+    if (!exec) {
+        // every pointer and size in the sequence is fixed for the life of the
+        // solver; what changes between rounds (lambda, the masks) lives in
+        // device buffers that the host rewrites before each replay
+        hipGraph_t g = nullptr;
+        GBA_HIP(hipStreamBeginCapture(st_, hipStreamCaptureModeRelaxed));
+        try {
+            enqueue();
+        } catch (...) {
+            (void)hipStreamEndCapture(st_, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        GBA_HIP(hipStreamEndCapture(st_, &g));
+        const hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        GBA_HIP(e);
+    }
+    GBA_HIP(hipGraphLaunch(exec, st_));
+}
+
+void BatchSolver::autotune_factor() {
+    // every problem linearised at its initial parameters and damped with
+    // lambda = 1e-3; one trial per form, twice, the faster time kept (the
+    // trial writes only trial buffers; solve starts over from the uploads)
+    GBA_HIP(hipMemcpyAsync(cams_.p, init_cams_.data(), sizeof(double) * init_cams_.size(),
+                           hipMemcpyHostToDevice, st_));
+    GBA_HIP(hipMemcpyAsync(pts_.p, init_pts_.data(), sizeof(double) * init_pts_.size(),
+                           hipMemcpyHostToDevice, st_));
+    for (int b = 0; b < B_; ++b) h_lam_[b] = 1e-3;
+    GBA_HIP(hipMemcpyAsync(lam_.p, h_lam_, sizeof(double) * B_, hipMemcpyHostToDevice, st_));
+    upload_mask(std::vector<int>(B_, 1), active_.p);
+    DevProblem pl = pb_;
+    pl.active = active_.p;
+    enqueue_lin(pl);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    GBA_HIP(hipEventCreate(&e0));
+    GBA_HIP(hipEventCreate(&e1));
+    double best[2] = {1e300, 1e300};
+    try {
+        for (int rep = 0; rep < 2; ++rep)
+            for (int form = 0; form < 2; ++form) {
+                GBA_HIP(hipEventRecord(e0, st_));
+                enqueue_trial(pl, form == 0);
+                GBA_HIP(hipEventRecord(e1, st_));
+                GBA_HIP(hipEventSynchronize(e1));
+                float ms = 0.0f;
+                GBA_HIP(hipEventElapsedTime(&ms, e0, e1));
+                best[form] = std::min(best[form], static_cast<double>(ms));
+            }
+    } catch (...) {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    tune_ms_[0] = best[0];
+    tune_ms_[1] = best[1];
+    tile_factor_ = best[0] <= best[1];
 }
 
 void BatchSolver::solve(const gba_lm_options& opts, gba_lm_result* res) {
@@ -262,11 +360,7 @@ void BatchSolver::solve(const gba_lm_options& opts, gba_lm_result* res) {
         }
         if (nlin > 0) {
             upload_mask(lin_mask, active_.p);
-            launch_lin_points(st_, pl, cams_.p, pts_.p, W_.p, nullptr, VB_.p, unit_sse_.p, unit_anom_.p);
-            launch_lin_cameras(st_, pl, cams_.p, pts_.p, gram_.p);
-            launch_lin_reduce(st_, pl, gram_.p, unit_sse_.p, unit_anom_.p, U_.p, bc_.p, nullptr);
-            launch_batch_status(st_, B, desc_.p, 0, unit_sse_.p, unit_den_.p, unit_anom_.p, dc_.p, U_.p,
-                                bc_.p, lam_.p, flags_.p, active_.p, lin_status_.p);
+            run_sequence(lin_exec_, [&] { enqueue_lin(pl); });
             GBA_HIP(hipMemcpyAsync(h_status_, lin_status_.p, sizeof(PassStatus) * B, hipMemcpyDeviceToHost, st_));
             GBA_HIP(hipGetLastError());
             GBA_HIP(hipStreamSynchronize(st_));
@@ -292,20 +386,7 @@ void BatchSolver::solve(const gba_lm_options& opts, gba_lm_result* res) {
         for (int b = 0; b < B; ++b) h_lam_[b] = lambda[b];
         GBA_HIP(hipMemcpyAsync(lam_.p, h_lam_, sizeof(double) * B, hipMemcpyHostToDevice, st_));
         upload_mask(trial_mask, active_.p);
-        launch_fill_int(st_, reinterpret_cast<int*>(flags_.p), 4 * B, 0);
-        launch_damp_invert_v_batched(st_, pb_.P, VB_.p, pt_prob_.p, lam_.p, active_.p, Vinv_.p, flags_.p);
-        launch_schur_diag(st_, pl, desc_.p, W_.p, nullptr, Vinv_.p, VB_.p, U_.p, bc_.p, 0.0, nullptr, S_.p,
-                          nullptr, nullptr, 0, rhs_.p, flags_.p);
-        if (num_stasks_ > 0)
-            launch_schur(st_, sshape_, pl, stasks_.p, num_stasks_, spairs_.p, W_.p, nullptr, Vinv_.p, VB_.p,
-                         U_.p, bc_.p, 0.0, nullptr, S_.p, nullptr, nullptr, 0, rhs_.p, flags_.p);
-        if (tile_factor_) launch_batch_cholesky(st_, B, ftab_, desc_.p, S_.p, D_.p, flags_.p, active_.p);
-        launch_batch_chol_solve(st_, B, desc_.p, S_.p, D_.p, rhs_.p, dc_.p, flags_.p, active_.p, !tile_factor_);
-        launch_trial_cams(st_, pb_.C, cams_.p, dc_.p, tcams_.p);
-        launch_backsub(st_, pl, W_.p, nullptr, Vinv_.p, VB_.p, dc_.p, pts_.p, tcams_.p, tpts_.p, 0.0,
-                       unit_sse_.p, unit_den_.p, unit_anom_.p, flags_.p);
-        launch_batch_status(st_, B, desc_.p, 1, unit_sse_.p, unit_den_.p, unit_anom_.p, dc_.p, U_.p, bc_.p,
-                            lam_.p, flags_.p, active_.p, trial_status_.p);
+        run_sequence(trial_exec_, [&] { enqueue_trial(pl, tile_factor_); });
         PassStatus* hs = h_status_ + B;
         GBA_HIP(hipMemcpyAsync(hs, trial_status_.p, sizeof(PassStatus) * B, hipMemcpyDeviceToHost, st_));
         GBA_HIP(hipGetLastError());

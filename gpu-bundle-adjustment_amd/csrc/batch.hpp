@@ -17,10 +17,15 @@ namespace gba_amd {
 // re-linearises the problems whose last trial was accepted and evaluates one
 // trial for every problem still running; the host then applies each
 // problem's accept / reject / lambda / nu rules.  One stream synchronisation
-// per round.
+// per round.  The linearisation and trial launch sequences are captured once
+// into HIP graphs and replayed every round (GBA_BATCH_GRAPH=0: launched one
+// by one).  With autotune set, the constructor times one trial of the whole
+// batch with each dense factor form (tile-parallel across problems, or one
+// workgroup per problem) and keeps the faster; GBA_BATCH_FACTOR=tiles or
+// =workgroup fixes the form instead.
 class BatchSolver {
 public:
-    BatchSolver(const gba_problem_view* probs, int count);
+    BatchSolver(const gba_problem_view* probs, int count, bool autotune = false);
     ~BatchSolver();
     BatchSolver(const BatchSolver&) = delete;
     BatchSolver& operator=(const BatchSolver&) = delete;
@@ -30,9 +35,21 @@ public:
     void download_params(int i, double* cams, double* pts);
     int count() const { return B_; }
     int rounds() const { return rounds_; }
+    bool tile_factor() const { return tile_factor_; }
+    bool graphs() const { return use_graphs_; }
+    // trial milliseconds measured by the autotuner: tile-parallel factor,
+    // workgroup factor (0 when it did not run)
+    double tune_ms(int form) const { return tune_ms_[form & 1]; }
 
 private:
     void upload_mask(const std::vector<int>& h, int* d);
+    // the launch sequences of a round (all problems, masked by active_)
+    void enqueue_lin(const DevProblem& pl);
+    void enqueue_trial(const DevProblem& pl, bool tile_factor);
+    // runs a sequence through its graph (captured on first use) or eagerly
+    template <class F>
+    void run_sequence(hipGraphExec_t& exec, F&& enqueue);
+    void autotune_factor();
 
     int B_ = 0;
     int rounds_ = 0;
@@ -63,6 +80,9 @@ private:
     DevBuf<int2> fpotrf_, ftrsm_, fupd_;
     std::vector<int> ftrsm_off_, fupd_off_;
     BatchFactorTables ftab_;
+    double tune_ms_[2] = {0.0, 0.0};
+    bool use_graphs_ = true;
+    hipGraphExec_t lin_exec_ = nullptr, trial_exec_ = nullptr;
     PassStatus* h_status_ = nullptr;  // 2 B entries: linearisation, trial
     double* h_lam_ = nullptr;
     int* h_mask_ = nullptr;           // staging for the per-problem switches

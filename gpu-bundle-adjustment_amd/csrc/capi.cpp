@@ -160,7 +160,7 @@ int gba_batch_create(const gba_problem_view* probs, int count, gba_batch** out) 
         *out = nullptr;
         if (!probs || count <= 0) return set_error("gba_batch_create: no problems");
         auto h = std::make_unique<gba_batch>();
-        h->s = std::make_unique<BatchSolver>(probs, count);
+        h->s = std::make_unique<BatchSolver>(probs, count, /*autotune=*/true);
         *out = h.release();
         return 0;
     });
@@ -182,6 +182,19 @@ int gba_batch_params(gba_batch* b, int index, double* cameras, double* points) {
     return guarded([&] {
         need(b && cameras && points, "gba_batch_params: null argument");
         b->s->download_params(index, cameras, points);
+        return 0;
+    });
+}
+
+int gba_batch_info(gba_batch* b, int* tile_factor, double* tune_ms, int* graphs) {
+    return guarded([&] {
+        need(b, "gba_batch_info: null handle");
+        if (tile_factor) *tile_factor = b->s->tile_factor() ? 1 : 0;
+        if (tune_ms) {
+            tune_ms[0] = b->s->tune_ms(0);
+            tune_ms[1] = b->s->tune_ms(1);
+        }
+        if (graphs) *graphs = b->s->graphs() ? 1 : 0;
         return 0;
     });
 }

@@ -150,15 +150,24 @@ void gba_solver_destroy(gba_solver* s);
 
 /* Many independent problems solved together (at most 113 cameras each), each
  * with the LM semantics of gba_solve_bal and the same options, plain FP64;
- * every stage of an LM round is one launch for the whole batch.  For callers
- * that solve many small reconstructions (the reference solves them one call
- * at a time).  Handle form: create uploads and sorts once; solve restarts from
- * the uploaded initial parameters; results[count]; *rounds (optional)
- * receives the number of LM rounds the batch took. */
+ * every stage of an LM round is one launch for the whole batch, and a
+ * round's launch sequences are replayed from HIP graphs captured on first
+ * use.  For callers that solve many small reconstructions (the reference
+ * solves them one call at a time).  Handle form: create uploads and sorts
+ * once and autotunes the dense factor form on the batch (one timed trial per
+ * form); solve restarts from the uploaded initial parameters;
+ * results[count]; *rounds (optional) receives the number of LM rounds the
+ * batch took. */
 typedef struct gba_batch gba_batch;
 int gba_batch_create(const gba_problem_view* probs, int count, gba_batch** out);
 int gba_batch_solve(gba_batch* b, const gba_lm_options* opts, gba_lm_result* results, int* rounds);
 int gba_batch_params(gba_batch* b, int index, double* cameras, double* points);
+/* What the handle chose: *tile_factor 1 = tile-parallel factor across the
+ * batch, 0 = one workgroup per problem; tune_ms[2] the autotuner's trial
+ * times for the two forms (0 when a GBA_BATCH_FACTOR setting fixed the
+ * form); *graphs 1 when rounds replay captured HIP graphs.  Any output may be
+ * NULL. */
+int gba_batch_info(gba_batch* b, int* tile_factor, double* tune_ms, int* graphs);
 void gba_batch_destroy(gba_batch* b);
 /* One-shot form: writes each problem's last accepted parameters back. */
 int gba_solve_bal_batched(gba_problem_view* probs, int count, const gba_lm_options* opts,

@@ -3,8 +3,11 @@ problem's own sequential FP64 solve through the C ABI.  The batch solves each
 factored S with a different kernel (one workgroup per problem), so the steps
 agree to rounding; the tolerances are the LM convergence tolerance on the
 final cost (1e-6 relative) and at most one iteration of difference.  Both
-batched factors are covered: the tile-parallel one (default) and the
-one-workgroup-per-problem one (GBA_BATCH_FACTOR=workgroup)."""
+batched factors are covered: the tile-parallel one and the
+one-workgroup-per-problem one (GBA_BATCH_FACTOR=tiles / workgroup; by
+default the handle's autotuner picks one), and so are the rounds replayed
+from captured HIP graphs (default) against eager launches
+(GBA_BATCH_GRAPH=0)."""
 import numpy as np
 import pytest
 
@@ -89,3 +92,50 @@ def test_batch_ragged_problem(gba):
     for rb, rs in zip(res, seq):
         check_pair(rb, rs)
         assert rb.termination == "CONVERGENCE"
+
+
+def test_batch_graph_replay_matches_eager(gba, monkeypatch):
+    """Rounds replayed from the captured HIP graphs against the same rounds
+    launched one by one: every replay (all rounds of two solves of one
+    handle, with the host's uploads, accept launches and synchronisations
+    between replays) gives the eager results.  The FP64 Schur sums are LDS
+    atomics whose order may differ between runs, hence 1e-9 rather than
+    bit equality."""
+    monkeypatch.setenv("GBA_BATCH_FACTOR", "tiles")
+    probs = [gba.generate(5, seed) for seed in (21, 22, 23)]
+    monkeypatch.setenv("GBA_BATCH_GRAPH", "0")
+    eager = gba.Batch(probs)
+    assert not eager.info()["graphs"]
+    ref = eager.solve()
+    monkeypatch.delenv("GBA_BATCH_GRAPH")
+    g = gba.Batch(probs)
+    assert g.info()["graphs"]
+    for _ in range(2):
+        res = g.solve()
+        assert g.rounds == eager.rounds
+        for i, (a, r) in enumerate(zip(res, ref)):
+            assert a.termination == r.termination
+            assert a.iterations == r.iterations and a.trials == r.trials
+            assert abs(a.final_cost - r.final_cost) <= 1e-9 * r.final_cost
+            ca, pa = g.params(i)
+            cr, pr = eager.params(i)
+            assert np.max(np.abs(ca - cr)) <= 1e-7 * max(1.0, np.max(np.abs(cr)))
+            assert np.max(np.abs(pa - pr)) <= 1e-7 * max(1.0, np.max(np.abs(pr)))
+
+
+def test_batch_autotuner_picks_the_faster_form(gba, monkeypatch):
+    """The handle times one trial of the batch with each factor form and
+    keeps the faster; a GBA_BATCH_FACTOR setting skips the timing."""
+    monkeypatch.delenv("GBA_BATCH_FACTOR", raising=False)
+    probs = [small(gba, 8, 800, 3600, 31), small(gba, 40, 2000, 9000, 32)]
+    b = gba.Batch(probs)
+    inf = b.info()
+    t = inf["tune_ms"]
+    assert t["tiles"] > 0.0 and t["workgroup"] > 0.0
+    assert inf["factor"] == ("tiles" if t["tiles"] <= t["workgroup"] else "workgroup")
+    seq = [gba.Solver(p).solve() for p in probs]
+    for rb, rs in zip(b.solve(), seq):
+        check_pair(rb, rs)
+    monkeypatch.setenv("GBA_BATCH_FACTOR", "workgroup")
+    inf2 = gba.Batch(probs).info()
+    assert inf2["factor"] == "workgroup" and inf2["tune_ms"]["tiles"] == 0.0
