@@ -24,6 +24,7 @@ SOURCES = [
     "k_precond.hip",
     "k_backsub.hip",
     "k_batch.hip",
+    "k_pairs.hip",
     "solver.cpp",
     "batch.cpp",
     "obs_fp16.cpp",

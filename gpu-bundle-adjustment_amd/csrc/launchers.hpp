@@ -56,6 +56,25 @@ void launch_schur_diag(hipStream_t st, const DevProblem& pb, const BatchDesc* de
 void launch_diag_scale(hipStream_t st, int n, int npad, const double* U, double lambda,
                        double* dscale);
 
+// k_pairs.hip -- the Schur task pair lists on the device.  Task t = (a, lo,
+// hi) in tq[t] (x, y, z) owns slots qoff[t] .. qoff[t + 1] - 1, one per
+// observation of camera a in cam_list order; qoff[ntask] = nslots.  count
+// writes cnt[0 .. nslots] (pairs per slot, then 0); scan turns cnt into write
+// positions pos (exclusive sum over nslots + 1 entries, pair_scan_work bytes
+// of scratch); bounds[t] = pos[qoff[t]] for t <= ntask (task t's pairs are
+// [bounds[t], bounds[t + 1])); fill writes the pairs (e, f).
+size_t pair_scan_work(long long n);
+void launch_pair_count(hipStream_t st, int ntask, const int4* tq, const long long* qoff, long long nslots,
+                       const int* cam_off, const int* cam_list, const int* spt, const int* pt_off,
+                       const int* scam, long long* cnt);
+void launch_pair_scan(hipStream_t st, const long long* cnt, long long* pos, long long n, void* work,
+                      size_t work_bytes);
+void launch_pair_bounds(hipStream_t st, int ntask, const long long* qoff, const long long* pos,
+                        long long* bounds);
+void launch_pair_fill(hipStream_t st, int ntask, const int4* tq, const long long* qoff, const int* cam_off,
+                      const int* cam_list, const int* spt, const int* pt_off, const int* scam, const long long* pos,
+                      int2* pairs);
+
 // k_dense.hip
 // Factors in place; D holds 2*nblk tiles of 64x64: tiles [0, nblk) receive the
 // diagonal tiles of L (row-major, upper part zero), tiles [nblk, 2 nblk) their

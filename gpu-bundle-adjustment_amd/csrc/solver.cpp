@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -106,6 +107,59 @@ void build_schur_tasks(int C, int cb, const std::vector<int>& cam_off,
             const int t = toff[a] + c;
             tasks[t] = SchurTask{a, c * cb, std::min((c + 1) * cb, a + dg), 0, begin[t], begin[t + 1], 0, 0, 0};
         }
+    std::stable_sort(tasks.begin(), tasks.end(), [](const SchurTask& x, const SchurTask& y) {
+        return x.end - x.begin > y.end - y.begin;
+    });
+}
+
+void build_schur_tasks_device(hipStream_t st, int C, int cb, const std::vector<int>& cam_off,
+                              const int* d_cam_off, const int* d_cam_list, const int* d_spt,
+                              const int* d_pt_off, const int* d_scam, std::vector<SchurTask>& tasks,
+                              DevBuf<int2>& pairs, bool with_diag) {
+    // tasks in the host form's order: camera a, then column chunk
+    const int dg = with_diag ? 1 : 0;
+    std::vector<int4> tq;
+    std::vector<long long> qoff(1, 0);
+    for (int a = 0; a < C; ++a) {
+        const int na = cam_off[a + 1] - cam_off[a];
+        for (int lo = 0; lo < a + dg; lo += cb) {
+            tq.push_back(make_int4(a, lo, std::min(lo + cb, a + dg), 0));
+            qoff.push_back(qoff.back() + na);
+        }
+    }
+    const int ntask = static_cast<int>(tq.size());
+    tasks.clear();
+    pairs.release();
+    if (ntask == 0) return;
+    const long long nslots = qoff.back();
+    if (nslots + 1 > INT_MAX) throw Error("too many observation slots for the pair scan");
+    DevBuf<int4> d_tq;
+    DevBuf<long long> d_qoff, cnt, pos, bounds;
+    DevBuf<unsigned char> work;
+    upload(d_tq, tq, st);
+    upload(d_qoff, qoff, st);
+    cnt.alloc(static_cast<size_t>(nslots) + 1);
+    pos.alloc(static_cast<size_t>(nslots) + 1);
+    bounds.alloc(static_cast<size_t>(ntask) + 1);
+    const size_t wb = pair_scan_work(nslots + 1);
+    work.alloc(std::max<size_t>(wb, 1));
+    launch_pair_count(st, ntask, d_tq.p, d_qoff.p, nslots, d_cam_off, d_cam_list, d_spt, d_pt_off, d_scam, cnt.p);
+    launch_pair_scan(st, cnt.p, pos.p, nslots + 1, work.p, wb);
+    launch_pair_bounds(st, ntask, d_qoff.p, pos.p, bounds.p);
+    std::vector<long long> hb(static_cast<size_t>(ntask) + 1);
+    GBA_HIP(hipMemcpyAsync(hb.data(), bounds.p, sizeof(long long) * hb.size(), hipMemcpyDeviceToHost, st));
+    GBA_HIP(hipGetLastError());
+    GBA_HIP(hipStreamSynchronize(st));
+    if (hb[ntask] > 0) {
+        pairs.alloc(static_cast<size_t>(hb[ntask]));
+        launch_pair_fill(st, ntask, d_tq.p, d_qoff.p, d_cam_off, d_cam_list, d_spt, d_pt_off, d_scam, pos.p,
+                         pairs.p);
+    }
+    GBA_HIP(hipGetLastError());
+    GBA_HIP(hipStreamSynchronize(st));  // the temporaries are released on return
+    tasks.resize(static_cast<size_t>(ntask));
+    for (int t = 0; t < ntask; ++t)
+        tasks[t] = SchurTask{tq[t].x, tq[t].y, tq[t].z, 0, hb[t], hb[t + 1], 0, 0, 0};
     std::stable_sort(tasks.begin(), tasks.end(), [](const SchurTask& x, const SchurTask& y) {
         return x.end - x.begin > y.end - y.begin;
     });
@@ -278,16 +332,12 @@ Solver::Solver(const gba_problem_view& prob, const gba_solver_config& cfg) : cfg
     std::vector<int>& cam_list = lay.cam_list;
     sshape_ = schur_shape(C, mixed_);
     std::vector<SchurTask> stasks;
-    std::vector<int2> spairs;
     // diagonal blocks and rhs in their own per-camera kernel (GBA_SCHUR_DIAG=0:
     // inside the task lists, as pairs e == f)
     {
         const char* env = std::getenv("GBA_SCHUR_DIAG");
         sep_diag_ = !(env && env[0] == '0');
     }
-    build_schur_tasks(C, sshape_.cb, cam_off, cam_list, pt_off, scam, spt, stasks, spairs, !sep_diag_);
-    num_stasks_ = static_cast<int>(stasks.size());
-    for (SchurTask& t : stasks) t.ld = pb_.npad;
     std::vector<int4>& chunks = lay.chunks;
     std::vector<int>& cam_chunk_off = lay.cam_chunk_off;
     pb_.num_units = static_cast<int>(units.size());
@@ -320,9 +370,13 @@ Solver::Solver(const gba_problem_view& prob, const gba_solver_config& cfg) : cfg
     }
     upload(units_, units, st_);
     upload(cam_list_, cam_list, st_);
-    upload(stasks_, stasks, st_);
-    upload(spairs_, spairs, st_);
     upload(cam_off_, cam_off, st_);
+    // Schur task and pair lists, built on the device from the sorted layout
+    build_schur_tasks_device(st_, C, sshape_.cb, cam_off, cam_off_.p, cam_list_.p, spt_.p, pt_off_.p, scam_.p,
+                             stasks, spairs_, !sep_diag_);
+    num_stasks_ = static_cast<int>(stasks.size());
+    for (SchurTask& t : stasks) t.ld = pb_.npad;
+    upload(stasks_, stasks, st_);
     upload(chunks_, chunks, st_);
     upload(cam_chunk_off_, cam_chunk_off, st_);
     pb_.scam = scam_.p;

@@ -60,6 +60,15 @@ void build_schur_tasks(int C, int cb, const std::vector<int>& cam_off,
                        const std::vector<int>& scam, const std::vector<int>& spt,
                        std::vector<SchurTask>& tasks, std::vector<int2>& pairs, bool with_diag);
 
+// Device form of build_schur_tasks: the same tasks and the same pairs in the
+// same order, built by launch_pair_count / scan / fill from the sorted layout
+// already on the device (d_*); the pairs land in pairs (allocated here), the
+// tasks come back sorted by decreasing pair count.  Synchronises st.
+void build_schur_tasks_device(hipStream_t st, int C, int cb, const std::vector<int>& cam_off,
+                              const int* d_cam_off, const int* d_cam_list, const int* d_spt,
+                              const int* d_pt_off, const int* d_scam, std::vector<SchurTask>& tasks,
+                              DevBuf<int2>& pairs, bool with_diag);
+
 // FP64 preconditioned conjugate gradients on A x = b, A symmetric given by the
 // lower triangle of an FP32 matrix, preconditioned by (L L^T)^-1 applied
 // through the explicit FP32 inverse X = L^-1 (launch_cholesky with X32).  Vectors
