@@ -25,6 +25,7 @@ SOURCES = [
     "k_backsub.hip",
     "k_batch.hip",
     "k_pairs.hip",
+    "k_layout.hip",
     "solver.cpp",
     "batch.cpp",
     "obs_fp16.cpp",

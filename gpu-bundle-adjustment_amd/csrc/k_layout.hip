@@ -1,0 +1,112 @@
+// k_layout.hip -- the point-major observation layout built on the device
+// (replaces the reference's host point -> edge lists, src/lm.cu:378-381, with
+// a CSR over observations sorted by (point, camera, index), plus a camera
+// index over the sorted observations).
+//
+//   perm    : sorted position k -> caller's observation index e, ordered by
+//             (point, camera, e): a radix sort of the keys point:camera with
+//             the indices as values (the sort is stable, so ties keep e's
+//             order);
+//   scam/spt/spx : camera, point, pixels of sorted observation k;
+//   pt_off  : exclusive scan of the per-point counts (P + 1);
+//   cam_list/cam_off : sorted positions k grouped by camera, ascending k
+//             within a camera (a second stable radix sort, keys scam[k]),
+//             and the exclusive scan of the per-camera counts (C + 1).
+// The same layout as the host form (build_layout).
+#include <hipcub/hipcub.hpp>
+
+#include "device_common.hpp"
+#include "launchers.hpp"
+#include "solver.hpp"
+
+namespace gba_amd {
+
+namespace {
+
+__global__ void k_layout_keys(int O, int cbits, const int* __restrict__ cam, const int* __restrict__ pt,
+                              unsigned long long* __restrict__ keys, int* __restrict__ idx,
+                              int* __restrict__ pcnt, int* __restrict__ ccnt) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= O) return;
+    const int c = cam[e], p = pt[e];
+    keys[e] = (static_cast<unsigned long long>(p) << cbits) | static_cast<unsigned long long>(c);
+    idx[e] = e;
+    atomicAdd(pcnt + p, 1);
+    atomicAdd(ccnt + c, 1);
+}
+
+__global__ void k_layout_gather(int O, const int* __restrict__ perm, const int* __restrict__ cam,
+                                const int* __restrict__ pt, const double* __restrict__ px,
+                                int* __restrict__ scam, int* __restrict__ spt, double2* __restrict__ spx,
+                                int* __restrict__ iota) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= O) return;
+    const int e = perm[k];
+    scam[k] = cam[e];
+    spt[k] = pt[e];
+    if (spx) spx[k] = make_double2(px[2 * static_cast<size_t>(e)], px[2 * static_cast<size_t>(e) + 1]);
+    iota[k] = k;
+}
+
+int bits_for(int n) {
+    int b = 1;
+    while ((1LL << b) < n) ++b;
+    return b;
+}
+
+}  // namespace
+
+void build_layout_device(hipStream_t st, int C, int P, int O, const int* d_cam, const int* d_pt,
+                         const double* d_px, DevBuf<int>& perm, DevBuf<int>& scam, DevBuf<int>& spt,
+                         DevBuf<double2>* spx, DevBuf<int>& pt_off, DevBuf<int>& cam_off, DevBuf<int>& cam_list) {
+    const int cbits = bits_for(C), pbits = bits_for(P);
+    if (cbits + pbits > 62) throw Error("layout: camera and point indices too wide for the sort key");
+    DevBuf<unsigned long long> keys, keys_sorted;
+    DevBuf<int> idx, ccnt, pcnt, iota, cam_keys_sorted;
+    keys.alloc(static_cast<size_t>(O));
+    keys_sorted.alloc(static_cast<size_t>(O));
+    idx.alloc(static_cast<size_t>(O));
+    iota.alloc(static_cast<size_t>(O));
+    cam_keys_sorted.alloc(static_cast<size_t>(O));
+    pcnt.alloc(static_cast<size_t>(P) + 1);
+    ccnt.alloc(static_cast<size_t>(C) + 1);
+    perm.alloc(static_cast<size_t>(O));
+    scam.alloc(static_cast<size_t>(O));
+    spt.alloc(static_cast<size_t>(O));
+    if (spx) spx->alloc(static_cast<size_t>(O));
+    pt_off.alloc(static_cast<size_t>(P) + 1);
+    cam_off.alloc(static_cast<size_t>(C) + 1);
+    cam_list.alloc(static_cast<size_t>(O));
+    // scratch of the sorts and scans
+    size_t b1 = 0, b2 = 0, b3 = 0, b4 = 0;
+    GBA_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, b1, keys.p, keys_sorted.p, idx.p, perm.p, O, 0,
+                                               cbits + pbits, st));
+    GBA_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, b2, scam.p, cam_keys_sorted.p, iota.p, cam_list.p, O, 0,
+                                               cbits, st));
+    GBA_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, b3, pcnt.p, pt_off.p, P + 1, st));
+    GBA_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, b4, ccnt.p, cam_off.p, C + 1, st));
+    const size_t wb = std::max(std::max(b1, b2), std::max(b3, b4));
+    DevBuf<unsigned char> work;
+    work.alloc(std::max<size_t>(wb, 1));
+    launch_fill_int(st, pcnt.p, P + 1, 0);
+    launch_fill_int(st, ccnt.p, C + 1, 0);
+    const int blocks = (O + 255) / 256;
+    hipLaunchKernelGGL(k_layout_keys, dim3(blocks), dim3(256), 0, st, O, cbits, d_cam, d_pt, keys.p, idx.p, pcnt.p,
+                       ccnt.p);
+    size_t bytes = wb;
+    GBA_HIP(hipcub::DeviceRadixSort::SortPairs(work.p, bytes, keys.p, keys_sorted.p, idx.p, perm.p, O, 0,
+                                               cbits + pbits, st));
+    hipLaunchKernelGGL(k_layout_gather, dim3(blocks), dim3(256), 0, st, O, static_cast<const int*>(perm.p), d_cam,
+                       d_pt, spx ? d_px : nullptr, scam.p, spt.p, spx ? spx->p : nullptr, iota.p);
+    bytes = wb;
+    GBA_HIP(hipcub::DeviceRadixSort::SortPairs(work.p, bytes, scam.p, cam_keys_sorted.p, iota.p, cam_list.p, O, 0,
+                                               cbits, st));
+    bytes = wb;
+    GBA_HIP(hipcub::DeviceScan::ExclusiveSum(work.p, bytes, pcnt.p, pt_off.p, P + 1, st));
+    bytes = wb;
+    GBA_HIP(hipcub::DeviceScan::ExclusiveSum(work.p, bytes, ccnt.p, cam_off.p, C + 1, st));
+    GBA_HIP(hipGetLastError());
+    GBA_HIP(hipStreamSynchronize(st));  // the temporaries are released on return
+}
+
+}  // namespace gba_amd

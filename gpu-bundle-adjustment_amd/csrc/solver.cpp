@@ -254,31 +254,7 @@ void build_layout(const gba_problem_view& prob, HostLayout& L) {
                                     prob.obs_pixel[2 * static_cast<size_t>(e) + 1]);
         }
     }
-    L.units.clear();
-    L.pair_count = 0.0;
-    {
-        int b = -1, e_end = -1;
-        for (int p = 0; p < P; ++p) {
-            const int k0 = L.pt_off[p], k1 = L.pt_off[p + 1], k = k1 - k0;
-            if (k == 0) continue;
-            L.pair_count += 0.5 * static_cast<double>(k) * (k + 1);
-            if (k > kWave) {
-                if (b >= 0) L.units.push_back(make_int2(b, e_end));
-                L.units.push_back(make_int2(k0, k1));
-                b = -1;
-            } else if (b < 0) {
-                b = k0;
-                e_end = k1;
-            } else if (k1 - b > kWave) {
-                L.units.push_back(make_int2(b, e_end));
-                b = k0;
-                e_end = k1;
-            } else {
-                e_end = k1;
-            }
-        }
-        if (b >= 0) L.units.push_back(make_int2(b, e_end));
-    }
+    layout_units(L.pt_off, L.units, L.pair_count);
     L.cam_off.assign(static_cast<size_t>(C) + 1, 0);
     L.cam_list.resize(static_cast<size_t>(O));
     for (int k = 0; k < O; ++k) L.cam_off[L.scam[k] + 1]++;
@@ -287,14 +263,46 @@ void build_layout(const gba_problem_view& prob, HostLayout& L) {
         std::vector<int> fill(L.cam_off.begin(), L.cam_off.end() - 1);
         for (int k = 0; k < O; ++k) L.cam_list[fill[L.scam[k]]++] = k;
     }
-    L.chunks.clear();
-    L.cam_chunk_off.assign(static_cast<size_t>(C) + 1, 0);
-    for (int c = 0; c < C; ++c) {
-        L.cam_chunk_off[c] = static_cast<int>(L.chunks.size());
-        for (int b = L.cam_off[c]; b < L.cam_off[c + 1]; b += kCamChunk)
-            L.chunks.push_back(make_int4(c, b, std::min(b + kCamChunk, L.cam_off[c + 1]), 0));
+    layout_chunks(L.cam_off, L.chunks, L.cam_chunk_off);
+}
+
+void layout_units(const std::vector<int>& pt_off, std::vector<int2>& units, double& pair_count) {
+    const int P = static_cast<int>(pt_off.size()) - 1;
+    units.clear();
+    pair_count = 0.0;
+    int b = -1, e_end = -1;
+    for (int p = 0; p < P; ++p) {
+        const int k0 = pt_off[p], k1 = pt_off[p + 1], k = k1 - k0;
+        if (k == 0) continue;
+        pair_count += 0.5 * static_cast<double>(k) * (k + 1);
+        if (k > kWave) {
+            if (b >= 0) units.push_back(make_int2(b, e_end));
+            units.push_back(make_int2(k0, k1));
+            b = -1;
+        } else if (b < 0) {
+            b = k0;
+            e_end = k1;
+        } else if (k1 - b > kWave) {
+            units.push_back(make_int2(b, e_end));
+            b = k0;
+            e_end = k1;
+        } else {
+            e_end = k1;
+        }
     }
-    L.cam_chunk_off[C] = static_cast<int>(L.chunks.size());
+    if (b >= 0) units.push_back(make_int2(b, e_end));
+}
+
+void layout_chunks(const std::vector<int>& cam_off, std::vector<int4>& chunks, std::vector<int>& cam_chunk_off) {
+    const int C = static_cast<int>(cam_off.size()) - 1;
+    chunks.clear();
+    cam_chunk_off.assign(static_cast<size_t>(C) + 1, 0);
+    for (int c = 0; c < C; ++c) {
+        cam_chunk_off[c] = static_cast<int>(chunks.size());
+        for (int b = cam_off[c]; b < cam_off[c + 1]; b += kCamChunk)
+            chunks.push_back(make_int4(c, b, std::min(b + kCamChunk, cam_off[c + 1]), 0));
+    }
+    cam_chunk_off[C] = static_cast<int>(chunks.size());
 }
 
 Solver::Solver(const gba_problem_view& prob, const gba_solver_config& cfg) : cfg_(cfg) {
@@ -318,42 +326,52 @@ Solver::Solver(const gba_problem_view& prob, const gba_solver_config& cfg) : cfg
     init_cams_.assign(prob.cameras, prob.cameras + 9 * static_cast<size_t>(C));
     init_pts_.assign(prob.points, prob.points + 3 * static_cast<size_t>(P));
 
-    // ---- host-side layout: point-major sort, work units, camera index ----
-    HostLayout lay;
-    build_layout(prob, lay);
-    pair_count_ = lay.pair_count;
-    std::vector<int>& perm = lay.perm;
-    std::vector<int>& scam = lay.scam;
-    std::vector<int>& spt = lay.spt;
-    std::vector<double2>& spx = lay.spx;
-    std::vector<int>& pt_off = lay.pt_off;
-    std::vector<int2>& units = lay.units;
-    std::vector<int>& cam_off = lay.cam_off;
-    std::vector<int>& cam_list = lay.cam_list;
     sshape_ = schur_shape(C, mixed_);
-    std::vector<SchurTask> stasks;
     // diagonal blocks and rhs in their own per-camera kernel (GBA_SCHUR_DIAG=0:
     // inside the task lists, as pairs e == f)
     {
         const char* env = std::getenv("GBA_SCHUR_DIAG");
         sep_diag_ = !(env && env[0] == '0');
     }
-    std::vector<int4>& chunks = lay.chunks;
-    std::vector<int>& cam_chunk_off = lay.cam_chunk_off;
+    GBA_HIP(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
+
+    // ---- point-major layout on the device: sort, CSR, camera index ----
+    const bool fp16 = cfg.obs_storage == GBA_OBS_FP16;
+    if (!fp16 && cfg.obs_storage != GBA_OBS_FP64) throw Error("unknown observation storage");
+    {
+        DevBuf<int> ocam, opt;
+        DevBuf<double> opx;
+        ocam.alloc(static_cast<size_t>(O));
+        opt.alloc(static_cast<size_t>(O));
+        GBA_HIP(hipMemcpyAsync(ocam.p, prob.obs_camera, sizeof(int) * O, hipMemcpyHostToDevice, st_));
+        GBA_HIP(hipMemcpyAsync(opt.p, prob.obs_point, sizeof(int) * O, hipMemcpyHostToDevice, st_));
+        if (!fp16) {
+            opx.alloc(2 * static_cast<size_t>(O));
+            GBA_HIP(hipMemcpyAsync(opx.p, prob.obs_pixel, sizeof(double) * 2 * O, hipMemcpyHostToDevice, st_));
+        }
+        build_layout_device(st_, C, P, O, ocam.p, opt.p, opx.p, perm_, scam_, spt_, fp16 ? nullptr : &spx_,
+                            pt_off_, cam_off_, cam_list_);
+    }
+    std::vector<int> pt_off(static_cast<size_t>(P) + 1), cam_off(static_cast<size_t>(C) + 1);
+    GBA_HIP(hipMemcpyAsync(pt_off.data(), pt_off_.p, sizeof(int) * pt_off.size(), hipMemcpyDeviceToHost, st_));
+    GBA_HIP(hipMemcpyAsync(cam_off.data(), cam_off_.p, sizeof(int) * cam_off.size(), hipMemcpyDeviceToHost, st_));
+    GBA_HIP(hipStreamSynchronize(st_));
+    std::vector<int2> units;
+    layout_units(pt_off, units, pair_count_);
+    std::vector<int4> chunks;
+    std::vector<int> cam_chunk_off;
+    layout_chunks(cam_off, chunks, cam_chunk_off);
     pb_.num_units = static_cast<int>(units.size());
     pb_.num_chunks = static_cast<int>(chunks.size());
-
-    // ---- device buffers ----
-    GBA_HIP(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
-    upload(scam_, scam, st_);
-    upload(spt_, spt, st_);
-    upload(perm_, perm, st_);
-    upload(pt_off_, pt_off, st_);
-    if (cfg.obs_storage == GBA_OBS_FP16) {
-        // only the binary16 pixels and the camera scales reach the device
+    if (fp16) {
+        // only the binary16 pixels and the camera scales reach the device,
+        // packed in the sorted order
+        std::vector<int> perm(static_cast<size_t>(O));
+        GBA_HIP(hipMemcpyAsync(perm.data(), perm_.p, sizeof(int) * O, hipMemcpyDeviceToHost, st_));
         std::vector<float> scale(static_cast<size_t>(C));
         std::vector<uint16_t> h(2 * static_cast<size_t>(O));
         encode_obs_fp16(prob, scale.data(), h.data());
+        GBA_HIP(hipStreamSynchronize(st_));
         std::vector<unsigned> packed(static_cast<size_t>(O));
         for (int k = 0; k < O; ++k) {
             const size_t e = static_cast<size_t>(perm[k]);
@@ -363,15 +381,10 @@ Solver::Solver(const gba_problem_view& prob, const gba_solver_config& cfg) : cfg
         upload(cam_scale_, scale, st_);
         pb_.spx16 = spx16_.p;
         pb_.cam_scale = cam_scale_.p;
-    } else if (cfg.obs_storage == GBA_OBS_FP64) {
-        upload(spx_, spx, st_);
-    } else {
-        throw Error("unknown observation storage");
     }
     upload(units_, units, st_);
-    upload(cam_list_, cam_list, st_);
-    upload(cam_off_, cam_off, st_);
     // Schur task and pair lists, built on the device from the sorted layout
+    std::vector<SchurTask> stasks;
     build_schur_tasks_device(st_, C, sshape_.cb, cam_off, cam_off_.p, cam_list_.p, spt_.p, pt_off_.p, scam_.p,
                              stasks, spairs_, !sep_diag_);
     num_stasks_ = static_cast<int>(stasks.size());

@@ -45,6 +45,17 @@ struct HostLayout {
     double pair_count = 0.0;  // sum_p k_p (k_p + 1) / 2
 };
 void build_layout(const gba_problem_view& prob, HostLayout& L);
+// Work units over the point CSR (points of up to a wavefront of observations
+// grouped, larger points alone) and sum_p k_p (k_p + 1) / 2; camera-pass
+// chunks over the camera CSR.  Shared by both layout forms.
+void layout_units(const std::vector<int>& pt_off, std::vector<int2>& units, double& pair_count);
+void layout_chunks(const std::vector<int>& cam_off, std::vector<int4>& chunks, std::vector<int>& cam_chunk_off);
+// Device form of build_layout (k_layout.hip): the same perm, scam, spt, spx
+// (when spx is set), pt_off, cam_off and cam_list, from the caller's arrays
+// already on the device.  Allocates the outputs; synchronises st.
+void build_layout_device(hipStream_t st, int C, int P, int O, const int* d_cam, const int* d_pt,
+                         const double* d_px, DevBuf<int>& perm, DevBuf<int>& scam, DevBuf<int>& spt,
+                         DevBuf<double2>* spx, DevBuf<int>& pt_off, DevBuf<int>& cam_off, DevBuf<int>& cam_list);
 
 // IEEE binary16 bits of x, rounded to nearest even.
 uint16_t double_to_half_rne(double x);
