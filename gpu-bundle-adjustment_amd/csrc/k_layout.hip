@@ -8,10 +8,10 @@
 //             the indices as values (the sort is stable, so ties keep e's
 //             order);
 //   scam/spt/spx : camera, point, pixels of sorted observation k;
-//   pt_off  : exclusive scan of the per-point counts (P + 1);
+//   pt_off  : the point CSR (P + 1), from the boundaries of the sorted points;
 //   cam_list/cam_off : sorted positions k grouped by camera, ascending k
 //             within a camera (a second stable radix sort, keys scam[k]),
-//             and the exclusive scan of the per-camera counts (C + 1).
+//             and the camera CSR (C + 1) from the sorted cameras.
 // The same layout as the host form (build_layout).
 #include <hipcub/hipcub.hpp>
 
@@ -24,15 +24,22 @@ namespace gba_amd {
 namespace {
 
 __global__ void k_layout_keys(int O, int cbits, const int* __restrict__ cam, const int* __restrict__ pt,
-                              unsigned long long* __restrict__ keys, int* __restrict__ idx,
-                              int* __restrict__ pcnt, int* __restrict__ ccnt) {
+                              unsigned long long* __restrict__ keys, int* __restrict__ idx) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= O) return;
-    const int c = cam[e], p = pt[e];
-    keys[e] = (static_cast<unsigned long long>(p) << cbits) | static_cast<unsigned long long>(c);
+    keys[e] = (static_cast<unsigned long long>(pt[e]) << cbits) | static_cast<unsigned long long>(cam[e]);
     idx[e] = e;
-    atomicAdd(pcnt + p, 1);
-    atomicAdd(ccnt + c, 1);
+}
+
+// CSR offsets of n sorted keys in [0, nkeys): off[c] = first position whose
+// key is >= c, off[nkeys] = n.  Thread k in [0, n] writes the offsets of the
+// keys in (key[k-1], key[k]], so every entry is written once.
+__global__ void k_csr_offsets(int n, int nkeys, const int* __restrict__ key, int* __restrict__ off) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k > n) return;
+    const int prev = k == 0 ? -1 : key[k - 1];
+    const int cur = k == n ? nkeys : key[k];
+    for (int c = prev + 1; c <= cur; ++c) off[c] = k;
 }
 
 __global__ void k_layout_gather(int O, const int* __restrict__ perm, const int* __restrict__ cam,
@@ -62,14 +69,12 @@ void build_layout_device(hipStream_t st, int C, int P, int O, const int* d_cam, 
     const int cbits = bits_for(C), pbits = bits_for(P);
     if (cbits + pbits > 62) throw Error("layout: camera and point indices too wide for the sort key");
     DevBuf<unsigned long long> keys, keys_sorted;
-    DevBuf<int> idx, ccnt, pcnt, iota, cam_keys_sorted;
+    DevBuf<int> idx, iota, cam_keys_sorted;
     keys.alloc(static_cast<size_t>(O));
     keys_sorted.alloc(static_cast<size_t>(O));
     idx.alloc(static_cast<size_t>(O));
     iota.alloc(static_cast<size_t>(O));
     cam_keys_sorted.alloc(static_cast<size_t>(O));
-    pcnt.alloc(static_cast<size_t>(P) + 1);
-    ccnt.alloc(static_cast<size_t>(C) + 1);
     perm.alloc(static_cast<size_t>(O));
     scam.alloc(static_cast<size_t>(O));
     spt.alloc(static_cast<size_t>(O));
@@ -77,22 +82,17 @@ void build_layout_device(hipStream_t st, int C, int P, int O, const int* d_cam, 
     pt_off.alloc(static_cast<size_t>(P) + 1);
     cam_off.alloc(static_cast<size_t>(C) + 1);
     cam_list.alloc(static_cast<size_t>(O));
-    // scratch of the sorts and scans
-    size_t b1 = 0, b2 = 0, b3 = 0, b4 = 0;
+    // scratch of the sorts
+    size_t b1 = 0, b2 = 0;
     GBA_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, b1, keys.p, keys_sorted.p, idx.p, perm.p, O, 0,
                                                cbits + pbits, st));
     GBA_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, b2, scam.p, cam_keys_sorted.p, iota.p, cam_list.p, O, 0,
                                                cbits, st));
-    GBA_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, b3, pcnt.p, pt_off.p, P + 1, st));
-    GBA_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, b4, ccnt.p, cam_off.p, C + 1, st));
-    const size_t wb = std::max(std::max(b1, b2), std::max(b3, b4));
+    const size_t wb = std::max(b1, b2);
     DevBuf<unsigned char> work;
     work.alloc(std::max<size_t>(wb, 1));
-    launch_fill_int(st, pcnt.p, P + 1, 0);
-    launch_fill_int(st, ccnt.p, C + 1, 0);
-    const int blocks = (O + 255) / 256;
-    hipLaunchKernelGGL(k_layout_keys, dim3(blocks), dim3(256), 0, st, O, cbits, d_cam, d_pt, keys.p, idx.p, pcnt.p,
-                       ccnt.p);
+    const int blocks = (O + 255) / 256, blocks1 = (O + 1 + 255) / 256;
+    hipLaunchKernelGGL(k_layout_keys, dim3(blocks), dim3(256), 0, st, O, cbits, d_cam, d_pt, keys.p, idx.p);
     size_t bytes = wb;
     GBA_HIP(hipcub::DeviceRadixSort::SortPairs(work.p, bytes, keys.p, keys_sorted.p, idx.p, perm.p, O, 0,
                                                cbits + pbits, st));
@@ -101,10 +101,10 @@ void build_layout_device(hipStream_t st, int C, int P, int O, const int* d_cam, 
     bytes = wb;
     GBA_HIP(hipcub::DeviceRadixSort::SortPairs(work.p, bytes, scam.p, cam_keys_sorted.p, iota.p, cam_list.p, O, 0,
                                                cbits, st));
-    bytes = wb;
-    GBA_HIP(hipcub::DeviceScan::ExclusiveSum(work.p, bytes, pcnt.p, pt_off.p, P + 1, st));
-    bytes = wb;
-    GBA_HIP(hipcub::DeviceScan::ExclusiveSum(work.p, bytes, ccnt.p, cam_off.p, C + 1, st));
+    hipLaunchKernelGGL(k_csr_offsets, dim3(blocks1), dim3(256), 0, st, O, P, static_cast<const int*>(spt.p),
+                       pt_off.p);
+    hipLaunchKernelGGL(k_csr_offsets, dim3(blocks1), dim3(256), 0, st, O, C,
+                       static_cast<const int*>(cam_keys_sorted.p), cam_off.p);
     GBA_HIP(hipGetLastError());
     GBA_HIP(hipStreamSynchronize(st));  // the temporaries are released on return
 }

@@ -7,8 +7,8 @@
 // in [lo, hi); its slots qoff[t] .. qoff[t] + n_a - 1 are camera a's
 // observations in index order (slot j <-> cam_list[cam_off[a] + j]), so the
 // pairs of a task come grouped by e (ascending point) and ordered by f, the
-// order the host form produces.  Two passes over the same loops: count per
-// slot, exclusive scan (write positions), fill.
+// order the host form produces.  Two passes over the same loops, one thread
+// per slot: count per slot, exclusive scan (write positions), fill.
 #include <hipcub/hipcub.hpp>
 
 #include "device_common.hpp"
@@ -19,36 +19,41 @@ namespace gba_amd {
 
 namespace {
 
-// One workgroup per task, threads over the task's slots.  kFill = false:
-// cnt[q] = number of pairs of slot q (and cnt[nslots] = 0, the scan's
+// One thread per slot q (the task found by binary search in qoff).  kFill =
+// false: cnt[q] = number of pairs of slot q (and cnt[nslots] = 0, the scan's
 // trailing element); kFill = true: writes slot q's pairs from position pos[q].
 template <bool kFill>
-__global__ __launch_bounds__(256) void k_pairs(const int4* __restrict__ tq, const long long* __restrict__ qoff,
-                                               long long nslots, const int* __restrict__ cam_off,
-                                               const int* __restrict__ cam_list, const int* __restrict__ spt,
-                                               const int* __restrict__ pt_off, const int* __restrict__ scam,
-                                               long long* __restrict__ cnt, const long long* __restrict__ pos,
-                                               int2* __restrict__ pairs) {
-    const int t = blockIdx.x;
-    const int4 d = tq[t];  // camera a, lo, hi
-    const int k0 = cam_off[d.x], n = cam_off[d.x + 1] - k0;
-    const long long q0 = qoff[t];
-    if (!kFill && t == 0 && threadIdx.x == 0) cnt[nslots] = 0;
-    for (int j = threadIdx.x; j < n; j += 256) {
-        const int e = cam_list[k0 + j];
-        const int p = spt[e];
-        const int f1 = pt_off[p + 1];
-        long long w = kFill ? pos[q0 + j] : 0;
-        // the point's observations are ordered by camera
-        for (int f = pt_off[p]; f < f1; ++f) {
-            const int b = scam[f];
-            if (b >= d.z) break;
-            if (b < d.y) continue;
-            if (kFill) pairs[w] = make_int2(e, f);
-            ++w;
-        }
-        if (!kFill) cnt[q0 + j] = w;
+__global__ __launch_bounds__(256) void k_pairs(int ntask, const int4* __restrict__ tq,
+                                               const long long* __restrict__ qoff, long long nslots,
+                                               const int* __restrict__ cam_off, const int* __restrict__ cam_list,
+                                               const int* __restrict__ spt, const int* __restrict__ pt_off,
+                                               const int* __restrict__ scam, long long* __restrict__ cnt,
+                                               const long long* __restrict__ pos, int2* __restrict__ pairs) {
+    const long long q = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
+    if (!kFill && q == nslots) cnt[nslots] = 0;
+    if (q >= nslots) return;
+    // the last task t with qoff[t] <= q (empty tasks share their successor's
+    // first slot and are skipped); qoff[ntask] = nslots > q
+    int lo = 0, hi = ntask;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (qoff[mid] <= q) lo = mid;
+        else hi = mid;
     }
+    const int4 d = tq[lo];  // camera a, lo, hi
+    const int e = cam_list[cam_off[d.x] + static_cast<int>(q - qoff[lo])];
+    const int p = spt[e];
+    const int f1 = pt_off[p + 1];
+    long long w = kFill ? pos[q] : 0;
+    // the point's observations are ordered by camera
+    for (int f = pt_off[p]; f < f1; ++f) {
+        const int b = scam[f];
+        if (b >= d.z) break;
+        if (b < d.y) continue;
+        if (kFill) pairs[w] = make_int2(e, f);
+        ++w;
+    }
+    if (!kFill) cnt[q] = w;
 }
 
 __global__ void k_pair_bounds(int ntask, const long long* __restrict__ qoff, const long long* __restrict__ pos,
@@ -70,8 +75,9 @@ void launch_pair_count(hipStream_t st, int ntask, const int4* tq, const long lon
                        const int* cam_off, const int* cam_list, const int* spt, const int* pt_off,
                        const int* scam, long long* cnt) {
     if (ntask <= 0) return;
-    hipLaunchKernelGGL(k_pairs<false>, dim3(ntask), dim3(256), 0, st, tq, qoff, nslots, cam_off, cam_list, spt,
-                       pt_off, scam, cnt, static_cast<const long long*>(nullptr), static_cast<int2*>(nullptr));
+    const unsigned blocks = static_cast<unsigned>((nslots + 1 + 255) / 256);
+    hipLaunchKernelGGL(k_pairs<false>, dim3(blocks), dim3(256), 0, st, ntask, tq, qoff, nslots, cam_off, cam_list,
+                       spt, pt_off, scam, cnt, static_cast<const long long*>(nullptr), static_cast<int2*>(nullptr));
 }
 
 void launch_pair_scan(hipStream_t st, const long long* cnt, long long* pos, long long n, void* work,
@@ -85,12 +91,13 @@ void launch_pair_bounds(hipStream_t st, int ntask, const long long* qoff, const 
     hipLaunchKernelGGL(k_pair_bounds, dim3((ntask + 1 + 255) / 256), dim3(256), 0, st, ntask, qoff, pos, bounds);
 }
 
-void launch_pair_fill(hipStream_t st, int ntask, const int4* tq, const long long* qoff, const int* cam_off,
-                      const int* cam_list, const int* spt, const int* pt_off, const int* scam, const long long* pos,
-                      int2* pairs) {
-    if (ntask <= 0) return;
-    hipLaunchKernelGGL(k_pairs<true>, dim3(ntask), dim3(256), 0, st, tq, qoff, 0LL, cam_off, cam_list, spt, pt_off,
-                       scam, static_cast<long long*>(nullptr), pos, pairs);
+void launch_pair_fill(hipStream_t st, int ntask, const int4* tq, const long long* qoff, long long nslots,
+                      const int* cam_off, const int* cam_list, const int* spt, const int* pt_off, const int* scam,
+                      const long long* pos, int2* pairs) {
+    if (ntask <= 0 || nslots <= 0) return;
+    const unsigned blocks = static_cast<unsigned>((nslots + 255) / 256);
+    hipLaunchKernelGGL(k_pairs<true>, dim3(blocks), dim3(256), 0, st, ntask, tq, qoff, nslots, cam_off, cam_list,
+                       spt, pt_off, scam, static_cast<long long*>(nullptr), pos, pairs);
 }
 
 }  // namespace gba_amd

@@ -71,9 +71,9 @@ void launch_pair_scan(hipStream_t st, const long long* cnt, long long* pos, long
                       size_t work_bytes);
 void launch_pair_bounds(hipStream_t st, int ntask, const long long* qoff, const long long* pos,
                         long long* bounds);
-void launch_pair_fill(hipStream_t st, int ntask, const int4* tq, const long long* qoff, const int* cam_off,
-                      const int* cam_list, const int* spt, const int* pt_off, const int* scam, const long long* pos,
-                      int2* pairs);
+void launch_pair_fill(hipStream_t st, int ntask, const int4* tq, const long long* qoff, long long nslots,
+                      const int* cam_off, const int* cam_list, const int* spt, const int* pt_off, const int* scam,
+                      const long long* pos, int2* pairs);
 
 // k_dense.hip
 // Factors in place; D holds 2*nblk tiles of 64x64: tiles [0, nblk) receive the

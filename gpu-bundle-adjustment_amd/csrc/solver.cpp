@@ -152,8 +152,8 @@ void build_schur_tasks_device(hipStream_t st, int C, int cb, const std::vector<i
     GBA_HIP(hipStreamSynchronize(st));
     if (hb[ntask] > 0) {
         pairs.alloc(static_cast<size_t>(hb[ntask]));
-        launch_pair_fill(st, ntask, d_tq.p, d_qoff.p, d_cam_off, d_cam_list, d_spt, d_pt_off, d_scam, pos.p,
-                         pairs.p);
+        launch_pair_fill(st, ntask, d_tq.p, d_qoff.p, nslots, d_cam_off, d_cam_list, d_spt, d_pt_off, d_scam,
+                         pos.p, pairs.p);
     }
     GBA_HIP(hipGetLastError());
     GBA_HIP(hipStreamSynchronize(st));  // the temporaries are released on return
