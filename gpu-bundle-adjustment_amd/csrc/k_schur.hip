@@ -207,7 +207,10 @@ __global__ __launch_bounds__(256) void k_schur_diag(
     ST* __restrict__ S, ST* __restrict__ Slow, int ld, double* __restrict__ rhs,
     const TrialFlags* flags) {
     __shared__ double s_red[4][54];
+    __shared__ double s_dup[45];  // repeated observations (see below)
     const int a = blockIdx.x, tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    if (tid < 45) s_dup[tid] = 0.0;
+    __syncthreads();
     const int q = prob_of_cam(pb, a);
     if (prob_off(pb, q) || flags[q].nonspd) return;
     lambda = lam_of(pb, q, lambda);
@@ -243,6 +246,35 @@ __global__ __launch_bounds__(256) void k_schur_diag(
                 acc[i * (i + 1) / 2 + j] += t0 * wv[3 * j] + t1 * wv[3 * j + 1] + t2 * wv[3 * j + 2];
             racc[i] += wv[3 * i] * vb0 + wv[3 * i + 1] * vb1 + wv[3 * i + 2] * vb2;
         }
+        // a point observed more than once by camera a: the pairs e != f of
+        // that run (adjacent in the point's camera-ordered list) also land
+        // on S_aa, W_e V'^-1 W_f^T (src/lm.cu:515-559 takes every f with
+        // camera(f) <= camera(e)).  Rare, so it stays out of the registers
+        // of the common path: rows of W_e and W_f read from memory, sums
+        // into s_dup with LDS atomics.
+        const int k0 = pb.pt_off[p], k1 = pb.pt_off[p + 1];
+        if ((e > k0 && pb.scam[e - 1] == a) || (e + 1 < k1 && pb.scam[e + 1] == a)) {
+            int f = e;
+            while (f > k0 && pb.scam[f - 1] == a) --f;
+            const WT* we = W + kWStride * static_cast<size_t>(e);
+            for (; f < k1 && pb.scam[f] == a; ++f) {
+                if (f == e) continue;
+                const WT* wf = W + kWStride * static_cast<size_t>(f);
+#pragma unroll 1
+                for (int i = 0; i < 9; ++i) {
+                    const double e0 = static_cast<double>(we[3 * i]), e1 = static_cast<double>(we[3 * i + 1]),
+                                 e2 = static_cast<double>(we[3 * i + 2]);
+                    const double t0 = e0 * v00 + e1 * v10 + e2 * v20;
+                    const double t1 = e0 * v10 + e1 * v11 + e2 * v21;
+                    const double t2 = e0 * v20 + e1 * v21 + e2 * v22;
+#pragma unroll 1
+                    for (int j = 0; j <= i; ++j)
+                        atomicAdd(&s_dup[i * (i + 1) / 2 + j],
+                                  t0 * static_cast<double>(wf[3 * j]) + t1 * static_cast<double>(wf[3 * j + 1]) +
+                                      t2 * static_cast<double>(wf[3 * j + 2]));
+                }
+            }
+        }
     }
 #pragma unroll
     for (int i = 0; i < 54; ++i) {
@@ -251,7 +283,8 @@ __global__ __launch_bounds__(256) void k_schur_diag(
     }
     __syncthreads();
     if (tid < 54) {
-        const double sum = ((s_red[0][tid] + s_red[1][tid]) + s_red[2][tid]) + s_red[3][tid];
+        double sum = ((s_red[0][tid] + s_red[1][tid]) + s_red[2][tid]) + s_red[3][tid];
+        if (tid < 45) sum += s_dup[tid];
         if (tid >= 45) {
             const int row = 9 * a + (tid - 45);
             const double v = bc[row] - sum;

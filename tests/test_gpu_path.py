@@ -85,6 +85,54 @@ def test_schur_step_fp64_matches_oracle(gba, orc, which, lam):
     assert abs(g["denom"] - o["denom"]) <= 1e-9 * abs(o["denom"])
 
 
+def shuffled(gba, seed):
+    """cfg2 with its observations in a random order and one observation
+    repeated (same point, same camera): the device sort must order by
+    (point, camera) with ties in the caller's order."""
+    p = gba.generate(2, 1)
+    rng = np.random.default_rng(seed)
+    order = rng.permutation(p.num_observations)
+    order = np.concatenate([order, order[:1]])
+    return gba.BALProblem(p.cameras.copy(), p.points.copy(), p.obs_camera[order].copy(),
+                          p.obs_point[order].copy(), p.obs_pixel[order].copy())
+
+
+@pytest.mark.parametrize("seed", [3, 4])
+def test_layout_any_observation_order_matches_oracle(gba, orc, seed):
+    prob = shuffled(gba, seed)
+    g = gba.linearize(prob)
+    o = orc.linearize(prob)
+    for k in ("U", "V", "W", "bc", "bp"):
+        assert arel(g[k], o[k]) < 1e-12, k
+    gs = gba.schur_step(prob, 1e-4, want_S=True)
+    os_ = orc.schur_step(prob, o, 1e-4, want_S=True)
+    assert gs["status"] == "accept" and os_["status"] == "ok"
+    assert arel(np.tril(gs["S"]), np.tril(os_["S"])) < 1e-12
+    assert arel(gs["dc"], os_["dc"]) < 1e-9
+    assert arel(gs["dp"], os_["dp"]) < 1e-9
+
+
+@pytest.mark.parametrize("C", [1, 2])
+def test_one_and_two_camera_problems_match_oracle(gba, orc, C):
+    """The smallest layouts: one camera (S is one 9x9 block, no pair tasks;
+    every point seen once, so the cost falls to rounding level) and two
+    cameras (one pair task).  Costs compared to the initial cost's scale."""
+    p = gba.generate(2, 1)
+    keep = p.obs_camera < C
+    used = np.unique(p.obs_point[keep])
+    remap = -np.ones(p.num_points, np.int32)
+    remap[used] = np.arange(used.size, dtype=np.int32)
+    q = gba.BALProblem(p.cameras[:C].copy(), p.points[used].copy(), p.obs_camera[keep].copy(),
+                       remap[p.obs_point[keep]].copy(), p.obs_pixel[keep].copy())
+    opts = dict(initial_lambda=1e-4, max_iterations=5)
+    r = gba.solve_bal_gpu(q.copy(), gba.LMOptions(**opts))
+    o = orc.lm_solve(q.copy(), **opts)
+    assert r.termination == o["termination"]
+    assert abs(r.iterations - o["iterations"]) <= 1
+    assert abs(r.initial_cost - o["initial_cost"]) <= 1e-12 * o["initial_cost"]
+    assert abs(r.final_cost - o["final_cost"]) <= 1e-9 * o["initial_cost"]
+
+
 def test_schur_step_mixed_close_to_fp64(gba, orc):
     prob = gba.generate(2, 1)
     g = gba.schur_step(prob, 1e-4, gba.PRECISION_MIXED, want_S=True)
