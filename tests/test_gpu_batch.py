@@ -139,3 +139,33 @@ def test_batch_autotuner_picks_the_faster_form(gba, monkeypatch):
     monkeypatch.setenv("GBA_BATCH_FACTOR", "workgroup")
     inf2 = gba.Batch(probs).info()
     assert inf2["factor"] == "workgroup" and inf2["tune_ms"]["tiles"] == 0.0
+
+
+def test_batch_any_order_repeated_observation_and_one_camera(gba):
+    """Problems whose observations come in a random order with one repeated
+    (same point, same camera), and a one-camera problem, batched with a
+    regular one: each against its own sequential solve."""
+    p = gba.generate(2, 1)
+    rng = np.random.default_rng(5)
+    order = rng.permutation(p.num_observations)
+    order = np.concatenate([order, order[:1]])
+    shuf = gba.BALProblem(p.cameras.copy(), p.points.copy(), p.obs_camera[order].copy(),
+                          p.obs_point[order].copy(), p.obs_pixel[order].copy())
+    keep = p.obs_camera < 1
+    used = np.unique(p.obs_point[keep])
+    remap = -np.ones(p.num_points, np.int32)
+    remap[used] = np.arange(used.size, dtype=np.int32)
+    one = gba.BALProblem(p.cameras[:1].copy(), p.points[used].copy(), p.obs_camera[keep].copy(),
+                         remap[p.obs_point[keep]].copy(), p.obs_pixel[keep].copy())
+    probs = [shuf, gba.generate(5, 4), one]
+    opts = gba.LMOptions(max_iterations=8)
+    seq = [gba.Solver(q).solve(opts) for q in probs]
+    res = gba.Batch(probs).solve(opts)
+    for rb, rs in zip(res[:2], seq[:2]):
+        check_pair(rb, rs)
+    # the one-camera problem fits exactly (every point seen once): its cost
+    # falls to rounding level, compared on the scale of the initial cost
+    rb, rs = res[2], seq[2]
+    assert rb.termination == rs.termination
+    assert abs(rb.initial_cost - rs.initial_cost) <= 1e-12 * rs.initial_cost
+    assert abs(rb.final_cost - rs.final_cost) <= 1e-9 * rs.initial_cost
