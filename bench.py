@@ -57,7 +57,9 @@ def run_batch(g, args):
     import torch
     probs = [g.generate(5, args.seed + i) for i in range(args.batch)]
     opts = g.LMOptions(initial_lambda=args.lambda0)
+    t_create = time.perf_counter()
     b = g.Batch(probs)
+    t_create = time.perf_counter() - t_create
     for _ in range(args.warmup):
         b.solve(opts)
     torch.cuda.synchronize()
@@ -92,6 +94,7 @@ def run_batch(g, args):
         "speedup_vs_sequential": round(rate / seq_rate, 2),
         "rounds": b.rounds,
         "handle": b.info(),
+        "create_s": round(t_create, 4),
         "max_final_cost_rel_diff_vs_sequential": float(max(dev)),
         "terminations": {t: sum(1 for r in res if r.termination == t) for t in set(r.termination for r in res)},
     }

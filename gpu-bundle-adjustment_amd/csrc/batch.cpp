@@ -33,14 +33,12 @@ constexpr int kBatchMaxNpad = 1024;
 
 BatchSolver::BatchSolver(const gba_problem_view* probs, int count, bool autotune) : B_(count) {
     if (count <= 0) throw Error("batch: no problems");
-    std::vector<int> scam, spt, pt_off(1, 0), cam_list, cam_off(1, 0), cam_chunk_off, cam_prob, pt_prob;
-    std::vector<double2> spx;
-    std::vector<int2> units, spairs;
-    std::vector<int4> chunks;
-    std::vector<SchurTask> stasks;
     int Cmax = 1;
     for (int b = 0; b < count; ++b) Cmax = std::max(Cmax, probs[b].num_cameras);
     sshape_ = schur_shape(Cmax, false);
+    // ---- per-problem descriptors and offsets (host) ----
+    std::vector<long long> o_off(1, 0);
+    std::vector<int> c0s, p0s, cam_prob, pt_prob;
     long long C0 = 0, P0 = 0, O0 = 0, soff = 0, doff = 0;
     for (int b = 0; b < count; ++b) {
         const gba_problem_view& pr = probs[b];
@@ -53,80 +51,104 @@ BatchSolver::BatchSolver(const gba_problem_view* probs, int count, bool autotune
         if (npad > kBatchMaxNpad) throw Error("batch: a problem has more than 113 cameras");
         if (C0 + C > (1LL << 30) || P0 + P > (1LL << 30) || O0 + O > (1LL << 30))
             throw Error("batch: too large");
-        HostLayout L;
-        build_layout(pr, L);
-        std::vector<SchurTask> t;
-        std::vector<int2> pr_pairs;
-        build_schur_tasks(C, sshape_.cb, L.cam_off, L.cam_list, L.pt_off, L.scam, L.spt, t, pr_pairs, false);
         BatchDesc d;
         d.c0 = static_cast<int>(C0);
         d.c1 = static_cast<int>(C0 + C);
-        d.u0 = static_cast<int>(units.size());
-        d.u1 = d.u0 + static_cast<int>(L.units.size());
+        d.u0 = d.u1 = 0;  // below, from the point CSR
         d.soff = soff;
         d.doff = doff;
         d.n = n;
         d.npad = npad;
         hdesc_.push_back(d);
-        const long long pair0 = static_cast<long long>(spairs.size());
-        for (const int2& q : pr_pairs) spairs.push_back(make_int2(q.x + static_cast<int>(O0), q.y + static_cast<int>(O0)));
-        for (SchurTask x : t) {
-            x.a += static_cast<int>(C0);
-            x.lo += static_cast<int>(C0);
-            x.hi += static_cast<int>(C0);
-            x.a0 = static_cast<int>(C0);
-            x.begin += pair0;
-            x.end += pair0;
-            x.soff = soff;
-            x.ld = npad;
-            stasks.push_back(x);
-        }
-        for (int k = 0; k < O; ++k) {
-            scam.push_back(L.scam[k] + static_cast<int>(C0));
-            spt.push_back(L.spt[k] + static_cast<int>(P0));
-            spx.push_back(L.spx[k]);
-        }
-        for (int p = 0; p < P; ++p) pt_off.push_back(L.pt_off[p + 1] + static_cast<int>(O0));
-        for (const int2& u : L.units) units.push_back(make_int2(u.x + static_cast<int>(O0), u.y + static_cast<int>(O0)));
-        for (int k = 0; k < O; ++k) cam_list.push_back(L.cam_list[k] + static_cast<int>(O0));
-        for (int c = 0; c < C; ++c) cam_off.push_back(L.cam_off[c + 1] + static_cast<int>(O0));
-        const int ch0 = static_cast<int>(chunks.size());
-        for (int c = 0; c < C; ++c) cam_chunk_off.push_back(L.cam_chunk_off[c] + ch0);
-        for (const int4& ch : L.chunks)
-            chunks.push_back(make_int4(ch.x + static_cast<int>(C0), ch.y + static_cast<int>(O0),
-                                       ch.z + static_cast<int>(O0), 0));
-        for (int c = 0; c < C; ++c) cam_prob.push_back(b);
-        for (int p = 0; p < P; ++p) pt_prob.push_back(b);
+        c0s.push_back(static_cast<int>(C0));
+        p0s.push_back(static_cast<int>(P0));
+        cam_prob.insert(cam_prob.end(), static_cast<size_t>(C), b);
+        pt_prob.insert(pt_prob.end(), static_cast<size_t>(P), b);
         init_cams_.insert(init_cams_.end(), pr.cameras, pr.cameras + 9 * static_cast<size_t>(C));
         init_pts_.insert(init_pts_.end(), pr.points, pr.points + 3 * static_cast<size_t>(P));
         pt0_.push_back(P0);
         C0 += C;
         P0 += P;
         O0 += O;
+        o_off.push_back(O0);
         soff += static_cast<long long>(npad) * npad;
         doff += 2LL * (npad / kTile) * kTile * kTile;
     }
-    cam_chunk_off.push_back(static_cast<int>(chunks.size()));
     pt0_.push_back(P0);
+    const int C = static_cast<int>(C0), P = static_cast<int>(P0), O = static_cast<int>(O0);
+
+    // ---- the whole batch's point-major layout on the device: the problems'
+    // observations side by side with batch indices, sorted by (point, camera,
+    // index) -- points are numbered problem by problem, so each problem's
+    // observations stay contiguous and in its own sorted order ----
+    GBA_HIP(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
+    std::vector<int> pt_off, cam_off;
+    {
+        DevBuf<int> ocam, opt, d_c0, d_p0, perm;
+        DevBuf<double> opx;
+        DevBuf<long long> d_ooff;
+        ocam.alloc(static_cast<size_t>(O));
+        opt.alloc(static_cast<size_t>(O));
+        opx.alloc(2 * static_cast<size_t>(O));
+        for (int b = 0; b < count; ++b) {
+            const gba_problem_view& pr = probs[b];
+            const size_t e0 = static_cast<size_t>(o_off[b]), no = static_cast<size_t>(pr.num_observations);
+            GBA_HIP(hipMemcpyAsync(ocam.p + e0, pr.obs_camera, sizeof(int) * no, hipMemcpyHostToDevice, st_));
+            GBA_HIP(hipMemcpyAsync(opt.p + e0, pr.obs_point, sizeof(int) * no, hipMemcpyHostToDevice, st_));
+            GBA_HIP(hipMemcpyAsync(opx.p + 2 * e0, pr.obs_pixel, sizeof(double) * 2 * no, hipMemcpyHostToDevice,
+                                   st_));
+        }
+        upload(d_ooff, o_off, st_);
+        upload(d_c0, c0s, st_);
+        upload(d_p0, p0s, st_);
+        launch_offset_obs(st_, O, count, d_ooff.p, d_c0.p, d_p0.p, ocam.p, opt.p);
+        build_layout_device(st_, C, P, O, ocam.p, opt.p, opx.p, perm, scam_, spt_, &spx_, pt_off_, cam_off_,
+                            cam_list_);
+        pt_off.resize(static_cast<size_t>(P) + 1);
+        cam_off.resize(static_cast<size_t>(C) + 1);
+        GBA_HIP(hipMemcpyAsync(pt_off.data(), pt_off_.p, sizeof(int) * pt_off.size(), hipMemcpyDeviceToHost, st_));
+        GBA_HIP(hipMemcpyAsync(cam_off.data(), cam_off_.p, sizeof(int) * cam_off.size(), hipMemcpyDeviceToHost,
+                               st_));
+        GBA_HIP(hipStreamSynchronize(st_));
+    }
+    // work units per problem (a unit never spans two problems), camera chunks
+    std::vector<int2> units;
+    std::vector<int4> chunks;
+    std::vector<int> cam_chunk_off;
+    for (int b = 0; b < count; ++b) {
+        double pc = 0.0;
+        hdesc_[b].u0 = static_cast<int>(units.size());
+        layout_units_append(pt_off.data() + pt0_[b], static_cast<int>(pt0_[b + 1] - pt0_[b]), units, pc);
+        hdesc_[b].u1 = static_cast<int>(units.size());
+    }
+    layout_chunks(cam_off, chunks, cam_chunk_off);
+    // Schur tasks of every problem (column chunks of cb cameras from the
+    // problem's first camera, blocks b < a), pair lists on the device
+    std::vector<int4> tq;
+    std::vector<int> tprob;
+    for (int b = 0; b < count; ++b)
+        for (int a = hdesc_[b].c0; a < hdesc_[b].c1; ++a)
+            for (int lo = hdesc_[b].c0; lo < a; lo += sshape_.cb) {
+                tq.push_back(make_int4(a, lo, std::min(lo + sshape_.cb, a), 0));
+                tprob.push_back(b);
+            }
+    std::vector<long long> bounds;
+    schur_pairs_device(st_, tq, cam_off, cam_off_.p, cam_list_.p, spt_.p, pt_off_.p, scam_.p, bounds, spairs_);
+    std::vector<SchurTask> stasks(tq.size());
+    for (size_t t = 0; t < tq.size(); ++t) {
+        const BatchDesc& d = hdesc_[tprob[t]];
+        stasks[t] = SchurTask{tq[t].x, tq[t].y, tq[t].z, d.c0, bounds[t], bounds[t + 1], d.soff, d.npad, 0};
+    }
     // longest Schur tasks first across the whole batch
     std::stable_sort(stasks.begin(), stasks.end(), [](const SchurTask& x, const SchurTask& y) {
         return x.end - x.begin > y.end - y.begin;
     });
     num_stasks_ = static_cast<int>(stasks.size());
-
-    GBA_HIP(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
-    upload(scam_, scam, st_);
-    upload(spt_, spt, st_);
-    upload(spx_, spx, st_);
-    upload(pt_off_, pt_off, st_);
     upload(units_, units, st_);
-    upload(cam_list_, cam_list, st_);
-    upload(cam_off_, cam_off, st_);
     upload(chunks_, chunks, st_);
     upload(cam_chunk_off_, cam_chunk_off, st_);
     upload(cam_prob_, cam_prob, st_);
     upload(pt_prob_, pt_prob, st_);
-    upload(spairs_, spairs, st_);
     upload(stasks_, stasks, st_);
     upload(desc_, hdesc_, st_);
     {
@@ -167,7 +189,6 @@ BatchSolver::BatchSolver(const gba_problem_view* probs, int count, bool autotune
         ftab_.nsteps = nsteps;
         GBA_HIP(hipStreamSynchronize(st_));  // the tables above are released here
     }
-    const int C = static_cast<int>(C0), P = static_cast<int>(P0), O = static_cast<int>(O0);
     pb_.C = C;
     pb_.P = P;
     pb_.O = O;

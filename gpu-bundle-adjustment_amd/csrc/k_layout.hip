@@ -12,7 +12,6 @@
 //   cam_list/cam_off : sorted positions k grouped by camera, ascending k
 //             within a camera (a second stable radix sort, keys scam[k]),
 //             and the camera CSR (C + 1) from the sorted cameras.
-// The same layout as the host form (build_layout).
 #include <hipcub/hipcub.hpp>
 
 #include "device_common.hpp"
@@ -53,6 +52,23 @@ __global__ void k_layout_gather(int O, const int* __restrict__ perm, const int* 
     spt[k] = pt[e];
     if (spx) spx[k] = make_double2(px[2 * static_cast<size_t>(e)], px[2 * static_cast<size_t>(e) + 1]);
     iota[k] = k;
+}
+
+// Problem-local camera / point indices of a batch's concatenated
+// observations to batch indices: observation e of problem q (o_off[q] <= e <
+// o_off[q + 1], found by binary search) gets + c0[q], + p0[q].
+__global__ void k_offset_obs(int O, int count, const long long* __restrict__ o_off, const int* __restrict__ c0,
+                             const int* __restrict__ p0, int* __restrict__ cam, int* __restrict__ pt) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= O) return;
+    int lo = 0, hi = count;  // o_off[lo] <= e < o_off[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (o_off[mid] <= e) lo = mid;
+        else hi = mid;
+    }
+    cam[e] += c0[lo];
+    pt[e] += p0[lo];
 }
 
 int bits_for(int n) {
@@ -107,6 +123,12 @@ void build_layout_device(hipStream_t st, int C, int P, int O, const int* d_cam, 
                        static_cast<const int*>(cam_keys_sorted.p), cam_off.p);
     GBA_HIP(hipGetLastError());
     GBA_HIP(hipStreamSynchronize(st));  // the temporaries are released on return
+}
+
+void launch_offset_obs(hipStream_t st, int O, int count, const long long* o_off, const int* c0, const int* p0,
+                       int* cam, int* pt) {
+    if (O <= 0) return;
+    hipLaunchKernelGGL(k_offset_obs, dim3((O + 255) / 256), dim3(256), 0, st, O, count, o_off, c0, p0, cam, pt);
 }
 
 }  // namespace gba_amd

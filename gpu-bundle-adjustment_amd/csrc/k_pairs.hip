@@ -6,8 +6,7 @@
 // Task t = (a, lo, hi) owns the pairs (e, f) with camera(e) = a and camera(f)
 // in [lo, hi); its slots qoff[t] .. qoff[t] + n_a - 1 are camera a's
 // observations in index order (slot j <-> cam_list[cam_off[a] + j]), so the
-// pairs of a task come grouped by e (ascending point) and ordered by f, the
-// order the host form produces.  Two passes over the same loops, one thread
+// pairs of a task come grouped by e (ascending point) and ordered by f.  Two passes over the same loops, one thread
 // per slot: count per slot, exclusive scan (write positions), fill.
 #include <hipcub/hipcub.hpp>
 

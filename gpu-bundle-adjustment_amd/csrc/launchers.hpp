@@ -75,6 +75,12 @@ void launch_pair_fill(hipStream_t st, int ntask, const int4* tq, const long long
                       const int* cam_off, const int* cam_list, const int* spt, const int* pt_off, const int* scam,
                       const long long* pos, int2* pairs);
 
+// k_layout.hip: a batch's concatenated observations from problem-local to
+// batch camera / point indices (problem q owns observations [o_off[q],
+// o_off[q + 1]) and adds c0[q], p0[q]).
+void launch_offset_obs(hipStream_t st, int O, int count, const long long* o_off, const int* c0, const int* p0,
+                       int* cam, int* pt);
+
 // k_dense.hip
 // Factors in place; D holds 2*nblk tiles of 64x64: tiles [0, nblk) receive the
 // diagonal tiles of L (row-major, upper part zero), tiles [nblk, 2 nblk) their

@@ -17,7 +17,6 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
-#include <thread>
 
 #include "launchers.hpp"
 
@@ -41,122 +40,57 @@ void upload(DevBuf<T>& b, const std::vector<T>& v, hipStream_t st) {
     if (!v.empty()) GBA_HIP(hipMemcpyAsync(b.p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, st));
 }
 
-// Runs f(i) for i in [0, n) on up to 16 host threads, interleaved.
-template <typename F>
-void parallel_for(int n, F f) {
-    const int nt = std::max(1, std::min({16, n, static_cast<int>(std::thread::hardware_concurrency())}));
-    if (nt == 1) {
-        for (int i = 0; i < n; ++i) f(i);
-        return;
-    }
-    std::vector<std::thread> th;
-    for (int t = 0; t < nt; ++t)
-        th.emplace_back([&, t]() {
-            for (int i = t; i < n; i += nt) f(i);
-        });
-    for (auto& x : th) x.join();
-}
-
 }  // namespace
 
-// Task and pair lists of the Schur assembly for chunks of cb camera columns:
-// task (a, c) holds the pairs (e, f) with camera(e) = a, f an observation of
-// the same point by a camera b <= a in [c cb, (c + 1) cb), grouped by e
-// (ascending point) and ordered by b.  Every (a, c) with c cb <= a gets a
-// task, empty or not, because the task also writes its blocks of S.  Tasks
-// are returned by decreasing pair count (longest first).
-void build_schur_tasks(int C, int cb, const std::vector<int>& cam_off,
-                       const std::vector<int>& cam_list, const std::vector<int>& pt_off,
-                       const std::vector<int>& scam, const std::vector<int>& spt,
-                       std::vector<SchurTask>& tasks, std::vector<int2>& pairs, bool with_diag) {
-    // without the diagonal: blocks b < a only (b <= bmax = a - 1)
-    const int dg = with_diag ? 1 : 0;
-    std::vector<int> toff(static_cast<size_t>(C) + 1, 0);
-    for (int a = 0; a < C; ++a) toff[a + 1] = toff[a] + (a + dg + cb - 1) / cb;
-    const int ntask = toff[C];
-    std::vector<long long> cnt(static_cast<size_t>(ntask), 0);
-    parallel_for(C, [&](int a) {
-        long long* c = cnt.data() + toff[a];
-        for (int i = cam_off[a]; i < cam_off[a + 1]; ++i) {
-            const int p = spt[cam_list[i]];
-            for (int f = pt_off[p]; f < pt_off[p + 1]; ++f) {
-                const int b = scam[f];
-                if (b > a - 1 + dg) break;
-                ++c[b / cb];
-            }
-        }
-    });
-    std::vector<long long> begin(static_cast<size_t>(ntask) + 1, 0);
-    for (int t = 0; t < ntask; ++t) begin[t + 1] = begin[t] + cnt[t];
-    pairs.resize(static_cast<size_t>(begin[ntask]));
-    parallel_for(C, [&](int a) {
-        std::vector<long long> cur(begin.begin() + toff[a], begin.begin() + toff[a + 1]);
-        for (int i = cam_off[a]; i < cam_off[a + 1]; ++i) {
-            const int e = cam_list[i];
-            const int p = spt[e];
-            for (int f = pt_off[p]; f < pt_off[p + 1]; ++f) {
-                const int b = scam[f];
-                if (b > a - 1 + dg) break;
-                pairs[static_cast<size_t>(cur[b / cb]++)] = make_int2(e, f);
-            }
-        }
-    });
-    tasks.resize(static_cast<size_t>(ntask));
-    for (int a = 0; a < C; ++a)
-        for (int c = 0; c < toff[a + 1] - toff[a]; ++c) {
-            const int t = toff[a] + c;
-            tasks[t] = SchurTask{a, c * cb, std::min((c + 1) * cb, a + dg), 0, begin[t], begin[t + 1], 0, 0, 0};
-        }
-    std::stable_sort(tasks.begin(), tasks.end(), [](const SchurTask& x, const SchurTask& y) {
-        return x.end - x.begin > y.end - y.begin;
-    });
+void schur_pairs_device(hipStream_t st, const std::vector<int4>& tq, const std::vector<int>& cam_off,
+                        const int* d_cam_off, const int* d_cam_list, const int* d_spt, const int* d_pt_off,
+                        const int* d_scam, std::vector<long long>& bounds, DevBuf<int2>& pairs) {
+    const int ntask = static_cast<int>(tq.size());
+    bounds.assign(static_cast<size_t>(ntask) + 1, 0);
+    pairs.release();
+    if (ntask == 0) return;
+    std::vector<long long> qoff(static_cast<size_t>(ntask) + 1, 0);
+    for (int t = 0; t < ntask; ++t) qoff[t + 1] = qoff[t] + (cam_off[tq[t].x + 1] - cam_off[tq[t].x]);
+    const long long nslots = qoff.back();
+    if (nslots + 1 > INT_MAX) throw Error("too many observation slots for the pair scan");
+    DevBuf<int4> d_tq;
+    DevBuf<long long> d_qoff, cnt, pos, d_bounds;
+    DevBuf<unsigned char> work;
+    upload(d_tq, tq, st);
+    upload(d_qoff, qoff, st);
+    cnt.alloc(static_cast<size_t>(nslots) + 1);
+    pos.alloc(static_cast<size_t>(nslots) + 1);
+    d_bounds.alloc(static_cast<size_t>(ntask) + 1);
+    const size_t wb = pair_scan_work(nslots + 1);
+    work.alloc(std::max<size_t>(wb, 1));
+    launch_pair_count(st, ntask, d_tq.p, d_qoff.p, nslots, d_cam_off, d_cam_list, d_spt, d_pt_off, d_scam, cnt.p);
+    launch_pair_scan(st, cnt.p, pos.p, nslots + 1, work.p, wb);
+    launch_pair_bounds(st, ntask, d_qoff.p, pos.p, d_bounds.p);
+    GBA_HIP(hipMemcpyAsync(bounds.data(), d_bounds.p, sizeof(long long) * bounds.size(), hipMemcpyDeviceToHost,
+                           st));
+    GBA_HIP(hipGetLastError());
+    GBA_HIP(hipStreamSynchronize(st));
+    if (bounds[ntask] > 0) {
+        pairs.alloc(static_cast<size_t>(bounds[ntask]));
+        launch_pair_fill(st, ntask, d_tq.p, d_qoff.p, nslots, d_cam_off, d_cam_list, d_spt, d_pt_off, d_scam,
+                         pos.p, pairs.p);
+    }
+    GBA_HIP(hipGetLastError());
+    GBA_HIP(hipStreamSynchronize(st));  // the temporaries are released on return
 }
 
 void build_schur_tasks_device(hipStream_t st, int C, int cb, const std::vector<int>& cam_off,
                               const int* d_cam_off, const int* d_cam_list, const int* d_spt,
                               const int* d_pt_off, const int* d_scam, std::vector<SchurTask>& tasks,
                               DevBuf<int2>& pairs, bool with_diag) {
-    // tasks in the host form's order: camera a, then column chunk
+    // tasks ordered by camera a, then column chunk
     const int dg = with_diag ? 1 : 0;
     std::vector<int4> tq;
-    std::vector<long long> qoff(1, 0);
-    for (int a = 0; a < C; ++a) {
-        const int na = cam_off[a + 1] - cam_off[a];
-        for (int lo = 0; lo < a + dg; lo += cb) {
-            tq.push_back(make_int4(a, lo, std::min(lo + cb, a + dg), 0));
-            qoff.push_back(qoff.back() + na);
-        }
-    }
+    for (int a = 0; a < C; ++a)
+        for (int lo = 0; lo < a + dg; lo += cb) tq.push_back(make_int4(a, lo, std::min(lo + cb, a + dg), 0));
+    std::vector<long long> hb;
+    schur_pairs_device(st, tq, cam_off, d_cam_off, d_cam_list, d_spt, d_pt_off, d_scam, hb, pairs);
     const int ntask = static_cast<int>(tq.size());
-    tasks.clear();
-    pairs.release();
-    if (ntask == 0) return;
-    const long long nslots = qoff.back();
-    if (nslots + 1 > INT_MAX) throw Error("too many observation slots for the pair scan");
-    DevBuf<int4> d_tq;
-    DevBuf<long long> d_qoff, cnt, pos, bounds;
-    DevBuf<unsigned char> work;
-    upload(d_tq, tq, st);
-    upload(d_qoff, qoff, st);
-    cnt.alloc(static_cast<size_t>(nslots) + 1);
-    pos.alloc(static_cast<size_t>(nslots) + 1);
-    bounds.alloc(static_cast<size_t>(ntask) + 1);
-    const size_t wb = pair_scan_work(nslots + 1);
-    work.alloc(std::max<size_t>(wb, 1));
-    launch_pair_count(st, ntask, d_tq.p, d_qoff.p, nslots, d_cam_off, d_cam_list, d_spt, d_pt_off, d_scam, cnt.p);
-    launch_pair_scan(st, cnt.p, pos.p, nslots + 1, work.p, wb);
-    launch_pair_bounds(st, ntask, d_qoff.p, pos.p, bounds.p);
-    std::vector<long long> hb(static_cast<size_t>(ntask) + 1);
-    GBA_HIP(hipMemcpyAsync(hb.data(), bounds.p, sizeof(long long) * hb.size(), hipMemcpyDeviceToHost, st));
-    GBA_HIP(hipGetLastError());
-    GBA_HIP(hipStreamSynchronize(st));
-    if (hb[ntask] > 0) {
-        pairs.alloc(static_cast<size_t>(hb[ntask]));
-        launch_pair_fill(st, ntask, d_tq.p, d_qoff.p, nslots, d_cam_off, d_cam_list, d_spt, d_pt_off, d_scam,
-                         pos.p, pairs.p);
-    }
-    GBA_HIP(hipGetLastError());
-    GBA_HIP(hipStreamSynchronize(st));  // the temporaries are released on return
     tasks.resize(static_cast<size_t>(ntask));
     for (int t = 0; t < ntask; ++t)
         tasks[t] = SchurTask{tq[t].x, tq[t].y, tq[t].z, 0, hb[t], hb[t + 1], 0, 0, 0};
@@ -223,53 +157,13 @@ int pcg_solve(hipStream_t st, const float* A, const float* X32, int ld, int npad
     return it;
 }
 
-void build_layout(const gba_problem_view& prob, HostLayout& L) {
-    const int C = prob.num_cameras, P = prob.num_points, O = prob.num_observations;
-    // sort key (point, camera, caller index): counting sort by camera, then a
-    // stable counting sort by point, so each point's observations are
-    // contiguous and ordered by camera
-    std::vector<int> by_cam(static_cast<size_t>(O));
-    {
-        std::vector<int> coff(static_cast<size_t>(C) + 1, 0);
-        for (int e = 0; e < O; ++e) coff[prob.obs_camera[e] + 1]++;
-        for (int c = 0; c < C; ++c) coff[c + 1] += coff[c];
-        for (int e = 0; e < O; ++e) by_cam[coff[prob.obs_camera[e]]++] = e;
-    }
-    L.pt_off.assign(static_cast<size_t>(P) + 1, 0);
-    for (int e = 0; e < O; ++e) L.pt_off[prob.obs_point[e] + 1]++;
-    for (int p = 0; p < P; ++p) L.pt_off[p + 1] += L.pt_off[p];
-    L.perm.resize(static_cast<size_t>(O));
-    L.scam.resize(static_cast<size_t>(O));
-    L.spt.resize(static_cast<size_t>(O));
-    L.spx.resize(static_cast<size_t>(O));
-    {
-        std::vector<int> fill(L.pt_off.begin(), L.pt_off.end() - 1);
-        for (int idx = 0; idx < O; ++idx) {
-            const int e = by_cam[idx];
-            const int k = fill[prob.obs_point[e]]++;
-            L.perm[k] = e;
-            L.scam[k] = prob.obs_camera[e];
-            L.spt[k] = prob.obs_point[e];
-            L.spx[k] = make_double2(prob.obs_pixel[2 * static_cast<size_t>(e)],
-                                    prob.obs_pixel[2 * static_cast<size_t>(e) + 1]);
-        }
-    }
-    layout_units(L.pt_off, L.units, L.pair_count);
-    L.cam_off.assign(static_cast<size_t>(C) + 1, 0);
-    L.cam_list.resize(static_cast<size_t>(O));
-    for (int k = 0; k < O; ++k) L.cam_off[L.scam[k] + 1]++;
-    for (int c = 0; c < C; ++c) L.cam_off[c + 1] += L.cam_off[c];
-    {
-        std::vector<int> fill(L.cam_off.begin(), L.cam_off.end() - 1);
-        for (int k = 0; k < O; ++k) L.cam_list[fill[L.scam[k]]++] = k;
-    }
-    layout_chunks(L.cam_off, L.chunks, L.cam_chunk_off);
-}
-
 void layout_units(const std::vector<int>& pt_off, std::vector<int2>& units, double& pair_count) {
-    const int P = static_cast<int>(pt_off.size()) - 1;
     units.clear();
     pair_count = 0.0;
+    layout_units_append(pt_off.data(), static_cast<int>(pt_off.size()) - 1, units, pair_count);
+}
+
+void layout_units_append(const int* pt_off, int P, std::vector<int2>& units, double& pair_count) {
     int b = -1, e_end = -1;
     for (int p = 0; p < P; ++p) {
         const int k0 = pt_off[p], k1 = pt_off[p + 1], k = k1 - k0;

@@ -35,23 +35,18 @@ struct DevBuf {
     ~DevBuf() { release(); }
 };
 
-// Host-side layout of one problem: observations sorted by (point, camera),
-// point-pass work units, camera index and camera-pass chunks.
-struct HostLayout {
-    std::vector<int> perm, scam, spt, pt_off, cam_off, cam_list, cam_chunk_off;
-    std::vector<double2> spx;
-    std::vector<int2> units;
-    std::vector<int4> chunks;
-    double pair_count = 0.0;  // sum_p k_p (k_p + 1) / 2
-};
-void build_layout(const gba_problem_view& prob, HostLayout& L);
 // Work units over the point CSR (points of up to a wavefront of observations
 // grouped, larger points alone) and sum_p k_p (k_p + 1) / 2; camera-pass
-// chunks over the camera CSR.  Shared by both layout forms.
+// chunks over the camera CSR.
 void layout_units(const std::vector<int>& pt_off, std::vector<int2>& units, double& pair_count);
+// The same over points [0, P) of a CSR slice, appending to units and adding
+// to pair_count (a batch builds each problem's units from its slice).
+void layout_units_append(const int* pt_off, int P, std::vector<int2>& units, double& pair_count);
 void layout_chunks(const std::vector<int>& cam_off, std::vector<int4>& chunks, std::vector<int>& cam_chunk_off);
-// Device form of build_layout (k_layout.hip): the same perm, scam, spt, spx
-// (when spx is set), pt_off, cam_off and cam_list, from the caller's arrays
+// The point-major layout (k_layout.hip): perm (sorted position -> caller's
+// index, ordered by (point, camera, index)), scam, spt, spx (when spx is
+// set), the point CSR pt_off, and the camera index cam_off / cam_list (sorted
+// positions by camera, ascending within a camera), from the caller's arrays
 // already on the device.  Allocates the outputs; synchronises st.
 void build_layout_device(hipStream_t st, int C, int P, int O, const int* d_cam, const int* d_pt,
                          const double* d_px, DevBuf<int>& perm, DevBuf<int>& scam, DevBuf<int>& spt,
@@ -62,19 +57,21 @@ uint16_t double_to_half_rne(double x);
 // FP16 pixel encoding with a per-camera FP32 scale, in the caller's
 // observation order (gba_encode_obs_fp16): cam_scale[C], px16[2 O].
 void encode_obs_fp16(const gba_problem_view& prob, float* cam_scale, uint16_t* px16);
-// Schur task and pair lists for chunks of cb camera columns (task ld, soff
-// and a0 are left 0 for the caller to fill); with_diag = false leaves out the
-// diagonal blocks and their pairs e == f (they then come from
-// launch_schur_diag, which also forms the rhs).
-void build_schur_tasks(int C, int cb, const std::vector<int>& cam_off,
-                       const std::vector<int>& cam_list, const std::vector<int>& pt_off,
-                       const std::vector<int>& scam, const std::vector<int>& spt,
-                       std::vector<SchurTask>& tasks, std::vector<int2>& pairs, bool with_diag);
 
-// Device form of build_schur_tasks: the same tasks and the same pairs in the
-// same order, built by launch_pair_count / scan / fill from the sorted layout
-// already on the device (d_*); the pairs land in pairs (allocated here), the
-// tasks come back sorted by decreasing pair count.  Synchronises st.
+// The pair lists of the tasks tq[t] = (a, lo, hi): pairs (e, f) with
+// camera(e) = a, camera(f) in [lo, hi), grouped by e (ascending point) and
+// ordered by f, built on the device from the sorted layout (d_*; cam_off on
+// the host too); task t's pairs are [bounds[t], bounds[t + 1]) of pairs
+// (allocated here).  Synchronises st.
+void schur_pairs_device(hipStream_t st, const std::vector<int4>& tq, const std::vector<int>& cam_off,
+                        const int* d_cam_off, const int* d_cam_list, const int* d_spt, const int* d_pt_off,
+                        const int* d_scam, std::vector<long long>& bounds, DevBuf<int2>& pairs);
+// Schur task and pair lists of one problem for chunks of cb camera columns:
+// task (a, c) holds the pairs (e, f) with camera(e) = a and camera(f) in
+// [c cb, min((c + 1) cb, a + with_diag)); every (a, c) gets a task, empty or
+// not, because the task also writes its blocks of S.  Pairs from
+// schur_pairs_device; tasks returned by decreasing pair count (longest
+// first).  Synchronises st.
 void build_schur_tasks_device(hipStream_t st, int C, int cb, const std::vector<int>& cam_off,
                               const int* d_cam_off, const int* d_cam_list, const int* d_spt,
                               const int* d_pt_off, const int* d_scam, std::vector<SchurTask>& tasks,
