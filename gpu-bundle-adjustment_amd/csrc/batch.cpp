@@ -103,7 +103,7 @@ BatchSolver::BatchSolver(const gba_problem_view* probs, int count, bool autotune
         upload(d_p0, p0s, st_);
         launch_offset_obs(st_, O, count, d_ooff.p, d_c0.p, d_p0.p, ocam.p, opt.p);
         build_layout_device(st_, C, P, O, ocam.p, opt.p, opx.p, perm, scam_, spt_, &spx_, pt_off_, cam_off_,
-                            cam_list_);
+                            cam_list_, &pb_.repeated_obs);
         pt_off.resize(static_cast<size_t>(P) + 1);
         cam_off.resize(static_cast<size_t>(C) + 1);
         GBA_HIP(hipMemcpyAsync(pt_off.data(), pt_off_.p, sizeof(int) * pt_off.size(), hipMemcpyDeviceToHost, st_));

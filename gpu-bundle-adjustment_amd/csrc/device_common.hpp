@@ -55,6 +55,10 @@ struct DevProblem {
     const float* cam_scale;
     // Huber loss threshold on the residual norm (<= 0: plain least squares).
     double huber;
+    // 1 when some point is observed more than once by one camera (set up
+    // from the sorted layout; the diagonal Schur blocks then add the cross
+    // terms of those observations)
+    int repeated_obs;
 };
 
 // Huber loss on the squared residual norm s = |r|^2, as the reference's

@@ -71,6 +71,13 @@ __global__ void k_offset_obs(int O, int count, const long long* __restrict__ o_o
     pt[e] += p0[lo];
 }
 
+// Sets *flag when two adjacent sorted keys are equal (a point observed twice
+// by one camera).
+__global__ void k_repeated_keys(int O, const unsigned long long* __restrict__ key, int* __restrict__ flag) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= 1 && k < O && key[k] == key[k - 1]) atomicOr(flag, 1);
+}
+
 int bits_for(int n) {
     int b = 1;
     while ((1LL << b) < n) ++b;
@@ -81,11 +88,12 @@ int bits_for(int n) {
 
 void build_layout_device(hipStream_t st, int C, int P, int O, const int* d_cam, const int* d_pt,
                          const double* d_px, DevBuf<int>& perm, DevBuf<int>& scam, DevBuf<int>& spt,
-                         DevBuf<double2>* spx, DevBuf<int>& pt_off, DevBuf<int>& cam_off, DevBuf<int>& cam_list) {
+                         DevBuf<double2>* spx, DevBuf<int>& pt_off, DevBuf<int>& cam_off, DevBuf<int>& cam_list,
+                         int* repeated) {
     const int cbits = bits_for(C), pbits = bits_for(P);
     if (cbits + pbits > 62) throw Error("layout: camera and point indices too wide for the sort key");
     DevBuf<unsigned long long> keys, keys_sorted;
-    DevBuf<int> idx, iota, cam_keys_sorted;
+    DevBuf<int> idx, iota, cam_keys_sorted, flag;
     keys.alloc(static_cast<size_t>(O));
     keys_sorted.alloc(static_cast<size_t>(O));
     idx.alloc(static_cast<size_t>(O));
@@ -117,12 +125,19 @@ void build_layout_device(hipStream_t st, int C, int P, int O, const int* d_cam, 
     bytes = wb;
     GBA_HIP(hipcub::DeviceRadixSort::SortPairs(work.p, bytes, scam.p, cam_keys_sorted.p, iota.p, cam_list.p, O, 0,
                                                cbits, st));
+    flag.alloc(1);
+    launch_fill_int(st, flag.p, 1, 0);
+    hipLaunchKernelGGL(k_repeated_keys, dim3(blocks), dim3(256), 0, st, O,
+                       static_cast<const unsigned long long*>(keys_sorted.p), flag.p);
     hipLaunchKernelGGL(k_csr_offsets, dim3(blocks1), dim3(256), 0, st, O, P, static_cast<const int*>(spt.p),
                        pt_off.p);
     hipLaunchKernelGGL(k_csr_offsets, dim3(blocks1), dim3(256), 0, st, O, C,
                        static_cast<const int*>(cam_keys_sorted.p), cam_off.p);
+    int h_flag = 0;
+    GBA_HIP(hipMemcpyAsync(&h_flag, flag.p, sizeof(int), hipMemcpyDeviceToHost, st));
     GBA_HIP(hipGetLastError());
     GBA_HIP(hipStreamSynchronize(st));  // the temporaries are released on return
+    if (repeated) *repeated = h_flag;
 }
 
 void launch_offset_obs(hipStream_t st, int O, int count, const long long* o_off, const int* c0, const int* p0,

@@ -252,6 +252,7 @@ __global__ __launch_bounds__(256) void k_schur_diag(
         // camera(f) <= camera(e)).  Rare, so it stays out of the registers
         // of the common path: rows of W_e and W_f read from memory, sums
         // into s_dup with LDS atomics.
+        if (!pb.repeated_obs) continue;
         const int k0 = pb.pt_off[p], k1 = pb.pt_off[p + 1];
         if ((e > k0 && pb.scam[e - 1] == a) || (e + 1 < k1 && pb.scam[e + 1] == a)) {
             int f = e;

@@ -244,7 +244,7 @@ Solver::Solver(const gba_problem_view& prob, const gba_solver_config& cfg) : cfg
             GBA_HIP(hipMemcpyAsync(opx.p, prob.obs_pixel, sizeof(double) * 2 * O, hipMemcpyHostToDevice, st_));
         }
         build_layout_device(st_, C, P, O, ocam.p, opt.p, opx.p, perm_, scam_, spt_, fp16 ? nullptr : &spx_,
-                            pt_off_, cam_off_, cam_list_);
+                            pt_off_, cam_off_, cam_list_, &pb_.repeated_obs);
     }
     std::vector<int> pt_off(static_cast<size_t>(P) + 1), cam_off(static_cast<size_t>(C) + 1);
     GBA_HIP(hipMemcpyAsync(pt_off.data(), pt_off_.p, sizeof(int) * pt_off.size(), hipMemcpyDeviceToHost, st_));

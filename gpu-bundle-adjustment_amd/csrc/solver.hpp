@@ -47,10 +47,12 @@ void layout_chunks(const std::vector<int>& cam_off, std::vector<int4>& chunks, s
 // index, ordered by (point, camera, index)), scam, spt, spx (when spx is
 // set), the point CSR pt_off, and the camera index cam_off / cam_list (sorted
 // positions by camera, ascending within a camera), from the caller's arrays
-// already on the device.  Allocates the outputs; synchronises st.
+// already on the device; *repeated = 1 when some point is observed more than
+// once by one camera.  Allocates the outputs; synchronises st.
 void build_layout_device(hipStream_t st, int C, int P, int O, const int* d_cam, const int* d_pt,
                          const double* d_px, DevBuf<int>& perm, DevBuf<int>& scam, DevBuf<int>& spt,
-                         DevBuf<double2>* spx, DevBuf<int>& pt_off, DevBuf<int>& cam_off, DevBuf<int>& cam_list);
+                         DevBuf<double2>* spx, DevBuf<int>& pt_off, DevBuf<int>& cam_off, DevBuf<int>& cam_list,
+                         int* repeated);
 
 // IEEE binary16 bits of x, rounded to nearest even.
 uint16_t double_to_half_rne(double x);
