@@ -210,6 +210,7 @@ def main():
                     help="initial LM damping (default 1e-4; 1e-3 in --mode bal, as bench_bal)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-iterations", type=int, default=2)
+    ap.add_argument("--cpu-samples", type=int, default=5, help="interleaved CPU baseline samples")
     ap.add_argument("--kernel-reps", type=int, default=10)
     ap.add_argument("--file", help="BAL problem for --mode bal")
     ap.add_argument("--iterations", type=int, default=50, help="--mode bal: max LM iterations")
@@ -277,10 +278,8 @@ def main():
         s32.solve(opts)
     torch.cuda.synchronize()
 
-    # FP64 reference solves (also the target cost)
-    r64 = []
-    for _ in range(max(1, min(args.steps, 3))):
-        r64.append(s64.solve(opts))
+    # FP64 reference solves (also the target cost): at least five samples
+    r64 = [s64.solve(opts) for _ in range(max(5, args.steps))]
     target = r64[0].final_cost * (1.0 + REL_TOL)
     t64 = [time_to_cost(r, target) for r in r64]
 
@@ -319,30 +318,40 @@ def main():
     }
     dominant = max(share, key=share.get)
     w_bytes = 108  # FP32 W in the mixed form
+    # edge accumulate = point pass + camera pass (one linearisation)
     edge_bytes = O * (16 + 8 + w_bytes) + P * (24 + 72) + 72 * C
-    edge_gbs = edge_bytes / (km["edge_point"] * 1e-3) / 1e9
-    factor_tflops = (n ** 3 / 3.0) / (km["factor"] * 1e-3) / 1e12
-    # Roofline of the Schur assembly (k_schur_diag + k_schur, the kernels with
-    # the largest single-kernel share of the mixed solve at this size; the
-    # factor family spreads over ~100 launches per trial): algorithmic bytes
-    # per assembly O * w_W + 72 P + n^2/2 * w_S, divided by its live
-    # HIP-event duration; traffic = HBM bytes per assembly from the committed
-    # rocprofv3 FETCH_SIZE / WRITE_SIZE passes (scaled by the copy-kernel
-    # calibration), if present.
-    schur_bytes = O * w_bytes + 72 * P + (n * n // 2) * 4
-    schur_gbs = schur_bytes / (km["schur"] * 1e-3) / 1e9
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "k_schur_C356_traffic.json")
-    if os.path.exists(tpath) and args.config == 3:
+    edge_ms = km["edge_point"] + km["edge_camera"]
+    edge_gbs = edge_bytes / (edge_ms * 1e-3) / 1e9
+    factor_flops = n ** 3 / 3.0
+    factor_tflops = factor_flops / (km["factor"] * 1e-3) / 1e12
+
+    def traffic_of(name):
+        """HBM bytes per launch from the committed rocprofv3 FETCH_SIZE /
+        WRITE_SIZE passes, scaled by the copy-kernel calibration."""
+        tpath = os.path.join(ROOT, "profiles", name)
+        if not (os.path.exists(tpath) and args.config == 3):
+            return None
         with open(tpath) as fh:
             t = json.load(fh)
-        traffic = round(sum(k["fetch_kib"] * t["fetch_scale"] + k["write_kib"] * t["write_scale"]
-                            for k in t["kernels"].values()) * 1024)
-    roof = {"kernel": "Schur assembly (k_schur_diag + k_schur), mixed form", "bound": "hbm",
-            "achieved": round(schur_gbs, 1), "peak": PEAK["hbm_gbs"], "unit": "GB/s",
-            "frac": round(schur_gbs / PEAK["hbm_gbs"], 4), "traffic": traffic,
-            "basis": f"O*108 + 72P + n^2/2*4 = {schur_bytes} B per launch over {km['schur']:.4f} ms",
+        return round(sum(k["fetch_kib"] * t["fetch_scale"] + k["write_kib"] * t["write_scale"]
+                         for k in t["kernels"].values()) * 1024)
+
+    # Roofline of the dominant kernel: the dense factor of S (FP32 MFMA in the
+    # mixed form), n^3/3 flops per factorisation over the live HIP-event time
+    # of the whole launch family of one factorisation (which in the mixed form
+    # also forms the explicit inverse L^-1; its flops are not counted).
+    roof = {"kernel": "dense Cholesky of S (k_potrf_tile + k_trsm_tiles + k_chol_update family), mixed form",
+            "bound": "mfma", "achieved": round(factor_tflops, 3), "peak": PEAK["fp32_tflops"],
+            "unit": "TFLOP/s", "frac": round(factor_tflops / PEAK["fp32_tflops"], 4),
+            "traffic": traffic_of("factor_C356_traffic.json"),
+            "basis": f"n^3/3 = {factor_flops:.4e} flop (n = {n}) per factorisation over {km['factor']:.4f} ms",
             "largest_share": dominant}
+    schur_bytes = O * w_bytes + 72 * P + (n * n // 2) * 4
+    schur_gbs = schur_bytes / (km["schur"] * 1e-3) / 1e9
+    roof_schur = {"kernel": "Schur assembly (k_schur_diag + k_schur), mixed form", "bound": "hbm",
+                  "achieved": round(schur_gbs, 1), "peak": PEAK["hbm_gbs"], "unit": "GB/s",
+                  "frac": round(schur_gbs / PEAK["hbm_gbs"], 4), "traffic": traffic_of("k_schur_C356_traffic.json"),
+                  "basis": f"O*108 + 72P + n^2/2*4 = {schur_bytes} B per assembly over {km['schur']:.4f} ms"}
 
     out = {
         "metric": "time-to-final-cost (rel err<=1e-6 vs FP64), C=356; mixed vs FP64 GPU",
@@ -355,50 +364,83 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 factor + f64 refinement/cost (mixed); f64 reference leg",
+        "dtype": "f32 W/S/factor + f64 PCG on the f32-stored S, f64 cost (mixed); f64 reference leg",
         "data": "synthetic (seeded Snavely generator, seed %d)" % args.seed,
         "config": {"workload": f"LM solve C={C} P={P} O={O} (preset {args.config}), "
                                f"lambda0={args.lambda0}, rel_decrease_tol=1e-6",
                    "cameras": C, "points": P, "observations": O},
         "fp64": {"time_to_final_cost_s": round(statistics.median(t64), 6),
+                 "samples": len(t64), "best_s": round(min(t64), 6), "worst_s": round(max(t64), 6),
                  "solve_s": round(statistics.median([r.solve_time_s for r in r64]), 6),
                  "final_cost": r64[0].final_cost, "iterations": r64[0].iterations,
                  "trials": r64[0].trials, "termination": r64[0].termination},
         "mixed": {"time_to_final_cost_s": round(value, 6),
+                  "samples": len(t32), "best_s": round(min(t32), 6), "worst_s": round(max(t32), 6),
                   "solve_s": round(statistics.median([r.solve_time_s for r in r32]), 6),
                   "final_cost": r32[0].final_cost, "iterations": r32[0].iterations,
                   "trials": r32[0].trials, "refine_iterations": r32[0].refine_iterations,
-                  "max_refine_residual": r32[0].max_refine_residual,
+                  "max_refine_residual_f32_system": r32[0].max_refine_residual,
                   "termination": r32[0].termination,
                   "all_s": [round(t, 6) for t in t32]},
         "including_setup": once,
         "speedup_mixed_vs_fp64": round(statistics.median(t64) / value, 3) if value == value else None,
         "kernels_ms": {k: {kk: round(vv, 4) for kk, vv in v.items()} for k, v in kern.items()},
-        "edge_kernel": {"bytes": edge_bytes, "gbs": round(edge_gbs, 1),
-                        "frac": round(edge_gbs / PEAK["hbm_gbs"], 4)},
+        "edge_kernel": {"passes": "point + camera", "bytes": edge_bytes, "ms": round(edge_ms, 4),
+                        "gbs": round(edge_gbs, 1), "frac": round(edge_gbs / PEAK["hbm_gbs"], 4)},
         "kernel_share_ms": {k: round(v, 3) for k, v in share.items()},
-        "factor": {"flops": n ** 3 / 3.0, "tflops_fp32": round(factor_tflops, 3),
-                   "tflops_fp64": round((n ** 3 / 3.0) / (kern["fp64"]["factor"] * 1e-3) / 1e12, 3)},
+        "factor": {"flops": factor_flops, "tflops_fp32": round(factor_tflops, 3),
+                   "tflops_fp64": round(factor_flops / (kern["fp64"]["factor"] * 1e-3) / 1e12, 3)},
         "schur_pairs": st["pair_count"],
         "roofline": roof,
+        "roofline_schur": roof_schur,
     }
 
     if not args.no_cpu:
+        # CPU baseline: the oracle (the reference's LM restated, with its host
+        # threading: min(C, hardware_concurrency) threads, src/lm.cu:382-383;
+        # Phase-2 workers by the 16 MiB rule; single-threaded LLT) on a bounded
+        # sample of the same problem -- --cpu-iterations LM iterations --
+        # timed --cpu-samples times, interleaved with plain FP64 GPU solves as
+        # the reference interleaves its columns (docs/RUNGS.md:136-142).
         sys.path.insert(0, ROOT)
         import oracle
-        cores = min(16, os.cpu_count() or 1)
-        p_cpu = g.generate(args.config, args.seed)
-        t = time.perf_counter()
-        rc = oracle.lm_solve(p_cpu, initial_lambda=args.lambda0, max_iterations=args.cpu_iterations,
-                             nthreads=cores)
-        tc = time.perf_counter() - t
-        per_it = tc / max(1, rc["iterations"])
+        hw = os.cpu_count() or 1
+        threads = min(C, hw)
+        try:
+            affinity = len(os.sched_getaffinity(0))
+        except (AttributeError, OSError):
+            affinity = hw
+        model = "unknown"
+        try:
+            with open("/proc/cpuinfo") as fh:
+                for line in fh:
+                    if line.startswith("model name"):
+                        model = line.split(":", 1)[1].strip()
+                        break
+        except OSError:
+            pass
+        cpu_s, gpu_s, its = [], [], 0
+        for _ in range(args.cpu_samples):
+            p_cpu = g.generate(args.config, args.seed)
+            t = time.perf_counter()
+            rc = oracle.lm_solve(p_cpu, initial_lambda=args.lambda0, max_iterations=args.cpu_iterations,
+                                 nthreads=threads)
+            cpu_s.append(time.perf_counter() - t)
+            its = rc["iterations"]
+            gpu_s.append(time_to_cost(s64.solve(opts), target))
+            print(f"[cpu] sample {len(cpu_s)}: {cpu_s[-1]:.2f} s for {its} iterations", file=sys.stderr, flush=True)
+        per_it = [t / max(1, its) for t in cpu_s]
+        scale = r64[0].iterations
         out["cpu_baseline"] = {
-            "value": round(per_it * r64[0].iterations, 3), "unit": "s", "cores": cores, "kind": "port",
-            "sample": f"oracle LM (restated reference: Phase-2 S workers by the 16 MiB rule -> 1, "
-                      f"single-thread blocked LLT) for {rc['iterations']} iterations "
-                      f"({tc:.2f} s, {rc['trials']} trials) at C={C}; value = s/iteration x "
-                      f"{r64[0].iterations} FP64 GPU iterations"}
+            "value": round(statistics.median(per_it) * scale, 3), "unit": "s", "cores": threads, "kind": "port",
+            "sample": f"oracle LM (restated reference: min(C, hardware_concurrency) = {threads} threads, "
+                      f"Phase-2 S workers by the 16 MiB rule, single-threaded blocked LLT) for {its} "
+                      f"iterations at C={C}, {len(cpu_s)} samples interleaved with FP64 GPU solves; "
+                      f"value = median s/iteration x {scale} FP64 GPU iterations",
+            "hardware_concurrency": hw, "affinity_cpus": affinity, "cpu_model": model,
+            "samples_s": [round(t, 3) for t in cpu_s],
+            "best_s": round(min(per_it) * scale, 3), "worst_s": round(max(per_it) * scale, 3),
+            "interleaved_gpu_fp64_time_to_final_cost_s": [round(t, 6) for t in gpu_s]}
     print(json.dumps(out))
 
 

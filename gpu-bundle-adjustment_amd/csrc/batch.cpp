@@ -236,6 +236,12 @@ BatchSolver::BatchSolver(const gba_problem_view* probs, int count, bool autotune
     GBA_HIP(hipHostMalloc(&h_status_, 2 * sizeof(PassStatus) * count, hipHostMallocDefault));
     GBA_HIP(hipHostMalloc(&h_lam_, sizeof(double) * count, hipHostMallocDefault));
     GBA_HIP(hipHostMalloc(&h_mask_, sizeof(int) * count, hipHostMallocDefault));
+    // a point without observations is in no work unit, so the point pass
+    // never writes its V and bp: they stay 0 (its damped V is then
+    // lambda 1e-12 I, SPD for lambda > 0, and its step is 0, as in the
+    // reference, src/lm.cu:289-301)
+    launch_fill_double(st_, VB_.p, VB_.n, 0.0);
+    launch_fill_double(st_, Vinv_.p, Vinv_.n, 0.0);
     // S: zero, then the identity on each problem's padding
     launch_fill_double(st_, S_.p, static_cast<size_t>(soff), 0.0);
     for (const BatchDesc& d : hdesc_)
@@ -360,8 +366,13 @@ void BatchSolver::solve(const gba_lm_options& opts, gba_lm_result* res) {
     const double t0 = now_s();
     GBA_HIP(hipMemcpyAsync(cams_.p, init_cams_.data(), sizeof(double) * init_cams_.size(),
                            hipMemcpyHostToDevice, st_));
-    GBA_HIP(hipMemcpyAsync(pts_.p, init_pts_.data(), sizeof(double) * init_pts_.size(),
-                           hipMemcpyHostToDevice, st_));
+    // the trial points too: the back-substitution writes only observed points,
+    // and an accept copies every point of its problem, so a point without
+    // observations must already hold its value there (the reference leaves it
+    // unchanged: dp = 0, src/lm.cu:289-301,645-646)
+    for (double* dst : {pts_.p, tpts_.p})
+        GBA_HIP(hipMemcpyAsync(dst, init_pts_.data(), sizeof(double) * init_pts_.size(), hipMemcpyHostToDevice,
+                               st_));
     const int B = B_;
     std::vector<double> sse(B, 0.0), lambda(B, opts.initial_lambda), nu(B, 2.0);
     std::vector<unsigned long long> counter(B, 0);
@@ -399,6 +410,12 @@ void BatchSolver::solve(const gba_lm_options& opts, gba_lm_result* res) {
             }
         }
         for (int b = 0; b < B; ++b) {
+            // no inner trial allowed: the reference's inner loop runs zero
+            // times and the solve fails (src/lm.cu:462,704-707)
+            if (!done[b] && opts.max_inner_trials <= 0) {
+                failed[b] = 1;
+                done[b] = 1;
+            }
             trial_mask[b] = done[b] ? 0 : 1;
             ntrial += trial_mask[b];
         }

@@ -296,14 +296,23 @@ __device__ __forceinline__ bool spd3_inverse(const double* a, double* inv6) {
     return true;
 }
 
-// Bounded wait for a device-scope flag written by another workgroup.
-// Returns false when the limit is hit (the caller records an error and exits).
-__device__ __forceinline__ bool wait_flag(const int* flag, int want) {
-    for (long spins = 0; spins < (1L << 20); ++spins) {
+// Wall-clock limit of one wait: 10^7 ticks of the constant-rate counter
+// (wall_clock64), 100 ms at the 100 MHz of MI300-class GPUs.
+constexpr long long kWaitTicks = 10000000LL;
+
+// Bounded wait for a device-scope flag written by another workgroup.  Gives
+// up (returns false) when *err becomes non-zero -- another waiter already
+// gave up, so the producer chain is broken -- or after kWaitTicks of wall
+// clock; the caller then raises *err, so every later wait on the same chain
+// returns at once instead of running out its own limit.
+__device__ __forceinline__ bool wait_flag(const int* flag, int want, const int* err) {
+    const long long t0 = wall_clock64();
+    for (;;) {
         if (__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) >= want) return true;
+        if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return false;
+        if (wall_clock64() - t0 > kWaitTicks) return false;
         __builtin_amdgcn_s_sleep(2);
     }
-    return false;
 }
 
 }  // namespace gba_amd

@@ -2,13 +2,20 @@
 //
 // snavely_residual_host follows the reader's model (reference
 // src/bal_reader.cpp:100-128: zero-order branch for theta <= 1e-12);
-// total_squared_error sums in observation order (:130-139).  The BAL reader
-// has the reference's contract (:12-98): whole-file read, .bz2 streamed
-// through bzcat, strict number parsing, positive header, index range checks,
-// no trailing garbage; any violation is an error.
-#include <charconv>
+// total_squared_error sums in observation order (:130-139).
+//
+// BAL text format (Agarwal et al., "Bundle Adjustment in the Large"): three
+// counts "C P O", then O records "camera point x y", then 9 C camera values,
+// then 3 P point coordinates, all separated by white space.  The loader keeps
+// the reference's contract (bal_problem.hpp:32-36): .bz2 input is read
+// through bzcat, and any malformed number, a non-positive count, an index out
+// of range, a missing value or extra data after the points is an error.
+#include <cctype>
+#include <cerrno>
+#include <cstdint>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -22,86 +29,133 @@ namespace gba_amd {
 namespace {
 thread_local std::string g_last_error;
 
-bool ends_with_bz2(const std::string& p) {
-    return p.size() >= 4 && p.compare(p.size() - 4, 4, ".bz2") == 0;
-}
-
-std::string read_file(const std::string& path) {
-    const bool bz = ends_with_bz2(path);
-    FILE* fp = nullptr;
-    if (bz) {
-        std::string cmd = "bzcat -- \"" + path + "\"";
-        fp = popen(cmd.c_str(), "r");
-    } else {
-        fp = std::fopen(path.c_str(), "rb");
-    }
-    if (!fp) throw Error("cannot open " + path);
-    std::string buf;
-    std::vector<char> chunk(1 << 20);
-    size_t n;
-    while ((n = std::fread(chunk.data(), 1, chunk.size(), fp)) > 0) buf.append(chunk.data(), n);
-    const int rc = bz ? pclose(fp) : std::fclose(fp);
-    if (rc != 0) throw Error("read failed for " + path);
-    return buf;
-}
-
-class Tokens {
+// Pulls white-space separated tokens out of a BAL stream through a fixed
+// buffer, so a file is never held in memory as text.
+class BalStream {
 public:
-    explicit Tokens(const std::string& s) : p_(s.data()), end_(s.data() + s.size()) {}
-    template <typename T>
-    T next() {
-        skip();
-        T v{};
-        auto r = std::from_chars(p_, end_, v);
-        if (r.ec != std::errc()) throw Error("malformed BAL number");
-        p_ = r.ptr;
+    explicit BalStream(const std::string& path) : path_(path) {
+        piped_ = path.size() > 4 && path.compare(path.size() - 4, 4, ".bz2") == 0;
+        if (piped_) {
+            std::string quoted = "'";
+            for (char ch : path) {
+                if (ch == '\'') quoted += "'\\''";
+                else quoted += ch;
+            }
+            quoted += "'";
+            fp_ = popen(("bzcat -- " + quoted).c_str(), "r");
+        } else {
+            fp_ = std::fopen(path.c_str(), "rb");
+        }
+        if (!fp_) throw Error("BAL: cannot open '" + path + "'");
+    }
+    ~BalStream() {
+        if (fp_) (void)(piped_ ? pclose(fp_) : std::fclose(fp_));
+    }
+    BalStream(const BalStream&) = delete;
+    BalStream& operator=(const BalStream&) = delete;
+
+    // Next token into tok_; false at the end of the stream.
+    bool token() {
+        tok_.clear();
+        int ch;
+        do {
+            ch = get();
+        } while (ch != EOF && std::isspace(ch));
+        while (ch != EOF && !std::isspace(ch)) {
+            tok_.push_back(static_cast<char>(ch));
+            ch = get();
+        }
+        if (!tok_.empty()) ++count_;
+        return !tok_.empty();
+    }
+    double real(const char* what) {
+        need(what);
+        if (!numeric_text(tok_)) bad(what);
+        char* end = nullptr;
+        const double v = std::strtod(tok_.c_str(), &end);
+        if (end != tok_.c_str() + tok_.size()) bad(what);
         return v;
     }
-    bool done() {
-        skip();
-        return p_ == end_;
+    int integer(const char* what) {
+        need(what);
+        bool digits = !tok_.empty();
+        for (size_t i = (tok_[0] == '-') ? 1 : 0; i < tok_.size(); ++i) digits = digits && std::isdigit(
+            static_cast<unsigned char>(tok_[i]));
+        if (!digits || tok_ == "-") bad(what);
+        errno = 0;
+        char* end = nullptr;
+        const long v = std::strtol(tok_.c_str(), &end, 10);
+        if (errno != 0 || v < INT32_MIN || v > INT32_MAX) bad(what);
+        return static_cast<int>(v);
     }
+    // Ends the read: extra tokens are an error, and so is a failed bzcat.
+    void finish() {
+        if (token()) throw Error("BAL: '" + path_ + "' has data after the last point");
+        const int rc = piped_ ? pclose(fp_) : std::fclose(fp_);
+        fp_ = nullptr;
+        if (rc != 0) throw Error("BAL: reading '" + path_ + "' failed");
+    }
+    const std::string& path() const { return path_; }
 
 private:
-    void skip() {
-        while (p_ != end_ && (*p_ == ' ' || *p_ == '\n' || *p_ == '\t' || *p_ == '\r')) ++p_;
+    int get() {
+        if (pos_ == len_) {
+            len_ = std::fread(buf_, 1, sizeof buf_, fp_);
+            pos_ = 0;
+            if (len_ == 0) return EOF;
+        }
+        return static_cast<unsigned char>(buf_[pos_++]);
     }
-    const char* p_;
-    const char* end_;
+    void need(const char* what) {
+        if (!token()) throw Error("BAL: '" + path_ + "' ends before its " + what);
+    }
+    [[noreturn]] void bad(const char* what) {
+        throw Error("BAL: '" + path_ + "': token " + std::to_string(count_) + " ('" + tok_ +
+                    "') is not a valid " + what);
+    }
+    // Decimal notation only ([+-] digits [. digits] [e [+-] digits]) or a
+    // signed nan / inf / infinity; hexadecimal floats are refused.
+    static bool numeric_text(const std::string& t) {
+        size_t i = (t[0] == '+' || t[0] == '-') ? 1 : 0;
+        std::string rest;
+        for (size_t k = i; k < t.size(); ++k) rest.push_back(static_cast<char>(std::tolower(
+            static_cast<unsigned char>(t[k]))));
+        if (rest == "nan" || rest == "inf" || rest == "infinity") return true;
+        bool mant = false;
+        while (i < t.size() && std::isdigit(static_cast<unsigned char>(t[i]))) i++, mant = true;
+        if (i < t.size() && t[i] == '.') {
+            ++i;
+            while (i < t.size() && std::isdigit(static_cast<unsigned char>(t[i]))) i++, mant = true;
+        }
+        if (!mant) return false;
+        if (i < t.size() && (t[i] == 'e' || t[i] == 'E')) {
+            ++i;
+            if (i < t.size() && (t[i] == '+' || t[i] == '-')) ++i;
+            bool ex = false;
+            while (i < t.size() && std::isdigit(static_cast<unsigned char>(t[i]))) i++, ex = true;
+            if (!ex) return false;
+        }
+        return i == t.size();
+    }
+
+    std::string path_;
+    FILE* fp_ = nullptr;
+    bool piped_ = false;
+    char buf_[1 << 16];
+    size_t pos_ = 0, len_ = 0;
+    std::string tok_;
+    long long count_ = 0;
 };
 
-struct Parsed {
-    int C = 0, P = 0, O = 0;
-    std::vector<double> cams, pts, px;
-    std::vector<int> oc, op;
+struct BalCounts {
+    int C, P, O;
 };
 
-Parsed parse_bal(const std::string& path, bool header_only) {
-    const std::string text = read_file(path);
-    Tokens t(text);
-    Parsed r;
-    r.C = t.next<int>();
-    r.P = t.next<int>();
-    r.O = t.next<int>();
-    if (r.C <= 0 || r.P <= 0 || r.O <= 0) throw Error("bad BAL header in " + path);
-    if (header_only) return r;
-    r.oc.resize(r.O);
-    r.op.resize(r.O);
-    r.px.resize(2 * static_cast<size_t>(r.O));
-    for (int i = 0; i < r.O; ++i) {
-        r.oc[i] = t.next<int>();
-        r.op[i] = t.next<int>();
-        r.px[2 * static_cast<size_t>(i)] = t.next<double>();
-        r.px[2 * static_cast<size_t>(i) + 1] = t.next<double>();
-        if (r.oc[i] < 0 || r.oc[i] >= r.C || r.op[i] < 0 || r.op[i] >= r.P)
-            throw Error("observation index out of range in " + path);
-    }
-    r.cams.resize(9 * static_cast<size_t>(r.C));
-    for (auto& v : r.cams) v = t.next<double>();
-    r.pts.resize(3 * static_cast<size_t>(r.P));
-    for (auto& v : r.pts) v = t.next<double>();
-    if (!t.done()) throw Error("trailing garbage in " + path);
-    return r;
+BalCounts read_counts(BalStream& in) {
+    BalCounts n{in.integer("camera count"), in.integer("point count"), in.integer("observation count")};
+    if (n.C <= 0 || n.P <= 0 || n.O <= 0)
+        throw Error("BAL: '" + in.path() + "' must start with three positive counts");
+    return n;
 }
 
 }  // namespace
@@ -135,22 +189,41 @@ void snavely_residual_host(const double* cam, const double* X, const double* obs
 }
 
 int bal_header(const char* path, int* C, int* P, int* O) {
-    Parsed r = parse_bal(path, true);
-    *C = r.C;
-    *P = r.P;
-    *O = r.O;
-    return 0;
+    BalStream in(path);
+    const BalCounts n = read_counts(in);
+    *C = n.C;
+    *P = n.P;
+    *O = n.O;
+    return 0;  // the rest of the stream is not read (a bzcat pipe just closes)
 }
 
 int load_bal_into(const char* path, gba_problem_view* out) {
-    Parsed r = parse_bal(path, false);
-    if (out->num_cameras != r.C || out->num_points != r.P || out->num_observations != r.O)
+    BalStream in(path);
+    const BalCounts n = read_counts(in);
+    if (out->num_cameras != n.C || out->num_points != n.P || out->num_observations != n.O)
         return set_error("gba_load_bal: view sizes do not match the file header");
-    std::memcpy(out->cameras, r.cams.data(), sizeof(double) * r.cams.size());
-    std::memcpy(out->points, r.pts.data(), sizeof(double) * r.pts.size());
-    std::memcpy(out->obs_camera, r.oc.data(), sizeof(int) * r.oc.size());
-    std::memcpy(out->obs_point, r.op.data(), sizeof(int) * r.op.size());
-    std::memcpy(out->obs_pixel, r.px.data(), sizeof(double) * r.px.size());
+    // parsed into local arrays first: the caller's view changes only when the
+    // whole file is valid
+    const size_t O = static_cast<size_t>(n.O);
+    std::vector<int> oc(O), op(O);
+    std::vector<double> px(2 * O), cams(9 * static_cast<size_t>(n.C)), pts(3 * static_cast<size_t>(n.P));
+    for (size_t e = 0; e < O; ++e) {
+        oc[e] = in.integer("camera index");
+        op[e] = in.integer("point index");
+        px[2 * e] = in.real("pixel coordinate");
+        px[2 * e + 1] = in.real("pixel coordinate");
+        if (oc[e] < 0 || oc[e] >= n.C || op[e] < 0 || op[e] >= n.P)
+            throw Error("BAL: '" + in.path() + "': observation " + std::to_string(e) +
+                        " names a camera or point that does not exist");
+    }
+    for (double& v : cams) v = in.real("camera parameter");
+    for (double& v : pts) v = in.real("point coordinate");
+    in.finish();
+    std::memcpy(out->cameras, cams.data(), sizeof(double) * cams.size());
+    std::memcpy(out->points, pts.data(), sizeof(double) * pts.size());
+    std::memcpy(out->obs_camera, oc.data(), sizeof(int) * O);
+    std::memcpy(out->obs_point, op.data(), sizeof(int) * O);
+    std::memcpy(out->obs_pixel, px.data(), sizeof(double) * px.size());
     return 0;
 }
 

@@ -914,8 +914,14 @@ __global__ __launch_bounds__(256) void k_trsv_fwd(const T* __restrict__ L, int l
         T lv[16];
 #pragma unroll
         for (int r = 0; r < 16; ++r) lv[r] = Lr[static_cast<size_t>(r) * ld];  // independent of y
-        if (tid == 0 && !wait_flag(done + j, 1)) atomicOr(&flags->solve_err, 1);
+        if (tid == 0 && !wait_flag(done + j, 1, &flags->solve_err)) {
+            atomicOr(&flags->solve_err, 1);
+            s_skip = 1;
+        }
         __syncthreads();
+        // after a failed wait the row is abandoned: no further waits, no
+        // publication (the waiters on it see solve_err and return)
+        if (s_skip) return;
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         const double yj = y[j * kTile + lane];
 #pragma unroll
@@ -969,8 +975,12 @@ __global__ __launch_bounds__(256) void k_trsv_bwd(const T* __restrict__ L, int l
         T lv[16];
 #pragma unroll
         for (int r = 0; r < 16; ++r) lv[r] = Lc[static_cast<size_t>(r) * ld];
-        if (tid == 0 && !wait_flag(done + j, 1)) atomicOr(&flags->solve_err, 1);
+        if (tid == 0 && !wait_flag(done + j, 1, &flags->solve_err)) {
+            atomicOr(&flags->solve_err, 1);
+            s_skip = 1;
+        }
         __syncthreads();
+        if (s_skip) return;
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         const double* xj = x + j * kTile + 16 * w;
 #pragma unroll

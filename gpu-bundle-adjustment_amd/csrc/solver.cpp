@@ -126,7 +126,8 @@ int pcg_solve(hipStream_t st, const float* A, const float* X32, int ld, int npad
     launch_vec_copy(st, npad, z, p);
     launch_pcg_dot(st, npad, r, z, b, b, blk, 0, sc, status, flags);
     sync_status();
-    *rel = 1.0;
+    // a trial that failed before the solve runs no refinement: no residual
+    *rel = 0.0;
     const TrialFlags f = h_status->flags;
     if (f.nonspd || f.chol_fail || f.solve_err) return 0;
     const double bb = h_status->dot[1];
@@ -364,8 +365,13 @@ void Solver::reset_params(const double* cams, const double* pts) {
     cur_ = 0;
     GBA_HIP(hipMemcpyAsync(cams_[0].p, cams ? cams : init_cams_.data(), sizeof(double) * cams_[0].n,
                            hipMemcpyHostToDevice, st_));
-    GBA_HIP(hipMemcpyAsync(pts_[0].p, pts ? pts : init_pts_.data(), sizeof(double) * pts_[0].n,
-                           hipMemcpyHostToDevice, st_));
+    // both point buffers: the back-substitution writes trial coordinates only
+    // for observed points, so a point without observations keeps its value in
+    // either buffer across accepted steps (the reference's dp = 0 for it,
+    // src/lm.cu:289-301,645-646)
+    for (int i = 0; i < 2; ++i)
+        GBA_HIP(hipMemcpyAsync(pts_[i].p, pts ? pts : init_pts_.data(), sizeof(double) * pts_[i].n,
+                               hipMemcpyHostToDevice, st_));
 }
 
 void Solver::sync_status() {
@@ -644,7 +650,8 @@ void Solver::solve(const gba_lm_options& opts, gba_lm_result* res, double t_entr
             TrialOut t = trial(lambda);
             res->trials++;
             res->refine_iterations += t.refine_iters;
-            res->max_refine_residual = std::max(res->max_refine_residual, t.refine_rel);
+            if (!t.flags.nonspd && !t.flags.chol_fail && !t.flags.solve_err)
+                res->max_refine_residual = std::max(res->max_refine_residual, t.refine_rel);
             int code;
             double rho = NAN, tcost = NAN;
             if (t.flags.nonspd) {

@@ -8,21 +8,23 @@
 
 namespace gba {
 
+// Field names, types and defaults are the interface of the reference
+// (include/gba/lm.hpp:13-29), so callers compile unchanged.
 struct LMOptions {
-    int max_iterations = 50;
-    double initial_lambda = 1e-3;   // Marquardt diagonal damping
-    double rel_decrease_tol = 1e-6; // convergence: relative cost decrease
-    int max_inner_trials = 16;      // lambda escalations per outer iteration
-    bool verbose = false;           // per-trial log to stderr
+    int max_iterations = 50;        // cap on accepted steps
+    double initial_lambda = 1e-3;   // starting damping, scaled by each diagonal entry
+    double rel_decrease_tol = 1e-6; // stop once an accepted step lowers the cost by less than this fraction
+    int max_inner_trials = 16;      // damped solves tried per step before the solve gives up
+    bool verbose = false;           // one line per trial on stderr
 };
 
 struct LMResult {
-    double initial_cost = 0.0;
-    double final_cost = 0.0;
-    int iterations = 0;             // accepted outer iterations
-    std::string termination;        // "CONVERGENCE" | "MAX_ITERATIONS" | "FAILURE"
-    unsigned long long anomalies = 0;  // cumulative non-finite edge count
-    double total_time_s = 0.0;      // wall time of the solve, allocation and upload included
+    double initial_cost = 0.0;      // half the sum of squared residuals at the start
+    double final_cost = 0.0;        // the same at the last accepted parameters
+    int iterations = 0;             // number of accepted steps
+    std::string termination;        // "CONVERGENCE", "MAX_ITERATIONS" or "FAILURE"
+    unsigned long long anomalies = 0;  // observations skipped as non-finite, summed over every pass
+    double total_time_s = 0.0;      // seconds in solve_bal_gpu, device set-up and copies included
 };
 
 // The whole LM solve runs device-resident on the current HIP device (plain

@@ -24,6 +24,9 @@ _ullp = ctypes.POINTER(ctypes.c_ulonglong)
 TRIAL_CODES = {0: "accept", 1: "reject", 2: "nonspd_v", 3: "chol_fail", 4: "denom", 5: "nonfinite"}
 STEP_CODES = {0: "ok", 1: "nonspd_v", 2: "llt_fail", 3: "denom"}
 TERMINATION = {0: "CONVERGENCE", 1: "MAX_ITERATIONS", 2: "FAILURE"}
+# trial status codes of the LM trace (gba_oracle.h ORC_TR_*)
+TRIAL_CODES = {0: "accept", 1: "reject", 2: "nonspd_v", 3: "llt_fail", 4: "denom", 5: "nonfinite"}
+TRACE_STRIDE = 7
 
 
 class Options(ctypes.Structure):
@@ -54,39 +57,48 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB):
             build()
-        L = ctypes.CDLL(LIB)
-        L.orc_snavely_residual.argtypes = [_dp, _dp, _dp, _dp]
-        L.orc_edge_residual.argtypes = [_dp, _dp, _dp, _dp]
-        L.orc_edge_residual_jacobian.argtypes = [_dp, _dp, _dp, _dp, _dp, _dp]
-        L.orc_total_squared_error.restype = ctypes.c_double
-        L.orc_total_squared_error.argtypes = [ctypes.c_int, _dp, _dp, _ip, _ip, _dp]
-        L.orc_linearize.argtypes = [ctypes.c_int] * 3 + [_dp, _dp, _ip, _ip, _dp] + [_dp] * 6 + [_ullp, ctypes.c_int]
-        L.orc_cost.restype = ctypes.c_double
-        L.orc_cost.argtypes = [ctypes.c_int, _dp, _dp, _ip, _ip, _dp, _ullp, ctypes.c_int]
-        L.orc_linearize_loss.argtypes = ([ctypes.c_int] * 3 + [_dp, _dp, _ip, _ip, _dp, ctypes.c_double] +
-                                         [_dp] * 6 + [_ullp, ctypes.c_int])
-        L.orc_cost_loss.restype = ctypes.c_double
-        L.orc_cost_loss.argtypes = [ctypes.c_int, _dp, _dp, _ip, _ip, _dp, ctypes.c_double, _ullp, ctypes.c_int]
-        L.orc_chol9_factor.restype = ctypes.c_int
-        L.orc_chol9_factor.argtypes = [_dp, _dp]
-        L.orc_chol9_solve.argtypes = [_dp, _dp, _dp]
-        L.orc_inverse3_spd.restype = ctypes.c_int
-        L.orc_inverse3_spd.argtypes = [_dp, _dp]
-        L.orc_batch_chol9_factor_solve.argtypes = [_dp, _dp, _dp, _ip, ctypes.c_int]
-        L.orc_batch_inverse3_spd.argtypes = [_dp, _dp, _ip, ctypes.c_int]
-        L.orc_damp_invert_v.restype = ctypes.c_long
-        L.orc_damp_invert_v.argtypes = [ctypes.c_int, _dp, ctypes.c_double, _dp]
-        L.orc_dense_cholesky.restype = ctypes.c_int
-        L.orc_dense_cholesky.argtypes = [ctypes.c_int, _dp]
-        L.orc_dense_cholesky_solve.argtypes = [ctypes.c_int, _dp, _dp]
-        L.orc_schur_step.restype = ctypes.c_int
-        L.orc_schur_step.argtypes = [ctypes.c_int] * 3 + [_ip, _ip] + [_dp] * 5 + [ctypes.c_double] + [_dp] * 5 + [ctypes.c_int]
-        L.orc_lm_solve.restype = ctypes.c_int
-        L.orc_lm_solve.argtypes = [ctypes.c_int] * 3 + [_dp, _dp, _ip, _ip, _dp, ctypes.POINTER(Options),
-                                                         ctypes.POINTER(Result), _dp, ctypes.c_int, _ip,
-                                                         ctypes.c_int]
-        _lib = L
+        _lib = _declare(ctypes.CDLL(LIB))
     return _lib
+
+
+def variant(path):
+    """A separately built copy of the oracle (e.g. one compiled with
+    -DORC_MUTATE=k by the schedule tests), declared like lib()."""
+    return _declare(ctypes.CDLL(os.path.abspath(path)))
+
+
+def _declare(L):
+    L.orc_snavely_residual.argtypes = [_dp, _dp, _dp, _dp]
+    L.orc_edge_residual.argtypes = [_dp, _dp, _dp, _dp]
+    L.orc_edge_residual_jacobian.argtypes = [_dp, _dp, _dp, _dp, _dp, _dp]
+    L.orc_total_squared_error.restype = ctypes.c_double
+    L.orc_total_squared_error.argtypes = [ctypes.c_int, _dp, _dp, _ip, _ip, _dp]
+    L.orc_linearize.argtypes = [ctypes.c_int] * 3 + [_dp, _dp, _ip, _ip, _dp] + [_dp] * 6 + [_ullp, ctypes.c_int]
+    L.orc_cost.restype = ctypes.c_double
+    L.orc_cost.argtypes = [ctypes.c_int, _dp, _dp, _ip, _ip, _dp, _ullp, ctypes.c_int]
+    L.orc_linearize_loss.argtypes = ([ctypes.c_int] * 3 + [_dp, _dp, _ip, _ip, _dp, ctypes.c_double] +
+                                     [_dp] * 6 + [_ullp, ctypes.c_int])
+    L.orc_cost_loss.restype = ctypes.c_double
+    L.orc_cost_loss.argtypes = [ctypes.c_int, _dp, _dp, _ip, _ip, _dp, ctypes.c_double, _ullp, ctypes.c_int]
+    L.orc_chol9_factor.restype = ctypes.c_int
+    L.orc_chol9_factor.argtypes = [_dp, _dp]
+    L.orc_chol9_solve.argtypes = [_dp, _dp, _dp]
+    L.orc_inverse3_spd.restype = ctypes.c_int
+    L.orc_inverse3_spd.argtypes = [_dp, _dp]
+    L.orc_batch_chol9_factor_solve.argtypes = [_dp, _dp, _dp, _ip, ctypes.c_int]
+    L.orc_batch_inverse3_spd.argtypes = [_dp, _dp, _ip, ctypes.c_int]
+    L.orc_damp_invert_v.restype = ctypes.c_long
+    L.orc_damp_invert_v.argtypes = [ctypes.c_int, _dp, ctypes.c_double, _dp]
+    L.orc_dense_cholesky.restype = ctypes.c_int
+    L.orc_dense_cholesky.argtypes = [ctypes.c_int, _dp]
+    L.orc_dense_cholesky_solve.argtypes = [ctypes.c_int, _dp, _dp]
+    L.orc_schur_step.restype = ctypes.c_int
+    L.orc_schur_step.argtypes = [ctypes.c_int] * 3 + [_ip, _ip] + [_dp] * 5 + [ctypes.c_double] + [_dp] * 5 + [ctypes.c_int]
+    L.orc_lm_solve.restype = ctypes.c_int
+    L.orc_lm_solve.argtypes = [ctypes.c_int] * 3 + [_dp, _dp, _ip, _ip, _dp, ctypes.POINTER(Options),
+                                                     ctypes.POINTER(Result), _dp, ctypes.c_int, _ip,
+                                                     ctypes.c_int]
+    return L
 
 
 def ref():
@@ -222,15 +234,18 @@ def schur_step(prob, lin, lam, nthreads=8, want_S=False):
 
 
 def lm_solve(prob, max_iterations=50, initial_lambda=1e-3, rel_decrease_tol=1e-6,
-             max_inner_trials=16, verbose=False, nthreads=8, max_trace=4096, huber_delta=0.0):
-    """Runs the restated reference LM; updates prob.cameras / prob.points in place."""
+             max_inner_trials=16, verbose=False, nthreads=8, max_trace=4096, huber_delta=0.0, library=None):
+    """Runs the restated reference LM; updates prob.cameras / prob.points in
+    place.  out["trace"] has one row per trial: outer iteration, lambda used,
+    trial cost, rho, status code (TRIAL_CODES), nu, gain denominator.
+    library: a variant() to run instead of the oracle proper."""
     cams, pts, oc, op, px = _arrs(prob)
     opts = Options(max_iterations, initial_lambda, rel_decrease_tol, max_inner_trials, int(verbose),
                    huber_delta)
     res = Result()
-    trace = np.zeros((max_trace, 5))
+    trace = np.zeros((max_trace, TRACE_STRIDE))
     ntr = ctypes.c_int()
-    lib().orc_lm_solve(cams.shape[0], pts.shape[0], len(oc), _p(cams), _p(pts), _p(oc, ctypes.c_int),
+    (library or lib()).orc_lm_solve(cams.shape[0], pts.shape[0], len(oc), _p(cams), _p(pts), _p(oc, ctypes.c_int),
                        _p(op, ctypes.c_int), _p(px), ctypes.byref(opts), ctypes.byref(res), _p(trace),
                        max_trace, ctypes.byref(ntr), nthreads)
     prob.cameras[...] = cams.reshape(prob.cameras.shape)

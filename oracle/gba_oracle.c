@@ -870,6 +870,20 @@ int orc_schur_step(int C, int P, int O, const int* oc, const int* op, const doub
 /* full LM solve (src/lm.cu:371-724)                                    */
 /* ------------------------------------------------------------------ */
 
+/* ORC_MUTATE = k (k > 0) builds a copy of the oracle with one LM control rule
+ * deliberately broken; tests/test_oracle_lm_schedule.py compiles each copy
+ * and checks that its schedule checks reject it.  The product build and the
+ * oracle proper use ORC_MUTATE = 0.
+ *   1 nu not reset per outer iteration   2 nu never doubles
+ *   3 accept clamp 1/2 instead of 1/3     4 accept factor from rho - 1
+ *   5 textbook gain ratio                 6-9 no escalation after a non-SPD V,
+ *   LLT failure, bad denominator, non-finite trial   10 no escalation after a
+ *   plain rejection   11 relative decrease over sse, not max(sse, 1)
+ *   12 nu doubled before it scales lambda 13 one extra inner trial */
+#ifndef ORC_MUTATE
+#define ORC_MUTATE 0
+#endif
+
 int orc_lm_solve(int C, int P, int O, double* cams, double* pts, const int* oc, const int* op,
                  const double* px, const orc_options* opts, orc_result* res, double* trace,
                  int max_trace, int* n_trace, int nthreads) {
@@ -910,10 +924,15 @@ int orc_lm_solve(int C, int P, int O, double* cams, double* pts, const int* oc, 
     double lambda = opts->initial_lambda;
     int converged = 0, failed = 0;
     double rel_decrease = 0.0;
+#if ORC_MUTATE == 1
+    double nu = 2.0; /* planted: nu not reset per outer iteration */
+#endif
     while (res->iterations < opts->max_iterations && !converged && !failed) {
-        double nu = 2.0;
+#if ORC_MUTATE != 1
+        double nu = 2.0; /* lm.cu:461 */
+#endif
         int accepted = 0;
-        for (int trial = 0; trial < opts->max_inner_trials; ++trial) {
+        for (int trial = 0; trial < opts->max_inner_trials + (ORC_MUTATE == 13); ++trial) {
             unsigned long long before0 = counter0;
             double denom = 0.0, lam_used = lambda;
             res->trials++;
@@ -938,14 +957,30 @@ int orc_lm_solve(int C, int P, int O, double* cams, double* pts, const int* oc, 
                 if (counter0 != before0 || !isfinite(tsse)) {
                     code = ORC_TR_NONFINITE;
                 } else {
+#if ORC_MUTATE == 5
+                    rho = (sse - tsse) / denom; /* planted: textbook ratio */
+#else
                     rho = (sse - tsse) / (0.5 * denom); /* lm.cu:672 */
+#endif
                     if (rho > 0.0 && tsse < sse) {      /* lm.cu:678-686 */
                         memcpy(cams, tc, sizeof(double) * 9 * sC);
                         memcpy(pts, tp, sizeof(double) * 3 * sP);
+#if ORC_MUTATE == 11
+                        rel_decrease = (sse - tsse) / sse; /* planted: no max(sse, 1) */
+#else
                         rel_decrease = (sse - tsse) / (sse > 1.0 ? sse : 1.0);
+#endif
                         sse = tsse;
-                        double fac = 1.0 - pow(2.0 * rho - 1.0, 3.0);
+#if ORC_MUTATE == 4
+                        double fac = 1.0 - pow(rho - 1.0, 3.0); /* planted: textbook rho in the update */
+#else
+                        double fac = 1.0 - pow(2.0 * rho - 1.0, 3.0); /* lm.cu:683 */
+#endif
+#if ORC_MUTATE == 3
+                        lambda *= (fac > 0.5 ? fac : 0.5); /* planted: wrong lower clamp */
+#else
                         lambda *= (fac > 1.0 / 3.0 ? fac : 1.0 / 3.0);
+#endif
                         accepted = 1;
                         code = ORC_TR_ACCEPT;
                     } else {
@@ -954,20 +989,39 @@ int orc_lm_solve(int C, int P, int O, double* cams, double* pts, const int* oc, 
                 }
             }
             if (trace && n_trace && *n_trace < max_trace) {
-                double* tr = trace + 5 * (size_t)(*n_trace);
+                double* tr = trace + ORC_TRACE_STRIDE * (size_t)(*n_trace);
                 tr[0] = res->iterations;
                 tr[1] = lam_used;
                 tr[2] = tcost;
                 tr[3] = rho;
                 tr[4] = code;
+                tr[5] = nu;
+                tr[6] = denom;
                 (*n_trace)++;
             }
             if (opts->verbose)
                 fprintf(stderr, "  iter %d trial: lambda %.3g cost %.9g rho %.3g code %d\n",
                         res->iterations, lambda, tcost, rho, code);
             if (accepted) break;
-            lambda *= nu; /* every rejection path escalates the same way */
+            /* every rejection path escalates the same way (lm.cu:478-479,
+             * 600-601, 640-641, 667-668, 687-688) */
+#if ORC_MUTATE >= 6 && ORC_MUTATE <= 10
+            {
+                /* planted: one failure path leaves lambda and nu alone */
+                static const int skip_code[5] = {ORC_TR_NONSPD_V, ORC_TR_LLT_FAIL, ORC_TR_DENOM,
+                                                 ORC_TR_NONFINITE, ORC_TR_REJECT};
+                if (code == skip_code[ORC_MUTATE - 6]) continue;
+            }
+#endif
+#if ORC_MUTATE == 12
+            nu *= 2.0; /* planted: nu doubled before it scales lambda */
+            lambda *= nu;
+#elif ORC_MUTATE == 2
+            lambda *= nu; /* planted: nu never doubles */
+#else
+            lambda *= nu;
             nu *= 2.0;
+#endif
         }
         if (!accepted) {
             failed = 1;

@@ -106,8 +106,10 @@ typedef struct {
     double t_linearize, t_dampinv, t_assembly, t_solve, t_trial_cost;
 } orc_result;
 
-/* Trial status codes written into the trace (5 doubles per trial:
- * iteration, lambda, trial_cost, rho, status). */
+/* Trial record of the trace (ORC_TRACE_STRIDE doubles per trial): outer
+ * iteration, lambda used, trial cost, rho, status (below), nu when the trial
+ * ran, gain-ratio denominator (0 when the trial stopped before it). */
+#define ORC_TRACE_STRIDE 7
 enum {
     ORC_TR_ACCEPT = 0,
     ORC_TR_REJECT = 1,

@@ -89,13 +89,34 @@ def test_bal_write_load_roundtrip_and_bz2(gba, tmp_path):
 
 
 @pytest.mark.parametrize("text,msg", [
-    ("0 1 1\n", "header"), ("1 1 1\n0 5 1 2\n", "range"), ("1 1 1\n0 0 1 x\n", "malformed"),
-    ("1 1 1\n0 0 1 2\n" + "0\n" * 12 + "7\n", "trailing")])
+    ("0 1 1\n", "positive counts"), ("1 1 1\n0 5 1 2\n", "does not exist"),
+    ("1 1 1\n0 0 1 x\n", "not a valid pixel"), ("1 1 1\n0 0 1 0x1p3\n", "not a valid pixel"),
+    ("1 1 1\n0 +0 1 2\n", "not a valid point index"),
+    ("1 1 1\n0 0 1 2\n" + "0\n" * 11, "ends before its point coordinate"),
+    ("1 1 1\n0 0 1 2\n" + "0\n" * 12 + "7\n", "after the last point")])
 def test_load_bal_rejects_malformed(gba, tmp_path, text, msg):
     path = tmp_path / "bad.txt"
     path.write_text(text)
     with pytest.raises(gba.GBAError, match=msg):
         gba.load_bal(path)
+
+
+def test_load_bal_bz2_and_number_forms(gba, tmp_path):
+    """A .bz2 file is read through bzcat; signed exponents, a leading '+',
+    nan and inf parse as numbers."""
+    import bz2
+    import shutil
+    text = "1 1 1\n0 0 +1.5e+2 -2.5E-1\n" + "0.5\n" * 8 + "nan\n" + "1\n-inf\n3.\n"
+    plain = tmp_path / "p.txt"
+    plain.write_text(text)
+    a = gba.load_bal(plain)
+    assert a.obs_pixel.tolist() == [[150.0, -0.25]]
+    assert np.isnan(a.cameras[0, 8]) and a.points.tolist()[0][1:] == [-np.inf, 3.0]
+    if shutil.which("bzcat"):
+        packed = tmp_path / "p.txt.bz2"
+        packed.write_bytes(bz2.compress(text.encode()))
+        b = gba.load_bal(packed)
+        assert np.array_equal(a.obs_pixel, b.obs_pixel) and np.array_equal(a.points, b.points)
 
 
 def test_generator_shapes_and_determinism(gba):

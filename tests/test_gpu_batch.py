@@ -44,6 +44,38 @@ def test_batch_matches_sequential_config5(gba):
         assert abs(r1.final_cost - r2.final_cost) <= 1e-9 * r1.final_cost
 
 
+def test_batch_512_preset5_against_sequential_and_oracle(gba, orc):
+    """The full batched workload: 512 problems of preset 5 (seeds 1..512, C
+    8..64, P 2,000..20,000, O = 4.5 P) in one handle.  Every problem against
+    its own sequential FP64 solve; six of them, the extremes of C and P and
+    two from the middle, against the oracle (the reference loop,
+    src/lm.cu:458-702)."""
+    probs = [gba.generate(5, seed) for seed in range(1, 513)]
+    b = gba.Batch(probs)
+    res = b.solve()
+    worst = 0.0
+    for p, rb in zip(probs, res):
+        s = gba.Solver(p)
+        rs = s.solve()
+        s.close()
+        assert rb.termination == rs.termination and rb.anomalies == rs.anomalies == 0
+        assert abs(rb.iterations - rs.iterations) <= 1
+        assert abs(rb.initial_cost - rs.initial_cost) <= 1e-12 * rs.initial_cost
+        d = abs(rb.final_cost - rs.final_cost) / rs.final_cost
+        worst = max(worst, d)
+        assert d <= 1e-9
+    C = np.array([p.num_cameras for p in probs])
+    P = np.array([p.num_points for p in probs])
+    pick = sorted({int(np.argmin(C)), int(np.argmax(C)), int(np.argmin(P)), int(np.argmax(P)), 100, 400})
+    for i in pick:
+        o = orc.lm_solve(probs[i].copy())
+        assert res[i].termination == o["termination"]
+        assert res[i].iterations == o["iterations"], (i, res[i].iterations, o["iterations"])
+        assert abs(res[i].final_cost - o["final_cost"]) <= 1e-9 * o["final_cost"]
+    print(f"512-problem batch: {b.rounds} rounds, worst final-cost gap to sequential {worst:.1e}")
+    b.close()
+
+
 @pytest.mark.parametrize("factor", ["tiles", "workgroup"])
 def test_batch_mixed_sizes_and_params(gba, monkeypatch, factor):
     monkeypatch.setenv("GBA_BATCH_FACTOR", factor)

@@ -1,9 +1,11 @@
 """The path at the benchmark's full sizes (C=356 / O=1.26M and C=1,936 /
-O=3.4M), checked through properties that do not need the oracle to finish a
-whole solve: the first LM iteration against the oracle (the restated
-reference, which needs a few seconds for one iteration at C=356), the mixed
-form's final cost against the plain FP64 solve (the 1e-6 relative target),
-its refinement residual (1e-10), and the FP16-pixel solve's final-cost error."""
+O=3.4M): the first LM iteration and the whole FP64 solve at C=356 against the
+oracle (the restated reference; about 30 s of host work for the whole solve
+on 16 threads), the mixed form's final cost against the plain FP64 solve (the
+1e-6 relative target), its refinement residual (1e-10), and the FP16-pixel
+solve's final-cost error at C=1,936."""
+import os
+
 import numpy as np
 import pytest
 
@@ -24,6 +26,27 @@ def test_cfg3_first_iteration_matches_oracle(gba, orc, cfg3):
     assert abs(r.initial_cost - o["initial_cost"]) <= 1e-12 * o["initial_cost"]
     # one accepted step: the device's Schur sums run in another order than
     # the oracle's, so the step agrees to rounding, amplified by S's condition
+    assert abs(r.final_cost - o["final_cost"]) <= 1e-9 * o["final_cost"]
+
+
+def test_cfg3_full_fp64_solve_matches_oracle(gba, orc, cfg3):
+    """The whole FP64 solve at C=356 / O=1.26M against the whole oracle solve
+    (the reference loop, src/lm.cu:458-702): same trials, statuses and lambda
+    sequence, final cost within 1e-9 -- the cross-check of the FP64 target
+    cost that the timed benchmark compares the mixed form against."""
+    opts = dict(initial_lambda=1e-4)
+    r = gba.solve_bal_gpu(cfg3.copy(), gba.LMOptions(**opts))
+    o = orc.lm_solve(cfg3.copy(), nthreads=min(16, os.cpu_count() or 1), **opts)
+    print(f"C=356 FP64: gpu {r.final_cost:.10f} ({r.iterations} it, {r.trials} trials), "
+          f"oracle {o['final_cost']:.10f} ({o['iterations']} it) in {o['total_time_s']:.1f} s")
+    assert r.termination == o["termination"] == "CONVERGENCE"
+    assert r.iterations == o["iterations"] and r.trials == len(o["trace"])
+    assert np.array_equal(r.trace[:, [0, 4]], o["trace"][:, [0, 4]])
+    assert np.allclose(r.trace[:, 1], o["trace"][:, 1], rtol=1e-9, atol=0)
+    assert abs(r.initial_cost - o["initial_cost"]) <= 1e-12 * o["initial_cost"]
+    ga = r.trace[r.trace[:, 4] == 0][:, 2]
+    oa = o["trace"][o["trace"][:, 4] == 0][:, 2]
+    assert np.max(np.abs(ga - oa) / oa) <= 1e-9
     assert abs(r.final_cost - o["final_cost"]) <= 1e-9 * o["final_cost"]
 
 

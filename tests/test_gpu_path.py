@@ -224,6 +224,78 @@ def test_lm_fp64_matches_oracle(gba, orc, cfg, lam):
     ga = r.trace[r.trace[:, 4] == 0][:, 2]
     oa = o["trace"][o["trace"][:, 4] == 0][:, 2]
     assert len(ga) == len(oa) and np.max(np.abs(ga - oa) / oa) < 1e-9
+    # every trial: same iteration, status and lambda as the oracle (whose
+    # schedule is pinned against the reference's rules on the CPU)
+    assert r.trace.shape[0] == o["trace"].shape[0] == r.trials
+    assert np.array_equal(r.trace[:, [0, 4]], o["trace"][:, [0, 4]])
+    assert np.allclose(r.trace[:, 1], o["trace"][:, 1], rtol=1e-9, atol=0)
+
+
+def test_lm_schedule_with_rejections_matches_oracle(gba, orc):
+    """Rejection runs inside iterations (lambda *= nu, nu *= 2) and accepts on
+    both branches of max(1/3, .): the GPU's trial-by-trial lambda equals the
+    oracle's."""
+    s = gba.synth_spec(1, 1)
+    s.num_cameras, s.num_points, s.num_observations, s.kmin, s.kmax = 10, 400, 1600, 2, 8
+    s.perturb = 0.2
+    prob = gba.generate(spec=s, seed=2)
+    opts = dict(initial_lambda=1e-6, max_iterations=30)
+    r = gba.solve_bal_gpu(prob.copy(), gba.LMOptions(**opts))
+    o = orc.lm_solve(prob.copy(), **opts)
+    assert np.sum(o["trace"][:, 4] != 0) >= 4  # the scenario does reject
+    assert r.termination == o["termination"] and r.iterations == o["iterations"]
+    assert np.array_equal(r.trace[:, [0, 4]], o["trace"][:, [0, 4]])
+    # 20% parameter noise: rho carries the rounding of a far-from-quadratic
+    # step, and accepts on the 1 - (2 rho - 1)^3 branch pass it into lambda
+    # (d factor / d rho = -6 (2 rho - 1)^2), so lambda agrees to 1e-7 here
+    # (1.4e-9 measured) while every decision is the same
+    assert np.allclose(r.trace[:, 1], o["trace"][:, 1], rtol=1e-7, atol=0)
+    assert abs(r.final_cost - o["final_cost"]) <= 1e-9 * o["final_cost"]
+
+
+def with_unobserved_point(gba):
+    p = gba.generate(2, 3)
+    p.points = np.vstack([p.points[:100], [[1.25, -2.5, 0.75]], p.points[100:]])
+    p.obs_point = np.where(p.obs_point >= 100, p.obs_point + 1, p.obs_point).astype(np.int32)
+    return p.normalize()
+
+
+def test_unobserved_point_keeps_its_value(gba, orc):
+    """A point no camera sees has V = 0, bp = 0, so dp = 0 at any lambda > 0
+    (src/lm.cu:289-301,645-646): it must come back unchanged, single and
+    batched, after any number of accepted steps."""
+    p = with_unobserved_point(gba)
+    opts = dict(initial_lambda=1e-4)
+    o = orc.lm_solve(p.copy(), **opts)
+    assert o["iterations"] >= 2
+    for prec in (gba.PRECISION_FP64, gba.PRECISION_MIXED):
+        q = p.copy()
+        r = gba.solve_bal_gpu(q, gba.LMOptions(**opts), prec)
+        assert r.termination == "CONVERGENCE" and r.iterations >= 2
+        assert np.array_equal(q.points[100], p.points[100])
+    q = p.copy()
+    r = gba.solve_bal_gpu(q, gba.LMOptions(**opts))
+    assert r.iterations == o["iterations"]
+    ref = p.copy()
+    orc.lm_solve(ref, **opts)
+    assert arel(q.points, ref.points) < 1e-6
+    b = gba.Batch([p, gba.generate(5, 2), p])
+    res = b.solve(gba.LMOptions(**opts))
+    for i in (0, 2):
+        assert res[i].iterations == o["iterations"]
+        cams_b, pts_b = b.params(i)
+        assert np.array_equal(pts_b[100], p.points[100])
+        assert arel(pts_b, ref.points) < 1e-6
+
+
+def test_batch_zero_inner_trials_is_failure(gba):
+    """max_inner_trials = 0: no trial runs, every problem fails at its initial
+    cost (the reference's inner loop runs zero times, src/lm.cu:462,704-707)."""
+    probs = [gba.generate(5, 4), gba.generate(5, 5)]
+    res = gba.Batch(probs).solve(gba.LMOptions(max_inner_trials=0))
+    for r in res:
+        assert r.termination == "FAILURE" and r.iterations == 0 and r.trials == 0
+        assert r.final_cost == r.initial_cost
 
 
 def test_lm_mixed_reaches_fp64_cost(gba):
