@@ -21,6 +21,7 @@ CONVERGENCE, MAX_ITERATIONS, FAILURE = 0, 1, 2
 TERMINATION = {CONVERGENCE: "CONVERGENCE", MAX_ITERATIONS: "MAX_ITERATIONS", FAILURE: "FAILURE"}
 PRECISION_FP64, PRECISION_MIXED = 0, 1
 OBS_FP64, OBS_FP16 = 0, 1  # device storage of the observed pixels
+NOALLOC_MAX_N = 1024  # GBA_NOALLOC_MAX_N of include/gba_c.h
 TRACE_STRIDE = 6
 TRIAL_STATUS = {0: "accept", 1: "reject", 2: "nonspd_v", 3: "chol_fail", 4: "denom", 5: "nonfinite"}
 KERNELS = {"edge_point": 0, "edge_camera": 1, "schur": 2, "factor": 3, "solve": 4, "backsub": 5,
@@ -96,6 +97,8 @@ _PROTOS = {
                                     ctypes.POINTER(ctypes.c_ulonglong)]),
     "gba_cost": (ctypes.c_int, [ctypes.POINTER(ProblemView), _dp, ctypes.POINTER(ctypes.c_ulonglong)]),
     "gba_dense_spd_solve": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _dp, _dp, _ip, _ip]),
+    "gba_dense_spd_solve_noalloc": (ctypes.c_int, [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                                   ctypes.c_void_p, ctypes.c_void_p]),
     "gba_batch_chol9_factor_solve": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                                     ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]),
     "gba_batch_inverse3_spd": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
@@ -437,6 +440,18 @@ def dense_spd_solve(A, b, precision=PRECISION_FP64):
     _check(lib().gba_dense_spd_solve(A.shape[0], precision, _ptr(A), _ptr(x), ctypes.byref(ok),
                                      ctypes.byref(it)))
     return x, bool(ok.value), it.value
+
+
+def dense_spd_solve_noalloc(A, b, ok):
+    """gba_dense_spd_solve_noalloc on torch device tensors: A (n, n) float64
+    read as column-major (pass A.T.contiguous() for a row-major matrix; for a
+    symmetric A either works), b (n,) float64 overwritten with the solution,
+    ok (1,) int32.  Runs on the library's default stream between a torch
+    synchronisation and a library synchronisation (see below)."""
+    import torch
+    torch.cuda.synchronize()
+    _check(lib().gba_dense_spd_solve_noalloc(A.shape[0], A.data_ptr(), b.data_ptr(), ok.data_ptr(), None))
+    _check(lib().gba_device_synchronize())
 
 
 # torch ships its own HIP runtime; device pointers are shared between the two

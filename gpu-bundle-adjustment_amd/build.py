@@ -26,6 +26,7 @@ SOURCES = [
     "k_batch.hip",
     "k_pairs.hip",
     "k_layout.hip",
+    "noalloc.hip",
     "solver.cpp",
     "batch.cpp",
     "obs_fp16.cpp",

@@ -386,6 +386,14 @@ int gba_dense_spd_solve(int n, int precision, const double* A, double* b, int* o
     });
 }
 
+int gba_dense_spd_solve_noalloc(int n, const double* A, double* b, int* ok, void* stream) {
+    return guarded([&] {
+        need(A && b && ok, "gba_dense_spd_solve_noalloc: null argument");
+        dense_spd_solve_noalloc(n, A, b, ok, static_cast<hipStream_t>(stream));
+        return 0;
+    });
+}
+
 int gba_batch_chol9_factor_solve(const double* A, const double* b, double* x, int* ok, int n,
                                  void* stream) {
     return guarded([&] {

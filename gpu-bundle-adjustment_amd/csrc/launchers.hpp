@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include "device_common.hpp"
+#include "gba_c.h"
 
 namespace gba_amd {
 
@@ -18,6 +19,11 @@ void launch_lin_reduce(hipStream_t st, const DevProblem& pb, const double* gram_
 void launch_cost(hipStream_t st, const DevProblem& pb, const double* cams, const double* pts,
                  double* blk_sse, unsigned* blk_anom, PassStatus* status);
 int cost_partials(const DevProblem& pb);
+
+// noalloc.hip: FP64 dense SPD factor + solve on caller device buffers, scratch
+// in the library's static device array (n <= kNoAllocMaxN); no allocation.
+constexpr int kNoAllocMaxN = GBA_NOALLOC_MAX_N;
+void dense_spd_solve_noalloc(int n, const double* A, double* b, int* ok, hipStream_t st);
 
 // k_blocks.hip
 void launch_clear_flags(hipStream_t st, TrialFlags* flags);

@@ -220,6 +220,21 @@ int gba_cost_cfg(const gba_problem_view* prob, const gba_solver_config* cfg, dou
 int gba_dense_spd_solve(int n, int precision, const double* A, double* b, int* ok,
                         int* refine_iters);
 
+/* Allocation-free variant of gba_dense_spd_solve (no counterpart in the
+ * reference): for callers that cannot allocate device memory -- code under
+ * HIP graph capture, tight serving loops.  Plain FP64 only.  A (n x n,
+ * column-major, lower triangle read), b (n; overwritten with the solution
+ * when the factor succeeds) and ok (one int: 1 solved, 0 not positive
+ * definite, -1 a triangular solve gave up waiting) are device pointers; the
+ * work is enqueued on stream (NULL = default) and the call returns without
+ * waiting.  The intermediate matrices live in one static device array of
+ * the library (about 9.5 MB per GPU, held until the process ends), so
+ * n <= GBA_NOALLOC_MAX_N (larger n is an error return), and calls that
+ * overlap on one GPU race on it: serialise them.  Calls no hipMalloc,
+ * hipFree or allocating library routine. */
+#define GBA_NOALLOC_MAX_N 1024
+int gba_dense_spd_solve_noalloc(int n, const double* A, double* b, int* ok, void* stream);
+
 /* ---- batched small blocks, device pointers (reference include/gba/schur.cuh) */
 
 /* Replaces the batch_chol9_factor_solve kernel (schur.cuh:104-109): factor
