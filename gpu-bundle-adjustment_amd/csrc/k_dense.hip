@@ -26,9 +26,14 @@
 // results are published with release/acquire flags at agent scope; the
 // diagonal block is applied as a product with the stored inverse tile.  Waits
 // are bounded (TrialFlags::solve_err on time-out), so every wave exits.
+#include <algorithm>
+#include <vector>
+
 #include "device_common.hpp"
 #include "launchers.hpp"
 #include "mfma_tile.hpp"
+#include "persist_tasks.hpp"
+#include "solver.hpp"
 
 namespace gba_amd {
 
@@ -281,9 +286,19 @@ __device__ __forceinline__ void acc_lds_lds(typename Acc4<T>::type (&acc)[2][2],
 
 // Role r (0 <= r < inv_roles(k, nblk)) of launch k, 256 threads; sa, sb are
 // two LDS tiles of the calling kernel.
-template <typename T>
+// kCoh: the one-launch factor's form -- R read and X / R written with
+// agent-scope atomic accesses (other workgroups of the same launch read them).
+template <typename T, bool kCoh = false>
 __device__ void inv_role(int r, int k, int nblk, const T* __restrict__ S, int ld, const T* __restrict__ DinvT,
                          T* __restrict__ X, T* __restrict__ R, T (*sa)[kTile + 2], T (*sb)[kTile + 2]) {
+    auto put = [](T* p, T v) {
+        if constexpr (kCoh) __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        else *p = v;
+    };
+    auto get = [](const T* p) -> T {
+        if constexpr (kCoh) return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        else return *p;
+    };
     typedef typename Acc4<T>::type acc_t;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
@@ -296,7 +311,7 @@ __device__ void inv_role(int r, int k, int nblk, const T* __restrict__ S, int ld
 #pragma unroll 4
         for (int q = threadIdx.x; q < kTile * kTile; q += 256) {
             const int rr = q >> 6, c = q & 63;
-            xt[static_cast<size_t>(rr) * ld + c] = it[(rr >= c) ? c * kTile + rr : rr * kTile + c];
+            put(xt + static_cast<size_t>(rr) * ld + c, it[(rr >= c) ? c * kTile + rr : rr * kTile + c]);
         }
         return;
     }
@@ -339,13 +354,20 @@ __device__ void inv_role(int r, int k, int nblk, const T* __restrict__ S, int ld
     acc_lds_lds<T>(acc, sa, sb);
     T* Rij = R + static_cast<size_t>(i * kTile) * ld + j * kTile;
     if (j < k - 1) {
+        T part[2][2][4];
 #pragma unroll
         for (int a = 0; a < 2; ++a)
 #pragma unroll
             for (int c = 0; c < 2; ++c)
 #pragma unroll
                 for (int q = 0; q < 4; ++q)
-                    acc[a][c][q] += Rij[static_cast<size_t>(wr + 16 * a + acc_row<T>(lk, q)) * ld + wc + 16 * c + li];
+                    part[a][c][q] = get(Rij + static_cast<size_t>(wr + 16 * a + acc_row<T>(lk, q)) * ld + wc + 16 * c + li);
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int c = 0; c < 2; ++c)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) acc[a][c][q] += part[a][c][q];
     }
     if (i > k) {
         // R role: R_ij = acc
@@ -355,7 +377,7 @@ __device__ void inv_role(int r, int k, int nblk, const T* __restrict__ S, int ld
             for (int c = 0; c < 2; ++c)
 #pragma unroll
                 for (int q = 0; q < 4; ++q)
-                    Rij[static_cast<size_t>(wr + 16 * a + acc_row<T>(lk, q)) * ld + wc + 16 * c + li] = acc[a][c][q];
+                    put(Rij + static_cast<size_t>(wr + 16 * a + acc_row<T>(lk, q)) * ld + wc + 16 * c + li, acc[a][c][q]);
         return;
     }
     // X row role: X_kj = -Linv_kk Z with Z = acc; Linv_kk (rows of DinvT[k]
@@ -397,8 +419,8 @@ __device__ void inv_role(int r, int k, int nblk, const T* __restrict__ S, int ld
     for (int u = 0; u < 16; ++u) {
         const int q = threadIdx.x + 256 * u;
         const int rr = q >> 6, c = q & 63;
-        low[static_cast<size_t>(rr) * ld + c] = sa[rr][c];
-        up[static_cast<size_t>(rr) * ld + c] = sa[c][rr];
+        put(low + static_cast<size_t>(rr) * ld + c, sa[rr][c]);
+        put(up + static_cast<size_t>(rr) * ld + c, sa[c][rr]);
     }
 }
 
@@ -1190,6 +1212,318 @@ __global__ void k_vec_scale(int n, int npad, const double* __restrict__ in,
     if (i < npad) out[i] = i < n ? in[i] * s[i] : 0.0;
 }
 
+// ---- one-launch tile Cholesky ----------------------------------------------
+// Workgroup 0 runs the critical chain (per tile column k: TRSM of tile
+// (k, k-1), the update of diagonal tile (k, k) by column k-1 from LDS, the
+// factor and inverse of tile (k, k)); every other workgroup takes tasks --
+// the remaining TRSMs, the trailing updates and, in the FP32 form, the rows
+// of X = L^-1 -- through one atomic ticket, in the order and with the
+// dependencies of persist_tasks.hpp.  Results are the launch-per-column
+// factor's: L_ik below the diagonal, L_ik^T in the upper tile (k, i), D / DinvT,
+// X and R.  A task's results are published with release stores after every
+// thread's agent-scope release fence and a barrier; a waiter polls with
+// acquire loads (thread 0), then every thread takes an agent-scope acquire
+// fence.  Waits end on the flag, on a failure flag (a bad pivot or a non-SPD
+// V, then the launch drains) or after kWaitTicks of wall clock (solve_err;
+// never expected: the tickets are ordered so every wait has a running owner).
+
+// Loads and stores of data that workgroups exchange inside the launch.  The
+// L2 of an XCD is not coherent with the other XCDs' L2s: agent-scope atomic
+// accesses (sc1) write through and read from the coherence point instead.
+// Tiles that several workgroups read-modify-write (the trailing tiles A_ij,
+// the partial sums R_ij) are read and written this way; tiles written once
+// per launch (L_ik, L_ik^T, the diagonal tiles and inverses, X_kj) are written
+// this way and read with ordinary loads -- no cache holds a copy of them
+// before they are written, since nothing reads them earlier in the launch and
+// the caches are invalidated at launch.  So no waiter needs an L2
+// invalidation, and a publication needs only its stores to have completed.
+template <typename T>
+__device__ __forceinline__ T ld_coh(const T* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ void st_coh(T* p, T v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ bool p_failed(const TrialFlags* f) {
+    return (__hip_atomic_load(&f->nonspd, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) |
+            __hip_atomic_load(&f->chol_fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) |
+            __hip_atomic_load(&f->solve_err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) != 0;
+}
+
+// Thread 0 only: waits for each flag in d; false when the launch must stop.
+__device__ bool p_wait_all(const PFlag* d, int nd, int* const* fl, TrialFlags* flags) {
+    for (int q = 0; q < nd; ++q) {
+        const int* p = fl[d[q].arr] + d[q].idx;
+        const long long t0 = wall_clock64();
+        for (;;) {
+            if (__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= d[q].want) break;
+            if (p_failed(flags)) return false;
+            if (wall_clock64() - t0 > kWaitTicks) {
+                atomicOr(&flags->solve_err, 1);
+                return false;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+    }
+    return true;
+}
+
+// Every thread: the workgroup's wait on d (thread 0 polls, the barrier
+// orders every later load of the workgroup after the flag it saw); false =
+// stop.
+__device__ __forceinline__ bool p_wg_wait(const PFlag* d, int nd, int* const* fl, TrialFlags* flags, int* s_ok) {
+    if (threadIdx.x == 0) *s_ok = p_wait_all(d, nd, fl, flags) ? 1 : 0;
+    __syncthreads();
+    return *s_ok != 0;
+}
+
+// Every thread: publish f once every wave's stores have completed (s_waitcnt
+// vmcnt(0), then the barrier; thread 0's release store adds the L2 write-back
+// for any ordinary store).
+__device__ __forceinline__ void p_wg_publish(const PFlag& f, int* const* fl) {
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0) on gfx9-family targets
+    __syncthreads();
+    if (threadIdx.x == 0) __hip_atomic_store(fl[f.arr] + f.idx, f.want, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// 256 threads copy a 64x64 tile into LDS with coherent loads.
+template <typename T>
+__device__ __forceinline__ void tile_to_lds_coh(T (*s)[kTile + 2], const T* __restrict__ src, int ld) {
+    T tmp[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+        const int q = threadIdx.x + 256 * u;
+        tmp[u] = ld_coh(src + static_cast<size_t>(q >> 6) * ld + (q & 63));
+    }
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+        const int q = threadIdx.x + 256 * u;
+        s[q >> 6][q & 63] = tmp[u];
+    }
+}
+
+// L_ik = A_ik (Linv_kk)^T into Ai (LDS), L_ik over the lower tile and L_ik^T
+// into the upper tile (k, i).
+template <typename T>
+__device__ void p_trsm(T* __restrict__ S, int ld, int i, int k, const T* __restrict__ DinvT, T (*Ai)[kTile + 2]) {
+    T* Aik = S + static_cast<size_t>(i * kTile) * ld + k * kTile;
+    T* LTki = S + static_cast<size_t>(k * kTile) * ld + i * kTile;
+    const T* inv = DinvT + static_cast<size_t>(k) * kTile * kTile;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
+    const int li = lane & 15, lk = lane >> 4;
+    T bv[16][2];
+    {
+        T tmp[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int q = threadIdx.x + 256 * u;
+            tmp[u] = ld_coh(Aik + static_cast<size_t>(q >> 6) * ld + (q & 63));
+        }
+#pragma unroll
+        for (int kk = 0; kk < kTile; kk += 4) {
+            bv[kk >> 2][0] = inv[(kk + lk) * kTile + wc + li];
+            bv[kk >> 2][1] = inv[(kk + lk) * kTile + wc + 16 + li];
+        }
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int q = threadIdx.x + 256 * u;
+            Ai[q >> 6][q & 63] = tmp[u];
+        }
+    }
+    __syncthreads();
+    typename Acc4<T>::type acc[2][2];
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int c = 0; c < 2; ++c) acc[r][c] = typename Acc4<T>::type{0, 0, 0, 0};
+#pragma unroll
+    for (int kk = 0; kk < kTile; kk += 4) {
+        const T a0 = Ai[wr + li][kk + lk], a1 = Ai[wr + 16 + li][kk + lk];
+        const T b0 = bv[kk >> 2][0], b1 = bv[kk >> 2][1];
+        acc[0][0] = mfma16(a0, b0, acc[0][0]);
+        acc[0][1] = mfma16(a0, b1, acc[0][1]);
+        acc[1][0] = mfma16(a1, b0, acc[1][0]);
+        acc[1][1] = mfma16(a1, b1, acc[1][1]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) Ai[wr + 16 * r + acc_row<T>(lk, q)][wc + 16 * c + li] = acc[r][c][q];
+    __syncthreads();
+#pragma unroll 4
+    for (int u = 0; u < 16; ++u) {
+        const int q = threadIdx.x + 256 * u;
+        const int r = q >> 6, c = q & 63;
+        st_coh(Aik + static_cast<size_t>(r) * ld + c, Ai[r][c]);
+        st_coh(LTki + static_cast<size_t>(r) * ld + c, Ai[c][r]);
+    }
+}
+
+// A_ij -= L_ik L_jk^T, both operands straight from the transposed tiles
+// (k, i), (k, j).
+template <typename T>
+__device__ void p_update(T* __restrict__ S, int ld, int i, int j, int k) {
+    const T* LTi = S + static_cast<size_t>(k * kTile) * ld + i * kTile;
+    const T* LTj = S + static_cast<size_t>(k * kTile) * ld + j * kTile;
+    T* Aij = S + static_cast<size_t>(i * kTile) * ld + j * kTile;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
+    const int li = lane & 15, lk = lane >> 4;
+    // the tile's current values first: sixteen independent coherent loads in
+    // flight while the operands load and the products run (a coherent load
+    // after a coherent store would wait for it)
+    T old[2][2][4];
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                old[r][c][q] = ld_coh(Aij + static_cast<size_t>(wr + 16 * r + acc_row<T>(lk, q)) * ld + wc + 16 * c + li);
+    typename Acc4<T>::type acc[2][2];
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int c = 0; c < 2; ++c) acc[r][c] = typename Acc4<T>::type{0, 0, 0, 0};
+#pragma unroll
+    for (int kk = 0; kk < kTile; kk += 4) {
+        const size_t ro = static_cast<size_t>(kk + lk) * ld;
+        const T a0 = LTi[ro + wr + li], a1 = LTi[ro + wr + 16 + li];
+        const T b0 = LTj[ro + wc + li], b1 = LTj[ro + wc + 16 + li];
+        acc[0][0] = mfma16(a0, b0, acc[0][0]);
+        acc[0][1] = mfma16(a0, b1, acc[0][1]);
+        acc[1][0] = mfma16(a1, b0, acc[1][0]);
+        acc[1][1] = mfma16(a1, b1, acc[1][1]);
+    }
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int row = wr + 16 * r + acc_row<T>(lk, q), col = wc + 16 * c + li;
+                st_coh(Aij + static_cast<size_t>(row) * ld + col, old[r][c][q] - acc[r][c][q]);
+            }
+}
+
+// store_diag_tile with coherent stores
+template <typename T>
+__device__ __forceinline__ void store_diag_tile_coh(const T (*a)[kTile + 2], const T (*b)[kTile + 2], bool ok, int k,
+                                                    T* __restrict__ D, T* __restrict__ DinvT, TrialFlags* flags) {
+    if (!ok && threadIdx.x == 0) atomicOr(&flags->chol_fail, 1);
+    T* d = D + static_cast<size_t>(k) * kTile * kTile;
+    T* it = DinvT + static_cast<size_t>(k) * kTile * kTile;
+    for (int q = threadIdx.x; q < kTile * kTile; q += blockDim.x) {
+        const int r = q >> 6, c = q & 63;
+        st_coh(d + q, (c <= r) ? a[r][c] : T(0));
+        st_coh(it + q, (r <= c) ? b[c][r] : T(0));
+    }
+}
+
+struct PersistArgs {
+    int* ticket;
+    int* fl[kPNumArrays];
+    const long long* col_start;
+    long long ntasks;
+    int inverse;
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_chol_persistent(T* __restrict__ S, int ld, int nblk, T* __restrict__ D,
+                                                         T* __restrict__ DinvT, TrialFlags* flags, PersistArgs pa,
+                                                         T* __restrict__ X, T* __restrict__ R) {
+    __shared__ T buf0[kTile][kTile + 2];
+    __shared__ T buf1[kTile][kTile + 2];
+    __shared__ int s_ok;
+    __shared__ long long s_t;
+    if (blockIdx.x == 0) {
+        // ---- critical chain ----
+        const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
+        const int li = lane & 15, lk = lane >> 4;
+        for (int k = 0; k < nblk; ++k) {
+            if (k == 0) {
+                if (threadIdx.x == 0) s_ok = p_failed(flags) ? 0 : 1;
+                __syncthreads();
+                if (!s_ok) return;
+                tile_to_lds_coh<T>(buf1, S, ld);
+            } else {
+                const PFlag wo = pcrit_wait_offdiag(k, nblk);
+                if (!p_wg_wait(&wo, 1, pa.fl, flags, &s_ok)) return;
+                p_trsm<T>(S, ld, k, k - 1, DinvT, buf0);
+                p_wg_publish(pcrit_publish_trsm(k, nblk), pa.fl);
+                const PFlag wd = pcrit_wait_diag(k, nblk);
+                if (!p_wg_wait(&wd, 1, pa.fl, flags, &s_ok)) return;
+                tile_to_lds_coh<T>(buf1, S + static_cast<size_t>(k * kTile) * ld + k * kTile, ld);
+                __syncthreads();
+                // A_kk -= L_{k,k-1} L_{k,k-1}^T from LDS
+                typename Acc4<T>::type acc[2][2];
+#pragma unroll
+                for (int r = 0; r < 2; ++r)
+#pragma unroll
+                    for (int c = 0; c < 2; ++c) acc[r][c] = typename Acc4<T>::type{0, 0, 0, 0};
+#pragma unroll 4
+                for (int kk = 0; kk < kTile; kk += 4) {
+                    const T a0 = buf0[wr + li][kk + lk], a1 = buf0[wr + 16 + li][kk + lk];
+                    const T b0 = buf0[wc + li][kk + lk], b1 = buf0[wc + 16 + li][kk + lk];
+                    acc[0][0] = mfma16(a0, b0, acc[0][0]);
+                    acc[0][1] = mfma16(a0, b1, acc[0][1]);
+                    acc[1][0] = mfma16(a1, b0, acc[1][0]);
+                    acc[1][1] = mfma16(a1, b1, acc[1][1]);
+                }
+#pragma unroll
+                for (int r = 0; r < 2; ++r)
+#pragma unroll
+                    for (int c = 0; c < 2; ++c)
+#pragma unroll
+                        for (int q = 0; q < 4; ++q)
+                            buf1[wr + 16 * r + acc_row<T>(lk, q)][wc + 16 * c + li] -= acc[r][c][q];
+            }
+            __syncthreads();
+            const bool ok = tile_potrf_inv<T>(buf1, buf0);
+            store_diag_tile_coh<T>(buf1, buf0, ok, k, D, DinvT, flags);
+            if (!ok) return;  // chol_fail is set: every waiter drains
+            p_wg_publish(pcrit_publish_diag(k), pa.fl);
+        }
+        return;
+    }
+    // ---- workers: tasks in ticket order; the next ticket is taken while
+    // the current task runs ----
+    if (threadIdx.x == 0) s_t = atomicAdd(pa.ticket, 1);
+    __syncthreads();
+    long long t = s_t;
+    while (t < pa.ntasks) {
+        __syncthreads();  // every thread has read s_t
+        if (threadIdx.x == 0) s_t = atomicAdd(pa.ticket, 1);
+        const PTask p = ptask_decode(t, pa.col_start, nblk, pa.inverse != 0);
+        PFlag d[5];
+        const int nd = ptask_deps(p, nblk, d);
+        if (!p_wg_wait(d, nd, pa.fl, flags, &s_ok)) return;
+        switch (p.type) {
+            case kPTrsm:
+                p_trsm<T>(S, ld, p.i, p.k, DinvT, buf0);
+                break;
+            case kPUpd:
+                p_update<T>(S, ld, p.i, p.j, p.k);
+                break;
+            default: {
+                const int r = p.type == kPXDiag ? p.k : (p.type == kPXRow ? p.j : (p.k + 1) + (p.i - p.k - 1) * p.k + p.j);
+                inv_role<T, true>(r, p.k, nblk, S, ld, DinvT, X, R, buf0, buf1);
+                break;
+            }
+        }
+        p_wg_publish(ptask_publish(p, nblk), pa.fl);
+        __syncthreads();  // s_t holds the next ticket
+        t = s_t;
+    }
+}
+
 }  // namespace
 
 void launch_cholesky(hipStream_t st, double* S64, float* S32, int ld, int nblk, double* D64,
@@ -1226,6 +1560,54 @@ void launch_cholesky(hipStream_t st, double* S64, float* S32, int ld, int nblk, 
             hipLaunchKernelGGL(k_inv_last_row<float>, dim3(inv_roles(nblk - 1, nblk)), dim3(256), 0, st,
                                static_cast<const float*>(S32), ld, nblk, static_cast<const float*>(Dinv), X32, R32,
                                static_cast<const TrialFlags*>(flags));
+    }
+}
+
+size_t persist_sync_ints(int nblk) {
+    return 64 + static_cast<size_t>(nblk) + 4 * static_cast<size_t>(nblk) * nblk;
+}
+
+void persist_plan(int nblk, bool inverse, bool fp64, std::vector<long long>& col_start, int& grid) {
+    col_start.assign(static_cast<size_t>(nblk) + 1, 0);
+    ptask_col_start(nblk, inverse, col_start.data());
+    int dev = 0, cus = 0, per_cu = 0;
+    GBA_HIP(hipGetDevice(&dev));
+    GBA_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    if (fp64)
+        GBA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_chol_persistent<double>, 256, 0));
+    else
+        GBA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_chol_persistent<float>, 256, 0));
+    // one workgroup for the critical chain plus workers; all of them fit on
+    // the device at once (the protocol does not need it, but a worker that is
+    // not resident takes no ticket and does no work)
+    const long long want = 1 + col_start[nblk];
+    grid = static_cast<int>(std::max<long long>(1, std::min<long long>(want, static_cast<long long>(std::max(1, per_cu)) * cus)));
+}
+
+void launch_cholesky_persistent(hipStream_t st, double* S64, float* S32, int ld, int nblk, double* D64, float* D32,
+                                TrialFlags* flags, const long long* col_start, long long ntasks, int grid, int* sync,
+                                float* X32, float* R32) {
+    const size_t tile = static_cast<size_t>(kTile) * kTile;
+    launch_fill_int(st, sync, static_cast<int>(persist_sync_ints(nblk)), 0);
+    PersistArgs pa;
+    pa.ticket = sync;
+    const size_t nn = static_cast<size_t>(nblk) * nblk;
+    pa.fl[kPDiag] = sync + 64;
+    pa.fl[kPTrsmDone] = sync + 64 + nblk;
+    pa.fl[kPVer] = pa.fl[kPTrsmDone] + nn;
+    pa.fl[kPXDone] = pa.fl[kPVer] + nn;
+    pa.fl[kPRVer] = pa.fl[kPXDone] + nn;
+    pa.col_start = col_start;
+    pa.ntasks = ntasks;
+    if (S64) {
+        pa.inverse = 0;
+        hipLaunchKernelGGL(k_chol_persistent<double>, dim3(grid), dim3(256), 0, st, S64, ld, nblk, D64,
+                           D64 + static_cast<size_t>(nblk) * tile, flags, pa, static_cast<double*>(nullptr),
+                           static_cast<double*>(nullptr));
+    } else {
+        pa.inverse = X32 ? 1 : 0;
+        hipLaunchKernelGGL(k_chol_persistent<float>, dim3(grid), dim3(256), 0, st, S32, ld, nblk, D32,
+                           D32 + static_cast<size_t>(nblk) * tile, flags, pa, X32, R32);
     }
 }
 

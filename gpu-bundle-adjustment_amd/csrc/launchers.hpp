@@ -3,6 +3,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include "device_common.hpp"
 #include "gba_c.h"
 
@@ -125,6 +127,17 @@ void launch_batch_cholesky(hipStream_t st, int count, const BatchFactorTables& t
                            double* S, double* D, TrialFlags* flags, const int* active);
 // Solves L L^T x = b (b, x length nblk*kTile; b and x may alias).  sync holds
 // 2*nblk + 2 ints of scratch (done flags and tickets).
+// One-launch form of launch_cholesky (same outputs): a critical-chain
+// workgroup and ticketed workers (persist_tasks.hpp).  persist_plan gives the
+// per-column first tickets (nblk + 1, to upload) and the grid; sync holds
+// persist_sync_ints(nblk) ints (cleared here by a fill kernel).  X32 / R32
+// (FP32 only, both or neither): also form X = L^-1.
+size_t persist_sync_ints(int nblk);
+void persist_plan(int nblk, bool inverse, bool fp64, std::vector<long long>& col_start, int& grid);
+void launch_cholesky_persistent(hipStream_t st, double* S64, float* S32, int ld, int nblk, double* D64, float* D32,
+                                TrialFlags* flags, const long long* col_start, long long ntasks, int grid, int* sync,
+                                float* X32 = nullptr, float* R32 = nullptr);
+
 void launch_chol_solve(hipStream_t st, const double* L64, const float* L32, int ld, int nblk,
                        const double* D64, const float* D32, const double* b, double* y, double* x,
                        int* sync, TrialFlags* flags);

@@ -332,6 +332,20 @@ Solver::Solver(const gba_problem_view& prob, const gba_solver_config& cfg) : cfg
     dscale_.alloc(np);
     pcgw_.alloc(pcg_work(pb_.npad));
     sync_.alloc(2 * static_cast<size_t>(nblk_) + 2);
+    {
+        // the one-launch factor is opt-in: at C=356 it measured 2.0 ms (FP32
+        // with the inverse) / 1.8 ms (FP64) against 1.13 / 1.47 ms for the
+        // launch per tile column
+        const char* env = std::getenv("GBA_FACTOR");
+        persist_ = env && std::string(env) == "persistent";
+        if (persist_) {
+            std::vector<long long> cs;
+            persist_plan(nblk_, mixed_, !mixed_, cs, pgrid_);
+            pntasks_ = cs.back();
+            upload(pcol_, cs, st_);
+            psync_.alloc(persist_sync_ints(nblk_));
+        }
+    }
     flags_.alloc(1);
     status_.alloc(1);
     GBA_HIP(hipHostMalloc(&h_status_, sizeof(PassStatus), hipHostMallocDefault));
@@ -443,19 +457,36 @@ void Solver::enqueue_trial_linear_algebra(double lambda, std::vector<double>* ho
         }
     }
     if (!mixed_) {
-        launch_cholesky(st_, S64_.p, nullptr, ld, nblk_, D64_.p, nullptr, flags_.p);
+        enqueue_factor();
         launch_chol_solve(st_, S64_.p, nullptr, ld, nblk_, D64_.p, nullptr, rhs_.p, y_.p, dc_.p,
                           sync_.p, flags_.p);
         return;
     }
     // mixed: FP32 factor of D S D; the camera system is solved in FP64 by CG
     // preconditioned with that factor, against the FP32-assembled D S D
-    launch_cholesky(st_, nullptr, S32_.p, ld, nblk_, nullptr, D32_.p, flags_.p, X32_.p, T32_.p);
+    enqueue_factor();
     const double tol = cfg_.refine_tol > 0 ? cfg_.refine_tol : 1e-10;
     const int max_it = cfg_.refine_max_iters > 0 ? cfg_.refine_max_iters : 12;
     out.refine_iters = pcg_solve(st_, S32low_.p, X32_.p, ld, pb_.npad, rhs_.p, x_.p, pcgw_.p, flags_.p,
                                  status_.p, h_status_, tol, max_it, &out.refine_rel);
     launch_vec_scale(st_, pb_.n, pb_.npad, x_.p, dscale_.p, dc_.p);
+}
+
+void Solver::enqueue_factor() {
+    const int ld = pb_.npad;
+    if (persist_) {
+        if (mixed_)
+            launch_cholesky_persistent(st_, nullptr, S32_.p, ld, nblk_, nullptr, D32_.p, flags_.p, pcol_.p, pntasks_,
+                                       pgrid_, psync_.p, X32_.p, T32_.p);
+        else
+            launch_cholesky_persistent(st_, S64_.p, nullptr, ld, nblk_, D64_.p, nullptr, flags_.p, pcol_.p, pntasks_,
+                                       pgrid_, psync_.p);
+        return;
+    }
+    if (mixed_)
+        launch_cholesky(st_, nullptr, S32_.p, ld, nblk_, nullptr, D32_.p, flags_.p, X32_.p, T32_.p);
+    else
+        launch_cholesky(st_, S64_.p, nullptr, ld, nblk_, D64_.p, nullptr, flags_.p);
 }
 
 TrialOut Solver::trial(double lambda, std::vector<double>* hook_S, std::vector<double>* hook_rhs) {
@@ -472,7 +503,7 @@ TrialOut Solver::trial(double lambda, std::vector<double>* hook_S, std::vector<d
     out.trial_sse = h_status_->sse;
     out.denom = h_status_->denom;
     out.anomalies = h_status_->anomalies;
-    if (out.flags.solve_err) throw Error("triangular solve timed out waiting for a tile row");
+    if (out.flags.solve_err) throw Error("dense factor or triangular solve gave up waiting for a tile");
     return out;
 }
 
@@ -562,10 +593,7 @@ double Solver::time_kernel(int kernel, double lambda, int reps) {
                 break;
             case GBA_KERNEL_FACTOR:
                 // mixed: factor + explicit inverse (what a trial runs before PCG)
-                if (mixed_)
-                    launch_cholesky(st_, nullptr, S32_.p, ld, nblk_, nullptr, D32_.p, flags_.p, X32_.p, T32_.p);
-                else
-                    launch_cholesky(st_, S64_.p, nullptr, ld, nblk_, D64_.p, nullptr, flags_.p);
+                enqueue_factor();
                 break;
             case GBA_KERNEL_SOLVE:
                 // FP64: both triangular solves; mixed: one preconditioner application

@@ -134,6 +134,7 @@ private:
                                       std::vector<double>* hook_rhs, TrialOut& out);
     void sync_status();
     void enqueue_schur(double lambda);
+    void enqueue_factor();  // the dense factor of S (and X = L^-1 in mixed precision)
 
     gba_solver_config cfg_;
     bool mixed_;
@@ -165,6 +166,13 @@ private:
     DevBuf<float> S32_, S32low_, D32_, X32_, T32_;
     DevBuf<double> rhs_, y_, x_, dc_, dscale_, pcgw_;
     DevBuf<int> sync_;
+    // one-launch factor (GBA_FACTOR=persistent; the default is the launch
+    // per tile column): per-column first tickets, flags, grid
+    bool persist_ = false;
+    DevBuf<long long> pcol_;
+    DevBuf<int> psync_;
+    long long pntasks_ = 0;
+    int pgrid_ = 1;
     DevBuf<TrialFlags> flags_;
     DevBuf<PassStatus> status_;
     PassStatus* h_status_ = nullptr;

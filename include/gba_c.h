@@ -66,14 +66,19 @@ typedef struct gba_lm_result {
     int trials;                   /* LM trials evaluated */
     double solve_time_s;          /* device-resident part (after upload) */
     double setup_time_s;          /* allocation, upload, observation sort */
-    int refine_iterations;        /* mixed precision: refinement steps, summed */
-    double max_refine_residual;   /* mixed precision: worst final relative residual */
+    int refine_iterations;        /* mixed precision: PCG steps, summed over trials */
+    double max_refine_residual;   /* mixed precision: worst final |S x - b| / |b| of the PCG,
+                                     S the FP32-stored reduced camera matrix (built from
+                                     FP32 W), products accumulated in FP64; not a residual
+                                     of the FP64 system (about 2e-8 against it, measured) */
 } gba_lm_result;
 
 /* Arithmetic of the reduced camera solve. */
 enum gba_precision {
     GBA_PRECISION_FP64 = 0,  /* plain drop-in: FP64 throughout */
-    GBA_PRECISION_MIXED = 1  /* FP32 W / S / MFMA factor, FP64 refinement, cost, gain ratio */
+    GBA_PRECISION_MIXED = 1  /* FP32 W, S and MFMA factor; FP64 PCG on the FP32-stored S
+                                preconditioned by that factor; FP64 cost, gain ratio,
+                                back-substitution */
 };
 
 /* Device storage of the observed pixels. */
@@ -86,8 +91,9 @@ enum gba_obs_storage {
 /* Solver configuration beyond gba_lm_options (no reference counterpart). */
 typedef struct gba_solver_config {
     int precision;          /* enum gba_precision */
-    double refine_tol;      /* mixed: relative residual target (default 1e-10) */
-    int refine_max_iters;   /* mixed: refinement cap per trial (default 12) */
+    double refine_tol;      /* mixed: PCG relative residual target on the FP32-stored S
+                               (default 1e-10) */
+    int refine_max_iters;   /* mixed: PCG step cap per trial (default 12) */
     double* trace;          /* optional, GBA_TRACE_STRIDE doubles per trial */
     int trace_capacity;     /* trials that fit in trace */
     int obs_storage;        /* enum gba_obs_storage (default FP64) */
@@ -216,7 +222,7 @@ int gba_cost_cfg(const gba_problem_view* prob, const gba_solver_config* cfg, dou
  * the solver's blocked MFMA Cholesky (replaces Eigen::LLT at
  * src/lm.cu:597-604).  b (n) is overwritten with the solution; *ok = 0 when
  * the matrix is not positive definite.  precision FP64 or MIXED (FP32
- * factor + FP64 refinement; *refine_iters receives the step count). */
+ * factor + FP64 PCG on the FP32 matrix; *refine_iters receives the step count). */
 int gba_dense_spd_solve(int n, int precision, const double* A, double* b, int* ok,
                         int* refine_iters);
 
