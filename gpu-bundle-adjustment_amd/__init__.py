@@ -22,6 +22,7 @@ TERMINATION = {CONVERGENCE: "CONVERGENCE", MAX_ITERATIONS: "MAX_ITERATIONS", FAI
 PRECISION_FP64, PRECISION_MIXED = 0, 1
 OBS_FP64, OBS_FP16 = 0, 1  # device storage of the observed pixels
 NOALLOC_MAX_N = 1024  # GBA_NOALLOC_MAX_N of include/gba_c.h
+LAYOUT_AUTO, LAYOUT_DENSE, LAYOUT_ENVELOPE = 0, 1, 2  # enum gba_factor_layout
 TRACE_STRIDE = 6
 TRIAL_STATUS = {0: "accept", 1: "reject", 2: "nonspd_v", 3: "chol_fail", 4: "denom", 5: "nonfinite"}
 KERNELS = {"edge_point": 0, "edge_camera": 1, "schur": 2, "factor": 3, "solve": 4, "backsub": 5,
@@ -52,14 +53,14 @@ class LMResultC(ctypes.Structure):
 class SolverConfigC(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("refine_tol", ctypes.c_double),
                 ("refine_max_iters", ctypes.c_int), ("trace", _dp), ("trace_capacity", ctypes.c_int),
-                ("obs_storage", ctypes.c_int), ("huber_delta", ctypes.c_double)]
+                ("obs_storage", ctypes.c_int), ("huber_delta", ctypes.c_double), ("factor_layout", ctypes.c_int)]
 
 
 class SynthSpecC(ctypes.Structure):
     _fields_ = [("num_cameras", ctypes.c_int), ("num_points", ctypes.c_int),
                 ("num_observations", ctypes.c_int), ("kmin", ctypes.c_int), ("kmax", ctypes.c_int),
                 ("law", ctypes.c_int), ("alpha", ctypes.c_double), ("ring_radius", ctypes.c_double),
-                ("pixel_noise", ctypes.c_double), ("perturb", ctypes.c_double)]
+                ("pixel_noise", ctypes.c_double), ("perturb", ctypes.c_double), ("view_window", ctypes.c_int)]
 
 
 # exported symbols and their prototypes (include/gba_c.h)
@@ -76,6 +77,7 @@ _PROTOS = {
     "gba_solver_time_kernel": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_double,
                                               ctypes.c_int, _dp]),
     "gba_solver_stats": (ctypes.c_int, [ctypes.c_void_p, _dp, _ip, _ip]),
+    "gba_solver_layout": (ctypes.c_int, [ctypes.c_void_p, _ip, _dp, _dp]),
     "gba_solver_destroy": (None, [ctypes.c_void_p]),
     "gba_batch_create": (ctypes.c_int, [ctypes.POINTER(ProblemView), ctypes.c_int,
                                         ctypes.POINTER(ctypes.c_void_p)]),
@@ -109,6 +111,7 @@ _PROTOS = {
     "gba_bal_header": (ctypes.c_int, [ctypes.c_char_p, _ip, _ip, _ip]),
     "gba_load_bal": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ProblemView)]),
     "gba_write_bal": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ProblemView)]),
+    "gba_plan_envelope": (ctypes.c_int, [ctypes.POINTER(ProblemView), ctypes.c_double, _ip, _ip, _ip, _dp, _dp]),
     "gba_synth_preset": (ctypes.c_int, [ctypes.c_int, ctypes.c_ulonglong, ctypes.POINTER(SynthSpecC)]),
     "gba_synth_generate": (ctypes.c_int, [ctypes.POINTER(SynthSpecC), ctypes.c_ulonglong,
                                           ctypes.POINTER(ProblemView)]),
@@ -232,10 +235,12 @@ class LMResult:
 
 
 def _config(precision, trace_capacity=0, refine_tol=1e-10, refine_max_iters=12, obs_storage=OBS_FP64,
-            huber_delta=0.0):
+            huber_delta=0.0, layout=None):
     c = SolverConfigC()
     lib().gba_default_config(ctypes.byref(c))
     c.precision = precision
+    if layout is not None:
+        c.factor_layout = layout
     c.obs_storage = obs_storage
     c.huber_delta = huber_delta
     c.refine_tol = refine_tol
@@ -256,11 +261,13 @@ def _trim_trace(trace):
 
 
 def solve_bal_gpu(prob, opts=None, precision=PRECISION_FP64, trace_capacity=1024, obs_storage=OBS_FP64,
-                  huber_delta=0.0):
+                  huber_delta=0.0, layout=None):
     """gba::solve_bal_gpu: updates prob.cameras / prob.points in place.
-    huber_delta > 0 minimises sum rho(|r|^2) / 2 with the Huber loss."""
+    huber_delta > 0 minimises sum rho(|r|^2) / 2 with the Huber loss; layout
+    is a LAYOUT_* value (default: the library's automatic choice)."""
     opts = opts or LMOptions()
-    cfg, trace = _config(precision, trace_capacity, obs_storage=obs_storage, huber_delta=huber_delta)
+    cfg, trace = _config(precision, trace_capacity, obs_storage=obs_storage, huber_delta=huber_delta,
+                         layout=layout)
     res = LMResultC()
     v = prob.view()
     _check(lib().gba_solve_bal(ctypes.byref(v), ctypes.byref(opts.c()), ctypes.byref(cfg),
@@ -272,9 +279,9 @@ class Solver:
     """Device-resident solver handle: upload once, solve repeatedly."""
 
     def __init__(self, prob, precision=PRECISION_FP64, trace_capacity=1024, obs_storage=OBS_FP64,
-                 huber_delta=0.0):
+                 huber_delta=0.0, layout=None):
         self._cfg, self._trace = _config(precision, trace_capacity, obs_storage=obs_storage,
-                                         huber_delta=huber_delta)
+                                         huber_delta=huber_delta, layout=layout)
         self._h = ctypes.c_void_p()
         self._prob = prob
         v = prob.view()
@@ -303,6 +310,12 @@ class Solver:
         pc, n, npad = ctypes.c_double(), ctypes.c_int(), ctypes.c_int()
         _check(lib().gba_solver_stats(self._h, ctypes.byref(pc), ctypes.byref(n), ctypes.byref(npad)))
         return {"pair_count": pc.value, "n": n.value, "n_padded": npad.value}
+
+    def layout(self):
+        """(LAYOUT_DENSE or LAYOUT_ENVELOPE, envelope work / dense work, camera-graph density)."""
+        lay, ratio, dens = ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
+        _check(lib().gba_solver_layout(self._h, ctypes.byref(lay), ctypes.byref(ratio), ctypes.byref(dens)))
+        return lay.value, ratio.value, dens.value
 
     def close(self):
         if self._h:
@@ -514,6 +527,21 @@ def load_bal(path):
 def write_bal(path, prob):
     v = prob.view()
     _check(lib().gba_write_bal(str(path).encode(), ctypes.byref(v)))
+
+
+def plan_envelope(prob, max_density=1.0):
+    """Host-side envelope plan: None when the camera graph is denser than
+    max_density, else dict(order, first, work_ratio, density)."""
+    C = prob.num_cameras
+    nblk = (9 * C + 63) // 64
+    order, first = np.zeros(C, np.int32), np.zeros(nblk, np.int32)
+    ok, wr, dens = ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
+    v = prob.view()
+    _check(lib().gba_plan_envelope(ctypes.byref(v), max_density, ctypes.byref(ok), _ptr(order, ctypes.c_int),
+                                   _ptr(first, ctypes.c_int), ctypes.byref(wr), ctypes.byref(dens)))
+    if not ok.value:
+        return None
+    return dict(order=order, first=first, work_ratio=wr.value, density=dens.value)
 
 
 def synth_spec(config, seed=1):

@@ -34,6 +34,7 @@ SOURCES = [
     "cpp_api.cpp",
     "host_model.cpp",
     "synth.cpp",
+    "camera_graph.cpp",
 ]
 
 ARCH = os.environ.get("GBA_OFFLOAD_ARCH", "gfx950")

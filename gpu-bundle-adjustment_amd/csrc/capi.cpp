@@ -3,6 +3,7 @@
 // them into a non-zero return code plus gba_last_error().
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -147,6 +148,20 @@ int gba_solver_stats(gba_solver* s, double* pair_count, int* n, int* n_padded) {
         return 0;
     });
 }
+
+int gba_solver_layout(gba_solver* s, int* layout, double* factor_work_ratio, double* graph_density) {
+    return guarded([&] {
+        need(s, "gba_solver_layout: null handle");
+        int l = 0;
+        double w = 1.0, d = 1.0;
+        s->s->layout(&l, &w, &d);
+        if (layout) *layout = l;
+        if (factor_work_ratio) *factor_work_ratio = w;
+        if (graph_density) *graph_density = d;
+        return 0;
+    });
+}
+
 
 void gba_solver_destroy(gba_solver* s) { delete s; }
 
@@ -447,6 +462,22 @@ int gba_write_bal(const char* path, const gba_problem_view* prob) {
     return guarded([&] { return write_bal(path, prob); });
 }
 
+int gba_plan_envelope(const gba_problem_view* prob, double max_density, int* usable, int* cam_order, int* first,
+                      double* work_ratio, double* density) {
+    return guarded([&] {
+        need(prob && usable, "gba_plan_envelope: null argument");
+        EnvelopeLayout lay;
+        *usable = plan_envelope(*prob, max_density, lay) ? 1 : 0;
+        if (*usable) {
+            if (cam_order) std::copy(lay.cam_of_pos.begin(), lay.cam_of_pos.end(), cam_order);
+            if (first) std::copy(lay.first.begin(), lay.first.end(), first);
+        }
+        if (work_ratio) *work_ratio = lay.work_ratio;
+        if (density) *density = lay.density;
+        return 0;
+    });
+}
+
 int gba_synth_preset(int config, unsigned long long seed, gba_synth_spec* s) {
     std::memset(s, 0, sizeof *s);
     s->kmin = 2;
@@ -486,8 +517,16 @@ int gba_synth_preset(int config, unsigned long long seed, gba_synth_spec* s) {
             s->kmax = s->num_cameras;
             return 0;
         }
+        case 6:
+            s->num_cameras = 400; s->num_points = 60000; s->num_observations = 300000;
+            s->kmax = 16; s->view_window = 12;
+            return 0;
+        case 7:
+            s->num_cameras = 2000; s->num_points = 400000; s->num_observations = 2000000;
+            s->kmax = 16; s->view_window = 12;
+            return 0;
         default:
-            return set_error("gba_synth_preset: config must be 1..5");
+            return set_error("gba_synth_preset: config must be 1..7");
     }
 }
 

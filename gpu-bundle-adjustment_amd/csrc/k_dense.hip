@@ -907,11 +907,13 @@ __global__ __launch_bounds__(kBatchThreads) void k_batch_chol_solve(const BatchD
 
 // Forward solve L y = b.  Workgroup with ticket t owns tile row i = t.
 // DinvT[i] holds (L_ii^-1)^T.
+// first (optional, envelope factor): L_ij = 0 for j < first[i], so row i
+// neither reads nor waits for those tiles.
 template <typename T>
 __global__ __launch_bounds__(256) void k_trsv_fwd(const T* __restrict__ L, int ld, int nblk,
                                                   const T* __restrict__ DinvT,
                                                   const double* __restrict__ b, double* y, int* done,
-                                                  int* ticket, TrialFlags* flags) {
+                                                  int* ticket, TrialFlags* flags, const int* __restrict__ first) {
     __shared__ int s_blk, s_skip;
     __shared__ double s_part[4][16];
     __shared__ double s_rhs[kTile];
@@ -931,7 +933,7 @@ __global__ __launch_bounds__(256) void k_trsv_fwd(const T* __restrict__ L, int l
     double acc[16];
 #pragma unroll
     for (int r = 0; r < 16; ++r) acc[r] = 0.0;
-    for (int j = 0; j < i; ++j) {
+    for (int j = first ? first[i] : 0; j < i; ++j) {
         const T* Lr = L + static_cast<size_t>(i * kTile + 16 * w) * ld + j * kTile + lane;
         T lv[16];
 #pragma unroll
@@ -974,7 +976,7 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_trsv_bwd(const T* __restrict__ L, int ld, int nblk,
                                                   const T* __restrict__ DinvT,
                                                   const double* __restrict__ y, double* x, int* done,
-                                                  int* ticket, TrialFlags* flags) {
+                                                  int* ticket, TrialFlags* flags, const int* __restrict__ first) {
     __shared__ int s_blk, s_skip;
     __shared__ double s_part[4][kTile];
     __shared__ double s_rhs[kTile];
@@ -993,6 +995,7 @@ __global__ __launch_bounds__(256) void k_trsv_bwd(const T* __restrict__ L, int l
     }
     double acc = 0.0;
     for (int j = nblk - 1; j > i; --j) {
+        if (first && first[j] > i) continue;  // L_ji = 0 (uniform across the workgroup)
         const T* Lc = L + static_cast<size_t>(j * kTile + 16 * w) * ld + i * kTile + lane;
         T lv[16];
 #pragma unroll
@@ -1644,7 +1647,7 @@ void launch_batch_chol_solve(hipStream_t st, int count, const BatchDesc* desc, d
 
 void launch_chol_solve(hipStream_t st, const double* L64, const float* L32, int ld, int nblk,
                        const double* D64, const float* D32, const double* b, double* y, double* x,
-                       int* sync, TrialFlags* flags) {
+                       int* sync, TrialFlags* flags, const int* first) {
     launch_fill_int(st, sync, 2 * nblk + 2, 0);
     int* done_f = sync;
     int* done_b = sync + nblk;
@@ -1652,14 +1655,14 @@ void launch_chol_solve(hipStream_t st, const double* L64, const float* L32, int 
     const size_t inv_off = static_cast<size_t>(nblk) * kTile * kTile;
     if (L64) {
         hipLaunchKernelGGL(k_trsv_fwd<double>, dim3(nblk), dim3(256), 0, st, L64, ld, nblk,
-                           D64 + inv_off, b, y, done_f, tickets, flags);
+                           D64 + inv_off, b, y, done_f, tickets, flags, first);
         hipLaunchKernelGGL(k_trsv_bwd<double>, dim3(nblk), dim3(256), 0, st, L64, ld, nblk,
-                           D64 + inv_off, y, x, done_b, tickets + 1, flags);
+                           D64 + inv_off, y, x, done_b, tickets + 1, flags, first);
     } else {
         hipLaunchKernelGGL(k_trsv_fwd<float>, dim3(nblk), dim3(256), 0, st, L32, ld, nblk,
-                           D32 + inv_off, b, y, done_f, tickets, flags);
+                           D32 + inv_off, b, y, done_f, tickets, flags, first);
         hipLaunchKernelGGL(k_trsv_bwd<float>, dim3(nblk), dim3(256), 0, st, L32, ld, nblk,
-                           D32 + inv_off, y, x, done_b, tickets + 1, flags);
+                           D32 + inv_off, y, x, done_b, tickets + 1, flags, first);
     }
 }
 

@@ -125,8 +125,6 @@ struct BatchFactorTables {
 // launch_cholesky lays them out; a bad pivot sets flags[problem].chol_fail.
 void launch_batch_cholesky(hipStream_t st, int count, const BatchFactorTables& t, const BatchDesc* desc,
                            double* S, double* D, TrialFlags* flags, const int* active);
-// Solves L L^T x = b (b, x length nblk*kTile; b and x may alias).  sync holds
-// 2*nblk + 2 ints of scratch (done flags and tickets).
 // One-launch form of launch_cholesky (same outputs): a critical-chain
 // workgroup and ticketed workers (persist_tasks.hpp).  persist_plan gives the
 // per-column first tickets (nblk + 1, to upload) and the grid; sync holds
@@ -138,9 +136,13 @@ void launch_cholesky_persistent(hipStream_t st, double* S64, float* S32, int ld,
                                 TrialFlags* flags, const long long* col_start, long long ntasks, int grid, int* sync,
                                 float* X32 = nullptr, float* R32 = nullptr);
 
+// Solves L L^T x = b (b, x length nblk*kTile; b and x may alias).  sync holds
+// 2*nblk + 2 ints of scratch (done flags and tickets).  first (nblk ints, optional):
+// the envelope factor's first tile column of each tile row.
 void launch_chol_solve(hipStream_t st, const double* L64, const float* L32, int ld, int nblk,
                        const double* D64, const float* D32, const double* b, double* y, double* x,
-                       int* sync, TrialFlags* flags);
+                       int* sync, TrialFlags* flags,
+                       const int* first = nullptr);
 // y = A x, A symmetric given by the lower triangle of an FP32 row-major
 // matrix (FP64 accumulation); work holds symv_work(npad) doubles.
 size_t symv_work(int npad);

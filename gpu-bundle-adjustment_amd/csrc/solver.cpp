@@ -218,8 +218,6 @@ Solver::Solver(const gba_problem_view& prob, const gba_solver_config& cfg) : cfg
     pb_.n = 9 * C;
     pb_.npad = (pb_.n + kTile - 1) / kTile * kTile;
     nblk_ = pb_.npad / kTile;
-    init_cams_.assign(prob.cameras, prob.cameras + 9 * static_cast<size_t>(C));
-    init_pts_.assign(prob.points, prob.points + 3 * static_cast<size_t>(P));
 
     sshape_ = schur_shape(C, mixed_);
     // diagonal blocks and rhs in their own per-camera kernel (GBA_SCHUR_DIAG=0:
@@ -229,16 +227,71 @@ Solver::Solver(const gba_problem_view& prob, const gba_solver_config& cfg) : cfg
         sep_diag_ = !(env && env[0] == '0');
     }
     GBA_HIP(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
+    // factor layout: the envelope (profile) factor for plain FP64 solves of
+    // sparse camera graphs, with the cameras renumbered; GBA_LAYOUT in the
+    // environment (dense / envelope) overrides the configuration
+    {
+        int want = cfg.factor_layout;
+        if (const char* env = std::getenv("GBA_LAYOUT")) {
+            if (std::string(env) == "dense") want = GBA_LAYOUT_DENSE;
+            else if (std::string(env) == "envelope") want = GBA_LAYOUT_ENVELOPE;
+        }
+        if (want == GBA_LAYOUT_ENVELOPE && mixed_) throw Error("the envelope factor layout is plain FP64 only");
+        if (want != GBA_LAYOUT_DENSE && want != GBA_LAYOUT_ENVELOPE && want != GBA_LAYOUT_AUTO)
+            throw Error("unknown factor layout");
+        if (!mixed_ && want != GBA_LAYOUT_DENSE) {
+            EnvelopeLayout lay;
+            if (plan_envelope(prob, want == GBA_LAYOUT_ENVELOPE ? 1.0 : 0.5, lay)) {
+                env_work_ = lay.work_ratio;
+                env_density_ = lay.density;
+                env_ = want == GBA_LAYOUT_ENVELOPE || lay.work_ratio < 0.5;
+            }
+            if (env_) {
+                cam_of_pos_ = std::move(lay.cam_of_pos);
+                pos_of_cam_ = std::move(lay.pos_of_cam);
+                upload(first_, lay.first, st_);
+                std::vector<int2> trsm, upd;
+                envelope_factor_tables(lay.first, trsm, etrsm_off_, upd, eupd_off_);
+                const int nblk = static_cast<int>(lay.first.size());
+                upload(epotrf_, std::vector<int2>{make_int2(0, 0)}, st_);
+                upload(etrsm_, trsm, st_);
+                upload(eupd_, upd, st_);
+                upload(eactive_, std::vector<int>{1}, st_);
+                etab_.potrf0 = epotrf_.p;
+                etab_.trsm = etrsm_.p;
+                etab_.upd = eupd_.p;
+                etab_.trsm_off = etrsm_off_.data();
+                etab_.upd_off = eupd_off_.data();
+                etab_.nsteps = nblk;
+                BatchDesc d{};
+                d.c0 = 0;
+                d.c1 = C;
+                d.n = 9 * C;
+                d.npad = nblk * kTile;
+                upload(edesc_, std::vector<BatchDesc>{d}, st_);
+            }
+        }
+    }
+    init_cams_.assign(9 * static_cast<size_t>(C), 0.0);
+    cams_to_internal(prob.cameras, init_cams_.data());
+    init_pts_.assign(prob.points, prob.points + 3 * static_cast<size_t>(P));
 
     // ---- point-major layout on the device: sort, CSR, camera index ----
     const bool fp16 = cfg.obs_storage == GBA_OBS_FP16;
     if (!fp16 && cfg.obs_storage != GBA_OBS_FP64) throw Error("unknown observation storage");
+    std::vector<int> oc_internal;  // alive until the synchronisation below
     {
         DevBuf<int> ocam, opt;
         DevBuf<double> opx;
         ocam.alloc(static_cast<size_t>(O));
         opt.alloc(static_cast<size_t>(O));
-        GBA_HIP(hipMemcpyAsync(ocam.p, prob.obs_camera, sizeof(int) * O, hipMemcpyHostToDevice, st_));
+        const int* oc = prob.obs_camera;
+        if (env_) {
+            oc_internal.resize(static_cast<size_t>(O));
+            for (int e = 0; e < O; ++e) oc_internal[e] = pos_of_cam_[prob.obs_camera[e]];
+            oc = oc_internal.data();
+        }
+        GBA_HIP(hipMemcpyAsync(ocam.p, oc, sizeof(int) * O, hipMemcpyHostToDevice, st_));
         GBA_HIP(hipMemcpyAsync(opt.p, prob.obs_point, sizeof(int) * O, hipMemcpyHostToDevice, st_));
         if (!fp16) {
             opx.alloc(2 * static_cast<size_t>(O));
@@ -273,6 +326,11 @@ Solver::Solver(const gba_problem_view& prob, const gba_solver_config& cfg) : cfg
             packed[k] = static_cast<unsigned>(h[2 * e]) | (static_cast<unsigned>(h[2 * e + 1]) << 16);
         }
         upload(spx16_, packed, st_);
+        if (env_) {
+            std::vector<float> internal(scale.size());
+            for (int c = 0; c < C; ++c) internal[c] = scale[cam_of_pos_[c]];
+            scale.swap(internal);
+        }
         upload(cam_scale_, scale, st_);
         pb_.spx16 = spx16_.p;
         pb_.cam_scale = cam_scale_.p;
@@ -361,6 +419,10 @@ Solver::Solver(const gba_problem_view& prob, const gba_solver_config& cfg) : cfg
         launch_identity_pad(st_, nullptr, S32_.p, pb_.npad, pb_.n, pb_.npad);
         launch_identity_pad(st_, nullptr, S32low_.p, pb_.npad, pb_.n, pb_.npad);
     } else {
+        // envelope: the tiles outside the profile are never written, so they
+        // (and the transposed tiles the update of a diagonal tile reads when
+        // its row starts after the column) stay zero
+        if (env_) launch_fill_double(st_, S64_.p, S64_.n, 0.0);
         launch_identity_pad(st_, S64_.p, nullptr, pb_.npad, pb_.n, pb_.npad);
     }
     launch_clear_flags(st_, flags_.p);
@@ -375,8 +437,30 @@ Solver::~Solver() {
     if (st_) (void)hipStreamDestroy(st_);
 }
 
+void Solver::cams_to_internal(const double* in, double* out) const {
+    const size_t C = static_cast<size_t>(pb_.C);
+    for (size_t c = 0; c < C; ++c) {
+        const size_t to = env_ ? static_cast<size_t>(pos_of_cam_[c]) : c;
+        std::memcpy(out + 9 * to, in + 9 * c, 9 * sizeof(double));
+    }
+}
+
+void Solver::cams_to_caller(const double* in, double* out) const {
+    const size_t C = static_cast<size_t>(pb_.C);
+    for (size_t c = 0; c < C; ++c) {
+        const size_t from = env_ ? static_cast<size_t>(pos_of_cam_[c]) : c;
+        std::memcpy(out + 9 * c, in + 9 * from, 9 * sizeof(double));
+    }
+}
+
 void Solver::reset_params(const double* cams, const double* pts) {
     cur_ = 0;
+    std::vector<double> internal;
+    if (cams && env_) {
+        internal.resize(init_cams_.size());
+        cams_to_internal(cams, internal.data());
+        cams = internal.data();
+    }
     GBA_HIP(hipMemcpyAsync(cams_[0].p, cams ? cams : init_cams_.data(), sizeof(double) * cams_[0].n,
                            hipMemcpyHostToDevice, st_));
     // both point buffers: the back-substitution writes trial coordinates only
@@ -386,6 +470,7 @@ void Solver::reset_params(const double* cams, const double* pts) {
     for (int i = 0; i < 2; ++i)
         GBA_HIP(hipMemcpyAsync(pts_[i].p, pts ? pts : init_pts_.data(), sizeof(double) * pts_[i].n,
                                hipMemcpyHostToDevice, st_));
+    if (!internal.empty()) GBA_HIP(hipStreamSynchronize(st_));
 }
 
 void Solver::sync_status() {
@@ -444,22 +529,29 @@ void Solver::enqueue_trial_linear_algebra(double lambda, std::vector<double>* ho
             } else {
                 GBA_HIP(hipMemcpy(S.data(), S64_.p, sizeof(double) * np * np, hipMemcpyDeviceToHost));
             }
+            // caller-order row r <-> internal row ir(r); the internal lower
+            // triangle holds S, so an entry above it is read mirrored
+            auto ir = [&](size_t r) { return env_ ? 9 * static_cast<size_t>(pos_of_cam_[r / 9]) + r % 9 : r; };
             hook_S->assign(n * n, 0.0);
             for (size_t r = 0; r < n; ++r)
-                for (size_t c = 0; c <= r && c < n; ++c)  // lower triangle, column-major out
-                    (*hook_S)[r + c * n] = S[r * np + c] / (scale[r] * scale[c]);
+                for (size_t c = 0; c <= r && c < n; ++c) {  // lower triangle, column-major out
+                    const size_t a = ir(r), b = ir(c);
+                    const double v = a >= b ? S[a * np + b] : S[b * np + a];
+                    (*hook_S)[r + c * n] = v / (scale[a] * scale[b]);
+                }
         }
         if (hook_rhs) {
             std::vector<double> rr(np);
             GBA_HIP(hipMemcpy(rr.data(), rhs_.p, sizeof(double) * np, hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < n; ++i) rr[i] /= scale[i];
             hook_rhs->assign(n, 0.0);
-            for (size_t i = 0; i < n; ++i) (*hook_rhs)[i] = rr[i] / scale[i];
+            cams_to_caller(rr.data(), hook_rhs->data());
         }
     }
     if (!mixed_) {
         enqueue_factor();
         launch_chol_solve(st_, S64_.p, nullptr, ld, nblk_, D64_.p, nullptr, rhs_.p, y_.p, dc_.p,
-                          sync_.p, flags_.p);
+                          sync_.p, flags_.p, env_ ? first_.p : nullptr);
         return;
     }
     // mixed: FP32 factor of D S D; the camera system is solved in FP64 by CG
@@ -474,6 +566,10 @@ void Solver::enqueue_trial_linear_algebra(double lambda, std::vector<double>* ho
 
 void Solver::enqueue_factor() {
     const int ld = pb_.npad;
+    if (env_) {
+        launch_batch_cholesky(st_, 1, etab_, edesc_.p, S64_.p, D64_.p, flags_.p, eactive_.p);
+        return;
+    }
     if (persist_) {
         if (mixed_)
             launch_cholesky_persistent(st_, nullptr, S32_.p, ld, nblk_, nullptr, D32_.p, flags_.p, pcol_.p, pntasks_,
@@ -510,9 +606,12 @@ TrialOut Solver::trial(double lambda, std::vector<double>* hook_S, std::vector<d
 void Solver::accept_trial() { cur_ = 1 - cur_; }
 
 void Solver::download_params(double* cams, double* pts) {
-    GBA_HIP(hipMemcpyAsync(cams, cams_[cur_].p, sizeof(double) * cams_[cur_].n, hipMemcpyDeviceToHost, st_));
+    std::vector<double> internal(env_ ? cams_[cur_].n : 0);
+    GBA_HIP(hipMemcpyAsync(env_ ? internal.data() : cams, cams_[cur_].p, sizeof(double) * cams_[cur_].n,
+                           hipMemcpyDeviceToHost, st_));
     GBA_HIP(hipMemcpyAsync(pts, pts_[cur_].p, sizeof(double) * pts_[cur_].n, hipMemcpyDeviceToHost, st_));
     GBA_HIP(hipStreamSynchronize(st_));
+    if (env_) cams_to_caller(internal.data(), cams);
 }
 
 void Solver::download_linearization(double* U, double* V, double* W, double* bc, double* bp) {
@@ -524,11 +623,14 @@ void Solver::download_linearization(double* U, double* V, double* W, double* bc,
     GBA_HIP(hipMemcpy(hVB.data(), VB_.p, sizeof(double) * hVB.size(), hipMemcpyDeviceToHost));
     GBA_HIP(hipMemcpy(hbc.data(), bc_.p, sizeof(double) * hbc.size(), hipMemcpyDeviceToHost));
     GBA_HIP(hipMemcpy(hperm.data(), perm_.p, sizeof(int) * O, hipMemcpyDeviceToHost));
+    // camera records in the caller's order (internal index pos_of_cam_[c])
+    auto cpos = [&](size_t c) { return env_ ? static_cast<size_t>(pos_of_cam_[c]) : c; };
     if (U)
         for (size_t c = 0; c < C; ++c)
             for (int i = 0; i < 9; ++i)
-                for (int j = 0; j < 9; ++j) U[81 * c + 9 * i + j] = j <= i ? hU[81 * c + 9 * i + j] : 0.0;
-    if (bc) std::memcpy(bc, hbc.data(), sizeof(double) * hbc.size());
+                for (int j = 0; j < 9; ++j) U[81 * c + 9 * i + j] = j <= i ? hU[81 * cpos(c) + 9 * i + j] : 0.0;
+    if (bc)
+        for (size_t c = 0; c < C; ++c) std::memcpy(bc + 9 * c, &hbc[9 * cpos(c)], 9 * sizeof(double));
     for (size_t p = 0; p < P; ++p) {
         const double* vb = &hVB[9 * p];
         if (V) {
@@ -570,7 +672,9 @@ void Solver::download_step(double* dc, double* dp) {
     }
     if (dc) {
         GBA_HIP(hipStreamSynchronize(st_));
-        GBA_HIP(hipMemcpy(dc, dc_.p, sizeof(double) * pb_.n, hipMemcpyDeviceToHost));
+        std::vector<double> internal(static_cast<size_t>(pb_.n));
+        GBA_HIP(hipMemcpy(internal.data(), dc_.p, sizeof(double) * pb_.n, hipMemcpyDeviceToHost));
+        cams_to_caller(internal.data(), dc);
     }
 }
 
@@ -601,7 +705,7 @@ double Solver::time_kernel(int kernel, double lambda, int reps) {
                     launch_precond_apply(st_, X32_.p, ld, pb_.npad, rhs_.p, x_.p, pcgw_.p + 5 * static_cast<size_t>(pb_.npad));
                 else
                     launch_chol_solve(st_, S64_.p, nullptr, ld, nblk_, D64_.p, nullptr, rhs_.p, y_.p, x_.p,
-                                      sync_.p, flags_.p);
+                                      sync_.p, flags_.p, env_ ? first_.p : nullptr);
                 break;
             case GBA_KERNEL_BACKSUB:
                 launch_backsub(st_, pb_, W64_.p, W32_.p, Vinv_.p, VB_.p, dc_.p, pts_[cur_].p,

@@ -90,6 +90,24 @@ int pcg_solve(hipStream_t st, const float* A, const float* X32, int ld, int npad
               double* x, double* work, TrialFlags* flags, PassStatus* status, PassStatus* h_status,
               double tol, int max_it, double* rel);
 
+// Envelope layout of S (camera_graph.cpp): the cameras' internal order
+// (reverse Cuthill-McKee on the camera graph), and per tile row of the
+// renumbered S the first tile column of its profile.
+struct EnvelopeLayout {
+    std::vector<int> cam_of_pos;  // internal index -> caller camera
+    std::vector<int> pos_of_cam;  // caller camera -> internal index
+    std::vector<int> first;       // nblk entries
+    double work_ratio = 1.0;      // profile tile products / dense tile products
+    double density = 1.0;         // camera-graph block density (off-diagonal)
+};
+// Builds the camera graph; false (dense) when it is too small, too large, or
+// denser than max_density (the scan stops there).  Otherwise fills out.
+bool plan_envelope(const gba_problem_view& prob, double max_density, EnvelopeLayout& out);
+// Per-column TRSM / update tables of the profile for launch_batch_cholesky
+// (one problem, index 0; update tile 0 of every column first).
+void envelope_factor_tables(const std::vector<int>& first, std::vector<int2>& trsm, std::vector<int>& trsm_off,
+                            std::vector<int2>& upd, std::vector<int>& upd_off);
+
 // Result of one enqueued-and-synchronised LM trial.
 struct TrialOut {
     TrialFlags flags;
@@ -126,6 +144,12 @@ public:
     void download_step(double* dc, double* dp);
     double time_kernel(int kernel, double lambda, int reps);
     double pair_count() const { return pair_count_; }
+    // GBA_LAYOUT_DENSE or GBA_LAYOUT_ENVELOPE, and the plan's figures
+    void layout(int* lay, double* work_ratio, double* density) const {
+        *lay = env_ ? GBA_LAYOUT_ENVELOPE : GBA_LAYOUT_DENSE;
+        *work_ratio = env_work_;
+        *density = env_density_;
+    }
     int n() const { return pb_.n; }
     int npad() const { return pb_.npad; }
 
@@ -173,6 +197,19 @@ private:
     DevBuf<int> psync_;
     long long pntasks_ = 0;
     int pgrid_ = 1;
+    // envelope layout: cameras renumbered (internal index = pos_of_cam_[caller
+    // camera]), profile per tile row, factor tables for launch_batch_cholesky
+    bool env_ = false;
+    double env_work_ = 1.0, env_density_ = 1.0;
+    std::vector<int> cam_of_pos_, pos_of_cam_;
+    DevBuf<int> first_, eactive_;
+    DevBuf<int2> epotrf_, etrsm_, eupd_;
+    std::vector<int> etrsm_off_, eupd_off_;
+    DevBuf<BatchDesc> edesc_;
+    BatchFactorTables etab_;
+    // caller camera order <-> internal order for 9-value camera records
+    void cams_to_internal(const double* in, double* out) const;
+    void cams_to_caller(const double* in, double* out) const;
     DevBuf<TrialFlags> flags_;
     DevBuf<PassStatus> status_;
     PassStatus* h_status_ = nullptr;

@@ -15,7 +15,11 @@
 //     Pareto(alpha) draw, clamped to kmax; the law's free parameter is solved
 //     by bisection so that E[k] = O / P; the total is then made exactly O by
 //     +-1 moves on uniformly drawn points that stay inside [kmin, kmax].
-//     Each point picks k_p distinct cameras uniformly.
+//     Each point picks k_p distinct cameras uniformly, or, with view_window
+//     W > 0, uniformly among the 2W + 1 cameras within ring distance W of its
+//     home camera -- the camera nearest to the point's azimuth
+//     atan2(x, z) -- so cameras share points only within ring distance 2W
+//     (kmax is then at most 2W + 1).
 //   * Pixels: exact projection + N(0, pixel_noise^2) per coordinate.  The
 //     solver starts from parameters perturbed by x <- x (1 + perturb N(0,1)).
 //   * Observations are emitted camera-major (camera, then point ascending).
@@ -162,7 +166,8 @@ int synth_generate(const gba_synth_spec& sp, uint64_t seed, gba_problem_view* ou
     const int C = sp.num_cameras, P = sp.num_points, O = sp.num_observations;
     if (C <= 0 || P <= 0 || O <= 0) return set_error("synth: sizes must be positive");
     const int kmin = std::max(1, sp.kmin);
-    const int kmax = std::min(sp.kmax, C);
+    const int win = sp.view_window > 0 ? std::min(sp.view_window, (C - 1) / 2) : 0;
+    const int kmax = win > 0 ? std::min({sp.kmax, C, 2 * win + 1}) : std::min(sp.kmax, C);
     if (kmax < kmin) return set_error("synth: kmax < kmin");
     if (static_cast<long long>(P) * kmin > O || static_cast<long long>(P) * kmax < O)
         return set_error("synth: O outside [P*kmin, P*kmax]");
@@ -247,7 +252,22 @@ int synth_generate(const gba_synth_spec& sp, uint64_t seed, gba_problem_view* ou
     std::vector<int> chosen(static_cast<size_t>(O));
     std::vector<int> swaps;
     size_t pos = 0;
+    std::vector<int> window(2 * static_cast<size_t>(win) + 1);
     for (int p = 0; p < P; ++p) {
+        if (win > 0) {
+            // k_p of the 2W + 1 cameras around the home camera
+            const double az = std::atan2(pts[3 * static_cast<size_t>(p)], pts[3 * static_cast<size_t>(p) + 2]);
+            const int home = static_cast<int>(std::lround(az / (2.0 * M_PI) * C));
+            for (int d = -win; d <= win; ++d) window[d + win] = ((home + d) % C + C) % C;
+            for (int j = 0; j < k[p]; ++j) {
+                int r = j + static_cast<int>(rng.below(static_cast<uint32_t>(2 * win + 1 - j)));
+                std::swap(window[j], window[r]);
+                chosen[pos + j] = window[j];
+                cam_count[window[j]]++;
+            }
+            pos += k[p];
+            continue;
+        }
         swaps.clear();
         for (int j = 0; j < k[p]; ++j) {
             int r = j + static_cast<int>(rng.below(static_cast<uint32_t>(C - j)));

@@ -103,7 +103,18 @@ typedef struct gba_solver_config {
                                rho(s) = s (s <= delta^2) or 2 delta sqrt(s) - delta^2,
                                IRLS-weighted linearisation.  0 (default): plain least
                                squares, as solve_bal_gpu. */
+    int factor_layout;      /* enum gba_factor_layout (default AUTO) */
 } gba_solver_config;
+
+/* Layout of the reduced camera system's factor (no reference counterpart;
+ * the reference always factors a dense S, src/lm.cu:597-604, and names a
+ * sparse factor as its next step, :13-17).  ENVELOPE renumbers the cameras by
+ * reverse Cuthill-McKee on the camera graph (cameras sharing a point) and
+ * factors only the tiles inside each tile row's profile; results are returned
+ * in the caller's camera order.  Plain FP64 only.  AUTO takes ENVELOPE for a
+ * plain FP64 solve when the camera graph has block density <= 0.5 and the
+ * profile needs less than half the dense factor's tile products, else DENSE. */
+enum gba_factor_layout { GBA_LAYOUT_AUTO = 0, GBA_LAYOUT_DENSE = 1, GBA_LAYOUT_ENVELOPE = 2 };
 
 /* Trace record per LM trial: iteration, lambda used, trial cost, rho,
  * status (enum below), seconds since solve entry when the trial finished. */
@@ -150,6 +161,11 @@ int gba_solver_time_kernel(gba_solver* s, int kernel, double lambda, int reps, d
 /* Shape facts used for roofline accounting: sum_p k_p(k_p+1)/2, matrix
  * order and padded order of S, and the workgroup counts of the edge passes. */
 int gba_solver_stats(gba_solver* s, double* pair_count, int* n, int* n_padded);
+/* The factor layout the handle uses (enum gba_factor_layout: DENSE or
+ * ENVELOPE), the envelope's share of the dense factor's tile products, and
+ * the camera graph's block density (both 1 when the graph was not examined
+ * or was found too dense).  Any output may be NULL. */
+int gba_solver_layout(gba_solver* s, int* layout, double* factor_work_ratio, double* graph_density);
 void gba_solver_destroy(gba_solver* s);
 
 /* ---- batched solve (no reference counterpart) --------------------------- */
@@ -272,6 +288,16 @@ int gba_load_bal(const char* path, gba_problem_view* out);
 /* BAL text writer (the reference has one only in tools/quantize_bal.cu:99-116). */
 int gba_write_bal(const char* path, const gba_problem_view* prob);
 
+/* The envelope layout the solver would plan for a problem (host only, no
+ * device work; no reference counterpart): *usable = 1 when the camera graph
+ * is sparse enough to plan at all (block density <= max_density), then
+ * cam_order[C] (internal position -> caller camera, reverse Cuthill-McKee),
+ * first[nblk] (first tile column of each 64-row tile row of the renumbered
+ * S, nblk = ceil(9C / 64)), the profile's share of the dense factor's tile
+ * products and the graph's block density.  cam_order / first may be NULL. */
+int gba_plan_envelope(const gba_problem_view* prob, double max_density, int* usable, int* cam_order, int* first,
+                      double* work_ratio, double* density);
+
 /* ---- synthetic problems (no reference counterpart) ---------------------- */
 
 enum gba_visibility_law { GBA_LAW_GEOMETRIC = 0, GBA_LAW_PARETO = 1 };
@@ -284,11 +310,16 @@ typedef struct gba_synth_spec {
     double ring_radius;  /* camera ring radius (default 30) */
     double pixel_noise;  /* pixel noise sigma (1 px) */
     double perturb;      /* relative parameter perturbation (0.01) */
+    int view_window;     /* > 0: a point is seen only by cameras within this ring distance of
+                            the camera nearest to it in azimuth (a camera graph of bounded
+                            degree, as in real reconstructions); 0: by any camera */
 } gba_synth_spec;
 
 /* Preset shapes: 1 (C16/P22106/O83718), 2 (C49/P7776/O31843),
  * 3 (C356/P226730/O1255268), 4 (C1936/P525000/O3400000), 5 (batched: C, P,
- * O drawn from the seed: C~U{8..64}, P~U{2000..20000}, O = round(4.5 P)). */
+ * O drawn from the seed: C~U{8..64}, P~U{2000..20000}, O = round(4.5 P)),
+ * and two with a limited view (sparse camera graph): 6 (C400/P60000/O300000,
+ * view_window 12) and 7 (C2000/P400000/O2000000, view_window 12). */
 int gba_synth_preset(int config, unsigned long long seed, gba_synth_spec* out);
 int gba_synth_generate(const gba_synth_spec* spec, unsigned long long seed, gba_problem_view* out);
 /* The reference's data-free end-to-end problem (tests/test_launch_bounds.cpp
