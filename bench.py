@@ -145,6 +145,55 @@ def run_fp16(g, args):
     print(json.dumps(out))
 
 
+def run_sparse(g, args):
+    """Sparse camera graph (preset --config, default 7: C=2,000, P=400,000,
+    O=2M, each point seen within a ring window of 12 cameras): the plain FP64
+    solve with the envelope factor layout against the dense layout on the same
+    problem -- solve time (inputs resident), factor time per trial (HIP events
+    on the solver stream), final costs."""
+    import torch
+    cfg = args.config if args.config in (6, 7) else 7
+    prob = g.generate(cfg, args.seed)
+    opts = g.LMOptions(initial_lambda=args.lambda0)
+    out = {}
+    for name, lay in (("envelope", g.LAYOUT_ENVELOPE), ("dense", g.LAYOUT_DENSE)):
+        t = time.perf_counter()
+        s = g.Solver(prob, g.PRECISION_FP64, layout=lay)
+        setup = time.perf_counter() - t
+        lay_id, ratio, dens = s.layout()
+        for _ in range(args.warmup):
+            s.solve(opts)
+        torch.cuda.synchronize()
+        steps = args.steps if name == "envelope" else max(1, min(args.steps, 3))
+        rs = [s.solve(opts) for _ in range(steps)]
+        fac = s.time_kernel("factor", args.lambda0, 3)
+        sol = s.time_kernel("solve", args.lambda0, 3)
+        r = rs[0]
+        out[name] = {"setup_s": round(setup, 4), "solve_s": round(statistics.median(x.solve_time_s for x in rs), 5),
+                     "samples": steps, "factor_ms": round(fac, 4), "triangular_solves_ms": round(sol, 4),
+                     "final_cost": r.final_cost, "iterations": r.iterations, "trials": r.trials,
+                     "termination": r.termination, "factor_work_vs_dense": round(ratio, 4),
+                     "graph_density": round(dens, 4)}
+        print(f"[sparse] {name}: solve {out[name]['solve_s']} s, factor {fac:.3f} ms", file=sys.stderr, flush=True)
+        s.close()
+        torch.cuda.synchronize()
+    e, d = out["envelope"], out["dense"]
+    res = {
+        "metric": f"LM solve time, preset {cfg} (sparse camera graph), envelope vs dense factor (s)",
+        "value": e["solve_s"], "unit": "s", "n_gpus": 1, "steps": e["samples"], "warmup": args.warmup,
+        "ms_per_step": round(1e3 * e["solve_s"], 3), "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded Snavely generator, view window, seed %d)" % args.seed,
+        "config": {"workload": f"LM solve C={prob.num_cameras} P={prob.num_points} O={prob.num_observations} "
+                               f"(preset {cfg}, view window 12)"},
+        "envelope": e, "dense": d,
+        "speedup_solve": round(d["solve_s"] / e["solve_s"], 2),
+        "speedup_factor": round(d["factor_ms"] / e["factor_ms"], 2),
+        "final_cost_rel_diff": abs(e["final_cost"] - d["final_cost"]) / d["final_cost"],
+    }
+    print(json.dumps(res))
+
+
 def run_bal(g, args):
     """The reference's bench_bal harness (benchmarks/bench_bal.cpp:63-139) on a
     BAL file: the reader-vs-solver initial-cost check (exit 2 on a mismatch)
@@ -218,12 +267,13 @@ def main():
     ap.add_argument("--loss", choices=["none", "huber"], default="none", help="--mode bal")
     ap.add_argument("--delta", type=float, default=1.0, help="--mode bal: Huber threshold")
     ap.add_argument("--pairs", type=int, default=3, help="--mode bal: interleaved FP64 / mixed pairs")
-    ap.add_argument("--mode", choices=["solve", "kernels", "batch", "fp16", "bal"], default="solve",
+    ap.add_argument("--mode", choices=["solve", "kernels", "batch", "fp16", "bal", "sparse"], default="solve",
                     help="kernels: only the per-kernel timing loops plus a 512 MiB device copy "
                          "(short run for rocprofv3 counter passes); batch: the batched solver on "
                          "--batch problems of preset 5 (problems/s) against sequential FP64 solves; "
                          "fp16: preset 4 with FP16 pixel storage, final-cost error vs FP64; "
-                         "bal: the reference's bench_bal harness on --file (interleaved FP64 / mixed pairs)")
+                         "bal: the reference's bench_bal harness on --file (interleaved FP64 / mixed pairs); "
+                         "sparse: preset 7 (or --config 6), envelope vs dense factor layout, plain FP64")
     ap.add_argument("--batch", type=int, default=512, help="problems in --mode batch")
     ap.add_argument("--seq-sample", type=int, default=32,
                     help="problems solved one by one for the sequential rate in --mode batch")
@@ -243,6 +293,9 @@ def main():
         return
     if args.mode == "fp16":
         run_fp16(g, args)
+        return
+    if args.mode == "sparse":
+        run_sparse(g, args)
         return
     if args.mode == "bal":
         if not args.file:
