@@ -204,7 +204,16 @@ struct PassStatus {
     unsigned long long anomalies;  // non-finite edges in this pass
     TrialFlags flags;
     int pcg_stop;      // refinement: the last step found p.q or r.z not > 0
+    int pcg_done;      // refinement: no further step (converged, stopped, or the step limit)
+    int pcg_it;        // refinement: steps counted
+    double pcg_rr;     // refinement: r.r after the last counted step (b.b before the first)
+    double pcg_bb;     // refinement: b.b
 };
+
+// A launch whose done flag (a device scalar, optional) is set does nothing:
+// the refinement enqueues several steps between host synchronisations and
+// the steps after its stopping test become empty launches.
+__device__ __forceinline__ bool pcg_skip(const double* done) { return done && *done != 0.0; }
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
 

@@ -1034,10 +1034,12 @@ __global__ __launch_bounds__(256) void k_trsv_bwd(const T* __restrict__ L, int l
 // workgroup per 64-block sums its partials in a fixed order.
 __global__ __launch_bounds__(256) void k_symv_tiles(const float* __restrict__ A, int ld,
                                                     const double* __restrict__ x,
-                                                    double* __restrict__ rp, double* __restrict__ cp) {
+                                                    double* __restrict__ rp, double* __restrict__ cp,
+                                                    const double* done) {
     __shared__ float s_a[kTile][kTile + 1];
     __shared__ double s_xi[kTile], s_xj[kTile];
     __shared__ double s_col[4][kTile];
+    if (pcg_skip(done)) return;
     const int t = blockIdx.x, tid = threadIdx.x;
     int i = static_cast<int>((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
     while (i * (i + 1) / 2 > t) --i;
@@ -1093,8 +1095,9 @@ __global__ __launch_bounds__(256) void k_symv_tiles(const float* __restrict__ A,
 // the partials with index = g mod 4, the four sums are combined in order.
 __global__ __launch_bounds__(256) void k_symv_reduce(int nblk, const double* __restrict__ rp,
                                                      const double* __restrict__ cp,
-                                                     double* __restrict__ y) {
+                                                     double* __restrict__ y, const double* done) {
     __shared__ double s_part[4][kTile];
+    if (pcg_skip(done)) return;
     const int b = blockIdx.x, lane = threadIdx.x & 63, g = threadIdx.x >> 6;
     double s = 0.0;
 #pragma unroll 4
@@ -1110,8 +1113,10 @@ __global__ __launch_bounds__(256) void k_symv_reduce(int nblk, const double* __r
 __global__ __launch_bounds__(256) void k_dot2(int n, const double* __restrict__ a,
                                               const double* __restrict__ b,
                                               const double* __restrict__ c,
-                                              const double* __restrict__ d, double* __restrict__ blk) {
+                                              const double* __restrict__ d, double* __restrict__ blk,
+                                              const double* done) {
     __shared__ double s0[4], s1[4];
+    if (pcg_skip(done)) return;
     const int i = blockIdx.x * 256 + threadIdx.x;
     double u = 0.0, v = 0.0;
     if (i < n) {
@@ -1131,17 +1136,24 @@ __global__ __launch_bounds__(256) void k_dot2(int n, const double* __restrict__ 
     }
 }
 
-// Refinement scalars kept on the device, so a PCG step needs one host
-// synchronisation (for the convergence test) instead of two:
-// sc[0] = r.z, sc[1] = alpha, sc[2] = beta, sc[3] = stop (p.q or r.z not > 0).
-//   mode 0: s0 = r.z, s1 = b.b of the start; status gets both.
+// Refinement scalars and the stopping test kept on the device, so the host
+// enqueues several PCG steps per synchronisation:
+// sc[0] = r.z, sc[1] = alpha, sc[2] = beta, sc[3] = stop (p.q or r.z not > 0),
+// sc[4] = done, sc[5] = b.b, sc[6] = steps counted, sc[7] = r.r of the last
+// counted step.
+//   mode 0: s0 = r.z, s1 = b.b of the start; done at once when the trial has
+//           failed, b.b is not > 0 or no step is allowed.
 //   mode 1: s0 = p.q -> alpha = r.z / p.q (0 and stop when either is not > 0).
-//   mode 2: s0 = r.z (new), s1 = r.r; status gets both and the stop flag;
-//           unless stopped, beta = r.z (new) / r.z and r.z is replaced.
+//   mode 2: s0 = r.z (new), s1 = r.r; unless stopped, beta = r.z (new) / r.z,
+//           r.z is replaced and the step is counted; done on a stop
+//           (uncounted), a non-finite r.r, r.r <= tol2 b.b or max_it steps.
+// After done, modes 1 and 2 do nothing.  status gets the dots, the flags and
+// the loop state (pcg_stop, pcg_done, pcg_it, pcg_rr).
 __global__ __launch_bounds__(256) void k_pcg_reduce(int nb, const double* __restrict__ blk, int mode,
                                                     double* __restrict__ sc, PassStatus* status,
-                                                    const TrialFlags* flags) {
+                                                    const TrialFlags* flags, double tol2, int max_it) {
     __shared__ double s0[256], s1[256];
+    if (mode != 0 && sc[4] != 0.0) return;
     const int tid = threadIdx.x;
     double u = 0.0, v = 0.0;
     for (int q = tid; q < nb; q += 256) {
@@ -1161,27 +1173,43 @@ __global__ __launch_bounds__(256) void k_pcg_reduce(int nb, const double* __rest
     if (tid != 0) return;
     const double a = s0[0], c = s1[0];
     if (mode == 0) {
+        const TrialFlags f = *flags;
         sc[0] = a;
         sc[3] = 0.0;
+        sc[5] = c;
+        sc[6] = 0.0;
+        sc[7] = c;
+        sc[4] = (f.nonspd || f.chol_fail || f.solve_err || !(c > 0.0) || max_it <= 0) ? 1.0 : 0.0;
     } else if (mode == 1) {
         const bool bad = !(a > 0.0) || !(sc[0] > 0.0);
         sc[1] = bad ? 0.0 : sc[0] / a;
         if (bad) sc[3] = 1.0;
         return;
-    } else if (sc[3] == 0.0) {
+    } else if (sc[3] != 0.0) {
+        sc[4] = 1.0;  // stopped: the step is not counted
+    } else {
         sc[2] = a / sc[0];
         sc[0] = a;
+        const double it = sc[6] + 1.0;
+        sc[6] = it;
+        sc[7] = c;
+        if (!isfinite(c) || c <= tol2 * sc[5] || it >= static_cast<double>(max_it)) sc[4] = 1.0;
     }
     status->dot[0] = a;
     status->dot[1] = c;
     status->flags = *flags;
     status->pcg_stop = sc[3] != 0.0 ? 1 : 0;
+    status->pcg_done = sc[4] != 0.0 ? 1 : 0;
+    status->pcg_it = static_cast<int>(sc[6]);
+    status->pcg_rr = sc[7];
+    status->pcg_bb = sc[5];
 }
 
 // x += alpha p ; r -= alpha q  (alpha = sc[1])
 __global__ void k_pcg_update(int n, const double* __restrict__ sc, double* __restrict__ x,
                              const double* __restrict__ p, double* __restrict__ r,
                              const double* __restrict__ q) {
+    if (sc[4] != 0.0) return;
     const int i = blockIdx.x * 256 + threadIdx.x;
     const double alpha = sc[1];
     if (i < n) {
@@ -1193,6 +1221,7 @@ __global__ void k_pcg_update(int n, const double* __restrict__ sc, double* __res
 // p = z + beta p  (beta = sc[2])
 __global__ void k_pcg_direction(int n, const double* __restrict__ sc, const double* __restrict__ z,
                                 double* __restrict__ p) {
+    if (sc[4] != 0.0) return;
     const int i = blockIdx.x * 256 + threadIdx.x;
     const double beta = sc[2];
     if (i < n) p[i] = z[i] + beta * p[i];
@@ -1674,21 +1703,22 @@ size_t symv_work(int npad) {
 }
 
 void launch_symv(hipStream_t st, const float* A, int ld, int npad, const double* x, double* y,
-                 double* work) {
+                 double* work, const double* done) {
     const int nblk = npad / kTile;
     const int tiles = nblk * (nblk + 1) / 2;
     double* rp = work;
     double* cp = work + static_cast<size_t>(tiles) * kTile;
-    hipLaunchKernelGGL(k_symv_tiles, dim3(tiles), dim3(256), 0, st, A, ld, x, rp, cp);
-    hipLaunchKernelGGL(k_symv_reduce, dim3(nblk), dim3(256), 0, st, nblk, rp, cp, y);
+    hipLaunchKernelGGL(k_symv_tiles, dim3(tiles), dim3(256), 0, st, A, ld, x, rp, cp, done);
+    hipLaunchKernelGGL(k_symv_reduce, dim3(nblk), dim3(256), 0, st, nblk, rp, cp, y, done);
 }
 
 void launch_pcg_dot(hipStream_t st, int n, const double* a, const double* b, const double* c,
                     const double* d, double* blk, int mode, double* sc, PassStatus* status,
-                    const TrialFlags* flags) {
+                    const TrialFlags* flags, double tol2, int max_it) {
     const int grid = residual_partials(n);
-    hipLaunchKernelGGL(k_dot2, dim3(grid), dim3(256), 0, st, n, a, b, c, d, blk);
-    hipLaunchKernelGGL(k_pcg_reduce, dim3(1), dim3(256), 0, st, grid, blk, mode, sc, status, flags);
+    hipLaunchKernelGGL(k_dot2, dim3(grid), dim3(256), 0, st, n, a, b, c, d, blk,
+                       mode == 0 ? static_cast<const double*>(nullptr) : sc + 4);
+    hipLaunchKernelGGL(k_pcg_reduce, dim3(1), dim3(256), 0, st, grid, blk, mode, sc, status, flags, tol2, max_it);
 }
 
 void launch_pcg_update(hipStream_t st, int n, const double* sc, double* x, const double* p, double* r,

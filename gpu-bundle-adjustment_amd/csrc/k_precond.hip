@@ -30,8 +30,9 @@ namespace {
 template <typename T>
 __global__ __launch_bounds__(256) void k_tile_colmv(const T* __restrict__ X, int ld, int mode,
                                                     const double* __restrict__ v,
-                                                    double* __restrict__ part) {
+                                                    double* __restrict__ part, const double* done) {
     __shared__ double s_part[4][kTile];
+    if (pcg_skip(done)) return;
     const int t = blockIdx.x;
     int i = static_cast<int>((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
     while (i * (i + 1) / 2 > t) --i;
@@ -62,8 +63,9 @@ __global__ __launch_bounds__(256) void k_tile_colmv(const T* __restrict__ X, int
 // order.
 __global__ __launch_bounds__(256) void k_tile_colmv_reduce(int nblk, int mode,
                                                            const double* __restrict__ part,
-                                                           double* __restrict__ out) {
+                                                           double* __restrict__ out, const double* done) {
     __shared__ double s_part[4][kTile];
+    if (pcg_skip(done)) return;
     const int b = blockIdx.x, lane = threadIdx.x & 63, g = threadIdx.x >> 6;
     double s = 0.0;
     if (mode == 0) {
@@ -86,15 +88,15 @@ size_t precond_work(int npad) {
 }
 
 void launch_precond_apply(hipStream_t st, const float* X32, int ld, int npad, const double* r,
-                          double* z, double* work) {
+                          double* z, double* work, const double* done) {
     const int nblk = npad / kTile;
     const int tiles = nblk * (nblk + 1) / 2;
     double* part = work;
     double* y = work + static_cast<size_t>(tiles) * kTile;
-    hipLaunchKernelGGL(k_tile_colmv<float>, dim3(tiles), dim3(256), 0, st, X32, ld, 0, r, part);
-    hipLaunchKernelGGL(k_tile_colmv_reduce, dim3(nblk), dim3(256), 0, st, nblk, 0, part, y);
-    hipLaunchKernelGGL(k_tile_colmv<float>, dim3(tiles), dim3(256), 0, st, X32, ld, 1, y, part);
-    hipLaunchKernelGGL(k_tile_colmv_reduce, dim3(nblk), dim3(256), 0, st, nblk, 1, part, z);
+    hipLaunchKernelGGL(k_tile_colmv<float>, dim3(tiles), dim3(256), 0, st, X32, ld, 0, r, part, done);
+    hipLaunchKernelGGL(k_tile_colmv_reduce, dim3(nblk), dim3(256), 0, st, nblk, 0, part, y, done);
+    hipLaunchKernelGGL(k_tile_colmv<float>, dim3(tiles), dim3(256), 0, st, X32, ld, 1, y, part, done);
+    hipLaunchKernelGGL(k_tile_colmv_reduce, dim3(nblk), dim3(256), 0, st, nblk, 1, part, z, done);
 }
 
 }  // namespace gba_amd

@@ -146,15 +146,19 @@ void launch_chol_solve(hipStream_t st, const double* L64, const float* L32, int 
 // y = A x, A symmetric given by the lower triangle of an FP32 row-major
 // matrix (FP64 accumulation); work holds symv_work(npad) doubles.
 size_t symv_work(int npad);
+// done (optional): a device flag that turns the launch into a no-op (the
+// refinement's steps after its stopping test).
 void launch_symv(hipStream_t st, const float* A, int ld, int npad, const double* x, double* y,
-                 double* work);
+                 double* work, const double* done = nullptr);
 int residual_partials(int npad);
 // blk holds 2 residual_partials(n) doubles (launch_pcg_dot).
-// Refinement dot products with device-side scalars sc[4] (see k_pcg_reduce):
-// mode 0 start (r.z, b.b), 1 alpha from p.q, 2 beta from the new r.z and r.r.
+// Refinement dot products with device-side scalars sc[0..7] (see
+// k_pcg_reduce): mode 0 start (r.z, b.b), 1 alpha from p.q, 2 beta from the
+// new r.z and r.r, the step count and the stopping test (r.r <= tol2 b.b,
+// max_it steps); sc[4] is the done flag of the no-op launches.
 void launch_pcg_dot(hipStream_t st, int n, const double* a, const double* b, const double* c,
                     const double* d, double* blk, int mode, double* sc, PassStatus* status,
-                    const TrialFlags* flags);
+                    const TrialFlags* flags, double tol2, int max_it);
 void launch_pcg_update(hipStream_t st, int n, const double* sc, double* x, const double* p, double* r,
                        const double* q);
 void launch_pcg_direction(hipStream_t st, int n, const double* sc, const double* z, double* p);
@@ -167,7 +171,7 @@ void launch_vec_scale(hipStream_t st, int n, int npad, const double* in, const d
 // z = X^T (X r) = (L L^T)^-1 r; work holds precond_work(npad) doubles.
 size_t precond_work(int npad);
 void launch_precond_apply(hipStream_t st, const float* X32, int ld, int npad, const double* r,
-                          double* z, double* work);
+                          double* z, double* work, const double* done = nullptr);
 
 // k_batch.hip
 // Per-problem status of a pass (trial = 0: linearisation SSE and anomalies;

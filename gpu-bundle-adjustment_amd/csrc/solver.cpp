@@ -106,7 +106,7 @@ size_t pcg_work(int npad) {
 
 int pcg_solve(hipStream_t st, const float* A, const float* X32, int ld, int npad, const double* b,
               double* x, double* work, TrialFlags* flags, PassStatus* status, PassStatus* h_status,
-              double tol, int max_it, double* rel) {
+              double tol, int max_it, double* rel, int first_batch) {
     const size_t np = static_cast<size_t>(npad);
     double* r = work;
     double* z = work + np;
@@ -120,42 +120,40 @@ int pcg_solve(hipStream_t st, const float* A, const float* X32, int ld, int npad
         GBA_HIP(hipMemcpyAsync(h_status, status, sizeof(PassStatus), hipMemcpyDeviceToHost, st));
         GBA_HIP(hipStreamSynchronize(st));
     };
+    const double* done = sc + 4;
     launch_fill_double(st, x, np, 0.0);
     launch_vec_copy(st, npad, b, r);
     launch_precond_apply(st, X32, ld, npad, r, z, sw);
     launch_vec_copy(st, npad, z, p);
-    launch_pcg_dot(st, npad, r, z, b, b, blk, 0, sc, status, flags);
-    sync_status();
-    // a trial that failed before the solve runs no refinement: no residual
-    *rel = 0.0;
-    const TrialFlags f = h_status->flags;
-    if (f.nonspd || f.chol_fail || f.solve_err) return 0;
-    const double bb = h_status->dot[1];
-    if (!(bb > 0.0)) {
-        *rel = 0.0;
-        return 0;
-    }
-    double rr = bb;
-    int it = 0;
-    // alpha and beta stay on the device; one synchronisation per step reads
-    // r.r for the stopping test.  A step whose p.q or r.z is not > 0 applies
-    // alpha = 0 (x and r unchanged) and ends the loop uncounted.
-    while (it < max_it) {
-        launch_symv(st, A, ld, npad, p, q, sw);
-        launch_pcg_dot(st, npad, p, q, p, q, blk, 1, sc, status, flags);
-        launch_pcg_update(st, npad, sc, x, p, r, q);
-        launch_precond_apply(st, X32, ld, npad, r, z, sw);
-        launch_pcg_dot(st, npad, r, z, r, r, blk, 2, sc, status, flags);
+    launch_pcg_dot(st, npad, r, z, b, b, blk, 0, sc, status, flags, tol * tol, max_it);
+    // Steps are enqueued in batches with one synchronisation per batch: the
+    // stopping test runs on the device (k_pcg_reduce), and the steps of a
+    // batch after it has fired are empty launches.  alpha = 0 on a step whose
+    // p.q or r.z is not > 0 (x and r unchanged), which ends the loop uncounted.
+    int enq = 0, batch = std::max(1, first_batch);
+    for (;;) {
+        const int nb = std::max(0, std::min(batch, max_it - enq));
+        for (int k = 0; k < nb; ++k) {
+            launch_symv(st, A, ld, npad, p, q, sw, done);
+            launch_pcg_dot(st, npad, p, q, p, q, blk, 1, sc, status, flags, tol * tol, max_it);
+            launch_pcg_update(st, npad, sc, x, p, r, q);
+            launch_precond_apply(st, X32, ld, npad, r, z, sw, done);
+            launch_pcg_dot(st, npad, r, z, r, r, blk, 2, sc, status, flags, tol * tol, max_it);
+            launch_pcg_direction(st, npad, sc, z, p);
+        }
+        enq += nb;
         sync_status();
-        if (h_status->pcg_stop) break;
-        ++it;
-        rr = h_status->dot[1];
-        if (!std::isfinite(rr)) break;
-        if (rr <= tol * tol * bb) break;
-        launch_pcg_direction(st, npad, sc, z, p);
+        if (h_status->pcg_done || enq >= max_it) break;
+        batch = 1;
     }
-    *rel = std::sqrt(rr / bb);
-    return it;
+    // a trial that failed before the solve runs no refinement: no residual
+    const TrialFlags f = h_status->flags;
+    *rel = 0.0;
+    if (f.nonspd || f.chol_fail || f.solve_err) return 0;
+    const double bb = h_status->pcg_bb;
+    if (!(bb > 0.0)) return 0;
+    *rel = std::sqrt(h_status->pcg_rr / bb);
+    return h_status->pcg_it;
 }
 
 void layout_units(const std::vector<int>& pt_off, std::vector<int2>& units, double& pair_count) {
@@ -559,8 +557,11 @@ void Solver::enqueue_trial_linear_algebra(double lambda, std::vector<double>* ho
     enqueue_factor();
     const double tol = cfg_.refine_tol > 0 ? cfg_.refine_tol : 1e-10;
     const int max_it = cfg_.refine_max_iters > 0 ? cfg_.refine_max_iters : 12;
+    // first batch: the previous trial's step count (the counts of one solve
+    // differ by about one)
     out.refine_iters = pcg_solve(st_, S32low_.p, X32_.p, ld, pb_.npad, rhs_.p, x_.p, pcgw_.p, flags_.p,
-                                 status_.p, h_status_, tol, max_it, &out.refine_rel);
+                                 status_.p, h_status_, tol, max_it, &out.refine_rel, last_refine_);
+    if (out.refine_iters > 0) last_refine_ = out.refine_iters;
     launch_vec_scale(st_, pb_.n, pb_.npad, x_.p, dscale_.p, dc_.p);
 }
 

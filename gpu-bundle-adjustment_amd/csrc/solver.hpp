@@ -84,11 +84,13 @@ void build_schur_tasks_device(hipStream_t st, int C, int cb, const std::vector<i
 // through the explicit FP32 inverse X = L^-1 (launch_cholesky with X32).  Vectors
 // have npad entries.  Stops at |r| <= tol |b| or after max_it steps; returns
 // the steps taken and the final |r| / |b| in *rel.  work holds pcg_work(npad)
-// doubles.  Synchronises the stream twice per step.
+// doubles.  The stopping test runs on the device; steps are enqueued
+// first_batch at a time first, then one at a time, with one synchronisation
+// per batch (the steps of a batch past the stop are empty launches).
 size_t pcg_work(int npad);
 int pcg_solve(hipStream_t st, const float* A, const float* X32, int ld, int npad, const double* b,
               double* x, double* work, TrialFlags* flags, PassStatus* status, PassStatus* h_status,
-              double tol, int max_it, double* rel);
+              double tol, int max_it, double* rel, int first_batch = 1);
 
 // Envelope layout of S (camera_graph.cpp): the cameras' internal order
 // (reverse Cuthill-McKee on the camera graph), and per tile row of the
@@ -189,6 +191,7 @@ private:
     DevBuf<double> S64_, D64_;
     DevBuf<float> S32_, S32low_, D32_, X32_, T32_;
     DevBuf<double> rhs_, y_, x_, dc_, dscale_, pcgw_;
+    int last_refine_ = 3;  // refinement steps of the last trial (first batch of the next)
     DevBuf<int> sync_;
     // one-launch factor (GBA_FACTOR=persistent; the default is the launch
     // per tile column): per-column first tickets, flags, grid
