@@ -27,6 +27,7 @@
 // diagonal block is applied as a product with the stored inverse tile.  Waits
 // are bounded (TrialFlags::solve_err on time-out), so every wave exits.
 #include <algorithm>
+#include <cstdlib>
 #include <vector>
 
 #include "device_common.hpp"
@@ -262,17 +263,19 @@ __device__ __forceinline__ void tile_to_lds(T (*s)[kTile + 2], const T* __restri
 // Roles of launch k: row k of X (k + 1 tiles), then the R tiles.
 __host__ __device__ constexpr int inv_roles(int k, int nblk) { return (k + 1) + (nblk - 1 - k) * k; }
 
-// acc = A B for two 64x64 tiles staged in LDS.
+// acc = A B (or acc += A B with add set) for two 64x64 tiles staged in LDS.
 template <typename T>
 __device__ __forceinline__ void acc_lds_lds(typename Acc4<T>::type (&acc)[2][2], const T (*sA)[kTile + 2],
-                                            const T (*sB)[kTile + 2]) {
+                                            const T (*sB)[kTile + 2], bool add = false) {
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
     const int li = lane & 15, lk = lane >> 4;
+    if (!add) {
 #pragma unroll
-    for (int r = 0; r < 2; ++r)
+        for (int r = 0; r < 2; ++r)
 #pragma unroll
-        for (int c = 0; c < 2; ++c) acc[r][c] = typename Acc4<T>::type{0, 0, 0, 0};
+            for (int c = 0; c < 2; ++c) acc[r][c] = typename Acc4<T>::type{0, 0, 0, 0};
+    }
 #pragma unroll 4
     for (int kk = 0; kk < kTile; kk += 4) {
         const T a0 = sA[wr + li][kk + lk], a1 = sA[wr + 16 + li][kk + lk];
@@ -288,9 +291,16 @@ __device__ __forceinline__ void acc_lds_lds(typename Acc4<T>::type (&acc)[2][2],
 // two LDS tiles of the calling kernel.
 // kCoh: the one-launch factor's form -- R read and X / R written with
 // agent-scope atomic accesses (other workgroups of the same launch read them).
+// Grouped factor (kg < k: the launch closes the group of tile columns kg ..
+// k): R holds the terms m <= kg - 2 only, and a role takes the terms
+// m = max(j, kg - 1) .. k - 1 itself -- X_kj = -Linv_kk (R_kj + sum_m L_km
+// X_mj), R_ij += sum_m L_im X_mj -- so inside a group the launches form the
+// rows of X without touching R, and the launch that closes the group adds the
+// group's terms to R at once.  kg = k is the column-by-column form (one term).
 template <typename T, bool kCoh = false>
 __device__ void inv_role(int r, int k, int nblk, const T* __restrict__ S, int ld, const T* __restrict__ DinvT,
-                         T* __restrict__ X, T* __restrict__ R, T (*sa)[kTile + 2], T (*sb)[kTile + 2]) {
+                         T* __restrict__ X, T* __restrict__ R, T (*sa)[kTile + 2], T (*sb)[kTile + 2],
+                         int kg) {
     auto put = [](T* p, T v) {
         if constexpr (kCoh) __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         else *p = v;
@@ -324,10 +334,11 @@ __device__ void inv_role(int r, int k, int nblk, const T* __restrict__ S, int ld
         i = k + 1 + b / k;
         j = b % k;
     }
-    // stage A = L_{i,k-1} (from its transposed copy in the upper tile
-    // (k-1, i), which every factor form writes) and B = X_{k-1,j} (lower part
-    // only for a diagonal tile of X), all loads in flight at once
-    {
+    acc_t acc[2][2];
+    if constexpr (kCoh) {
+        // one-launch factor (kg = k): the one term L_{i,k-1} X_{k-1,j},
+        // staged through LDS -- A from the transposed copy in the upper tile
+        // (k-1, i), B = X_{k-1,j} (lower part only for a diagonal tile of X)
         const T* A = S + static_cast<size_t>((k - 1) * kTile) * ld + i * kTile;
         const T* B = X + static_cast<size_t>((k - 1) * kTile) * ld + j * kTile;
         const bool bl = (j == k - 1);
@@ -348,12 +359,42 @@ __device__ void inv_role(int r, int k, int nblk, const T* __restrict__ S, int ld
                 sb[rr][c] = (bl && c > rr) ? T(0) : tb[u];
             }
         }
+        __syncthreads();
+        acc_lds_lds<T>(acc, sa, sb);
+    } else {
+        // acc = sum over m of L_{i,m} X_{m,j}, the MFMA operands read
+        // straight from memory: A = L_{i,m} from its transposed copy in the
+        // upper tile (m, i), which every factor form writes (16 lanes read 16
+        // consecutive values), B = X_{m,j} by rows (lower part only for a
+        // diagonal tile of X)
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int c = 0; c < 2; ++c) acc[a][c] = acc_t{0, 0, 0, 0};
+        const int mlo = j > kg - 1 ? j : kg - 1;
+        for (int m = mlo; m < k; ++m) {
+            const T* LT = S + static_cast<size_t>(m * kTile) * ld + i * kTile;
+            const T* XB = X + static_cast<size_t>(m * kTile) * ld + j * kTile;
+            const bool bl = (j == m);
+#pragma unroll 4
+            for (int kk = 0; kk < kTile; kk += 4) {
+                const int row = kk + lk;
+                const size_t ro = static_cast<size_t>(row) * ld;
+                const T a0 = LT[ro + wr + li], a1 = LT[ro + wr + 16 + li];
+                T b0 = XB[ro + wc + li], b1 = XB[ro + wc + 16 + li];
+                if (bl) {
+                    b0 = (wc + li > row) ? T(0) : b0;
+                    b1 = (wc + 16 + li > row) ? T(0) : b1;
+                }
+                acc[0][0] = mfma16(a0, b0, acc[0][0]);
+                acc[0][1] = mfma16(a0, b1, acc[0][1]);
+                acc[1][0] = mfma16(a1, b0, acc[1][0]);
+                acc[1][1] = mfma16(a1, b1, acc[1][1]);
+            }
+        }
     }
-    __syncthreads();
-    acc_t acc[2][2];
-    acc_lds_lds<T>(acc, sa, sb);
     T* Rij = R + static_cast<size_t>(i * kTile) * ld + j * kTile;
-    if (j < k - 1) {
+    if (j <= kg - 2) {
         T part[2][2][4];
 #pragma unroll
         for (int a = 0; a < 2; ++a)
@@ -433,7 +474,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     __shared__ T sa[kTile][kTile + 2];
     __shared__ T sb[kTile][kTile + 2];
     if (any_fail(flags)) return;
-    inv_role<T>(blockIdx.x, nblk - 1, nblk, S, ld, DinvT, X, R, sa, sb);
+    inv_role<T>(blockIdx.x, nblk - 1, nblk, S, ld, DinvT, X, R, sa, sb, nblk - 1);
 }
 
 // L_ik = A_ik (L_kk^-1)^T for tile rows i > k: an MFMA GEMM per tile, A_ik
@@ -529,10 +570,16 @@ __global__ __launch_bounds__(256) void k_trsm_tiles(T* __restrict__ S, int ld, i
 // from the raw tiles below the diagonal and Linv_kk, and the workgroup of
 // column k+1 writes L_ik^T to the upper tile (k, i), the only copy later
 // launches read (the lower tiles keep A).
+// Grouped form (large matrices, fused off): the update applies the panel
+// columns kc0 .. k together, A_ij -= sum_c L_ic L_jc^T, and covers the tiles
+// (i, j), k < j <= i, of the first jw + 1 trailing columns only (jw >= the
+// trailing width: all of them), so a group of panel columns reads and writes
+// the trailing matrix once instead of once per column.  kg: the first column
+// of the launch's group, for the inverse roles (inv_role; kg = k otherwise).
 template <typename T>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 2))) void k_chol_update(
     T* __restrict__ S, int ld, int k, T* __restrict__ D, T* __restrict__ DinvT, TrialFlags* flags, TileBatch tb,
-    int nup, int nblk, T* __restrict__ X, T* __restrict__ R, int fused) {
+    int nup, int nblk, T* __restrict__ X, T* __restrict__ R, int fused, int kc0, int jw, int kg) {
     __shared__ T Ai[kTile][kTile + 2];
     __shared__ T Aj[kTile][kTile + 2];
     int t;
@@ -541,17 +588,26 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
         if (X && t >= nup) {
             // workgroups past the update tiles form row k of X = L^-1 (see inv_role)
             if (any_fail(flags)) return;
-            inv_role<T>(t - nup, k, nblk, S, ld, DinvT, X, R, Ai, Aj);
+            inv_role<T>(t - nup, k, nblk, S, ld, DinvT, X, R, Ai, Aj, kg);
             return;
         }
     }
     // the failure check below comes after this workgroup's first loads are
     // issued (they are in bounds either way), so its latency overlaps them
     // tile index -> (ii, jj), jj <= ii, over the trailing lower triangle
-    int ii = static_cast<int>((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
-    while (ii * (ii + 1) / 2 > t) --ii;
-    while ((ii + 1) * (ii + 2) / 2 <= t) ++ii;
-    const int jj = t - ii * (ii + 1) / 2;
+    // (rows of the triangle of the first jw + 1 columns, then rows of jw + 1
+    // tiles)
+    int ii, jj;
+    if (t < (jw + 1) * (jw + 2) / 2) {
+        ii = static_cast<int>((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+        while (ii * (ii + 1) / 2 > t) --ii;
+        while ((ii + 1) * (ii + 2) / 2 <= t) ++ii;
+        jj = t - ii * (ii + 1) / 2;
+    } else {
+        const int t2 = t - (jw + 1) * (jw + 2) / 2;
+        ii = jw + 1 + t2 / (jw + 1);
+        jj = t2 % (jw + 1);
+    }
     const int i = k + 1 + ii, j = k + 1 + jj;
     const T* LTi = S + static_cast<size_t>(k * kTile) * ld + i * kTile;
     const T* LTj = S + static_cast<size_t>(k * kTile) * ld + j * kTile;
@@ -664,15 +720,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
         __syncthreads();  // every wave is done with Ai / Aj before the lookahead reuses them
     } else {
         if (any_fail(flags)) return;
+        for (int c = kc0; c <= k; ++c) {
+            // panel column c: its transposed tiles L_ic^T, L_jc^T sit in the upper tiles (c, i), (c, j)
+            const T* Ti = LTi - static_cast<size_t>((k - c) * kTile) * ld;
+            const T* Tj = LTj - static_cast<size_t>((k - c) * kTile) * ld;
 #pragma unroll
-        for (int kk = 0; kk < kTile; kk += 4) {
-            const size_t ro = static_cast<size_t>(kk + lk) * ld;
-            const T a0 = LTi[ro + wr + li], a1 = LTi[ro + wr + 16 + li];
-            const T b0 = LTj[ro + wc + li], b1 = LTj[ro + wc + 16 + li];
-            acc[0][0] = mfma16(a0, b0, acc[0][0]);
-            acc[0][1] = mfma16(a0, b1, acc[0][1]);
-            acc[1][0] = mfma16(a1, b0, acc[1][0]);
-            acc[1][1] = mfma16(a1, b1, acc[1][1]);
+            for (int kk = 0; kk < kTile; kk += 4) {
+                const size_t ro = static_cast<size_t>(kk + lk) * ld;
+                const T a0 = Ti[ro + wr + li], a1 = Ti[ro + wr + 16 + li];
+                const T b0 = Tj[ro + wc + li], b1 = Tj[ro + wc + 16 + li];
+                acc[0][0] = mfma16(a0, b0, acc[0][0]);
+                acc[0][1] = mfma16(a0, b1, acc[0][1]);
+                acc[1][0] = mfma16(a1, b0, acc[1][0]);
+                acc[1][1] = mfma16(a1, b1, acc[1][1]);
+            }
         }
     }
     if (t == 0) {
@@ -1546,7 +1607,7 @@ __global__ __launch_bounds__(256) void k_chol_persistent(T* __restrict__ S, int 
                 break;
             default: {
                 const int r = p.type == kPXDiag ? p.k : (p.type == kPXRow ? p.j : (p.k + 1) + (p.i - p.k - 1) * p.k + p.j);
-                inv_role<T, true>(r, p.k, nblk, S, ld, DinvT, X, R, buf0, buf1);
+                inv_role<T, true>(r, p.k, nblk, S, ld, DinvT, X, R, buf0, buf1, p.k);
                 break;
             }
         }
@@ -1558,40 +1619,87 @@ __global__ __launch_bounds__(256) void k_chol_persistent(T* __restrict__ S, int 
 
 }  // namespace
 
+// Number of tiles of a trailing update below tile column k (m = nblk - k - 1
+// tiles below the diagonal) covering the first jw + 1 trailing columns.
+static int update_tiles(int m, int jw) {
+    if (jw >= m - 1) return m * (m + 1) / 2;
+    return (jw + 1) * (jw + 2) / 2 + (m - 1 - jw) * (jw + 1);
+}
+
+// Grouping of the right-looking factor for large matrices: while more than
+// tail tile columns would remain after it, a group of g panel columns takes
+// its TRSMs and the updates inside the group column by column, then one
+// update of the rest of the trailing matrix by the whole group (rank 64 g).
+// Environment: GBA_FACTOR_GROUP (g, default 8; 1: never), GBA_FACTOR_GROUP_TAIL
+// (tile columns left to the column-by-column form, default 64).  Measured at
+// n = 17,424 (272 tile columns): FP64 factor 101.8 ms column by column, 48.1
+// with g = 4, 42.3 with g = 8, 41.7 with g = 12; the FP32 factor with the
+// explicit inverse 96 -> 65 ms.
+static void factor_grouping(int& g, int& tail) {
+    g = 8;
+    tail = 64;
+    if (const char* e = std::getenv("GBA_FACTOR_GROUP")) g = std::max(1, std::atoi(e));
+    if (const char* e = std::getenv("GBA_FACTOR_GROUP_TAIL")) tail = std::max(0, std::atoi(e));
+}
+
+template <typename T>
+static void factor_columns(hipStream_t st, T* S, int ld, int nblk, T* D, T* Dinv, TrialFlags* flags, T* X,
+                           T* R, int fuse_cols) {
+    const TileBatch none{nullptr, nullptr, nullptr};
+    int g = 8, tail = 64;
+    factor_grouping(g, tail);
+    auto trsm = [&](int k) {
+        hipLaunchKernelGGL(k_trsm_tiles<T>, dim3(nblk - k - 1), dim3(256), 0, st, S, ld, k, D, Dinv, flags, none);
+    };
+    // update of the trailing tiles from panel columns kc0 .. k, first jw + 1
+    // trailing columns; hosts the inverse roles of column k when X is set
+    // (in a group kg .. : row k of X only, or, in the launch that closes the
+    // group, row k and the group's terms of R)
+    auto update = [&](int kc0, int k, int jw, int fused, int kg, bool rterms) {
+        const int m = nblk - k - 1, nup = update_tiles(m, jw);
+        const int grid = nup + (X ? (rterms ? inv_roles(k, nblk) : k + 1) : 0);
+        hipLaunchKernelGGL(k_chol_update<T>, dim3(grid), dim3(256), 0, st, S, ld, k, D, Dinv, flags, none, nup,
+                           nblk, X, R, fused, kc0, std::min(jw, m - 1), kg);
+    };
+    hipLaunchKernelGGL(k_potrf_tile<T>, dim3(1), dim3(256), 0, st, S, ld, 0, D, Dinv, flags, none);
+    int k = 0;
+    while (k + 1 < nblk) {
+        if (g > 1 && nblk - (k + g) > tail) {
+            const int k0 = k;
+            for (int c = k0; c < k0 + g; ++c) {
+                trsm(c);
+                // columns c+1 .. k0+g-1 of the group (none after the last)
+                if (c + 1 < k0 + g) update(c, c, k0 + g - 2 - c, 0, k0, false);
+            }
+            update(k0, k0 + g - 1, nblk, 0, k0, true);
+            k = k0 + g;
+            continue;
+        }
+        const int m = nblk - k - 1;
+        const int fused = m <= fuse_cols ? 1 : 0;
+        if (!fused) trsm(k);
+        update(k, k, nblk, fused, k, true);
+        ++k;
+    }
+    if (X)
+        hipLaunchKernelGGL(k_inv_last_row<T>, dim3(inv_roles(nblk - 1, nblk)), dim3(256), 0, st,
+                           static_cast<const T*>(S), ld, nblk, static_cast<const T*>(Dinv), X, R,
+                           static_cast<const TrialFlags*>(flags));
+}
+
 void launch_cholesky(hipStream_t st, double* S64, float* S32, int ld, int nblk, double* D64,
                      float* D32, TrialFlags* flags, float* X32, float* R32) {
     const size_t tile = static_cast<size_t>(kTile) * kTile;
-    const TileBatch none{nullptr, nullptr, nullptr};
     if (S64) {
-        double* Dinv = D64 + static_cast<size_t>(nblk) * tile;
-        double* const nd = nullptr;
-        hipLaunchKernelGGL(k_potrf_tile<double>, dim3(1), dim3(256), 0, st, S64, ld, 0, D64, Dinv, flags, none);
-        for (int k = 0; k + 1 < nblk; ++k) {
-            const int m = nblk - k - 1, nup = m * (m + 1) / 2;
-            hipLaunchKernelGGL(k_trsm_tiles<double>, dim3(m), dim3(256), 0, st, S64, ld, k, D64, Dinv, flags, none);
-            hipLaunchKernelGGL(k_chol_update<double>, dim3(nup), dim3(256), 0, st, S64, ld, k, D64, Dinv, flags,
-                               none, nup, nblk, nd, nd, 0);
-        }
+        factor_columns<double>(st, S64, ld, nblk, D64, D64 + static_cast<size_t>(nblk) * tile, flags, nullptr,
+                               nullptr, 0);
     } else {
-        float* Dinv = D32 + static_cast<size_t>(nblk) * tile;
-        hipLaunchKernelGGL(k_potrf_tile<float>, dim3(1), dim3(256), 0, st, S32, ld, 0, D32, Dinv, flags, none);
         // the last columns (at most kFuseCols tiles below the diagonal) take
         // their TRSM inside the update (measured at n = 3,204: 1.32 -> 1.26 ms;
         // fusing every column, 1.30 ms: the early updates have many tiles)
         constexpr int kFuseCols = 32;
-        for (int k = 0; k + 1 < nblk; ++k) {
-            const int m = nblk - k - 1, nup = m * (m + 1) / 2;
-            const int grid = nup + (X32 ? inv_roles(k, nblk) : 0);
-            const int fused = m <= kFuseCols ? 1 : 0;
-            if (!fused)
-                hipLaunchKernelGGL(k_trsm_tiles<float>, dim3(m), dim3(256), 0, st, S32, ld, k, D32, Dinv, flags, none);
-            hipLaunchKernelGGL(k_chol_update<float>, dim3(grid), dim3(256), 0, st, S32, ld, k, D32, Dinv, flags,
-                               none, nup, nblk, X32, R32, fused);
-        }
-        if (X32)
-            hipLaunchKernelGGL(k_inv_last_row<float>, dim3(inv_roles(nblk - 1, nblk)), dim3(256), 0, st,
-                               static_cast<const float*>(S32), ld, nblk, static_cast<const float*>(Dinv), X32, R32,
-                               static_cast<const TrialFlags*>(flags));
+        factor_columns<float>(st, S32, ld, nblk, D32, D32 + static_cast<size_t>(nblk) * tile, flags, X32, R32,
+                              kFuseCols);
     }
 }
 
@@ -1657,7 +1765,7 @@ void launch_batch_cholesky(hipStream_t st, int count, const BatchFactorTables& t
         if (nup > 0) {
             tb.tab = t.upd + t.upd_off[k];
             hipLaunchKernelGGL(k_chol_update<double>, dim3(nup), dim3(256), 0, st, S, 0, k, D, D, flags, tb, nup, 0,
-                               static_cast<double*>(nullptr), static_cast<double*>(nullptr), 0);
+                               static_cast<double*>(nullptr), static_cast<double*>(nullptr), 0, k, 1 << 14, k);
         }
     }
 }

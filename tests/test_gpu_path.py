@@ -193,6 +193,44 @@ def test_dense_spd_solve_mixed_refines_to_fp64(gba, n):
     assert np.linalg.norm(A @ x - b) <= 1e-10 * np.linalg.norm(b)
 
 
+# Grouped factor (large matrices): a group of panel columns takes its TRSMs
+# and in-group updates column by column, then one update of the rest of the
+# trailing matrix by the whole group.  Forced on small matrices here
+# (GBA_FACTOR_GROUP_TAIL=0), every group width, both precisions; and at its
+# default threshold on a matrix large enough to use it (n = 5,000: 79 tile
+# columns, three groups of 4, then column by column).
+@pytest.mark.parametrize("group", [2, 3, 4])
+@pytest.mark.parametrize("n", [441, 1000, 2000])
+def test_dense_spd_solve_grouped_factor(gba, monkeypatch, group, n):
+    monkeypatch.setenv("GBA_FACTOR_GROUP", str(group))
+    monkeypatch.setenv("GBA_FACTOR_GROUP_TAIL", "0")
+    rng = np.random.default_rng(n + group)
+    M = rng.uniform(-1, 1, (n, n))
+    A = M.T @ M + n * 0.05 * np.eye(n) + np.eye(n)
+    b = rng.normal(size=n)
+    x, ok, _ = gba.dense_spd_solve(A, b)
+    assert ok
+    assert arel(x, np.linalg.solve(A, b)) < 1e-10
+    A32 = A.astype(np.float32).astype(np.float64)
+    x, ok, it = gba.dense_spd_solve(A32, b, gba.PRECISION_MIXED)
+    assert ok and 1 <= it <= 6
+    assert np.linalg.norm(A32 @ x - b) <= 1e-10 * np.linalg.norm(b)
+
+
+def test_dense_spd_solve_grouped_at_default_threshold(gba, monkeypatch):
+    n = 5000
+    rng = np.random.default_rng(11)
+    M = rng.uniform(-1, 1, (n, n))
+    A = M.T @ M + n * 0.05 * np.eye(n) + np.eye(n)
+    b = rng.normal(size=n)
+    x, ok, _ = gba.dense_spd_solve(A, b)
+    assert ok
+    assert arel(x, np.linalg.solve(A, b)) < 1e-10
+    monkeypatch.setenv("GBA_FACTOR_GROUP", "1")
+    x1, ok1, _ = gba.dense_spd_solve(A, b)
+    assert ok1 and arel(x, x1) < 1e-11
+
+
 def test_dense_spd_solve_rejects_indefinite(gba):
     A = np.eye(100)
     A[70, 70] = -1.0
