@@ -335,10 +335,11 @@ __device__ void inv_role(int r, int k, int nblk, const T* __restrict__ S, int ld
         j = b % k;
     }
     acc_t acc[2][2];
-    if constexpr (kCoh) {
-        // one-launch factor (kg = k): the one term L_{i,k-1} X_{k-1,j},
-        // staged through LDS -- A from the transposed copy in the upper tile
-        // (k-1, i), B = X_{k-1,j} (lower part only for a diagonal tile of X)
+    if (kCoh || kg == k) {
+        // column-by-column form (and the one-launch factor): the one term
+        // L_{i,k-1} X_{k-1,j}, staged through LDS -- A from the transposed
+        // copy in the upper tile (k-1, i), B = X_{k-1,j} (lower part only for
+        // a diagonal tile of X)
         const T* A = S + static_cast<size_t>((k - 1) * kTile) * ld + i * kTile;
         const T* B = X + static_cast<size_t>((k - 1) * kTile) * ld + j * kTile;
         const bool bl = (j == k - 1);
