@@ -558,9 +558,14 @@ void Solver::enqueue_trial_linear_algebra(double lambda, std::vector<double>* ho
     const double tol = cfg_.refine_tol > 0 ? cfg_.refine_tol : 1e-10;
     const int max_it = cfg_.refine_max_iters > 0 ? cfg_.refine_max_iters : 12;
     // first batch: the previous trial's step count (the counts of one solve
-    // differ by about one)
+    // differ by about one); GBA_PCG_BATCH=1 in the environment enqueues one
+    // step per synchronisation
+    static const bool one_step = [] {
+        const char* e = std::getenv("GBA_PCG_BATCH");
+        return e && std::atoi(e) == 1;
+    }();
     out.refine_iters = pcg_solve(st_, S32low_.p, X32_.p, ld, pb_.npad, rhs_.p, x_.p, pcgw_.p, flags_.p,
-                                 status_.p, h_status_, tol, max_it, &out.refine_rel, last_refine_);
+                                 status_.p, h_status_, tol, max_it, &out.refine_rel, one_step ? 1 : last_refine_);
     if (out.refine_iters > 0) last_refine_ = out.refine_iters;
     launch_vec_scale(st_, pb_.n, pb_.npad, x_.p, dscale_.p, dc_.p);
 }
