@@ -258,6 +258,7 @@ def main():
     ap.add_argument("--lambda0", type=float, default=None,
                     help="initial LM damping (default 1e-4; 1e-3 in --mode bal, as bench_bal)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-large", action="store_true", help="skip the preset-4 factor roofline")
     ap.add_argument("--cpu-iterations", type=int, default=2)
     ap.add_argument("--cpu-samples", type=int, default=5, help="interleaved CPU baseline samples")
     ap.add_argument("--kernel-reps", type=int, default=10)
@@ -447,6 +448,32 @@ def main():
         "roofline": roof,
         "roofline_schur": roof_schur,
     }
+
+    if not args.no_large and args.config == 3:
+        # The dense factor where it is throughput-bound rather than
+        # latency-bound: preset 4 (C=1,936, n = 17,424), the grouped form
+        # (8 tile columns per group), HIP events around the factor launches
+        # of one factorisation (S re-assembled before each repetition).
+        pl = g.generate(4, args.seed)
+        big = {}
+        for name, prec in (("fp64", g.PRECISION_FP64), ("mixed", g.PRECISION_MIXED)):
+            sl = g.Solver(pl, prec)
+            big[name] = sl.time_kernel("factor", args.lambda0, 3)
+            n4 = sl.stats()["n"]
+            sl.close()
+            torch.cuda.synchronize()
+        fl4 = n4 ** 3 / 3.0
+        tf64 = fl4 / (big["fp64"] * 1e-3) / 1e12
+        tf32 = fl4 / (big["mixed"] * 1e-3) / 1e12
+        out["roofline_factor_preset4"] = {
+            "kernel": f"dense Cholesky of S at preset 4 (n = {n4}), grouped form", "bound": "mfma",
+            "basis": f"n^3/3 = {fl4:.4e} flop per factorisation",
+            "fp64": {"ms": round(big["fp64"], 3), "achieved": round(tf64, 2), "peak": PEAK["fp64_tflops"],
+                     "unit": "TFLOP/s", "frac": round(tf64 / PEAK["fp64_tflops"], 4)},
+            "mixed": {"ms": round(big["mixed"], 3), "achieved": round(tf32, 2), "peak": PEAK["fp32_tflops"],
+                      "unit": "TFLOP/s", "frac": round(tf32 / PEAK["fp32_tflops"], 4),
+                      "note": "the same launches also form X = L^-1 (about n^3/3 more flops, not counted)"},
+        }
 
     if not args.no_cpu:
         # CPU baseline: the oracle (the reference's LM restated, with its host
