@@ -967,14 +967,52 @@ __global__ __launch_bounds__(kBatchThreads) void k_batch_chol_solve(const BatchD
     for (int q = tid; q < d.n; q += kBatchThreads) dc[9 * static_cast<size_t>(d.c0) + q] = vec[q];
 }
 
-// Forward solve L y = b.  Workgroup with ticket t owns tile row i = t.
+// Triangular solves: the results are their own completion flags.  Before a
+// solve its output vector is filled with kUnset, a signalling NaN -- an
+// arithmetic result is never one (IEEE operations return quiet NaNs) -- and
+// each tile row stores its 64 results with agent-scope stores (written through
+// to the coherence point).  A workgroup that needs a finished row polls the
+// very values it uses with agent-scope loads until none is kUnset: no separate
+// flag, no release / acquire fences, one memory round trip per row.  Waits are
+// bounded: no progress for kWaitTicks of wall clock, or the error flag, and
+// the row gives up (solve_err; never expected -- the tickets hand out rows in
+// dependency order, so every awaited row has a running owner).
+constexpr unsigned long long kUnset = 0x7FF0DEADBEEF0001ULL;
+
+__device__ __forceinline__ double ld_result(const double* p) {
+    return __longlong_as_double(static_cast<long long>(
+        __hip_atomic_load(reinterpret_cast<const unsigned long long*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)));
+}
+__device__ __forceinline__ void st_result(double* p, double v) {
+    __hip_atomic_store(reinterpret_cast<unsigned long long*>(p), static_cast<unsigned long long>(__double_as_longlong(v)),
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ bool is_unset(double v) {
+    return static_cast<unsigned long long>(__double_as_longlong(v)) == kUnset;
+}
+// Polls v = *p (one element per lane; lanes with use unset read nothing)
+// until no lane of the wavefront holds kUnset; false when it gives up.
+__device__ __forceinline__ bool wait_result(const double* p, bool use, double& v, const int* err) {
+    const long long t0 = wall_clock64();
+    for (;;) {
+        v = use ? ld_result(p) : 0.0;
+        if (__ballot(use && is_unset(v)) == 0) return true;
+        if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ||
+            wall_clock64() - t0 > kWaitTicks)
+            return false;
+        __builtin_amdgcn_s_sleep(1);
+    }
+}
+
+// Forward solve L y = b (y filled with kUnset).  Workgroup with ticket t owns
+// tile row i = t; lane = column of the L tiles, wavefront w = rows 16w..16w+15.
 // DinvT[i] holds (L_ii^-1)^T.
 // first (optional, envelope factor): L_ij = 0 for j < first[i], so row i
 // neither reads nor waits for those tiles.
 template <typename T>
 __global__ __launch_bounds__(256) void k_trsv_fwd(const T* __restrict__ L, int ld, int nblk,
                                                   const T* __restrict__ DinvT,
-                                                  const double* __restrict__ b, double* y, int* done,
+                                                  const double* __restrict__ b, double* y,
                                                   int* ticket, TrialFlags* flags, const int* __restrict__ first) {
     __shared__ int s_blk, s_skip;
     __shared__ double s_part[4][16];
@@ -988,6 +1026,7 @@ __global__ __launch_bounds__(256) void k_trsv_fwd(const T* __restrict__ L, int l
     __syncthreads();
     if (s_skip) return;
     const int i = s_blk;
+    const int lo = first ? first[i] : 0;
     {
         const T* inv = DinvT + static_cast<size_t>(i) * kTile * kTile;
         for (int q = tid; q < kTile * kTile; q += 256) s_inv[q >> 6][q & 63] = inv[q];
@@ -995,24 +1034,29 @@ __global__ __launch_bounds__(256) void k_trsv_fwd(const T* __restrict__ L, int l
     double acc[16];
 #pragma unroll
     for (int r = 0; r < 16; ++r) acc[r] = 0.0;
-    for (int j = first ? first[i] : 0; j < i; ++j) {
+    bool gave_up = false;
+    for (int j = lo; j < i; ++j) {
+        // the tile's loads do not depend on y_j: issued before the wait
         const T* Lr = L + static_cast<size_t>(i * kTile + 16 * w) * ld + j * kTile + lane;
         T lv[16];
 #pragma unroll
-        for (int r = 0; r < 16; ++r) lv[r] = Lr[static_cast<size_t>(r) * ld];  // independent of y
-        if (tid == 0 && !wait_flag(done + j, 1, &flags->solve_err)) {
-            atomicOr(&flags->solve_err, 1);
-            s_skip = 1;
+        for (int r = 0; r < 16; ++r) lv[r] = Lr[static_cast<size_t>(r) * ld];
+        double yj;
+        if (!wait_result(y + j * kTile + lane, true, yj, &flags->solve_err)) {
+            gave_up = true;
+            break;
         }
-        __syncthreads();
-        // after a failed wait the row is abandoned: no further waits, no
-        // publication (the waiters on it see solve_err and return)
-        if (s_skip) return;
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        const double yj = y[j * kTile + lane];
 #pragma unroll
         for (int r = 0; r < 16; ++r) acc[r] += static_cast<double>(lv[r]) * yj;
     }
+    // after a failed wait the row is abandoned: its results stay unset (the
+    // waiters on it see solve_err and give up)
+    if (gave_up && lane == 0) {
+        atomicOr(&flags->solve_err, 1);
+        s_skip = 1;
+    }
+    __syncthreads();
+    if (s_skip) return;
     double mine = 0.0;
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
@@ -1028,16 +1072,16 @@ __global__ __launch_bounds__(256) void k_trsv_fwd(const T* __restrict__ L, int l
         double v = 0.0;
 #pragma unroll 16
         for (int c = 0; c < kTile; ++c) v += static_cast<double>(s_inv[c][lane]) * s_rhs[c];
-        y[i * kTile + lane] = v;
-        if (lane == 0) __hip_atomic_store(done + i, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        st_result(y + i * kTile + lane, v);
     }
 }
 
-// Backward solve L^T x = y.  Ticket t owns tile row i = nblk - 1 - t.
+// Backward solve L^T x = y (x filled with kUnset).  Ticket t owns tile row
+// i = nblk - 1 - t.
 template <typename T>
 __global__ __launch_bounds__(256) void k_trsv_bwd(const T* __restrict__ L, int ld, int nblk,
                                                   const T* __restrict__ DinvT,
-                                                  const double* __restrict__ y, double* x, int* done,
+                                                  const double* __restrict__ y, double* x,
                                                   int* ticket, TrialFlags* flags, const int* __restrict__ first) {
     __shared__ int s_blk, s_skip;
     __shared__ double s_part[4][kTile];
@@ -1056,23 +1100,29 @@ __global__ __launch_bounds__(256) void k_trsv_bwd(const T* __restrict__ L, int l
         for (int q = tid; q < kTile * kTile; q += 256) s_inv[q >> 6][q & 63] = inv[q];
     }
     double acc = 0.0;
+    bool gave_up = false;
     for (int j = nblk - 1; j > i; --j) {
         if (first && first[j] > i) continue;  // L_ji = 0 (uniform across the workgroup)
         const T* Lc = L + static_cast<size_t>(j * kTile + 16 * w) * ld + i * kTile + lane;
         T lv[16];
 #pragma unroll
         for (int r = 0; r < 16; ++r) lv[r] = Lc[static_cast<size_t>(r) * ld];
-        if (tid == 0 && !wait_flag(done + j, 1, &flags->solve_err)) {
-            atomicOr(&flags->solve_err, 1);
-            s_skip = 1;
+        // x_j rows 16w .. 16w+15: lane r < 16 polls row r, then every lane
+        // takes the sixteen values
+        double xr;
+        if (!wait_result(x + j * kTile + 16 * w + (lane & 15), lane < 16, xr, &flags->solve_err)) {
+            gave_up = true;
+            break;
         }
-        __syncthreads();
-        if (s_skip) return;
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        const double* xj = x + j * kTile + 16 * w;
 #pragma unroll
-        for (int r = 0; r < 16; ++r) acc += static_cast<double>(lv[r]) * xj[r];
+        for (int r = 0; r < 16; ++r) acc += static_cast<double>(lv[r]) * readlane(xr, r);
     }
+    if (gave_up && lane == 0) {
+        atomicOr(&flags->solve_err, 1);
+        s_skip = 1;
+    }
+    __syncthreads();
+    if (s_skip) return;
     s_part[w][lane] = acc;
     __syncthreads();
     if (tid < kTile)
@@ -1084,8 +1134,7 @@ __global__ __launch_bounds__(256) void k_trsv_bwd(const T* __restrict__ L, int l
         double v = 0.0;
 #pragma unroll 16
         for (int c = 0; c < kTile; ++c) v += static_cast<double>(s_inv[lane][c]) * s_rhs[c];
-        x[i * kTile + lane] = v;
-        if (lane == 0) __hip_atomic_store(done + i, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        st_result(x + i * kTile + lane, v);
     }
 }
 
@@ -1787,21 +1836,25 @@ void launch_chol_solve(hipStream_t st, const double* L64, const float* L32, int 
                        const double* D64, const float* D32, const double* b, double* y, double* x,
                        int* sync, TrialFlags* flags, const int* first) {
     launch_fill_int(st, sync, 2 * nblk + 2, 0);
-    int* done_f = sync;
-    int* done_b = sync + nblk;
     int* tickets = sync + 2 * nblk;
     const size_t inv_off = static_cast<size_t>(nblk) * kTile * kTile;
-    if (L64) {
+    const size_t np = static_cast<size_t>(nblk) * kTile;
+    const double unset = __builtin_bit_cast(double, kUnset);
+    // the outputs start unset (x after the forward pass: b and x may alias)
+    launch_fill_double(st, y, np, unset);
+    if (L64)
         hipLaunchKernelGGL(k_trsv_fwd<double>, dim3(nblk), dim3(256), 0, st, L64, ld, nblk,
-                           D64 + inv_off, b, y, done_f, tickets, flags, first);
-        hipLaunchKernelGGL(k_trsv_bwd<double>, dim3(nblk), dim3(256), 0, st, L64, ld, nblk,
-                           D64 + inv_off, y, x, done_b, tickets + 1, flags, first);
-    } else {
+                           D64 + inv_off, b, y, tickets, flags, first);
+    else
         hipLaunchKernelGGL(k_trsv_fwd<float>, dim3(nblk), dim3(256), 0, st, L32, ld, nblk,
-                           D32 + inv_off, b, y, done_f, tickets, flags, first);
+                           D32 + inv_off, b, y, tickets, flags, first);
+    launch_fill_double(st, x, np, unset);
+    if (L64)
+        hipLaunchKernelGGL(k_trsv_bwd<double>, dim3(nblk), dim3(256), 0, st, L64, ld, nblk,
+                           D64 + inv_off, y, x, tickets + 1, flags, first);
+    else
         hipLaunchKernelGGL(k_trsv_bwd<float>, dim3(nblk), dim3(256), 0, st, L32, ld, nblk,
-                           D32 + inv_off, y, x, done_b, tickets + 1, flags, first);
-    }
+                           D32 + inv_off, y, x, tickets + 1, flags, first);
 }
 
 int residual_partials(int npad) { return (npad + 255) / 256; }
