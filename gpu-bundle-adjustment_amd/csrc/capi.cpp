@@ -379,7 +379,9 @@ int gba_dense_spd_solve(int n, int precision, const double* A, double* b, int* o
             launch_cholesky(st, nullptr, S32.p, npad, nblk, nullptr, D32.p, flags.p, X32.p, T32.p);
             double rel = 0.0;
             try {
-                iters = pcg_solve(st, S32low.p, X32.p, npad, npad, rhs.p, x.p, work.p, flags.p, status.p,
+                PcgPrecond M;
+                M.X32 = X32.p;
+                iters = pcg_solve(st, S32low.p, M, npad, npad, rhs.p, x.p, work.p, flags.p, status.p,
                                   hs, 1e-12, 50, &rel);
             } catch (...) {
                 (void)hipHostFree(hs);

@@ -1013,9 +1013,11 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_trsv_fwd(const T* __restrict__ L, int ld, int nblk,
                                                   const T* __restrict__ DinvT,
                                                   const double* __restrict__ b, double* y,
-                                                  int* ticket, TrialFlags* flags, const int* __restrict__ first) {
+                                                  int* ticket, TrialFlags* flags, const int* __restrict__ first,
+                                                  const double* skip) {
     __shared__ int s_blk, s_skip;
     __shared__ double s_part[4][16];
+    if (skip && *skip != 0.0) return;  // a refinement step after the stop
     __shared__ double s_rhs[kTile];
     __shared__ T s_inv[kTile][kTile + 1];
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
@@ -1082,9 +1084,11 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_trsv_bwd(const T* __restrict__ L, int ld, int nblk,
                                                   const T* __restrict__ DinvT,
                                                   const double* __restrict__ y, double* x,
-                                                  int* ticket, TrialFlags* flags, const int* __restrict__ first) {
+                                                  int* ticket, TrialFlags* flags, const int* __restrict__ first,
+                                                  const double* skip) {
     __shared__ int s_blk, s_skip;
     __shared__ double s_part[4][kTile];
+    if (skip && *skip != 0.0) return;
     __shared__ double s_rhs[kTile];
     __shared__ T s_inv[kTile][kTile + 1];
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
@@ -1747,9 +1751,11 @@ void launch_cholesky(hipStream_t st, double* S64, float* S32, int ld, int nblk, 
         // the last columns (at most kFuseCols tiles below the diagonal) take
         // their TRSM inside the update (measured at n = 3,204: 1.32 -> 1.26 ms;
         // fusing every column, 1.30 ms: the early updates have many tiles)
+        // (only with X: fused columns keep A in their lower tiles, which the
+        // triangular solves read)
         constexpr int kFuseCols = 32;
         factor_columns<float>(st, S32, ld, nblk, D32, D32 + static_cast<size_t>(nblk) * tile, flags, X32, R32,
-                              kFuseCols);
+                              X32 ? kFuseCols : 0);
     }
 }
 
@@ -1834,7 +1840,7 @@ void launch_batch_chol_solve(hipStream_t st, int count, const BatchDesc* desc, d
 
 void launch_chol_solve(hipStream_t st, const double* L64, const float* L32, int ld, int nblk,
                        const double* D64, const float* D32, const double* b, double* y, double* x,
-                       int* sync, TrialFlags* flags, const int* first) {
+                       int* sync, TrialFlags* flags, const int* first, const double* skip) {
     launch_fill_int(st, sync, 2 * nblk + 2, 0);
     int* tickets = sync + 2 * nblk;
     const size_t inv_off = static_cast<size_t>(nblk) * kTile * kTile;
@@ -1844,17 +1850,17 @@ void launch_chol_solve(hipStream_t st, const double* L64, const float* L32, int 
     launch_fill_double(st, y, np, unset);
     if (L64)
         hipLaunchKernelGGL(k_trsv_fwd<double>, dim3(nblk), dim3(256), 0, st, L64, ld, nblk,
-                           D64 + inv_off, b, y, tickets, flags, first);
+                           D64 + inv_off, b, y, tickets, flags, first, skip);
     else
         hipLaunchKernelGGL(k_trsv_fwd<float>, dim3(nblk), dim3(256), 0, st, L32, ld, nblk,
-                           D32 + inv_off, b, y, tickets, flags, first);
+                           D32 + inv_off, b, y, tickets, flags, first, skip);
     launch_fill_double(st, x, np, unset);
     if (L64)
         hipLaunchKernelGGL(k_trsv_bwd<double>, dim3(nblk), dim3(256), 0, st, L64, ld, nblk,
-                           D64 + inv_off, y, x, tickets + 1, flags, first);
+                           D64 + inv_off, y, x, tickets + 1, flags, first, skip);
     else
         hipLaunchKernelGGL(k_trsv_bwd<float>, dim3(nblk), dim3(256), 0, st, L32, ld, nblk,
-                           D32 + inv_off, y, x, tickets + 1, flags, first);
+                           D32 + inv_off, y, x, tickets + 1, flags, first, skip);
 }
 
 int residual_partials(int npad) { return (npad + 255) / 256; }

@@ -138,11 +138,13 @@ void launch_cholesky_persistent(hipStream_t st, double* S64, float* S32, int ld,
 
 // Solves L L^T x = b (b, x length nblk*kTile; b and x may alias).  sync holds
 // 2*nblk + 2 ints of scratch (done flags and tickets).  first (nblk ints, optional):
-// the envelope factor's first tile column of each tile row.
+// the envelope factor's first tile column of each tile row.  skip (optional):
+// a device flag that turns the two solves into empty launches (a refinement
+// step after its stop).
 void launch_chol_solve(hipStream_t st, const double* L64, const float* L32, int ld, int nblk,
                        const double* D64, const float* D32, const double* b, double* y, double* x,
                        int* sync, TrialFlags* flags,
-                       const int* first = nullptr);
+                       const int* first = nullptr, const double* skip = nullptr);
 // y = A x, A symmetric given by the lower triangle of an FP32 row-major
 // matrix (FP64 accumulation); work holds symv_work(npad) doubles.
 size_t symv_work(int npad);

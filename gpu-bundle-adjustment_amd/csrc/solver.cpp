@@ -104,7 +104,7 @@ size_t pcg_work(int npad) {
            2 * static_cast<size_t>(residual_partials(npad));
 }
 
-int pcg_solve(hipStream_t st, const float* A, const float* X32, int ld, int npad, const double* b,
+int pcg_solve(hipStream_t st, const float* A, const PcgPrecond& M, int ld, int npad, const double* b,
               double* x, double* work, TrialFlags* flags, PassStatus* status, PassStatus* h_status,
               double tol, int max_it, double* rel, int first_batch) {
     const size_t np = static_cast<size_t>(npad);
@@ -121,9 +121,16 @@ int pcg_solve(hipStream_t st, const float* A, const float* X32, int ld, int npad
         GBA_HIP(hipStreamSynchronize(st));
     };
     const double* done = sc + 4;
+    auto precond = [&](const double* skip) {
+        if (M.X32)
+            launch_precond_apply(st, M.X32, ld, npad, r, z, sw, skip);
+        else
+            launch_chol_solve(st, nullptr, M.L32, ld, M.nblk, nullptr, M.D32, r, M.tmp, z, M.sync, flags, nullptr,
+                              skip);
+    };
     launch_fill_double(st, x, np, 0.0);
     launch_vec_copy(st, npad, b, r);
-    launch_precond_apply(st, X32, ld, npad, r, z, sw);
+    precond(nullptr);
     launch_vec_copy(st, npad, z, p);
     launch_pcg_dot(st, npad, r, z, b, b, blk, 0, sc, status, flags, tol * tol, max_it);
     // Steps are enqueued in batches with one synchronisation per batch: the
@@ -137,7 +144,7 @@ int pcg_solve(hipStream_t st, const float* A, const float* X32, int ld, int npad
             launch_symv(st, A, ld, npad, p, q, sw, done);
             launch_pcg_dot(st, npad, p, q, p, q, blk, 1, sc, status, flags, tol * tol, max_it);
             launch_pcg_update(st, npad, sc, x, p, r, q);
-            launch_precond_apply(st, X32, ld, npad, r, z, sw, done);
+            precond(done);
             launch_pcg_dot(st, npad, r, z, r, r, blk, 2, sc, status, flags, tol * tol, max_it);
             launch_pcg_direction(st, npad, sc, z, p);
         }
@@ -216,6 +223,16 @@ Solver::Solver(const gba_problem_view& prob, const gba_solver_config& cfg) : cfg
     pb_.n = 9 * C;
     pb_.npad = (pb_.n + kTile - 1) / kTile * kTile;
     nblk_ = pb_.npad / kTile;
+    // mixed refinement preconditioner: the explicit inverse up to
+    // kInversePrecondBlk tile columns, the factor's triangular solves beyond
+    // (GBA_PRECOND=inverse / =trsv in the environment overrides)
+    if (mixed_) {
+        precond_trsv_ = nblk_ > kInversePrecondBlk;
+        if (const char* env = std::getenv("GBA_PRECOND")) {
+            if (std::string(env) == "inverse") precond_trsv_ = false;
+            else if (std::string(env) == "trsv") precond_trsv_ = true;
+        }
+    }
 
     sshape_ = schur_shape(C, mixed_);
     // diagonal blocks and rhs in their own per-camera kernel (GBA_SCHUR_DIAG=0:
@@ -374,8 +391,10 @@ Solver::Solver(const gba_problem_view& prob, const gba_solver_config& cfg) : cfg
     if (mixed_) {
         S32_.alloc(np * np);
         S32low_.alloc(np * np);
-        X32_.alloc(np * np);
-        T32_.alloc(np * np);
+        if (!precond_trsv_) {
+            X32_.alloc(np * np);
+            T32_.alloc(np * np);
+        }
         D32_.alloc(2 * static_cast<size_t>(nblk_) * kTile * kTile);
     } else {
         S64_.alloc(np * np);
@@ -564,7 +583,7 @@ void Solver::enqueue_trial_linear_algebra(double lambda, std::vector<double>* ho
         const char* e = std::getenv("GBA_PCG_BATCH");
         return e && std::atoi(e) == 1;
     }();
-    out.refine_iters = pcg_solve(st_, S32low_.p, X32_.p, ld, pb_.npad, rhs_.p, x_.p, pcgw_.p, flags_.p,
+    out.refine_iters = pcg_solve(st_, S32low_.p, precond(), ld, pb_.npad, rhs_.p, x_.p, pcgw_.p, flags_.p,
                                  status_.p, h_status_, tol, max_it, &out.refine_rel, one_step ? 1 : last_refine_);
     if (out.refine_iters > 0) last_refine_ = out.refine_iters;
     launch_vec_scale(st_, pb_.n, pb_.npad, x_.p, dscale_.p, dc_.p);
@@ -586,7 +605,7 @@ void Solver::enqueue_factor() {
         return;
     }
     if (mixed_)
-        launch_cholesky(st_, nullptr, S32_.p, ld, nblk_, nullptr, D32_.p, flags_.p, X32_.p, T32_.p);
+        launch_cholesky(st_, nullptr, S32_.p, ld, nblk_, nullptr, D32_.p, flags_.p, X32_.p, T32_.p);  // X32_ null: no inverse
     else
         launch_cholesky(st_, S64_.p, nullptr, ld, nblk_, D64_.p, nullptr, flags_.p);
 }
@@ -708,7 +727,15 @@ double Solver::time_kernel(int kernel, double lambda, int reps) {
             case GBA_KERNEL_SOLVE:
                 // FP64: both triangular solves; mixed: one preconditioner application
                 if (mixed_)
-                    launch_precond_apply(st_, X32_.p, ld, pb_.npad, rhs_.p, x_.p, pcgw_.p + 5 * static_cast<size_t>(pb_.npad));
+                {
+                    const PcgPrecond M = precond();
+                    if (M.X32)
+                        launch_precond_apply(st_, M.X32, ld, pb_.npad, rhs_.p, x_.p,
+                                             pcgw_.p + 5 * static_cast<size_t>(pb_.npad));
+                    else
+                        launch_chol_solve(st_, nullptr, M.L32, ld, nblk_, nullptr, M.D32, rhs_.p, y_.p, x_.p, sync_.p,
+                                          flags_.p);
+                }
                 else
                     launch_chol_solve(st_, S64_.p, nullptr, ld, nblk_, D64_.p, nullptr, rhs_.p, y_.p, x_.p,
                                       sync_.p, flags_.p, env_ ? first_.p : nullptr);

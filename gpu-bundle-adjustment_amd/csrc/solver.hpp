@@ -80,15 +80,34 @@ void build_schur_tasks_device(hipStream_t st, int C, int cb, const std::vector<i
                               DevBuf<int2>& pairs, bool with_diag);
 
 // FP64 preconditioned conjugate gradients on A x = b, A symmetric given by the
-// lower triangle of an FP32 matrix, preconditioned by (L L^T)^-1 applied
-// through the explicit FP32 inverse X = L^-1 (launch_cholesky with X32).  Vectors
+// lower triangle of an FP32 matrix, preconditioned by (L L^T)^-1 (PcgPrecond:
+// the explicit FP32 inverse X = L^-1 from launch_cholesky with X32, or the
+// factor's triangular solves).  Vectors
 // have npad entries.  Stops at |r| <= tol |b| or after max_it steps; returns
 // the steps taken and the final |r| / |b| in *rel.  work holds pcg_work(npad)
 // doubles.  The stopping test runs on the device; steps are enqueued
 // first_batch at a time first, then one at a time, with one synchronisation
 // per batch (the steps of a batch past the stop are empty launches).
 size_t pcg_work(int npad);
-int pcg_solve(hipStream_t st, const float* A, const float* X32, int ld, int npad, const double* b,
+// The refinement's preconditioner: the explicit FP32 inverse X = L^-1
+// (z = X^T X r, two tile-parallel matrix-vector passes) or, where forming X
+// costs more than its cheaper applications save (large matrices), the FP32
+// factor itself (forward and backward block substitution, launch_chol_solve).
+// Largest tile-column count at which the mixed refinement applies the explicit
+// inverse; above it, the triangular solves (measured: C=356, 51 columns,
+// inverse 0.07 ms to form + 22 us per application against 0.3 ms per pair of
+// solves; n = 17,424, 272 columns, inverse 29 ms to form against 1.7 ms per
+// pair of solves).
+constexpr int kInversePrecondBlk = 128;
+struct PcgPrecond {
+    const float* X32 = nullptr;
+    const float* L32 = nullptr;  // factor (lower tiles), with D32 (diagonal tiles and inverses)
+    const float* D32 = nullptr;
+    int nblk = 0;
+    double* tmp = nullptr;  // npad doubles
+    int* sync = nullptr;    // 2 nblk + 2 ints
+};
+int pcg_solve(hipStream_t st, const float* A, const PcgPrecond& M, int ld, int npad, const double* b,
               double* x, double* work, TrialFlags* flags, PassStatus* status, PassStatus* h_status,
               double tol, int max_it, double* rel, int first_batch = 1);
 
@@ -192,6 +211,22 @@ private:
     DevBuf<float> S32_, S32low_, D32_, X32_, T32_;
     DevBuf<double> rhs_, y_, x_, dc_, dscale_, pcgw_;
     int last_refine_ = 3;  // refinement steps of the last trial (first batch of the next)
+    // mixed: precondition by the factor's triangular solves instead of the
+    // explicit inverse (large matrices; see kInversePrecondBlk)
+    bool precond_trsv_ = false;
+    PcgPrecond precond() const {
+        PcgPrecond M;
+        if (!precond_trsv_) {
+            M.X32 = X32_.p;
+        } else {
+            M.L32 = S32_.p;
+            M.D32 = D32_.p;
+            M.nblk = nblk_;
+            M.tmp = y_.p;
+            M.sync = sync_.p;
+        }
+        return M;
+    }
     DevBuf<int> sync_;
     // one-launch factor (GBA_FACTOR=persistent; the default is the launch
     // per tile column): per-column first tickets, flags, grid

@@ -401,6 +401,26 @@ def test_launch_bounds_gate_on_gpu(gba):
     assert r.final_cost < 1e-4 * r.initial_cost
 
 
+def test_mixed_refinement_preconditioners_agree(gba, monkeypatch):
+    """The mixed refinement preconditioned by the factor's triangular solves
+    (what large matrices use) against the explicit inverse (small ones): both
+    solve each trial's FP32-stored system to the same residual, so the LM
+    trajectory is the same."""
+    prob = heavy(gba)
+    opts = gba.LMOptions(initial_lambda=1e-3)
+    monkeypatch.setenv("GBA_PRECOND", "inverse")
+    inv = gba.Solver(prob, gba.PRECISION_MIXED).solve(opts)
+    s_inv = gba.schur_step(prob, 1e-3, gba.PRECISION_MIXED)
+    monkeypatch.setenv("GBA_PRECOND", "trsv")
+    tri = gba.Solver(prob, gba.PRECISION_MIXED).solve(opts)
+    s_tri = gba.schur_step(prob, 1e-3, gba.PRECISION_MIXED)
+    assert s_tri["status"] == s_inv["status"] == "accept"
+    assert arel(s_tri["dc"], s_inv["dc"]) < 1e-8
+    assert tri.termination == inv.termination and tri.iterations == inv.iterations
+    assert tri.max_refine_residual <= 1e-10
+    assert abs(tri.final_cost - inv.final_cost) <= 1e-9 * inv.final_cost
+
+
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
 def test_one_launch_factor_matches_launch_per_column(gba, monkeypatch, prec):
     """GBA_FACTOR=persistent (one launch: a critical-chain workgroup plus
