@@ -472,7 +472,8 @@ def main():
                      "unit": "TFLOP/s", "frac": round(tf64 / PEAK["fp64_tflops"], 4)},
             "mixed": {"ms": round(big["mixed"], 3), "achieved": round(tf32, 2), "peak": PEAK["fp32_tflops"],
                       "unit": "TFLOP/s", "frac": round(tf32 / PEAK["fp32_tflops"], 4),
-                      "note": "the same launches also form X = L^-1 (about n^3/3 more flops, not counted)"},
+                      "note": "at this size the mixed form preconditions with the factor's triangular solves "
+                              "and forms no explicit inverse"},
         }
 
     if not args.no_cpu:
