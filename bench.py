@@ -7,7 +7,9 @@ HBM (the solver handle is created, sorted and uploaded before the timed
 region).  The plain FP64 solve defines the target cost C64; the time-to-final-
 cost of a solve is the elapsed time from solve entry until the first accepted
 LM iterate whose cost is <= C64 * (1 + 1e-6).  `value` is the median over the
-timed mixed-precision steps.  Prints one JSON line.  `including_setup`
+timed mixed-precision steps (the library's default refinement of the mixed
+form: block-Jacobi CG at this size); the mixed form refined with its FP32
+MFMA factor is timed beside it (`mixed_factor`).  Prints one JSON line.  `including_setup`
 gives the same measure for one-shot solves that allocate, sort and upload
 inside the clock (the reference's total_time_s convention).
 
@@ -308,12 +310,15 @@ def main():
     opts = g.LMOptions(initial_lambda=args.lambda0)
 
     s64 = g.Solver(prob, g.PRECISION_FP64)
-    s32 = g.Solver(prob, g.PRECISION_MIXED)
+    s32 = g.Solver(prob, g.PRECISION_MIXED)  # the library's choice of refinement (block Jacobi here)
+    s32f = g.Solver(prob, g.PRECISION_MIXED, precond=g.PRECOND_FACTOR)
+    jacobi = s32.precond() == g.PRECOND_JACOBI
+    k64 = ("edge_point", "edge_camera", "schur", "factor", "solve", "backsub")
+    k32 = (("edge_point", "edge_camera", "schur", "cg_setup", "cg_step", "backsub") if jacobi else k64)
     if args.mode == "kernels":
         out = {"mode": "kernels", "config": {"cameras": C, "points": P, "observations": O}}
-        for name, s in (("fp64", s64), ("mixed", s32)):
-            out[name] = {k: round(s.time_kernel(k, args.lambda0, args.kernel_reps), 4)
-                         for k in ("edge_point", "edge_camera", "schur", "factor", "solve", "backsub")}
+        for name, s, ks in (("fp64", s64, k64), ("mixed", s32, k32), ("mixed_factor", s32f, k64)):
+            out[name] = {k: round(s.time_kernel(k, args.lambda0, args.kernel_reps), 4) for k in ks}
         # calibration for FETCH_SIZE / WRITE_SIZE: a plain copy of 512 MiB
         a = torch.empty(64 * 1024 * 1024, dtype=torch.float64, device="cuda")
         b = torch.empty_like(a)
@@ -330,6 +335,7 @@ def main():
     for _ in range(args.warmup):
         s64.solve(opts)
         s32.solve(opts)
+        s32f.solve(opts)
     torch.cuda.synchronize()
 
     # FP64 reference solves (also the target cost): at least five samples
@@ -345,6 +351,9 @@ def main():
     wall = time.perf_counter() - t0
     t32 = [time_to_cost(r, target) for r in r32]
     value = statistics.median(t32)
+    # the other refinement of the mixed form (FP32 MFMA factor), at least five samples
+    r32f = [s32f.solve(opts) for _ in range(max(5, args.steps))]
+    t32f = [time_to_cost(r, target) for r in r32f]
 
     # the same measure including allocation, sort and upload (one-shot
     # solve_bal_gpu calls, as the reference's total_time_s counts them)
@@ -356,20 +365,23 @@ def main():
 
     # per-kernel durations (HIP events on the solver stream)
     kern = {}
-    for name, s in (("fp64", s64), ("mixed", s32)):
-        kern[name] = {k: s.time_kernel(k, args.lambda0, args.kernel_reps)
-                      for k in ("edge_point", "edge_camera", "schur", "factor", "solve", "backsub")}
+    for name, s, ks in (("fp64", s64, k64), ("mixed", s32, k32), ("mixed_factor", s32f, k64)):
+        kern[name] = {k: s.time_kernel(k, args.lambda0, args.kernel_reps) for k in ks}
     st = s32.stats()
-    n = st["n"]
+    n, npad = st["n"], st["n_padded"]
     trials32 = statistics.median([r.trials for r in r32])
     lins32 = statistics.median([r.iterations for r in r32])
     refine = statistics.median([r.refine_iterations for r in r32])
     km = kern["mixed"]
-    share = {
-        "edge_point": lins32 * km["edge_point"], "edge_camera": lins32 * km["edge_camera"],
-        "schur": trials32 * km["schur"], "factor": trials32 * km["factor"],
-        "solve": (trials32 + refine) * km["solve"], "backsub": trials32 * km["backsub"],
-    }
+    kf = kern["mixed_factor"]
+    # kernel time per mixed solve (HIP-event averages x launches per solve)
+    share = {"edge_point": lins32 * km["edge_point"], "edge_camera": lins32 * km["edge_camera"],
+             "schur": trials32 * km["schur"], "backsub": trials32 * km["backsub"]}
+    if jacobi:
+        share["cg_refinement"] = trials32 * km["cg_setup"] + (refine + trials32) * km["cg_step"]
+    else:
+        share["factor"] = trials32 * km["factor"]
+        share["solve"] = (trials32 + refine) * km["solve"]
     dominant = max(share, key=share.get)
     w_bytes = 108  # FP32 W in the mixed form
     # edge accumulate = point pass + camera pass (one linearisation)
@@ -377,7 +389,7 @@ def main():
     edge_ms = km["edge_point"] + km["edge_camera"]
     edge_gbs = edge_bytes / (edge_ms * 1e-3) / 1e9
     factor_flops = n ** 3 / 3.0
-    factor_tflops = factor_flops / (km["factor"] * 1e-3) / 1e12
+    factor_tflops = factor_flops / (kf["factor"] * 1e-3) / 1e12
 
     def traffic_of(name):
         """HBM bytes per launch from the committed rocprofv3 FETCH_SIZE /
@@ -390,22 +402,42 @@ def main():
         return round(sum(k["fetch_kib"] * t["fetch_scale"] + k["write_kib"] * t["write_scale"]
                          for k in t["kernels"].values()) * 1024)
 
-    # Roofline of the dominant kernel: the dense factor of S (FP32 MFMA in the
-    # mixed form), n^3/3 flops per factorisation over the live HIP-event time
-    # of the whole launch family of one factorisation (which in the mixed form
-    # also forms the explicit inverse L^-1; its flops are not counted).
-    roof = {"kernel": "dense Cholesky of S (k_potrf_tile + k_trsm_tiles + k_chol_update family), mixed form",
-            "bound": "mfma", "achieved": round(factor_tflops, 3), "peak": PEAK["fp32_tflops"],
-            "unit": "TFLOP/s", "frac": round(factor_tflops / PEAK["fp32_tflops"], 4),
-            "traffic": traffic_of("factor_C356_traffic.json"),
-            "basis": f"n^3/3 = {factor_flops:.4e} flop (n = {n}) per factorisation over {km['factor']:.4f} ms",
-            "largest_share": dominant}
+    # The dense factor of S (FP32 MFMA), the other refinement of the mixed
+    # form: n^3/3 flops per factorisation over the live HIP-event time of the
+    # whole launch family of one factorisation (which also forms the explicit
+    # inverse L^-1; its flops are not counted).
+    roof_factor = {"kernel": "dense Cholesky of S (k_potrf_tile + k_trsm_tiles + k_chol_update family), "
+                             "mixed form with the factor refinement",
+                   "bound": "mfma", "achieved": round(factor_tflops, 3), "peak": PEAK["fp32_tflops"],
+                   "unit": "TFLOP/s", "frac": round(factor_tflops / PEAK["fp32_tflops"], 4),
+                   "traffic": traffic_of("factor_C356_traffic.json"),
+                   "basis": f"n^3/3 = {factor_flops:.4e} flop (n = {n}) per factorisation over {kf['factor']:.4f} ms"}
+    roof_cg = None
+    if jacobi:
+        # one block-Jacobi CG step (k_cg_matvec + k_cg_vector): every lower
+        # 64x64 FP32 tile of the scaled S read once, plus the vectors (u, p, s,
+        # x, r, b read, p, s, x, r, u written: 11 npad doubles) and the tile
+        # partials (two 64-double vectors per tile, written and read)
+        nblk = npad // 64
+        tiles = nblk * (nblk + 1) // 2
+        cg_bytes = tiles * 64 * 64 * 4 + 11 * npad * 8 + tiles * 2 * 64 * 8 * 2
+        cg_gbs = cg_bytes / (km["cg_step"] * 1e-3) / 1e9
+        roof_cg = {"kernel": "block-Jacobi CG step (k_cg_matvec + k_cg_vector), mixed form", "bound": "hbm",
+                   "achieved": round(cg_gbs, 1), "peak": PEAK["hbm_gbs"], "unit": "GB/s",
+                   "frac": round(cg_gbs / PEAK["hbm_gbs"], 4), "traffic": traffic_of("k_cg_C356_traffic.json"),
+                   "basis": f"{tiles} tiles x 16 KiB + 11 npad x 8 B + tile partials = {cg_bytes} B per step "
+                            f"over {km['cg_step']:.4f} ms"}
     schur_bytes = O * w_bytes + 72 * P + (n * n // 2) * 4
     schur_gbs = schur_bytes / (km["schur"] * 1e-3) / 1e9
     roof_schur = {"kernel": "Schur assembly (k_schur_diag + k_schur), mixed form", "bound": "hbm",
                   "achieved": round(schur_gbs, 1), "peak": PEAK["hbm_gbs"], "unit": "GB/s",
                   "frac": round(schur_gbs / PEAK["hbm_gbs"], 4), "traffic": traffic_of("k_schur_C356_traffic.json"),
                   "basis": f"O*108 + 72P + n^2/2*4 = {schur_bytes} B per assembly over {km['schur']:.4f} ms"}
+    # the bench line's roofline is the kernel family with the largest share of
+    # the measured mixed solve; the others follow as roofline_<name>
+    entries = {"schur": roof_schur, "cg_refinement": roof_cg, "factor": roof_factor}
+    dom_key = dominant if dominant in entries and entries[dominant] else ("cg_refinement" if jacobi else "factor")
+    roof = dict(entries[dom_key], largest_share=dominant)
 
     out = {
         "metric": "time-to-final-cost (rel err<=1e-6 vs FP64), C=356; mixed vs FP64 GPU",
@@ -418,7 +450,9 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 W/S/factor + f64 PCG on the f32-stored S, f64 cost (mixed); f64 reference leg",
+        "dtype": ("f32 W/S + f64 block-Jacobi CG on the f32-stored S, f64 cost (mixed); "
+                  "f32 MFMA factor-preconditioned leg and f64 reference leg beside it") if jacobi else
+                 "f32 W/S/factor + f64 PCG on the f32-stored S, f64 cost (mixed); f64 reference leg",
         "data": "synthetic (seeded Snavely generator, seed %d)" % args.seed,
         "config": {"workload": f"LM solve C={C} P={P} O={O} (preset {args.config}), "
                                f"lambda0={args.lambda0}, rel_decrease_tol=1e-6",
@@ -434,8 +468,15 @@ def main():
                   "final_cost": r32[0].final_cost, "iterations": r32[0].iterations,
                   "trials": r32[0].trials, "refine_iterations": r32[0].refine_iterations,
                   "max_refine_residual_f32_system": r32[0].max_refine_residual,
+                  "refinement": "block-Jacobi CG" if jacobi else "FP32 factor PCG",
+                  "refine_factor_trials": r32[0].refine_factor_trials,
                   "termination": r32[0].termination,
                   "all_s": [round(t, 6) for t in t32]},
+        "mixed_factor": {"time_to_final_cost_s": round(statistics.median(t32f), 6), "samples": len(t32f),
+                         "best_s": round(min(t32f), 6), "worst_s": round(max(t32f), 6),
+                         "final_cost": r32f[0].final_cost, "trials": r32f[0].trials,
+                         "refine_iterations": r32f[0].refine_iterations,
+                         "max_refine_residual_f32_system": r32f[0].max_refine_residual},
         "including_setup": once,
         "speedup_mixed_vs_fp64": round(statistics.median(t64) / value, 3) if value == value else None,
         "kernels_ms": {k: {kk: round(vv, 4) for kk, vv in v.items()} for k, v in kern.items()},
@@ -446,8 +487,10 @@ def main():
                    "tflops_fp64": round(factor_flops / (kern["fp64"]["factor"] * 1e-3) / 1e12, 3)},
         "schur_pairs": st["pair_count"],
         "roofline": roof,
-        "roofline_schur": roof_schur,
     }
+    for k, v in entries.items():
+        if v and k != dom_key:
+            out["roofline_" + k] = v
 
     if not args.no_large and args.config == 3:
         # The dense factor where it is throughput-bound rather than
@@ -459,7 +502,13 @@ def main():
         for name, prec in (("fp64", g.PRECISION_FP64), ("mixed", g.PRECISION_MIXED)):
             sl = g.Solver(pl, prec)
             big[name] = sl.time_kernel("factor", args.lambda0, 3)
-            n4 = sl.stats()["n"]
+            st4 = sl.stats()
+            n4 = st4["n"]
+            if prec == g.PRECISION_MIXED and sl.precond() == g.PRECOND_JACOBI:
+                big["cg_step"] = sl.time_kernel("cg_step", args.lambda0, 10)
+                nb4 = st4["n_padded"] // 64
+                t4 = nb4 * (nb4 + 1) // 2
+                big["cg_bytes"] = t4 * 64 * 64 * 4 + 11 * st4["n_padded"] * 8 + t4 * 2 * 64 * 8 * 2
             sl.close()
             torch.cuda.synchronize()
         fl4 = n4 ** 3 / 3.0
@@ -472,9 +521,16 @@ def main():
                      "unit": "TFLOP/s", "frac": round(tf64 / PEAK["fp64_tflops"], 4)},
             "mixed": {"ms": round(big["mixed"], 3), "achieved": round(tf32, 2), "peak": PEAK["fp32_tflops"],
                       "unit": "TFLOP/s", "frac": round(tf32 / PEAK["fp32_tflops"], 4),
-                      "note": "at this size the mixed form preconditions with the factor's triangular solves "
-                              "and forms no explicit inverse"},
+                      "note": "the factor refinement's factor; at this size it preconditions with the factor's "
+                              "triangular solves and forms no explicit inverse"},
         }
+        if "cg_step" in big:
+            gbs4 = big["cg_bytes"] / (big["cg_step"] * 1e-3) / 1e9
+            out["roofline_cg_preset4"] = {
+                "kernel": f"block-Jacobi CG step at preset 4 (n = {n4}), the default mixed refinement there",
+                "bound": "hbm", "achieved": round(gbs4, 1), "peak": PEAK["hbm_gbs"], "unit": "GB/s",
+                "frac": round(gbs4 / PEAK["hbm_gbs"], 4), "ms": round(big["cg_step"], 4),
+                "basis": f"{big['cg_bytes']} B per step (lower tiles of the FP32 S, vectors, tile partials)"}
 
     if not args.no_cpu:
         # CPU baseline: the oracle (the reference's LM restated, with its host

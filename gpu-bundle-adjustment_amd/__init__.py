@@ -23,10 +23,11 @@ PRECISION_FP64, PRECISION_MIXED = 0, 1
 OBS_FP64, OBS_FP16 = 0, 1  # device storage of the observed pixels
 NOALLOC_MAX_N = 1024  # GBA_NOALLOC_MAX_N of include/gba_c.h
 LAYOUT_AUTO, LAYOUT_DENSE, LAYOUT_ENVELOPE = 0, 1, 2  # enum gba_factor_layout
+PRECOND_AUTO, PRECOND_FACTOR, PRECOND_JACOBI = 0, 1, 2  # enum gba_mixed_precond
 TRACE_STRIDE = 6
 TRIAL_STATUS = {0: "accept", 1: "reject", 2: "nonspd_v", 3: "chol_fail", 4: "denom", 5: "nonfinite"}
 KERNELS = {"edge_point": 0, "edge_camera": 1, "schur": 2, "factor": 3, "solve": 4, "backsub": 5,
-           "cost": 6}
+           "cost": 6, "cg_step": 7, "cg_setup": 8}
 
 
 class ProblemView(ctypes.Structure):
@@ -47,13 +48,14 @@ class LMResultC(ctypes.Structure):
                 ("anomalies", ctypes.c_ulonglong), ("total_time_s", ctypes.c_double),
                 ("trials", ctypes.c_int), ("solve_time_s", ctypes.c_double),
                 ("setup_time_s", ctypes.c_double), ("refine_iterations", ctypes.c_int),
-                ("max_refine_residual", ctypes.c_double)]
+                ("max_refine_residual", ctypes.c_double), ("refine_factor_trials", ctypes.c_int)]
 
 
 class SolverConfigC(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("refine_tol", ctypes.c_double),
                 ("refine_max_iters", ctypes.c_int), ("trace", _dp), ("trace_capacity", ctypes.c_int),
-                ("obs_storage", ctypes.c_int), ("huber_delta", ctypes.c_double), ("factor_layout", ctypes.c_int)]
+                ("obs_storage", ctypes.c_int), ("huber_delta", ctypes.c_double), ("factor_layout", ctypes.c_int),
+                ("mixed_precond", ctypes.c_int), ("jacobi_max_iters", ctypes.c_int)]
 
 
 class SynthSpecC(ctypes.Structure):
@@ -78,6 +80,7 @@ _PROTOS = {
                                               ctypes.c_int, _dp]),
     "gba_solver_stats": (ctypes.c_int, [ctypes.c_void_p, _dp, _ip, _ip]),
     "gba_solver_layout": (ctypes.c_int, [ctypes.c_void_p, _ip, _dp, _dp]),
+    "gba_solver_mixed_precond": (ctypes.c_int, [ctypes.c_void_p, _ip]),
     "gba_solver_destroy": (None, [ctypes.c_void_p]),
     "gba_batch_create": (ctypes.c_int, [ctypes.POINTER(ProblemView), ctypes.c_int,
                                         ctypes.POINTER(ctypes.c_void_p)]),
@@ -226,19 +229,24 @@ class LMResult:
     refine_iterations: int = 0
     max_refine_residual: float = 0.0
     trace: np.ndarray = None
+    refine_factor_trials: int = 0
 
     @staticmethod
     def from_c(r, trace=None):
         return LMResult(r.initial_cost, r.final_cost, r.iterations, TERMINATION[r.termination],
                         r.anomalies, r.total_time_s, r.trials, r.solve_time_s, r.setup_time_s,
-                        r.refine_iterations, r.max_refine_residual, trace)
+                        r.refine_iterations, r.max_refine_residual, trace, r.refine_factor_trials)
 
 
 def _config(precision, trace_capacity=0, refine_tol=1e-10, refine_max_iters=12, obs_storage=OBS_FP64,
-            huber_delta=0.0, layout=None):
+            huber_delta=0.0, layout=None, precond=None, jacobi_max_iters=None):
     c = SolverConfigC()
     lib().gba_default_config(ctypes.byref(c))
     c.precision = precision
+    if precond is not None:
+        c.mixed_precond = precond
+    if jacobi_max_iters is not None:
+        c.jacobi_max_iters = jacobi_max_iters
     if layout is not None:
         c.factor_layout = layout
     c.obs_storage = obs_storage
@@ -261,13 +269,14 @@ def _trim_trace(trace):
 
 
 def solve_bal_gpu(prob, opts=None, precision=PRECISION_FP64, trace_capacity=1024, obs_storage=OBS_FP64,
-                  huber_delta=0.0, layout=None):
+                  huber_delta=0.0, layout=None, precond=None):
     """gba::solve_bal_gpu: updates prob.cameras / prob.points in place.
     huber_delta > 0 minimises sum rho(|r|^2) / 2 with the Huber loss; layout
-    is a LAYOUT_* value (default: the library's automatic choice)."""
+    is a LAYOUT_* value, precond a PRECOND_* value for the mixed refinement
+    (default: the library's automatic choices)."""
     opts = opts or LMOptions()
     cfg, trace = _config(precision, trace_capacity, obs_storage=obs_storage, huber_delta=huber_delta,
-                         layout=layout)
+                         layout=layout, precond=precond)
     res = LMResultC()
     v = prob.view()
     _check(lib().gba_solve_bal(ctypes.byref(v), ctypes.byref(opts.c()), ctypes.byref(cfg),
@@ -279,9 +288,10 @@ class Solver:
     """Device-resident solver handle: upload once, solve repeatedly."""
 
     def __init__(self, prob, precision=PRECISION_FP64, trace_capacity=1024, obs_storage=OBS_FP64,
-                 huber_delta=0.0, layout=None):
+                 huber_delta=0.0, layout=None, precond=None, jacobi_max_iters=None):
         self._cfg, self._trace = _config(precision, trace_capacity, obs_storage=obs_storage,
-                                         huber_delta=huber_delta, layout=layout)
+                                         huber_delta=huber_delta, layout=layout, precond=precond,
+                                         jacobi_max_iters=jacobi_max_iters)
         self._h = ctypes.c_void_p()
         self._prob = prob
         v = prob.view()
@@ -316,6 +326,12 @@ class Solver:
         lay, ratio, dens = ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
         _check(lib().gba_solver_layout(self._h, ctypes.byref(lay), ctypes.byref(ratio), ctypes.byref(dens)))
         return lay.value, ratio.value, dens.value
+
+    def precond(self):
+        """The mixed refinement's preconditioner (PRECOND_FACTOR or PRECOND_JACOBI; 0 for FP64)."""
+        m = ctypes.c_int()
+        _check(lib().gba_solver_mixed_precond(self._h, ctypes.byref(m)))
+        return m.value
 
     def close(self):
         if self._h:

@@ -22,6 +22,7 @@ SOURCES = [
     "k_schur.hip",
     "k_dense.hip",
     "k_precond.hip",
+    "k_cg.hip",
     "k_backsub.hip",
     "k_batch.hip",
     "k_pairs.hip",

@@ -162,6 +162,13 @@ int gba_solver_layout(gba_solver* s, int* layout, double* factor_work_ratio, dou
     });
 }
 
+int gba_solver_mixed_precond(gba_solver* s, int* mode) {
+    return guarded([&] {
+        need(s && mode, "gba_solver_mixed_precond: null argument");
+        *mode = s->s->mixed_precond();
+        return 0;
+    });
+}
 
 void gba_solver_destroy(gba_solver* s) { delete s; }
 

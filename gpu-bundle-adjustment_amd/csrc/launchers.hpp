@@ -175,6 +175,39 @@ size_t precond_work(int npad);
 void launch_precond_apply(hipStream_t st, const float* X32, int ld, int npad, const double* r,
                           double* z, double* work, const double* done = nullptr);
 
+// k_cg.hip -- block-Jacobi conjugate gradients on the FP32-stored scaled S
+// (the mixed form's factor-free refinement).  A: lower tiles valid, camera
+// blocks on the diagonal in both triangles; vectors have npad entries.
+// Matrix-product workgroups: each takes a run of up to tpw tiles of one tile
+// row, wgtab[w] = (tile row i, run c), listed row by row; rowstart[i] is the
+// index of tile row i's first run (nblk + 1 entries).  Work arrays: rp
+// (nwg_m x 64), cp (nblk (nblk + 1) / 2 x 64), dpart (nwg_m), gpart
+// (4 nwg_v), st (16).
+struct CgArgs {
+    const float* A;
+    int ld, npad, nblk, ncam, tpw;
+    const int2* wgtab;
+    const int* rowstart;
+    int nwg_m, nwg_v;
+    const double* b;
+    double *x, *r, *u, *p, *s;
+    const double* minv;  // ncam x 81: inverse of each camera block
+    double *rp, *cp, *dpart, *gpart, *st;
+    PassStatus* status;
+    const TrialFlags* flags;
+    double tol2;
+    int max_it;
+};
+int cg_vector_groups(int npad);
+// minv[c] = A_cc^-1 (FP64 Cholesky of the FP32 block); a block that is not
+// SPD raises chol_fail.
+void launch_cg_block_inverse(hipStream_t st, const float* A, int ld, int ncam, double* minv, TrialFlags* flags);
+// x = 0, r = b, u = M^-1 r (clears the done state).
+void launch_cg_start(hipStream_t st, const CgArgs& g);
+// Step k (0, 1, ... in order): the matrix product and the vector update;
+// both do nothing once the iteration has stopped.
+void launch_cg_step(hipStream_t st, const CgArgs& g, int k);
+
 // k_batch.hip
 // Per-problem status of a pass (trial = 0: linearisation SSE and anomalies;
 // trial = 1: trial SSE, gain denominator, anomalies, flags).

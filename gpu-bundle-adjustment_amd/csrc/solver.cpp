@@ -423,6 +423,7 @@ Solver::Solver(const gba_problem_view& prob, const gba_solver_config& cfg) : cfg
     }
     flags_.alloc(1);
     status_.alloc(1);
+    if (mixed_) setup_cg();  // after every buffer its kernel arguments point at
     GBA_HIP(hipHostMalloc(&h_status_, sizeof(PassStatus), hipHostMallocDefault));
     std::memset(h_status_, 0, sizeof(PassStatus));
     // points without observations keep V = 0, bp = 0; padding of S is identity;
@@ -513,14 +514,125 @@ void Solver::cost(double* sse, unsigned long long* anomalies) {
     *anomalies = h_status_->anomalies;
 }
 
-void Solver::enqueue_schur(double lambda) {
+void Solver::enqueue_schur(double lambda, bool fac_copy) {
+    // mixed: the factor copy and the lower copy of the scaled S, or the lower
+    // copy alone (written through the factor-copy argument)
+    float* sfac = mixed_ ? (fac_copy ? S32_.p : S32low_.p) : nullptr;
+    float* slow = mixed_ && fac_copy ? S32low_.p : nullptr;
     if (sep_diag_)
         launch_schur_diag(st_, pb_, nullptr, W64_.p, W32_.p, Vinv_.p, VB_.p, U_.p, bc_.p, lambda,
-                          mixed_ ? dscale_.p : nullptr, S64_.p, S32_.p, S32low_.p, pb_.npad, rhs_.p, flags_.p);
+                          mixed_ ? dscale_.p : nullptr, S64_.p, sfac, slow, pb_.npad, rhs_.p, flags_.p);
     if (num_stasks_ > 0)
         launch_schur(st_, sshape_, pb_, stasks_.p, num_stasks_, spairs_.p, W64_.p, W32_.p, Vinv_.p,
-                 VB_.p, U_.p, bc_.p, lambda, mixed_ ? dscale_.p : nullptr, S64_.p, S32_.p,
-                 S32low_.p, pb_.npad, rhs_.p, flags_.p);
+                 VB_.p, U_.p, bc_.p, lambda, mixed_ ? dscale_.p : nullptr, S64_.p, sfac,
+                 slow, pb_.npad, rhs_.p, flags_.p);
+}
+
+void Solver::setup_cg() {
+    int mode = cfg_.mixed_precond;
+    if (const char* env = std::getenv("GBA_MIXED_PRECOND")) {
+        if (std::string(env) == "factor") mode = GBA_PRECOND_FACTOR;
+        else if (std::string(env) == "jacobi") mode = GBA_PRECOND_JACOBI;
+    }
+    if (mode != GBA_PRECOND_AUTO && mode != GBA_PRECOND_FACTOR && mode != GBA_PRECOND_JACOBI)
+        throw Error("unknown mixed preconditioner");
+    if (mode == GBA_PRECOND_AUTO)
+        mode = nblk_ >= kJacobiMinBlk ? GBA_PRECOND_JACOBI : GBA_PRECOND_FACTOR;
+    cg_mode_ = mode == GBA_PRECOND_JACOBI;
+    if (!cg_mode_) return;
+    const int nblk = nblk_, npad = pb_.npad;
+    // tiles per matrix-product workgroup: pairs while the tiles fill the chip
+    // (one round of workgroups at C=356), runs of eight beyond (fewer partial
+    // sums to reduce)
+    int tpw = nblk <= kJacobiPairBlk ? 2 : 8;
+    if (const char* env = std::getenv("GBA_CG_TPW")) tpw = std::max(1, std::atoi(env));
+    std::vector<int2> tab;
+    std::vector<int> rowstart(static_cast<size_t>(nblk) + 1);
+    for (int i = 0; i < nblk; ++i) {
+        rowstart[i] = static_cast<int>(tab.size());
+        for (int c = 0; c < (i + tpw) / tpw; ++c) tab.push_back(make_int2(i, c));
+    }
+    rowstart[nblk] = static_cast<int>(tab.size());
+    upload(cg_wgtab_, tab, st_);
+    upload(cg_rowstart_, rowstart, st_);
+    const int nwg_m = static_cast<int>(tab.size()), nwg_v = cg_vector_groups(npad);
+    const size_t np = static_cast<size_t>(npad);
+    cg_minv_.alloc(81 * static_cast<size_t>(pb_.C));
+    cg_vec_.alloc(4 * np);
+    cg_rp_.alloc(static_cast<size_t>(nwg_m) * kTile);
+    cg_cp_.alloc(static_cast<size_t>(nblk) * (nblk + 1) / 2 * kTile);
+    cg_dpart_.alloc(static_cast<size_t>(nwg_m));
+    cg_gpart_.alloc(4 * static_cast<size_t>(nwg_v));
+    cg_st_.alloc(16);
+    launch_fill_double(st_, cg_st_.p, cg_st_.n, 0.0);
+    launch_fill_double(st_, cg_gpart_.p, cg_gpart_.n, 0.0);
+    CgArgs& g = cg_;
+    g.A = S32low_.p;
+    g.ld = npad;
+    g.npad = npad;
+    g.nblk = nblk;
+    g.ncam = pb_.C;
+    g.tpw = tpw;
+    g.wgtab = cg_wgtab_.p;
+    g.rowstart = cg_rowstart_.p;
+    g.nwg_m = nwg_m;
+    g.nwg_v = nwg_v;
+    g.b = rhs_.p;
+    g.x = x_.p;
+    g.r = cg_vec_.p;
+    g.u = cg_vec_.p + np;
+    g.p = cg_vec_.p + 2 * np;
+    g.s = cg_vec_.p + 3 * np;
+    g.minv = cg_minv_.p;
+    g.rp = cg_rp_.p;
+    g.cp = cg_cp_.p;
+    g.dpart = cg_dpart_.p;
+    g.gpart = cg_gpart_.p;
+    g.st = cg_st_.p;
+    g.status = status_.p;
+    g.flags = flags_.p;
+}
+
+// Every device pointer of the block-Jacobi kernels' arguments must be set
+// before a launch (a null one would fault the GPU): checked on the host.
+static void check_cg_args(const CgArgs& g) {
+    const void* ptrs[] = {g.A, g.wgtab, g.rowstart, g.b, g.x, g.r, g.u, g.p, g.s, g.minv,
+                          g.rp, g.cp, g.dpart, g.gpart, g.st, g.status, g.flags};
+    for (const void* q : ptrs)
+        if (!q) throw Error("block-Jacobi refinement: unset device buffer");
+    if (g.nwg_m <= 0 || g.nwg_v <= 0 || g.tpw <= 0 || g.nblk * kTile != g.npad || 9 * g.ncam > g.npad)
+        throw Error("block-Jacobi refinement: inconsistent shape");
+}
+
+int Solver::cg_solve(double tol, int max_it, double* rel, bool* converged) {
+    CgArgs g = cg_;
+    g.tol2 = tol * tol;
+    g.max_it = max_it;
+    check_cg_args(g);
+    launch_cg_block_inverse(st_, S32low_.p, pb_.npad, pb_.C, cg_minv_.p, flags_.p);
+    launch_cg_start(st_, g);
+    // steps k = 0 .. max_it (the launch of step max_it only ends the loop),
+    // the first batch sized by the last refinement's count, then batches of 8
+    int enq = 0, batch = std::max(4, cg_last_ + std::max(2, cg_last_ / 8));
+    for (;;) {
+        const int nb = std::min(batch, max_it + 1 - enq);
+        for (int k = enq; k < enq + nb; ++k) launch_cg_step(st_, g, k);
+        enq += nb;
+        sync_status();
+        if (h_status_->pcg_done || enq > max_it) break;
+        batch = 8;
+    }
+    const PassStatus& h = *h_status_;
+    const TrialFlags f = h.flags;
+    *rel = 0.0;
+    *converged = true;
+    if (f.nonspd || f.chol_fail || f.solve_err) return 0;
+    const double bb = h.pcg_bb;
+    if (!(bb > 0.0)) return 0;
+    *rel = std::sqrt(h.pcg_rr / bb);
+    *converged = !h.pcg_stop && h.pcg_rr <= g.tol2 * bb;
+    if (*converged) cg_last_ = h.pcg_it;
+    return h.pcg_it;
 }
 
 void Solver::enqueue_trial_linear_algebra(double lambda, std::vector<double>* hook_S,
@@ -529,7 +641,7 @@ void Solver::enqueue_trial_linear_algebra(double lambda, std::vector<double>* ho
     launch_clear_flags(st_, flags_.p);
     launch_damp_invert_v(st_, pb_.P, VB_.p, lambda, Vinv_.p, flags_.p);
     if (mixed_) launch_diag_scale(st_, pb_.n, pb_.npad, U_.p, lambda, dscale_.p);
-    enqueue_schur(lambda);
+    enqueue_schur(lambda, !cg_active());
     if (hook_S || hook_rhs) {
         GBA_HIP(hipGetLastError());
         GBA_HIP(hipStreamSynchronize(st_));
@@ -541,7 +653,8 @@ void Solver::enqueue_trial_linear_algebra(double lambda, std::vector<double>* ho
             std::vector<double> S(np * np);
             if (mixed_) {
                 std::vector<float> Sf(np * np);
-                GBA_HIP(hipMemcpy(Sf.data(), S32_.p, sizeof(float) * np * np, hipMemcpyDeviceToHost));
+                GBA_HIP(hipMemcpy(Sf.data(), cg_active() ? S32low_.p : S32_.p, sizeof(float) * np * np,
+                                  hipMemcpyDeviceToHost));
                 for (size_t i = 0; i < np * np; ++i) S[i] = Sf[i];
             } else {
                 GBA_HIP(hipMemcpy(S.data(), S64_.p, sizeof(double) * np * np, hipMemcpyDeviceToHost));
@@ -571,10 +684,25 @@ void Solver::enqueue_trial_linear_algebra(double lambda, std::vector<double>* ho
                           sync_.p, flags_.p, env_ ? first_.p : nullptr);
         return;
     }
-    // mixed: FP32 factor of D S D; the camera system is solved in FP64 by CG
-    // preconditioned with that factor, against the FP32-assembled D S D
-    enqueue_factor();
+    // mixed: the camera system D S D x = D rhs is solved in FP64 by CG against
+    // the FP32-assembled D S D, preconditioned by its camera blocks or by its
+    // FP32 factor
     const double tol = cfg_.refine_tol > 0 ? cfg_.refine_tol : 1e-10;
+    if (cg_active()) {
+        const int jmax = cfg_.jacobi_max_iters > 0 ? cfg_.jacobi_max_iters : 200;
+        bool conv = false;
+        out.refine_iters = cg_solve(tol, jmax, &out.refine_rel, &conv);
+        if (conv) {
+            launch_vec_scale(st_, pb_.n, pb_.npad, x_.p, dscale_.p, dc_.p);
+            return;
+        }
+        // missed the tolerance within the cap: this trial and the rest of the
+        // solve are refined with the FP32 factor (its copy of S from the lower one)
+        cg_off_ = true;
+        GBA_HIP(hipMemcpyAsync(S32_.p, S32low_.p, sizeof(float) * S32_.n, hipMemcpyDeviceToDevice, st_));
+    }
+    out.refined_by_factor = true;
+    enqueue_factor();
     const int max_it = cfg_.refine_max_iters > 0 ? cfg_.refine_max_iters : 12;
     // first batch: the previous trial's step count (the counts of one solve
     // differ by about one); GBA_PCG_BATCH=1 in the environment enqueues one
@@ -704,6 +832,8 @@ void Solver::download_step(double* dc, double* dp) {
 }
 
 double Solver::time_kernel(int kernel, double lambda, int reps) {
+    CgArgs cgt = cg_;
+    int cg_k = 0;
     hipEvent_t a, b;
     GBA_HIP(hipEventCreate(&a));
     GBA_HIP(hipEventCreate(&b));
@@ -728,6 +858,7 @@ double Solver::time_kernel(int kernel, double lambda, int reps) {
                 // FP64: both triangular solves; mixed: one preconditioner application
                 if (mixed_)
                 {
+                    if (cg_active()) throw Error("no factor preconditioner on a block-Jacobi handle");
                     const PcgPrecond M = precond();
                     if (M.X32)
                         launch_precond_apply(st_, M.X32, ld, pb_.npad, rhs_.p, x_.p,
@@ -748,6 +879,12 @@ double Solver::time_kernel(int kernel, double lambda, int reps) {
             case GBA_KERNEL_COST:
                 launch_cost(st_, pb_, cams_[cur_].p, pts_[cur_].p, blk_sse_.p, blk_anom_.p, status_.p);
                 break;
+            case GBA_KERNEL_CG_SETUP:
+                launch_cg_block_inverse(st_, S32low_.p, pb_.npad, pb_.C, cg_minv_.p, flags_.p);
+                break;
+            case GBA_KERNEL_CG_STEP:
+                launch_cg_step(st_, cgt, cg_k++);
+                break;
             default:
                 throw Error("unknown kernel id");
         }
@@ -757,6 +894,16 @@ double Solver::time_kernel(int kernel, double lambda, int reps) {
     unsigned long long an;
     linearize(&sse, &an);
     (void)trial(lambda);
+    if ((kernel == GBA_KERNEL_CG_STEP || kernel == GBA_KERNEL_CG_SETUP) && !cg_active())
+        throw Error("the block-Jacobi refinement is not in use on this handle");
+    if (kernel == GBA_KERNEL_CG_STEP) {
+        check_cg_args(cgt);
+        // steps that never stop: no tolerance, no cap
+        cgt.tol2 = 0.0;
+        cgt.max_it = INT_MAX;
+        launch_cg_block_inverse(st_, S32low_.p, pb_.npad, pb_.C, cg_minv_.p, flags_.p);
+        launch_cg_start(st_, cgt);
+    }
     if (kernel == GBA_KERNEL_FACTOR) {
         // refactor the same matrix each repetition: re-assemble first
         double total = 0.0;
@@ -764,7 +911,7 @@ double Solver::time_kernel(int kernel, double lambda, int reps) {
             launch_clear_flags(st_, flags_.p);
             launch_damp_invert_v(st_, pb_.P, VB_.p, lambda, Vinv_.p, flags_.p);
             if (mixed_) launch_diag_scale(st_, pb_.n, pb_.npad, U_.p, lambda, dscale_.p);
-            enqueue_schur(lambda);
+            enqueue_schur(lambda, true);
             GBA_HIP(hipEventRecord(a, st_));
             enqueue();
             GBA_HIP(hipEventRecord(b, st_));
@@ -802,6 +949,8 @@ void Solver::solve(const gba_lm_options& opts, gba_lm_result* res, double t_entr
     res->trials = 0;
     res->refine_iterations = 0;
     res->max_refine_residual = 0.0;
+    res->refine_factor_trials = 0;
+    cg_off_ = false;
     double lambda = opts.initial_lambda;
     bool converged = false, failed = false;
     double rel_decrease = 0.0;
@@ -815,6 +964,7 @@ void Solver::solve(const gba_lm_options& opts, gba_lm_result* res, double t_entr
             TrialOut t = trial(lambda);
             res->trials++;
             res->refine_iterations += t.refine_iters;
+            if (t.refined_by_factor) res->refine_factor_trials++;
             if (!t.flags.nonspd && !t.flags.chol_fail && !t.flags.solve_err)
                 res->max_refine_residual = std::max(res->max_refine_residual, t.refine_rel);
             int code;

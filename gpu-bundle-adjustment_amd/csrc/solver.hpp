@@ -99,6 +99,15 @@ size_t pcg_work(int npad);
 // solves; n = 17,424, 272 columns, inverse 29 ms to form against 1.7 ms per
 // pair of solves).
 constexpr int kInversePrecondBlk = 128;
+// Smallest tile-column count of S at which GBA_PRECOND_AUTO refines the mixed
+// form with block-Jacobi CG instead of the FP32 factor: below it the factor's
+// chain of tile columns is short (measured on MI355X: mixed solve at C=356,
+// 51 columns, 17.3 ms with block Jacobi against 17.9 ms with the factor; at
+// n = 17,424, 273 columns, 0.10 s against 0.34 s).
+constexpr int kJacobiMinBlk = 24;
+// Tile-column count up to which a block-Jacobi matrix-product workgroup takes
+// two tiles (enough workgroups to fill the chip), runs of eight beyond.
+constexpr int kJacobiPairBlk = 96;
 struct PcgPrecond {
     const float* X32 = nullptr;
     const float* L32 = nullptr;  // factor (lower tiles), with D32 (diagonal tiles and inverses)
@@ -137,6 +146,7 @@ struct TrialOut {
     unsigned long long anomalies = 0;
     int refine_iters = 0;
     double refine_rel = 0.0;
+    bool refined_by_factor = false;  // mixed: the FP32 factor preconditioned this trial
 };
 
 class Solver {
@@ -172,14 +182,24 @@ public:
         *density = env_density_;
     }
     int n() const { return pb_.n; }
+    // mixed refinement preconditioner of the handle (enum gba_mixed_precond)
+    int mixed_precond() const { return !mixed_ ? 0 : (cg_mode_ ? GBA_PRECOND_JACOBI : GBA_PRECOND_FACTOR); }
     int npad() const { return pb_.npad; }
 
 private:
     void enqueue_trial_linear_algebra(double lambda, std::vector<double>* hook_S,
                                       std::vector<double>* hook_rhs, TrialOut& out);
     void sync_status();
-    void enqueue_schur(double lambda);
+    // fac_copy: also write the FP32 factor copy of the scaled S (mixed; the
+    // block-Jacobi refinement reads only the lower copy)
+    void enqueue_schur(double lambda, bool fac_copy = true);
     void enqueue_factor();  // the dense factor of S (and X = L^-1 in mixed precision)
+    // Mixed, block-Jacobi refinement (k_cg.hip): solves the scaled system into
+    // x_; returns the steps, *rel = |r| / |b|, *converged when |r| <= tol |b|
+    // (or the trial failed before the solve, which needs no refinement).
+    void setup_cg();
+    int cg_solve(double tol, int max_it, double* rel, bool* converged);
+    bool cg_active() const { return cg_mode_ && !cg_off_; }
 
     gba_solver_config cfg_;
     bool mixed_;
@@ -211,6 +231,14 @@ private:
     DevBuf<float> S32_, S32low_, D32_, X32_, T32_;
     DevBuf<double> rhs_, y_, x_, dc_, dscale_, pcgw_;
     int last_refine_ = 3;  // refinement steps of the last trial (first batch of the next)
+    // block-Jacobi refinement: chosen for the handle (cg_mode_), switched off
+    // for the rest of a solve after a trial missed the tolerance (cg_off_)
+    bool cg_mode_ = false, cg_off_ = false;
+    int cg_last_ = 40;  // steps of the last block-Jacobi refinement
+    CgArgs cg_{};
+    DevBuf<int2> cg_wgtab_;
+    DevBuf<int> cg_rowstart_;
+    DevBuf<double> cg_minv_, cg_vec_, cg_rp_, cg_cp_, cg_dpart_, cg_gpart_, cg_st_;
     // mixed: precondition by the factor's triangular solves instead of the
     // explicit inverse (large matrices; see kInversePrecondBlk)
     bool precond_trsv_ = false;

@@ -71,13 +71,17 @@ typedef struct gba_lm_result {
                                      S the FP32-stored reduced camera matrix (built from
                                      FP32 W), products accumulated in FP64; not a residual
                                      of the FP64 system (about 2e-8 against it, measured) */
+    int refine_factor_trials;     /* mixed precision: trials refined with the FP32 factor
+                                     (all of them with GBA_PRECOND_FACTOR; with block Jacobi,
+                                     those after a trial missed the tolerance) */
 } gba_lm_result;
 
 /* Arithmetic of the reduced camera solve. */
 enum gba_precision {
     GBA_PRECISION_FP64 = 0,  /* plain drop-in: FP64 throughout */
-    GBA_PRECISION_MIXED = 1  /* FP32 W, S and MFMA factor; FP64 PCG on the FP32-stored S
-                                preconditioned by that factor; FP64 cost, gain ratio,
+    GBA_PRECISION_MIXED = 1  /* FP32 W and S; FP64 PCG on the FP32-stored S, preconditioned
+                                by its FP32 MFMA factor or by its camera blocks (enum
+                                gba_mixed_precond); FP64 cost, gain ratio,
                                 back-substitution */
 };
 
@@ -104,7 +108,20 @@ typedef struct gba_solver_config {
                                IRLS-weighted linearisation.  0 (default): plain least
                                squares, as solve_bal_gpu. */
     int factor_layout;      /* enum gba_factor_layout (default AUTO) */
+    int mixed_precond;      /* mixed: enum gba_mixed_precond (default AUTO) */
+    int jacobi_max_iters;   /* mixed, block-Jacobi refinement: step cap per trial; a trial
+                               that misses refine_tol within it is refined with the FP32
+                               factor instead, and so is the rest of the solve
+                               (0: default 200) */
 } gba_solver_config;
+
+/* Preconditioner of the mixed form's FP64 refinement (no reference
+ * counterpart).  FACTOR: the FP32 MFMA Cholesky factor of the scaled S (two or
+ * three steps per trial).  JACOBI: the inverses of S's 9x9 camera blocks, no
+ * factorisation (tens of steps, each one pass over S).  AUTO: JACOBI from 24
+ * tile columns of S up (9C above about 1,500), FACTOR below.
+ * GBA_MIXED_PRECOND=factor|jacobi in the environment overrides. */
+enum gba_mixed_precond { GBA_PRECOND_AUTO = 0, GBA_PRECOND_FACTOR = 1, GBA_PRECOND_JACOBI = 2 };
 
 /* Layout of the reduced camera system's factor (no reference counterpart;
  * the reference always factors a dense S, src/lm.cu:597-604, and names a
@@ -155,7 +172,10 @@ enum gba_kernel_id {
     GBA_KERNEL_FACTOR = 3,      /* dense Cholesky of S (all its launches); mixed: + explicit L^-1 */
     GBA_KERNEL_SOLVE = 4,       /* FP64: both triangular solves; mixed: one preconditioner apply */
     GBA_KERNEL_BACKSUB = 5,     /* point back-substitution + trial cost */
-    GBA_KERNEL_COST = 6         /* residual-only cost pass */
+    GBA_KERNEL_COST = 6,        /* residual-only cost pass */
+    GBA_KERNEL_CG_STEP = 7,     /* mixed, block Jacobi: one refinement step (matrix product
+                                   over S + vector update) */
+    GBA_KERNEL_CG_SETUP = 8     /* mixed, block Jacobi: inverses of the camera blocks */
 };
 int gba_solver_time_kernel(gba_solver* s, int kernel, double lambda, int reps, double* avg_ms);
 /* Shape facts used for roofline accounting: sum_p k_p(k_p+1)/2, matrix
@@ -166,6 +186,9 @@ int gba_solver_stats(gba_solver* s, double* pair_count, int* n, int* n_padded);
  * the camera graph's block density (both 1 when the graph was not examined
  * or was found too dense).  Any output may be NULL. */
 int gba_solver_layout(gba_solver* s, int* layout, double* factor_work_ratio, double* graph_density);
+/* The preconditioner of the handle's mixed refinement (enum gba_mixed_precond:
+ * FACTOR or JACOBI, AUTO resolved; 0 for a plain FP64 handle). */
+int gba_solver_mixed_precond(gba_solver* s, int* mode);
 void gba_solver_destroy(gba_solver* s);
 
 /* ---- batched solve (no reference counterpart) --------------------------- */

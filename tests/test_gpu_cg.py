@@ -1,0 +1,99 @@
+"""Block-Jacobi conjugate gradients (k_cg.hip), the factor-free refinement of
+the mixed form: the same FP32-stored reduced camera system as the factor
+preconditioner, solved to the same 1e-10 relative residual, so the step
+agrees with the factor-preconditioned one to that residual amplified by the
+system's condition, and the LM trajectory and final cost follow.  Against the
+FP64 system (the oracle, the reference's src/lm.cu:515-604) the step carries
+the same FP32 rounding of W and S as the factor path's."""
+import numpy as np
+import pytest
+
+from test_gpu_path import arel, heavy, ragged
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("cfg,lam", [(2, 1e-4), (1, 1e-3)])
+def test_cg_step_matches_factor_step(gba, orc, monkeypatch, cfg, lam):
+    prob = gba.generate(cfg, 1)
+    monkeypatch.setenv("GBA_MIXED_PRECOND", "factor")
+    f = gba.schur_step(prob, lam, gba.PRECISION_MIXED, want_S=True)
+    monkeypatch.setenv("GBA_MIXED_PRECOND", "jacobi")
+    j = gba.schur_step(prob, lam, gba.PRECISION_MIXED, want_S=True)
+    assert f["status"] == j["status"] == "accept"
+    # the same FP32 assembly (written to the lower copy alone in the Jacobi form)
+    assert np.array_equal(np.tril(j["S"]), np.tril(f["S"]))
+    assert np.array_equal(j["rhs"], f["rhs"])
+    gap = arel(j["dc"], f["dc"])
+    S = np.tril(j["S"]) + np.tril(j["S"], -1).T
+    res32 = np.linalg.norm(S @ j["dc"] - j["rhs"]) / np.linalg.norm(j["rhs"])
+    o = orc.schur_step(prob, orc.linearize(prob), lam, want_S=True)
+    S64 = np.tril(o["S"]) + np.tril(o["S"], -1).T
+    res64 = np.linalg.norm(S64 @ j["dc"] - o["rhs"]) / np.linalg.norm(o["rhs"])
+    print(f"cfg {cfg}: Jacobi vs factor step {gap:.2e}; residual of the FP32-stored system {res32:.2e}, "
+          f"of the FP64 system {res64:.2e}")
+    assert gap < 1e-6
+    assert res32 < 1e-8  # 1e-10 on the scaled system, unscaled here
+    assert res64 < 1e-7  # as the factor path's (FP32 W and S)
+    assert abs(j["denom"] - f["denom"]) <= 1e-8 * abs(f["denom"])
+
+
+@pytest.mark.parametrize("which", ["cfg2", "heavy", "ragged"])
+def test_cg_lm_matches_factor_lm(gba, which):
+    prob = {"cfg2": lambda: gba.generate(2, 1), "heavy": lambda: heavy(gba), "ragged": lambda: ragged(gba)}[which]()
+    opts = gba.LMOptions(initial_lambda=1e-4)
+    fs = gba.Solver(prob, gba.PRECISION_MIXED, precond=gba.PRECOND_FACTOR)
+    js = gba.Solver(prob, gba.PRECISION_MIXED, precond=gba.PRECOND_JACOBI)
+    assert fs.precond() == gba.PRECOND_FACTOR and js.precond() == gba.PRECOND_JACOBI
+    f, j = fs.solve(opts), js.solve(opts)
+    print(f"{which}: Jacobi {j.refine_iterations} steps over {j.trials} trials, factor {f.refine_iterations}; "
+          f"final costs {j.final_cost:.12g} / {f.final_cost:.12g}")
+    assert j.termination == f.termination == "CONVERGENCE"
+    assert j.iterations == f.iterations and j.trials == f.trials
+    assert np.array_equal(j.trace[:, 4], f.trace[:, 4])
+    assert abs(j.final_cost - f.final_cost) <= 1e-9 * f.final_cost
+    assert j.refine_factor_trials == 0 and f.refine_factor_trials == f.trials
+    assert 0.0 < j.max_refine_residual <= 1e-10
+    # repeatable: fixed-order sums throughout
+    j2 = js.solve(opts)
+    assert j2.final_cost == j.final_cost and j2.refine_iterations == j.refine_iterations
+
+
+def test_cg_step_cap_falls_back_to_factor(gba):
+    """A trial that misses the tolerance within jacobi_max_iters is refined
+    with the FP32 factor instead, and so is the rest of the solve: the result
+    is the factor-preconditioned solve's."""
+    prob = gba.generate(2, 1)
+    opts = gba.LMOptions(initial_lambda=1e-4)
+    f = gba.Solver(prob, gba.PRECISION_MIXED, precond=gba.PRECOND_FACTOR).solve(opts)
+    c = gba.Solver(prob, gba.PRECISION_MIXED, precond=gba.PRECOND_JACOBI, jacobi_max_iters=3).solve(opts)
+    assert c.refine_factor_trials == c.trials == f.trials
+    assert c.iterations == f.iterations
+    assert abs(c.final_cost - f.final_cost) <= 1e-12 * f.final_cost
+
+
+def test_cg_camera_block_not_spd_is_a_factor_failure(gba, orc):
+    """lambda = 0 with an unobserved camera: its S block is zero, not SPD.  The
+    block inverse raises the failed-factorisation status, as the reference's
+    LLT failure does (src/lm.cu:597-601): the oracle's trial statuses."""
+    p = gba.generate(2, 1)
+    q = gba.BALProblem(np.vstack([p.cameras, p.cameras[:1] * 1.01]), p.points.copy(), p.obs_camera.copy(),
+                       p.obs_point.copy(), p.obs_pixel.copy())
+    opts = dict(initial_lambda=0.0, max_iterations=2)
+    r = gba.Solver(q, gba.PRECISION_MIXED, precond=gba.PRECOND_JACOBI).solve(gba.LMOptions(**opts))
+    o = orc.lm_solve(q.copy(), **opts)
+    assert np.array_equal(r.trace[:, 4], o["trace"][:, 4])
+    assert r.trace[0, 4] == 3  # GBA_TRIAL_CHOL_FAIL
+    assert r.termination == o["termination"]
+
+
+def test_cg_kernel_timers(gba):
+    s = gba.Solver(gba.generate(2, 1), gba.PRECISION_MIXED, precond=gba.PRECOND_JACOBI)
+    assert s.time_kernel("cg_step", 1e-4, 5) > 0.0
+    assert s.time_kernel("cg_setup", 1e-4, 5) > 0.0
+    assert s.time_kernel("factor", 1e-4, 2) > 0.0  # the alternative, assembled with its copy
+    with pytest.raises(gba.GBAError):
+        s.time_kernel("solve", 1e-4, 1)
+    f = gba.Solver(gba.generate(2, 1), gba.PRECISION_MIXED, precond=gba.PRECOND_FACTOR)
+    with pytest.raises(gba.GBAError):
+        f.time_kernel("cg_step", 1e-4, 1)
