@@ -1,0 +1,56 @@
+"""Launch-shape invariants of the block-Jacobi CG kernels (k_cg.hip), replayed
+on the CPU for the shapes Solver::setup_cg builds: the matrix-product table
+(workgroup -> (tile row, run), runs of up to tpw tiles listed row by row, and
+each tile row's first run) covers every lower tile exactly once, and the
+partial-sum terms k_cg_vector adds for each row -- runs w, w + 4, ... of its
+tile row (ceil((nc - w) / 4) of them) and the column partials of the tiles
+(J, I), J = I + w, I + w + 4, ... -- are each of the row's partials exactly
+once, with every index inside the buffers the solver allocates."""
+import pytest
+
+
+def tables(nblk, tpw):  # as Solver::setup_cg builds wgtab and rowstart
+    tab, rowstart = [], []
+    for i in range(nblk):
+        rowstart.append(len(tab))
+        tab.extend((i, c) for c in range((i + tpw) // tpw))
+    rowstart.append(len(tab))
+    return tab, rowstart
+
+
+@pytest.mark.parametrize("C", [1, 2, 7, 8, 49, 50, 120, 356, 1936])
+@pytest.mark.parametrize("tpw", [1, 2, 3, 8])
+def test_cg_launch_shapes(C, tpw):
+    npad = (9 * C + 63) // 64 * 64
+    nblk = npad // 64
+    tab, rowstart = tables(nblk, tpw)
+    nwg_m = len(tab)
+    seen = []
+    for i, c in tab:  # k_cg_matvec: j0 = c tpw, j1 = min(j0 + tpw, i + 1)
+        j0 = c * tpw
+        j1 = min(j0 + tpw, i + 1)
+        assert j0 < j1
+        seen.extend((i, j) for j in range(j0, j1))
+    assert sorted(seen) == [(i, j) for i in range(nblk) for j in range(i + 1)]
+    n_cp = nblk * (nblk + 1) // 2 * 64
+    nwg_v = (npad + 62) // 63  # cg_vector_groups: 63 rows (seven cameras) each
+    for blk in range(nwg_v):
+        for lane in range(63):
+            row = 63 * blk + lane
+            if row >= npad:
+                continue
+            I, rr = row >> 6, row & 63
+            nc = (I + tpw) // tpw
+            assert nc == rowstart[I + 1] - rowstart[I]
+            runs, cols = [], []
+            for w in range(4):
+                for q in range((nc - w + 3) // 4 if nc > w else 0):
+                    runs.append(w + 4 * q)
+                    assert (rowstart[I] + w + 4 * q) * 64 + rr < nwg_m * 64
+                for J in range(I + w, nblk, 4):
+                    cols.append(J)
+                    assert (J * (J + 1) // 2 + I) * 64 + rr < n_cp
+            assert sorted(runs) == list(range(nc))
+            assert sorted(cols) == list(range(I, nblk))
+            if row < 9 * C:  # the camera's nine rows lie inside the workgroup
+                assert 9 * (lane // 9) + 8 < 63 and (63 * blk) // 9 + lane // 9 == row // 9
