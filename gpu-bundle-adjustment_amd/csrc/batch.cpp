@@ -229,6 +229,10 @@ BatchSolver::BatchSolver(const gba_problem_view* probs, int count, bool autotune
     lam_.alloc(count);
     active_.alloc(count);
     accept_.alloc(count);
+    lin_active_.alloc(count);
+    trial_active_.alloc(count);
+    lm_.alloc(count);
+    counters_.alloc(2);
     flags_.alloc(count);
     lin_status_.alloc(count);
     trial_status_.alloc(count);
@@ -236,6 +240,7 @@ BatchSolver::BatchSolver(const gba_problem_view* probs, int count, bool autotune
     GBA_HIP(hipHostMalloc(&h_status_, 2 * sizeof(PassStatus) * count, hipHostMallocDefault));
     GBA_HIP(hipHostMalloc(&h_lam_, sizeof(double) * count, hipHostMallocDefault));
     GBA_HIP(hipHostMalloc(&h_mask_, sizeof(int) * count, hipHostMallocDefault));
+    GBA_HIP(hipHostMalloc(&h_counters_, 2 * sizeof(int), hipHostMallocDefault));
     // a point without observations is in no work unit, so the point pass
     // never writes its V and bp: they stay 0 (its damped V is then
     // lambda 1e-12 I, SPD for lambda > 0, and its step is 0, as in the
@@ -249,12 +254,13 @@ BatchSolver::BatchSolver(const gba_problem_view* probs, int count, bool autotune
     GBA_HIP(hipGetLastError());
     GBA_HIP(hipStreamSynchronize(st_));
     if (const char* env = std::getenv("GBA_BATCH_GRAPH")) use_graphs_ = std::string(env) != "0";
+    if (const char* env = std::getenv("GBA_BATCH_ROUNDS_PER_SYNC")) rounds_per_sync_ = std::max(1, std::atoi(env));
     if (autotune) autotune_factor();
 }
 
 BatchSolver::~BatchSolver() {
-    if (lin_exec_) (void)hipGraphExecDestroy(lin_exec_);
-    if (trial_exec_) (void)hipGraphExecDestroy(trial_exec_);
+    if (round_exec_) (void)hipGraphExecDestroy(round_exec_);
+    if (h_counters_) (void)hipHostFree(h_counters_);
     if (h_status_) (void)hipHostFree(h_status_);
     if (h_lam_) (void)hipHostFree(h_lam_);
     if (h_mask_) (void)hipHostFree(h_mask_);
@@ -274,24 +280,37 @@ void BatchSolver::enqueue_lin(const DevProblem& pl) {
     launch_lin_cameras(st_, pl, cams_.p, pts_.p, gram_.p);
     launch_lin_reduce(st_, pl, gram_.p, unit_sse_.p, unit_anom_.p, U_.p, bc_.p, nullptr);
     launch_batch_status(st_, B_, desc_.p, 0, unit_sse_.p, unit_den_.p, unit_anom_.p, dc_.p, U_.p, bc_.p,
-                        lam_.p, flags_.p, active_.p, lin_status_.p);
+                        lam_.p, flags_.p, pl.active, lin_status_.p);
 }
 
 void BatchSolver::enqueue_trial(const DevProblem& pl, bool tile_factor) {
     launch_fill_int(st_, reinterpret_cast<int*>(flags_.p), 4 * B_, 0);
-    launch_damp_invert_v_batched(st_, pb_.P, VB_.p, pt_prob_.p, lam_.p, active_.p, Vinv_.p, flags_.p);
+    launch_damp_invert_v_batched(st_, pb_.P, VB_.p, pt_prob_.p, lam_.p, pl.active, Vinv_.p, flags_.p);
     launch_schur_diag(st_, pl, desc_.p, W_.p, nullptr, Vinv_.p, VB_.p, U_.p, bc_.p, 0.0, nullptr, S_.p,
                       nullptr, nullptr, 0, rhs_.p, flags_.p);
     if (num_stasks_ > 0)
         launch_schur(st_, sshape_, pl, stasks_.p, num_stasks_, spairs_.p, W_.p, nullptr, Vinv_.p, VB_.p,
                      U_.p, bc_.p, 0.0, nullptr, S_.p, nullptr, nullptr, 0, rhs_.p, flags_.p);
-    if (tile_factor) launch_batch_cholesky(st_, B_, ftab_, desc_.p, S_.p, D_.p, flags_.p, active_.p);
-    launch_batch_chol_solve(st_, B_, desc_.p, S_.p, D_.p, rhs_.p, dc_.p, flags_.p, active_.p, !tile_factor);
+    if (tile_factor) launch_batch_cholesky(st_, B_, ftab_, desc_.p, S_.p, D_.p, flags_.p, pl.active);
+    launch_batch_chol_solve(st_, B_, desc_.p, S_.p, D_.p, rhs_.p, dc_.p, flags_.p, pl.active, !tile_factor);
     launch_trial_cams(st_, pb_.C, cams_.p, dc_.p, tcams_.p);
     launch_backsub(st_, pl, W_.p, nullptr, Vinv_.p, VB_.p, dc_.p, pts_.p, tcams_.p, tpts_.p, 0.0,
                    unit_sse_.p, unit_den_.p, unit_anom_.p, flags_.p);
     launch_batch_status(st_, B_, desc_.p, 1, unit_sse_.p, unit_den_.p, unit_anom_.p, dc_.p, U_.p, bc_.p,
-                        lam_.p, flags_.p, active_.p, trial_status_.p);
+                        lam_.p, flags_.p, pl.active, trial_status_.p);
+}
+
+void BatchSolver::enqueue_round(const gba_lm_options& opts) {
+    DevProblem pl = pb_, pt = pb_;
+    pl.active = lin_active_.p;
+    pt.active = trial_active_.p;
+    enqueue_lin(pl);
+    launch_batch_lm_lin(st_, B_, opts, lin_status_.p, lm_.p, lin_active_.p, trial_active_.p, counters_.p);
+    enqueue_trial(pt, tile_factor_);
+    launch_batch_lm_trial(st_, B_, opts, trial_status_.p, lm_.p, trial_active_.p, accept_.p, lin_active_.p,
+                          lam_.p, counters_.p);
+    launch_batch_accept(st_, pb_.C, pb_.P, cam_prob_.p, pt_prob_.p, accept_.p, tcams_.p, tpts_.p, cams_.p,
+                        pts_.p);
 }
 
 template <class F>
@@ -374,113 +393,45 @@ void BatchSolver::solve(const gba_lm_options& opts, gba_lm_result* res) {
         GBA_HIP(hipMemcpyAsync(dst, init_pts_.data(), sizeof(double) * init_pts_.size(), hipMemcpyHostToDevice,
                                st_));
     const int B = B_;
-    std::vector<double> sse(B, 0.0), lambda(B, opts.initial_lambda), nu(B, 2.0);
-    std::vector<unsigned long long> counter(B, 0);
-    std::vector<int> need_lin(B, 1), done(B, 0), inner(B, 0), converged(B, 0), failed(B, 0);
-    std::vector<int> lin_mask(B), trial_mask(B), acc_mask(B);
-    for (int b = 0; b < B; ++b) {
-        res[b] = gba_lm_result{};
+    // The LM rules run on the device (k_batch_lm_*): a round is one captured
+    // sequence (re-linearise the problems that accepted, apply the rules,
+    // evaluate a trial for every running problem, apply the rules, copy the
+    // accepted parameters), replayed rounds_per_sync times between reads of
+    // the running-problem count.  A round after every problem has finished
+    // runs with every switch off.
+    launch_batch_lm_init(st_, B, opts.initial_lambda, lm_.p, lam_.p, lin_active_.p, trial_active_.p, accept_.p,
+                         counters_.p);
+    if (round_exec_ && std::memcmp(&round_opts_, &opts, sizeof opts) != 0) {
+        (void)hipGraphExecDestroy(round_exec_);
+        round_exec_ = nullptr;
     }
-    rounds_ = 0;
-    DevProblem pl = pb_;
-    pl.active = active_.p;
+    round_opts_ = opts;
+    // an upper bound on the rounds any problem can need: every outer
+    // iteration ends with an accept after at most max_inner_trials trials
+    const long long max_rounds =
+        static_cast<long long>(std::max(0, opts.max_iterations)) * std::max(1, opts.max_inner_trials) + 2;
+    long long enq = 0;
     for (;;) {
-        int nlin = 0, ntrial = 0;
-        for (int b = 0; b < B; ++b) {
-            lin_mask[b] = (!done[b] && need_lin[b]) ? 1 : 0;
-            nlin += lin_mask[b];
-        }
-        if (nlin > 0) {
-            upload_mask(lin_mask, active_.p);
-            run_sequence(lin_exec_, [&] { enqueue_lin(pl); });
-            GBA_HIP(hipMemcpyAsync(h_status_, lin_status_.p, sizeof(PassStatus) * B, hipMemcpyDeviceToHost, st_));
-            GBA_HIP(hipGetLastError());
-            GBA_HIP(hipStreamSynchronize(st_));
-            for (int b = 0; b < B; ++b) {
-                if (!lin_mask[b]) continue;
-                counter[b] += h_status_[b].anomalies;
-                if (res[b].trials == 0 && res[b].iterations == 0) {
-                    sse[b] = h_status_[b].sse;  // initial linearisation: the starting cost
-                    res[b].initial_cost = sse[b] / 2.0;
-                }
-                need_lin[b] = 0;
-                // the reference's outer loop condition, evaluated after the
-                // linearisation that ends an accepted iteration
-                if (res[b].iterations >= opts.max_iterations) done[b] = 1;
-            }
-        }
-        for (int b = 0; b < B; ++b) {
-            // no inner trial allowed: the reference's inner loop runs zero
-            // times and the solve fails (src/lm.cu:462,704-707)
-            if (!done[b] && opts.max_inner_trials <= 0) {
-                failed[b] = 1;
-                done[b] = 1;
-            }
-            trial_mask[b] = done[b] ? 0 : 1;
-            ntrial += trial_mask[b];
-        }
-        if (ntrial == 0) break;
-        ++rounds_;
-        for (int b = 0; b < B; ++b) h_lam_[b] = lambda[b];
-        GBA_HIP(hipMemcpyAsync(lam_.p, h_lam_, sizeof(double) * B, hipMemcpyHostToDevice, st_));
-        upload_mask(trial_mask, active_.p);
-        run_sequence(trial_exec_, [&] { enqueue_trial(pl, tile_factor_); });
-        PassStatus* hs = h_status_ + B;
-        GBA_HIP(hipMemcpyAsync(hs, trial_status_.p, sizeof(PassStatus) * B, hipMemcpyDeviceToHost, st_));
+        for (int r = 0; r < rounds_per_sync_ && enq < max_rounds; ++r, ++enq)
+            run_sequence(round_exec_, [&] { enqueue_round(opts); });
+        GBA_HIP(hipMemcpyAsync(h_counters_, counters_.p, 2 * sizeof(int), hipMemcpyDeviceToHost, st_));
         GBA_HIP(hipGetLastError());
         GBA_HIP(hipStreamSynchronize(st_));
-        int naccept = 0;
-        for (int b = 0; b < B; ++b) {
-            acc_mask[b] = 0;
-            if (!trial_mask[b]) continue;
-            const PassStatus& t = hs[b];
-            res[b].trials++;
-            bool accept = false;
-            if (!t.flags.nonspd && !t.flags.chol_fail && t.denom > 0.0) {
-                const unsigned long long before = counter[b];
-                counter[b] += t.anomalies;
-                if (counter[b] == before && std::isfinite(t.sse)) {
-                    const double rho = (sse[b] - t.sse) / (0.5 * t.denom);
-                    if (rho > 0.0 && t.sse < sse[b]) {
-                        accept = true;
-                        const double rel = (sse[b] - t.sse) / std::max(sse[b], 1.0);
-                        sse[b] = t.sse;
-                        lambda[b] *= std::max(1.0 / 3.0, 1.0 - std::pow(2.0 * rho - 1.0, 3.0));
-                        res[b].iterations++;
-                        acc_mask[b] = 1;
-                        ++naccept;
-                        inner[b] = 0;
-                        nu[b] = 2.0;
-                        if (rel < opts.rel_decrease_tol) {
-                            converged[b] = 1;
-                            done[b] = 1;
-                        } else {
-                            need_lin[b] = 1;
-                        }
-                    }
-                }
-            }
-            if (!accept) {
-                lambda[b] *= nu[b];
-                nu[b] *= 2.0;
-                if (++inner[b] >= opts.max_inner_trials) {
-                    failed[b] = 1;
-                    done[b] = 1;
-                }
-            }
-        }
-        if (naccept > 0) {
-            upload_mask(acc_mask, accept_.p);
-            launch_batch_accept(st_, pb_.C, pb_.P, cam_prob_.p, pt_prob_.p, accept_.p, tcams_.p, tpts_.p,
-                                cams_.p, pts_.p);
-        }
+        if (h_counters_[1] == 0 || enq >= max_rounds) break;
     }
-    GBA_HIP(hipStreamSynchronize(st_));
+    rounds_ = h_counters_[0];
+    std::vector<BatchLmState> hs(static_cast<size_t>(B));
+    GBA_HIP(hipMemcpy(hs.data(), lm_.p, sizeof(BatchLmState) * B, hipMemcpyDeviceToHost));
     const double t1 = now_s();
     for (int b = 0; b < B; ++b) {
-        res[b].final_cost = sse[b] / 2.0;
-        res[b].anomalies = counter[b];
-        res[b].termination = converged[b] ? GBA_CONVERGENCE : (failed[b] ? GBA_FAILURE : GBA_MAX_ITERATIONS);
+        const BatchLmState& s = hs[b];
+        res[b] = gba_lm_result{};
+        res[b].initial_cost = s.cost0;
+        res[b].final_cost = s.sse / 2.0;
+        res[b].iterations = s.iterations;
+        res[b].trials = s.trials;
+        res[b].anomalies = s.counter;
+        res[b].termination = s.converged ? GBA_CONVERGENCE : (s.failed ? GBA_FAILURE : GBA_MAX_ITERATIONS);
         res[b].solve_time_s = t1 - t0;
         res[b].total_time_s = t1 - t0;
     }

@@ -12,17 +12,18 @@ namespace gba_amd {
 // reference LM semantics (src/lm.cu:455-714, the same options for all), in
 // plain FP64.  All problems live in one concatenated device layout (cameras,
 // points and sorted observations side by side); every stage of an LM round is
-// one launch for the whole batch, masked per problem, so the launch and
-// synchronisation cost of a round is shared by all problems.  A round
-// re-linearises the problems whose last trial was accepted and evaluates one
-// trial for every problem still running; the host then applies each
-// problem's accept / reject / lambda / nu rules.  One stream synchronisation
-// per round.  The linearisation and trial launch sequences are captured once
-// into HIP graphs and replayed every round (GBA_BATCH_GRAPH=0: launched one
-// by one).  With autotune set, the constructor times one trial of the whole
-// batch with each dense factor form (tile-parallel across problems, or one
-// workgroup per problem) and keeps the faster; GBA_BATCH_FACTOR=tiles or
-// =workgroup fixes the form instead.
+// one launch for the whole batch, masked per problem, so the launch cost of a
+// round is shared by all problems.  A round re-linearises the problems whose
+// last trial was accepted and evaluates one trial for every problem still
+// running; each problem's accept / reject / lambda / nu rules run on the
+// device (k_batch_lm_*), so a round needs no host decision.  The round's
+// launch sequence is captured once into a HIP graph (GBA_BATCH_GRAPH=0:
+// launched one by one) and replayed rounds_per_sync times between two reads
+// of the running-problem count (GBA_BATCH_ROUNDS_PER_SYNC, default 4).  With
+// autotune set, the constructor times one trial of the whole batch with each
+// dense factor form (tile-parallel across problems, or one workgroup per
+// problem) and keeps the faster; GBA_BATCH_FACTOR=tiles or =workgroup fixes
+// the form instead.
 class BatchSolver {
 public:
     BatchSolver(const gba_problem_view* probs, int count, bool autotune = false);
@@ -43,9 +44,11 @@ public:
 
 private:
     void upload_mask(const std::vector<int>& h, int* d);
-    // the launch sequences of a round (all problems, masked by active_)
+    // the launch sequences of a round, masked by pl.active
     void enqueue_lin(const DevProblem& pl);
     void enqueue_trial(const DevProblem& pl, bool tile_factor);
+    // one whole round with the device-side LM rules
+    void enqueue_round(const gba_lm_options& opts);
     // runs a sequence through its graph (captured on first use) or eagerly
     template <class F>
     void run_sequence(hipGraphExec_t& exec, F&& enqueue);
@@ -72,7 +75,8 @@ private:
     DevBuf<double> unit_sse_, unit_den_, gram_;
     DevBuf<unsigned> unit_anom_;
     DevBuf<double> lam_;
-    DevBuf<int> active_, accept_;
+    DevBuf<int> active_, accept_, lin_active_, trial_active_, counters_;
+    DevBuf<BatchLmState> lm_;
     DevBuf<TrialFlags> flags_;
     DevBuf<PassStatus> lin_status_, trial_status_;
     // tile-parallel factor (launch_batch_cholesky) and its work tables
@@ -82,7 +86,10 @@ private:
     BatchFactorTables ftab_;
     double tune_ms_[2] = {0.0, 0.0};
     bool use_graphs_ = true;
-    hipGraphExec_t lin_exec_ = nullptr, trial_exec_ = nullptr;
+    int rounds_per_sync_ = 4;
+    hipGraphExec_t round_exec_ = nullptr;
+    gba_lm_options round_opts_{};  // the options the round graph was captured with
+    int* h_counters_ = nullptr;
     PassStatus* h_status_ = nullptr;  // 2 B entries: linearisation, trial
     double* h_lam_ = nullptr;
     int* h_mask_ = nullptr;           // staging for the per-problem switches

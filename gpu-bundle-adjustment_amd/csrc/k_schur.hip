@@ -308,8 +308,10 @@ __global__ __launch_bounds__(256) void k_schur_diag(
     }
 }
 
-// D_i = (U_ii + lambda diag'(U_ii))^-1/2 for i < n, 1 for the padding.
+// D_i = (U_ii + lambda diag'(U_ii))^-1/2 for i < n, 1 for the padding;
+// batched (cam_prob set): lambda of row i's problem, lam[cam_prob[i / 9]].
 __global__ void k_diag_scale(int n, int npad, const double* __restrict__ U, double lambda,
+                             const int* __restrict__ cam_prob, const double* __restrict__ lam,
                              double* __restrict__ d) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= npad) return;
@@ -317,6 +319,7 @@ __global__ void k_diag_scale(int n, int npad, const double* __restrict__ U, doub
         d[i] = 1.0;
         return;
     }
+    if (cam_prob) lambda = lam[cam_prob[i / 9]];
     const double u = U[81 * static_cast<size_t>(i / 9) + 10 * (i % 9)];
     const double v = u + lambda * damp_diag(u);
     d[i] = v > 0.0 ? 1.0 / sqrt(v) : 1.0;
@@ -368,7 +371,17 @@ void launch_schur(hipStream_t st, const SchurShape& sh, const DevProblem& pb,
                   const float* W32, const double* Vinv6, const double* VB, const double* U,
                   const double* bc, double lambda, const double* dscale, double* S64, float* S32fac,
                   float* S32low, int ld, double* rhs, TrialFlags* flags) {
-    if (W32) {
+    if (W32 && !S32fac) {
+        // FP32 W with the fixed-point sums, S stored FP64 (the batched mixed form)
+        if (sh.threads == 1024)
+            launch_one<1024, float, long long, double>(st, sh, pb, tasks, num_tasks, pairs, W32, Vinv6, VB, U, bc,
+                                                       lambda, dscale, S64, static_cast<double*>(nullptr), ld,
+                                                       rhs, flags);
+        else
+            launch_one<512, float, long long, double>(st, sh, pb, tasks, num_tasks, pairs, W32, Vinv6, VB, U, bc,
+                                                      lambda, dscale, S64, static_cast<double*>(nullptr), ld,
+                                                      rhs, flags);
+    } else if (W32) {
         if (sh.threads == 1024)
             launch_one<1024, float, long long, float>(st, sh, pb, tasks, num_tasks, pairs, W32, Vinv6, VB,
                                                       U, bc, lambda, dscale, S32fac, S32low, ld, rhs, flags);
@@ -392,7 +405,10 @@ void launch_schur_diag(hipStream_t st, const DevProblem& pb, const BatchDesc* de
                        const double* bc, double lambda, const double* dscale, double* S64,
                        float* S32fac, float* S32low, int ld, double* rhs, const TrialFlags* flags) {
     if (pb.C <= 0) return;
-    if (W32)
+    if (W32 && !S32fac)
+        hipLaunchKernelGGL((k_schur_diag<float, double>), dim3(pb.C), dim3(256), 0, st, pb, desc, W32, Vinv6,
+                           VB, U, bc, lambda, dscale, S64, static_cast<double*>(nullptr), ld, rhs, flags);
+    else if (W32)
         hipLaunchKernelGGL((k_schur_diag<float, float>), dim3(pb.C), dim3(256), 0, st, pb, desc, W32, Vinv6,
                            VB, U, bc, lambda, dscale, S32fac, S32low, ld, rhs, flags);
     else
@@ -402,9 +418,9 @@ void launch_schur_diag(hipStream_t st, const DevProblem& pb, const BatchDesc* de
 }
 
 void launch_diag_scale(hipStream_t st, int n, int npad, const double* U, double lambda,
-                       double* dscale) {
+                       double* dscale, const int* cam_prob, const double* lam) {
     hipLaunchKernelGGL(k_diag_scale, dim3((npad + 255) / 256), dim3(256), 0, st, n, npad, U, lambda,
-                       dscale);
+                       cam_prob, lam, dscale);
 }
 
 }  // namespace gba_amd

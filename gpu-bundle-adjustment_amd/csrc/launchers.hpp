@@ -56,13 +56,16 @@ void launch_schur(hipStream_t st, const SchurShape& sh, const DevProblem& pb,
                   float* S32low, int ld, double* rhs, TrialFlags* flags);
 // Diagonal blocks (both triangles) + rhs, one workgroup per camera.  desc:
 // per-problem S layout (batched) or NULL (single problem, leading dimension
-// ld).  W32 selects the mixed form (D S D into the FP32 copies, rhs scaled).
+// ld).  W32 selects the mixed form (D S D into the FP32 copies, rhs scaled);
+// W32 without S32fac writes the mixed form's D S D into S64 (the batched
+// mixed solver, whose factor is FP64).  launch_schur follows the same rule.
 void launch_schur_diag(hipStream_t st, const DevProblem& pb, const BatchDesc* desc, const double* W64,
                        const float* W32, const double* Vinv6, const double* VB, const double* U,
                        const double* bc, double lambda, const double* dscale, double* S64,
                        float* S32fac, float* S32low, int ld, double* rhs, const TrialFlags* flags);
+// Batched problems: cam_prob and lam give each row its problem's lambda.
 void launch_diag_scale(hipStream_t st, int n, int npad, const double* U, double lambda,
-                       double* dscale);
+                       double* dscale, const int* cam_prob = nullptr, const double* lam = nullptr);
 
 // k_pairs.hip -- the Schur task pair lists on the device.  Task t = (a, lo,
 // hi) in tq[t] (x, y, z) owns slots qoff[t] .. qoff[t + 1] - 1, one per
@@ -218,6 +221,28 @@ void launch_batch_status(hipStream_t st, int count, const BatchDesc* desc, int t
 void launch_batch_accept(hipStream_t st, int nc, int np, const int* cam_prob, const int* pt_prob,
                          const int* accept, const double* tcams, const double* tpts, double* cams,
                          double* pts);
+// Per-problem LM control of the batched solver on the device (the rules of
+// src/lm.cu:455-714, as Solver::solve applies them): one state per problem,
+// the per-problem switches of the next pass (lin_active: re-linearise;
+// trial_active: evaluate a trial; accept: copy the trial parameters) and
+// lambda (lam, what the trial kernels damp with).  counters[0] counts the
+// rounds in which some problem evaluated a trial, counters[1] the problems
+// still running.
+struct BatchLmState {
+    double sse, lambda, nu, cost0;
+    unsigned long long counter;  // non-finite edges seen (the reference's anomaly counter)
+    int need_lin, done, converged, failed, inner, iterations, trials, started;
+};
+void launch_batch_lm_init(hipStream_t st, int count, double lambda0, BatchLmState* state, double* lam,
+                          int* lin_active, int* trial_active, int* accept, int* counters);
+// After a linearisation pass (statuses of the problems in lin_active).
+void launch_batch_lm_lin(hipStream_t st, int count, const gba_lm_options& opts, const PassStatus* lin_status,
+                         BatchLmState* state, const int* lin_active, int* trial_active, int* counters);
+// After a trial pass: accept / reject, lambda and nu, termination; sets
+// accept, lin_active and lam for the next pass.
+void launch_batch_lm_trial(hipStream_t st, int count, const gba_lm_options& opts, const PassStatus* trial_status,
+                           BatchLmState* state, const int* trial_active, int* accept, int* lin_active,
+                           double* lam, int* counters);
 
 // k_backsub.hip
 void launch_trial_cams(hipStream_t st, int C, const double* cams, const double* dc, double* tcams);
