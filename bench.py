@@ -53,52 +53,75 @@ def time_to_cost(res, target):
 def run_batch(g, args):
     """Preset 5: --batch independent problems (per-problem seeds seed..seed+B-1,
     C ~ U{8..64}, P ~ U{2000..20000}, O = 4.5 P), solved by one batched handle
-    (inputs resident); problems/s over --steps solves, against problems/s of
-    sequential FP64 solves (one solver handle per problem, inputs resident)
-    on the first --seq-sample problems."""
+    per precision (inputs resident): the mixed form (FP32 W, diagonally scaled
+    reduced system from fixed-point sums, FP64 factor; the headline value) and
+    the plain FP64 form, problems/s over --steps solves each, with each
+    problem's own solve time (device clock, start of the batch solve to the
+    round that finished it) as p50 / p99; against problems/s of sequential
+    FP64 solves (one solver handle per problem, inputs resident) on the first
+    --seq-sample problems, whose final costs the batched results are compared
+    with."""
     import torch
     probs = [g.generate(5, args.seed + i) for i in range(args.batch)]
     opts = g.LMOptions(initial_lambda=args.lambda0)
-    t_create = time.perf_counter()
-    b = g.Batch(probs)
-    t_create = time.perf_counter() - t_create
-    for _ in range(args.warmup):
-        b.solve(opts)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        res = b.solve(opts)
-    torch.cuda.synchronize()
-    wall = (time.perf_counter() - t0) / args.steps
+    forms = {}
+    for name, prec in (("mixed", g.PRECISION_MIXED), ("fp64", g.PRECISION_FP64)):
+        t_create = time.perf_counter()
+        b = g.Batch(probs, precision=prec)
+        t_create = time.perf_counter() - t_create
+        for _ in range(args.warmup):
+            b.solve(opts)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            res = b.solve(opts)
+        torch.cuda.synchronize()
+        wall = (time.perf_counter() - t0) / args.steps
+        pt = sorted(r.solve_time_s for r in res)
+        forms[name] = {"res": res, "wall": wall, "rounds": b.rounds, "handle": b.info(), "create_s": t_create,
+                       "p50_s": statistics.median(pt), "p99_s": pt[min(len(pt) - 1, int(0.99 * len(pt)))]}
+        b.close()
     k = min(args.seq_sample, args.batch)
     solvers = [g.Solver(p) for p in probs[:k]]
     for s in solvers[:2]:
         s.solve(opts)
-    times, dev = [], []
-    for i, s in enumerate(solvers):
+    times, seq = [], []
+    for s in solvers:
         t = time.perf_counter()
-        r = s.solve(opts)
+        seq.append(s.solve(opts))
         times.append(time.perf_counter() - t)
-        dev.append(abs(r.final_cost - res[i].final_cost) / r.final_cost)
     seq_rate = k / sum(times)
-    rate = args.batch / wall
+
+    def rel(a, b):
+        return float(max(abs(x.final_cost - y.final_cost) / y.final_cost for x, y in zip(a, b)))
+
+    m, f = forms["mixed"], forms["fp64"]
+    rate = args.batch / m["wall"]
     out = {
         "metric": "batched LM solves, preset 5 (problems/s)", "value": round(rate, 2), "unit": "problems/s",
-        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * wall, 3),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * m["wall"], 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32 W / f64",
         "data": "synthetic (seeded Snavely generator, per-problem seeds %d..%d)" % (args.seed, args.seed + args.batch - 1),
-        "config": {"workload": f"{args.batch} problems of preset 5", "problems": args.batch,
+        "config": {"workload": f"{args.batch} problems of preset 5, mixed precision", "problems": args.batch,
                    "cameras": sum(p.num_cameras for p in probs), "points": sum(p.num_points for p in probs),
                    "observations": sum(p.num_observations for p in probs)},
+        "per_problem_solve_s": {"p50": round(m["p50_s"], 5), "p99": round(m["p99_s"], 5)},
         "sequential_fp64": {"problems": k, "problems_per_s": round(seq_rate, 2),
                             "p50_s": round(statistics.median(times), 5),
                             "p99_s": round(sorted(times)[min(k - 1, int(0.99 * k))], 5)},
         "speedup_vs_sequential": round(rate / seq_rate, 2),
-        "rounds": b.rounds,
-        "handle": b.info(),
-        "create_s": round(t_create, 4),
-        "max_final_cost_rel_diff_vs_sequential": float(max(dev)),
-        "terminations": {t: sum(1 for r in res if r.termination == t) for t in set(r.termination for r in res)},
+        "rounds": m["rounds"],
+        "handle": m["handle"],
+        "create_s": round(m["create_s"], 4),
+        "max_final_cost_rel_diff_vs_fp64_batch": rel(m["res"], f["res"]),
+        "max_final_cost_rel_diff_vs_sequential": rel(m["res"][:k], seq),
+        "terminations": {t: sum(1 for r in m["res"] if r.termination == t)
+                         for t in set(r.termination for r in m["res"])},
+        "fp64_batch": {"problems_per_s": round(args.batch / f["wall"], 2), "ms_per_step": round(1e3 * f["wall"], 3),
+                       "speedup_vs_sequential": round(args.batch / f["wall"] / seq_rate, 2),
+                       "per_problem_solve_s": {"p50": round(f["p50_s"], 5), "p99": round(f["p99_s"], 5)},
+                       "rounds": f["rounds"], "handle": f["handle"],
+                       "max_final_cost_rel_diff_vs_sequential": rel(f["res"][:k], seq)},
     }
     print(json.dumps(out))
 
@@ -273,7 +296,8 @@ def main():
     ap.add_argument("--mode", choices=["solve", "kernels", "batch", "fp16", "bal", "sparse"], default="solve",
                     help="kernels: only the per-kernel timing loops plus a 512 MiB device copy "
                          "(short run for rocprofv3 counter passes); batch: the batched solver on "
-                         "--batch problems of preset 5 (problems/s) against sequential FP64 solves; "
+                         "--batch problems of preset 5 (problems/s), mixed and FP64 handles, against "
+                         "sequential FP64 solves; "
                          "fp16: preset 4 with FP16 pixel storage, final-cost error vs FP64; "
                          "bal: the reference's bench_bal harness on --file (interleaved FP64 / mixed pairs); "
                          "sparse: preset 7 (or --config 6), envelope vs dense factor layout, plain FP64")

@@ -84,6 +84,8 @@ _PROTOS = {
     "gba_solver_destroy": (None, [ctypes.c_void_p]),
     "gba_batch_create": (ctypes.c_int, [ctypes.POINTER(ProblemView), ctypes.c_int,
                                         ctypes.POINTER(ctypes.c_void_p)]),
+    "gba_batch_create_precision": (ctypes.c_int, [ctypes.POINTER(ProblemView), ctypes.c_int, ctypes.c_int,
+                                                   ctypes.POINTER(ctypes.c_void_p)]),
     "gba_batch_solve": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(LMOptionsC),
                                        ctypes.POINTER(LMResultC), _ip]),
     "gba_batch_params": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp, _dp]),
@@ -346,15 +348,18 @@ class Solver:
 
 
 class Batch:
-    """Batched solver handle (many independent problems, FP64, one launch per
-    LM stage for the whole batch): upload once, solve repeatedly."""
+    """Batched solver handle (many independent problems, one launch per LM
+    stage for the whole batch): upload once, solve repeatedly.  precision
+    PRECISION_FP64 (default) or PRECISION_MIXED (FP32 W, diagonally scaled
+    reduced system from fixed-point sums, FP64 factor)."""
 
-    def __init__(self, probs):
+    def __init__(self, probs, precision=PRECISION_FP64):
         self._probs = list(probs)
         n = len(self._probs)
         self._views = (ProblemView * n)(*[p.view() for p in self._probs])
         self._h = ctypes.c_void_p()
-        _check(lib().gba_batch_create(self._views, n, ctypes.byref(self._h)))
+        self.precision = precision
+        _check(lib().gba_batch_create_precision(self._views, n, precision, ctypes.byref(self._h)))
         self.rounds = 0
 
     def info(self):

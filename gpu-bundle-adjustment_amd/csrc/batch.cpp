@@ -31,11 +31,12 @@ constexpr int kBatchMaxNpad = 1024;
 
 }  // namespace
 
-BatchSolver::BatchSolver(const gba_problem_view* probs, int count, bool autotune) : B_(count) {
+BatchSolver::BatchSolver(const gba_problem_view* probs, int count, bool autotune, bool mixed)
+    : B_(count), mixed_(mixed) {
     if (count <= 0) throw Error("batch: no problems");
     int Cmax = 1;
     for (int b = 0; b < count; ++b) Cmax = std::max(Cmax, probs[b].num_cameras);
-    sshape_ = schur_shape(Cmax, false);
+    sshape_ = schur_shape(Cmax, mixed);
     // ---- per-problem descriptors and offsets (host) ----
     std::vector<long long> o_off(1, 0);
     std::vector<int> c0s, p0s, cam_prob, pt_prob;
@@ -212,7 +213,16 @@ BatchSolver::BatchSolver(const gba_problem_view* probs, int count, bool autotune
     tcams_.alloc(9 * static_cast<size_t>(C));
     pts_.alloc(3 * static_cast<size_t>(P));
     tpts_.alloc(3 * static_cast<size_t>(P));
-    W_.alloc(kWStride * static_cast<size_t>(O));
+    if (mixed_) {
+        W32_.alloc(kWStride * static_cast<size_t>(O));
+        dscale_.alloc(9 * static_cast<size_t>(C));
+        ys_.alloc(9 * static_cast<size_t>(C));
+        zp_.alloc(3 * static_cast<size_t>(P));
+        rot_.alloc(4 * static_cast<size_t>(C));
+        if (const char* env = std::getenv("GBA_BATCH_REFINE_STEPS")) refine_steps_ = std::max(0, std::atoi(env));
+    } else {
+        W_.alloc(kWStride * static_cast<size_t>(O));
+    }
     VB_.alloc(9 * static_cast<size_t>(P));
     Vinv_.alloc(6 * static_cast<size_t>(P));
     U_.alloc(81 * static_cast<size_t>(C));
@@ -253,6 +263,13 @@ BatchSolver::BatchSolver(const gba_problem_view* probs, int count, bool autotune
         if (d.npad > d.n) launch_identity_pad(st_, S_.p + d.soff, nullptr, d.npad, d.n, d.npad);
     GBA_HIP(hipGetLastError());
     GBA_HIP(hipStreamSynchronize(st_));
+    {
+        int dev = 0, khz = 0;
+        GBA_HIP(hipGetDevice(&dev));
+        if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) == hipSuccess && khz > 0)
+            clock_hz_ = 1e3 * khz;
+        (void)hipGetLastError();
+    }
     if (const char* env = std::getenv("GBA_BATCH_GRAPH")) use_graphs_ = std::string(env) != "0";
     if (const char* env = std::getenv("GBA_BATCH_ROUNDS_PER_SYNC")) rounds_per_sync_ = std::max(1, std::atoi(env));
     if (autotune) autotune_factor();
@@ -276,7 +293,7 @@ void BatchSolver::upload_mask(const std::vector<int>& h, int* d) {
 }
 
 void BatchSolver::enqueue_lin(const DevProblem& pl) {
-    launch_lin_points(st_, pl, cams_.p, pts_.p, W_.p, nullptr, VB_.p, unit_sse_.p, unit_anom_.p);
+    launch_lin_points(st_, pl, cams_.p, pts_.p, W_.p, W32_.p, VB_.p, unit_sse_.p, unit_anom_.p);
     launch_lin_cameras(st_, pl, cams_.p, pts_.p, gram_.p);
     launch_lin_reduce(st_, pl, gram_.p, unit_sse_.p, unit_anom_.p, U_.p, bc_.p, nullptr);
     launch_batch_status(st_, B_, desc_.p, 0, unit_sse_.p, unit_den_.p, unit_anom_.p, dc_.p, U_.p, bc_.p,
@@ -286,15 +303,32 @@ void BatchSolver::enqueue_lin(const DevProblem& pl) {
 void BatchSolver::enqueue_trial(const DevProblem& pl, bool tile_factor) {
     launch_fill_int(st_, reinterpret_cast<int*>(flags_.p), 4 * B_, 0);
     launch_damp_invert_v_batched(st_, pb_.P, VB_.p, pt_prob_.p, lam_.p, pl.active, Vinv_.p, flags_.p);
-    launch_schur_diag(st_, pl, desc_.p, W_.p, nullptr, Vinv_.p, VB_.p, U_.p, bc_.p, 0.0, nullptr, S_.p,
+    // mixed: S and rhs come out scaled (D S D, D rhs); the factor solves for
+    // y = D^-1 dc, then dc = D y
+    const int n = 9 * pb_.C;
+    if (mixed_) launch_diag_scale(st_, n, n, U_.p, 0.0, dscale_.p, cam_prob_.p, lam_.p);
+    launch_schur_diag(st_, pl, desc_.p, W_.p, W32_.p, Vinv_.p, VB_.p, U_.p, bc_.p, 0.0, dscale_.p, S_.p,
                       nullptr, nullptr, 0, rhs_.p, flags_.p);
     if (num_stasks_ > 0)
-        launch_schur(st_, sshape_, pl, stasks_.p, num_stasks_, spairs_.p, W_.p, nullptr, Vinv_.p, VB_.p,
-                     U_.p, bc_.p, 0.0, nullptr, S_.p, nullptr, nullptr, 0, rhs_.p, flags_.p);
+        launch_schur(st_, sshape_, pl, stasks_.p, num_stasks_, spairs_.p, W_.p, W32_.p, Vinv_.p, VB_.p,
+                     U_.p, bc_.p, 0.0, dscale_.p, S_.p, nullptr, nullptr, 0, rhs_.p, flags_.p);
+    double* y = mixed_ ? ys_.p : dc_.p;
     if (tile_factor) launch_batch_cholesky(st_, B_, ftab_, desc_.p, S_.p, D_.p, flags_.p, pl.active);
-    launch_batch_chol_solve(st_, B_, desc_.p, S_.p, D_.p, rhs_.p, dc_.p, flags_.p, pl.active, !tile_factor);
+    launch_batch_chol_solve(st_, B_, desc_.p, S_.p, D_.p, rhs_.p, y, flags_.p, pl.active, !tile_factor);
+    if (mixed_) {
+        launch_vec_scale(st_, n, n, ys_.p, dscale_.p, dc_.p);
+        // FP64 refinement with the same factor: the residual of the FP64
+        // system (Jacobians evaluated again, W not read), its scaled
+        // correction solved, dc += D y
+        for (int it = 0; it < refine_steps_; ++it) {
+            launch_refine_residual(st_, pl, cams_.p, pts_.p, Vinv_.p, VB_.p, U_.p, bc_.p, 0.0, dc_.p, zp_.p,
+                                   rot_.p, dscale_.p, rhs_.p, flags_.p);
+            launch_batch_chol_solve(st_, B_, desc_.p, S_.p, D_.p, rhs_.p, ys_.p, flags_.p, pl.active, false);
+            launch_vec_axpy_scaled(st_, n, dc_.p, ys_.p, dscale_.p);
+        }
+    }
     launch_trial_cams(st_, pb_.C, cams_.p, dc_.p, tcams_.p);
-    launch_backsub(st_, pl, W_.p, nullptr, Vinv_.p, VB_.p, dc_.p, pts_.p, tcams_.p, tpts_.p, 0.0,
+    launch_backsub(st_, pl, W_.p, W32_.p, Vinv_.p, VB_.p, dc_.p, pts_.p, tcams_.p, tpts_.p, 0.0,
                    unit_sse_.p, unit_den_.p, unit_anom_.p, flags_.p);
     launch_batch_status(st_, B_, desc_.p, 1, unit_sse_.p, unit_den_.p, unit_anom_.p, dc_.p, U_.p, bc_.p,
                         lam_.p, flags_.p, pl.active, trial_status_.p);
@@ -432,7 +466,11 @@ void BatchSolver::solve(const gba_lm_options& opts, gba_lm_result* res) {
         res[b].trials = s.trials;
         res[b].anomalies = s.counter;
         res[b].termination = s.converged ? GBA_CONVERGENCE : (s.failed ? GBA_FAILURE : GBA_MAX_ITERATIONS);
-        res[b].solve_time_s = t1 - t0;
+        // this problem's own solve time: device clock from the start of the
+        // batch solve to the round that finished it
+        res[b].solve_time_s = (s.done && clock_hz_ > 0.0 && s.t_done >= s.t_start)
+                                  ? static_cast<double>(s.t_done - s.t_start) / clock_hz_
+                                  : t1 - t0;
         res[b].total_time_s = t1 - t0;
     }
 }

@@ -10,7 +10,15 @@ namespace gba_amd {
 
 // Solves a batch of independent bundle-adjustment problems, each with the
 // reference LM semantics (src/lm.cu:455-714, the same options for all), in
-// plain FP64.  All problems live in one concatenated device layout (cameras,
+// plain FP64 or in the mixed form (mixed set): W stored FP32, and the damped
+// reduced camera system assembled diagonally scaled, D S D with D =
+// (diag(U + lambda diag'(U)))^-1/2, from 64-bit fixed-point LDS sums (2^-46,
+// one order-independent result), stored and factored in FP64; the step from
+// it carries the FP32 rounding of W (percents where lambda is small: the
+// damped system is near singular along the gauge freedom), so refine_steps
+// FP64 refinement steps follow, each solving the residual of the FP64
+// system (k_refine.hip: the Jacobians evaluated again, W not read) with the
+// same factor.  All problems live in one concatenated device layout (cameras,
 // points and sorted observations side by side); every stage of an LM round is
 // one launch for the whole batch, masked per problem, so the launch cost of a
 // round is shared by all problems.  A round re-linearises the problems whose
@@ -26,7 +34,7 @@ namespace gba_amd {
 // the form instead.
 class BatchSolver {
 public:
-    BatchSolver(const gba_problem_view* probs, int count, bool autotune = false);
+    BatchSolver(const gba_problem_view* probs, int count, bool autotune = false, bool mixed = false);
     ~BatchSolver();
     BatchSolver(const BatchSolver&) = delete;
     BatchSolver& operator=(const BatchSolver&) = delete;
@@ -38,6 +46,7 @@ public:
     int rounds() const { return rounds_; }
     bool tile_factor() const { return tile_factor_; }
     bool graphs() const { return use_graphs_; }
+    bool mixed() const { return mixed_; }
     // trial milliseconds measured by the autotuner: tile-parallel factor,
     // workgroup factor (0 when it did not run)
     double tune_ms(int form) const { return tune_ms_[form & 1]; }
@@ -72,6 +81,11 @@ private:
     DevBuf<BatchDesc> desc_;
     DevBuf<double> cams_, pts_, tcams_, tpts_;
     DevBuf<double> W_, VB_, Vinv_, U_, bc_, rhs_, dc_, S_, D_;
+    // mixed form: FP32 W, the diagonal scale D and the scaled step D^-1 dc
+    bool mixed_ = false;
+    int refine_steps_ = 1;  // FP64 refinement steps per trial (GBA_BATCH_REFINE_STEPS)
+    DevBuf<float> W32_;
+    DevBuf<double> dscale_, ys_, zp_, rot_;
     DevBuf<double> unit_sse_, unit_den_, gram_;
     DevBuf<unsigned> unit_anom_;
     DevBuf<double> lam_;
@@ -90,6 +104,7 @@ private:
     hipGraphExec_t round_exec_ = nullptr;
     gba_lm_options round_opts_{};  // the options the round graph was captured with
     int* h_counters_ = nullptr;
+    double clock_hz_ = 0.0;  // device wall clock rate
     PassStatus* h_status_ = nullptr;  // 2 B entries: linearisation, trial
     double* h_lam_ = nullptr;
     int* h_mask_ = nullptr;           // staging for the per-problem switches

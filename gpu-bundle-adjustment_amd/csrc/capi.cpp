@@ -176,16 +176,22 @@ struct gba_batch {
     std::unique_ptr<BatchSolver> s;
 };
 
-int gba_batch_create(const gba_problem_view* probs, int count, gba_batch** out) {
+int gba_batch_create_precision(const gba_problem_view* probs, int count, int precision, gba_batch** out) {
     return guarded([&] {
         need(out, "gba_batch_create: null handle");
         *out = nullptr;
         if (!probs || count <= 0) return set_error("gba_batch_create: no problems");
+        if (precision != GBA_PRECISION_FP64 && precision != GBA_PRECISION_MIXED)
+            return set_error("gba_batch_create: unknown precision");
         auto h = std::make_unique<gba_batch>();
-        h->s = std::make_unique<BatchSolver>(probs, count, /*autotune=*/true);
+        h->s = std::make_unique<BatchSolver>(probs, count, /*autotune=*/true, precision == GBA_PRECISION_MIXED);
         *out = h.release();
         return 0;
     });
+}
+
+int gba_batch_create(const gba_problem_view* probs, int count, gba_batch** out) {
+    return gba_batch_create_precision(probs, count, GBA_PRECISION_FP64, out);
 }
 
 int gba_batch_solve(gba_batch* b, const gba_lm_options* opts, gba_lm_result* results, int* rounds) {

@@ -102,6 +102,7 @@ __global__ void k_batch_lm_init(int count, double lambda0, BatchLmState* __restr
     }
     if (q >= count) return;
     BatchLmState s{};
+    s.t_start = wall_clock64();
     s.lambda = lambda0;
     s.nu = 2.0;
     s.need_lin = 1;
@@ -123,8 +124,10 @@ __global__ __launch_bounds__(1024) void k_batch_lm_lin(int count, gba_lm_options
                                                        int* __restrict__ trial_active, int* __restrict__ counters) {
     __shared__ int s_tmp[16];
     int running = 0;
+    const int round = counters[0] + 1;
     for (int q = threadIdx.x; q < count; q += blockDim.x) {
         BatchLmState s = st[q];
+        const int was_done = s.done;
         if (lin_active[q]) {
             s.counter += lin_status[q].anomalies;
             if (!s.started) {
@@ -139,13 +142,17 @@ __global__ __launch_bounds__(1024) void k_batch_lm_lin(int count, gba_lm_options
             s.failed = 1;
             s.done = 1;
         }
+        if (s.done && !was_done) {
+            s.t_done = wall_clock64();
+            s.done_round = round;
+        }
         trial_active[q] = s.done ? 0 : 1;
         running += s.done ? 0 : 1;
         st[q] = s;
     }
     running = block_count(running, s_tmp);
     if (threadIdx.x == 0) {
-        if (running > 0) counters[0] += 1;
+        if (running > 0) counters[0] = round;
         counters[1] = running;
     }
 }
@@ -166,6 +173,7 @@ __global__ __launch_bounds__(1024) void k_batch_lm_trial(int count, gba_lm_optio
                                                          double* __restrict__ lam, int* __restrict__ counters) {
     __shared__ int s_tmp[16];
     int running = 0;
+    const int round = counters[0];
     for (int q = threadIdx.x; q < count; q += blockDim.x) {
         BatchLmState s = st[q];
         int acc = 0;
@@ -201,6 +209,10 @@ __global__ __launch_bounds__(1024) void k_batch_lm_trial(int count, gba_lm_optio
                     s.failed = 1;
                     s.done = 1;
                 }
+            }
+            if (s.done) {
+                s.t_done = wall_clock64();
+                s.done_round = round;
             }
             st[q] = s;
             lam[q] = s.lambda;

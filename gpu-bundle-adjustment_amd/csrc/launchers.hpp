@@ -231,7 +231,12 @@ void launch_batch_accept(hipStream_t st, int nc, int np, const int* cam_prob, co
 struct BatchLmState {
     double sse, lambda, nu, cost0;
     unsigned long long counter;  // non-finite edges seen (the reference's anomaly counter)
+    // device wall clock (wall_clock64, hipDeviceAttributeWallClockRate kHz)
+    // when the batch solve started and when this problem finished
+    unsigned long long t_start, t_done;
     int need_lin, done, converged, failed, inner, iterations, trials, started;
+    int done_round;  // the round in which the problem finished
+    int pad_;
 };
 void launch_batch_lm_init(hipStream_t st, int count, double lambda0, BatchLmState* state, double* lam,
                           int* lin_active, int* trial_active, int* accept, int* counters);
@@ -243,6 +248,20 @@ void launch_batch_lm_lin(hipStream_t st, int count, const gba_lm_options& opts, 
 void launch_batch_lm_trial(hipStream_t st, int count, const gba_lm_options& opts, const PassStatus* trial_status,
                            BatchLmState* state, const int* trial_active, int* accept, int* lin_active,
                            double* lam, int* counters);
+
+// k_refine.hip
+// out = rhs - S x of the damped reduced camera system in FP64 (D (rhs - S x)
+// with dscale), with neither S nor W read: two passes that evaluate the
+// Jacobians again at (cams, pts) -- z_p = V'^-1 (bp - sum W_f^T x) into z
+// (3P), then the camera sums; rot (4C doubles) receives the cameras'
+// rotation terms.  Problems switched off by pb.active or whose trial failed
+// (flags) are skipped.
+void launch_refine_residual(hipStream_t st, const DevProblem& pb, const double* cams, const double* pts,
+                            const double* Vinv6, const double* VB, const double* U, const double* bc, double lambda,
+                            const double* x, double* z, double* rot, const double* dscale, double* out,
+                            const TrialFlags* flags);
+// x_i += s_i d_i for i < n (s NULL: x += d)
+void launch_vec_axpy_scaled(hipStream_t st, int n, double* x, const double* d, const double* s);
 
 // k_backsub.hip
 void launch_trial_cams(hipStream_t st, int C, const double* cams, const double* dc, double* tcams);

@@ -64,7 +64,9 @@ typedef struct gba_lm_result {
     unsigned long long anomalies; /* cumulative non-finite edge count */
     double total_time_s;          /* includes device allocation and upload */
     int trials;                   /* LM trials evaluated */
-    double solve_time_s;          /* device-resident part (after upload) */
+    double solve_time_s;          /* device-resident part (after upload); in a batch, from
+                                     the start of the batch solve to the round that
+                                     finished this problem (device clock) */
     double setup_time_s;          /* allocation, upload, observation sort */
     int refine_iterations;        /* mixed precision: PCG steps, summed over trials */
     double max_refine_residual;   /* mixed precision: worst final |S x - b| / |b| of the PCG,
@@ -194,8 +196,7 @@ void gba_solver_destroy(gba_solver* s);
 /* ---- batched solve (no reference counterpart) --------------------------- */
 
 /* Many independent problems solved together (at most 113 cameras each), each
- * with the LM semantics of gba_solve_bal and the same options, plain FP64;
- * every stage of an LM round is one launch for the whole batch, and a
+ * with the LM semantics of gba_solve_bal and the same options; every stage of an LM round is one launch for the whole batch, and a
  * round's launch sequences are replayed from HIP graphs captured on first
  * use.  For callers that solve many small reconstructions (the reference
  * solves them one call at a time).  Handle form: create uploads and sorts
@@ -205,6 +206,17 @@ void gba_solver_destroy(gba_solver* s);
  * batch took. */
 typedef struct gba_batch gba_batch;
 int gba_batch_create(const gba_problem_view* probs, int count, gba_batch** out);
+/* gba_batch_create with a precision (enum gba_precision).  GBA_PRECISION_FP64
+ * is gba_batch_create.  GBA_PRECISION_MIXED stores W in FP32 and assembles
+ * each problem's damped reduced camera system diagonally scaled, D S D with
+ * D = (diag(U + lambda diag'(U)))^-1/2, from 64-bit fixed-point sums
+ * (resolution 2^-46), stored and factored in FP64; then one FP64 refinement
+ * step: the residual of the FP64 system, evaluated without W (the Jacobians
+ * are evaluated again at the linearisation point), is solved with the same
+ * factor and the correction added (GBA_BATCH_REFINE_STEPS sets the number
+ * of steps).  The cost, gain ratio, point back-substitution and LM rules stay
+ * FP64 in both. */
+int gba_batch_create_precision(const gba_problem_view* probs, int count, int precision, gba_batch** out);
 int gba_batch_solve(gba_batch* b, const gba_lm_options* opts, gba_lm_result* results, int* rounds);
 int gba_batch_params(gba_batch* b, int index, double* cameras, double* points);
 /* What the handle chose: *tile_factor 1 = tile-parallel factor across the
