@@ -48,14 +48,17 @@ class LMResultC(ctypes.Structure):
                 ("anomalies", ctypes.c_ulonglong), ("total_time_s", ctypes.c_double),
                 ("trials", ctypes.c_int), ("solve_time_s", ctypes.c_double),
                 ("setup_time_s", ctypes.c_double), ("refine_iterations", ctypes.c_int),
-                ("max_refine_residual", ctypes.c_double), ("refine_factor_trials", ctypes.c_int)]
+                ("max_refine_residual", ctypes.c_double), ("refine_factor_trials", ctypes.c_int),
+                ("refine_outer_steps", ctypes.c_int), ("refine_floor_trials", ctypes.c_int),
+                ("max_refine_floor", ctypes.c_double)]
 
 
 class SolverConfigC(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("refine_tol", ctypes.c_double),
                 ("refine_max_iters", ctypes.c_int), ("trace", _dp), ("trace_capacity", ctypes.c_int),
                 ("obs_storage", ctypes.c_int), ("huber_delta", ctypes.c_double), ("factor_layout", ctypes.c_int),
-                ("mixed_precond", ctypes.c_int), ("jacobi_max_iters", ctypes.c_int)]
+                ("mixed_precond", ctypes.c_int), ("jacobi_max_iters", ctypes.c_int),
+                ("refine_corrections", ctypes.c_int)]
 
 
 class SynthSpecC(ctypes.Structure):
@@ -232,19 +235,25 @@ class LMResult:
     max_refine_residual: float = 0.0
     trace: np.ndarray = None
     refine_factor_trials: int = 0
+    refine_outer_steps: int = 0
+    refine_floor_trials: int = 0
+    max_refine_floor: float = 0.0
 
     @staticmethod
     def from_c(r, trace=None):
         return LMResult(r.initial_cost, r.final_cost, r.iterations, TERMINATION[r.termination],
                         r.anomalies, r.total_time_s, r.trials, r.solve_time_s, r.setup_time_s,
-                        r.refine_iterations, r.max_refine_residual, trace, r.refine_factor_trials)
+                        r.refine_iterations, r.max_refine_residual, trace, r.refine_factor_trials,
+                        r.refine_outer_steps, r.refine_floor_trials, r.max_refine_floor)
 
 
 def _config(precision, trace_capacity=0, refine_tol=1e-10, refine_max_iters=12, obs_storage=OBS_FP64,
-            huber_delta=0.0, layout=None, precond=None, jacobi_max_iters=None):
+            huber_delta=0.0, layout=None, precond=None, jacobi_max_iters=None, refine_corrections=None):
     c = SolverConfigC()
     lib().gba_default_config(ctypes.byref(c))
     c.precision = precision
+    if refine_corrections is not None:
+        c.refine_corrections = refine_corrections
     if precond is not None:
         c.mixed_precond = precond
     if jacobi_max_iters is not None:
@@ -290,10 +299,10 @@ class Solver:
     """Device-resident solver handle: upload once, solve repeatedly."""
 
     def __init__(self, prob, precision=PRECISION_FP64, trace_capacity=1024, obs_storage=OBS_FP64,
-                 huber_delta=0.0, layout=None, precond=None, jacobi_max_iters=None):
+                 huber_delta=0.0, layout=None, precond=None, jacobi_max_iters=None, refine_corrections=None):
         self._cfg, self._trace = _config(precision, trace_capacity, obs_storage=obs_storage,
                                          huber_delta=huber_delta, layout=layout, precond=precond,
-                                         jacobi_max_iters=jacobi_max_iters)
+                                         jacobi_max_iters=jacobi_max_iters, refine_corrections=refine_corrections)
         self._h = ctypes.c_void_p()
         self._prob = prob
         v = prob.view()

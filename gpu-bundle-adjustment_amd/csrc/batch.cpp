@@ -322,7 +322,7 @@ void BatchSolver::enqueue_trial(const DevProblem& pl, bool tile_factor) {
         // correction solved, dc += D y
         for (int it = 0; it < refine_steps_; ++it) {
             launch_refine_residual(st_, pl, cams_.p, pts_.p, Vinv_.p, VB_.p, U_.p, bc_.p, 0.0, dc_.p, zp_.p,
-                                   rot_.p, dscale_.p, rhs_.p, flags_.p);
+                                   rot_.p, gram_.p, dscale_.p, rhs_.p, nullptr, flags_.p);
             launch_batch_chol_solve(st_, B_, desc_.p, S_.p, D_.p, rhs_.p, ys_.p, flags_.p, pl.active, false);
             launch_vec_axpy_scaled(st_, n, dc_.p, ys_.p, dscale_.p);
         }

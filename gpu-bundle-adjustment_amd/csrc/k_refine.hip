@@ -14,7 +14,9 @@
 // products are not finite contributes nothing (its W is 0).  The rotation
 // terms (sine, cosine, square root, divisions) depend on the camera only:
 // the point pass reads them from a per-camera table, the camera pass
-// evaluates them once per thread.
+// evaluates them once per wavefront.  The camera pass runs over the camera
+// chunks (a wavefront each) with per-chunk partial sums, summed per row in a
+// fixed order: no atomics, one result for one input.
 #include <hip/hip_runtime.h>
 
 #include "device_common.hpp"
@@ -63,7 +65,8 @@ __global__ void k_rot_terms(int C, const double* __restrict__ cams, double* __re
 }
 
 // Point pass over the work units (a wavefront per unit, as the
-// back-substitution): z_p for every observed point of a running problem.
+// back-substitution): z_p for every observed point of a running problem
+// (x NULL: x = 0, z_p = V'^-1 bp).
 __global__ __launch_bounds__(256) void k_refine_points(DevProblem pb, const double* __restrict__ cams,
                                                        const double* __restrict__ rot,
                                                        const double* __restrict__ pts,
@@ -83,12 +86,12 @@ __global__ __launch_bounds__(256) void k_refine_points(DevProblem pb, const doub
         if (e < rg.y) {
             key = pb.spt[e];
             const int c = pb.scam[e];
-            jp_t_jc_x(pb, cams, rot, pts, e, c, key, x + 9 * static_cast<size_t>(c), m);
+            if (x) jp_t_jc_x(pb, cams, rot, pts, e, c, key, x + 9 * static_cast<size_t>(c), m);
         }
         seg_allreduce<3>(m, key);
     } else {
         key = pb.spt[rg.x];
-        for (int base = rg.x; base < rg.y; base += kWave) {
+        for (int base = rg.x; x && base < rg.y; base += kWave) {
             const int e = base + lane;
             if (e < rg.y) {
                 const int c = pb.scam[e];
@@ -116,26 +119,26 @@ __global__ __launch_bounds__(256) void k_refine_points(DevProblem pb, const doub
     }
 }
 
-// Camera pass, one workgroup per camera: strided share of its observations
-// per thread, then a fixed-order reduction (wavefront sums, the four
-// wavefronts in order).  out = r, or D r with dscale.
-__global__ __launch_bounds__(256) void k_refine_cams(DevProblem pb, const double* __restrict__ cams,
-                                                     const double* __restrict__ pts, const double* __restrict__ U,
-                                                     const double* __restrict__ bc, double lambda,
-                                                     const double* __restrict__ x, const double* __restrict__ z,
-                                                     const double* __restrict__ dscale, double* __restrict__ out,
-                                                     const TrialFlags* __restrict__ flags) {
-    __shared__ double s_red[4][9];
-    const int a = blockIdx.x, tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+// Camera pass over the camera chunks (runs of at most 256 observations of
+// one camera, one wavefront each): the chunk's sums of Jc^T (Jp z_p) and of
+// their magnitudes (18 values) into part.
+__global__ __launch_bounds__(256) void k_refine_cam_chunks(DevProblem pb, const double* __restrict__ cams,
+                                                           const double* __restrict__ pts,
+                                                           const double* __restrict__ z, double* __restrict__ part,
+                                                           const TrialFlags* __restrict__ flags) {
+    const int lane = lane_id();
+    const int chunk = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (chunk >= pb.num_chunks) return;
+    const int4 ch = pb.chunks[chunk];
+    const int a = ch.x;
     const int q = prob_of_cam(pb, a);
     if (prob_off(pb, q) || trial_failed(flags + q)) return;
-    lambda = lam_of(pb, q, lambda);
-    double acc[9];
-#pragma unroll
-    for (int i = 0; i < 9; ++i) acc[i] = 0.0;
     const double* cam = cams + 9 * static_cast<size_t>(a);
     const RotTerms rt = rot_terms(cam);
-    for (int idx = pb.cam_off[a] + tid; idx < pb.cam_off[a + 1]; idx += 256) {
+    double acc[9], amag[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) acc[i] = amag[i] = 0.0;
+    for (int idx = ch.y + lane; idx < ch.z; idx += kWave) {
         const int e = pb.cam_list[idx];
         const int p = pb.spt[e];
         const double2 o = obs_px(pb, e, a);
@@ -153,25 +156,60 @@ __global__ __launch_bounds__(256) void k_refine_cams(DevProblem pb, const double
         for (int i = 0; i < 9; ++i) ok = ok && finite_d(t[i]);
         if (!ok) continue;
 #pragma unroll
-        for (int i = 0; i < 9; ++i) acc[i] += t[i];
+        for (int i = 0; i < 9; ++i) {
+            acc[i] += t[i];
+            amag[i] += fabs(t[i]);
+        }
     }
+    double* out = part + 18 * static_cast<size_t>(chunk);
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
         const double t = wave_sum(acc[i]);
-        if (lane == 0) s_red[w][i] = t;
+        const double m = wave_sum(amag[i]);
+        if (lane == 0) {
+            out[i] = t;
+            out[9 + i] = m;
+        }
     }
-    __syncthreads();
-    if (tid < 9) {
-        const size_t row = 9 * static_cast<size_t>(a) + tid;
-        const double s = ((s_red[0][tid] + s_red[1][tid]) + s_red[2][tid]) + s_red[3][tid];
-        const double* Ua = U + 81 * static_cast<size_t>(a) + 9 * tid;
+}
+
+// One thread per camera row: the row's chunk sums in order, then
+// out = D (bc - (U + lambda diag'(U)) x - sum) and mag (optional) = D (|bc|
+// + |terms of U x| + sum of magnitudes).  The rounding of the residual's
+// evaluation in FP64 is a small multiple of 2^-53 times mag; near a minimum
+// the residual is a small difference of such terms (the reduced gradient),
+// and that is its floor.  x NULL: x = 0.
+__global__ void k_refine_cam_rows(DevProblem pb, const double* __restrict__ U, const double* __restrict__ bc,
+                                  double lambda, const double* __restrict__ x, const double* __restrict__ part,
+                                  const double* __restrict__ dscale, double* __restrict__ out,
+                                  double* __restrict__ mag, const TrialFlags* __restrict__ flags) {
+    const int row = blockIdx.x * 256 + threadIdx.x;
+    if (row >= 9 * pb.C) return;
+    const int a = row / 9, i = row - 9 * a;
+    const int q = prob_of_cam(pb, a);
+    if (prob_off(pb, q) || trial_failed(flags + q)) return;
+    lambda = lam_of(pb, q, lambda);
+    double s = 0.0, sm = 0.0;
+    for (int c = pb.cam_chunk_off[a]; c < pb.cam_chunk_off[a + 1]; ++c) {
+        s += part[18 * static_cast<size_t>(c) + i];
+        sm += part[18 * static_cast<size_t>(c) + 9 + i];
+    }
+    double ux = 0.0, um = 0.0;
+    if (x) {
+        const double* Ua = U + 81 * static_cast<size_t>(a) + 9 * i;
         const double* xa = x + 9 * static_cast<size_t>(a);
-        double ux = lambda * damp_diag(Ua[tid]) * xa[tid];
+        ux = lambda * damp_diag(Ua[i]) * xa[i];
+        um = fabs(ux);
 #pragma unroll
-        for (int j = 0; j < 9; ++j) ux += Ua[j] * xa[j];
-        const double r = bc[row] - ux - s;
-        out[row] = dscale ? r * dscale[row] : r;
+        for (int j = 0; j < 9; ++j) {
+            ux += Ua[j] * xa[j];
+            um += fabs(Ua[j] * xa[j]);
+        }
     }
+    const double r = bc[row] - ux - s;
+    const double sc = dscale ? dscale[row] : 1.0;
+    out[row] = r * sc;
+    if (mag) mag[row] = (fabs(bc[row]) + um + sm) * sc;
 }
 
 // x_i += s_i d_i (s NULL: 1)
@@ -185,16 +223,20 @@ __global__ void k_vec_axpy_scaled(int n, double* __restrict__ x, const double* _
 
 void launch_refine_residual(hipStream_t st, const DevProblem& pb, const double* cams, const double* pts,
                             const double* Vinv6, const double* VB, const double* U, const double* bc, double lambda,
-                            const double* x, double* z, double* rot, const double* dscale, double* out,
-                            const TrialFlags* flags) {
+                            const double* x, double* z, double* rot, double* part, const double* dscale, double* out,
+                            double* mag, const TrialFlags* flags) {
     const int grid = (pb.num_units + 3) / 4;
-    if (pb.C > 0) hipLaunchKernelGGL(k_rot_terms, dim3((pb.C + 255) / 256), dim3(256), 0, st, pb.C, cams, rot);
+    if (x && pb.C > 0)
+        hipLaunchKernelGGL(k_rot_terms, dim3((pb.C + 255) / 256), dim3(256), 0, st, pb.C, cams, rot);
     if (grid > 0)
         hipLaunchKernelGGL(k_refine_points, dim3(grid), dim3(256), 0, st, pb, cams, rot, pts, Vinv6, VB, x, z,
                            flags);
+    if (pb.num_chunks > 0)
+        hipLaunchKernelGGL(k_refine_cam_chunks, dim3((pb.num_chunks + 3) / 4), dim3(256), 0, st, pb, cams, pts, z,
+                           part, flags);
     if (pb.C > 0)
-        hipLaunchKernelGGL(k_refine_cams, dim3(pb.C), dim3(256), 0, st, pb, cams, pts, U, bc, lambda, x, z, dscale,
-                           out, flags);
+        hipLaunchKernelGGL(k_refine_cam_rows, dim3((9 * pb.C + 255) / 256), dim3(256), 0, st, pb, U, bc, lambda, x,
+                           part, dscale, out, mag, flags);
 }
 
 void launch_vec_axpy_scaled(hipStream_t st, int n, double* x, const double* d, const double* s) {

@@ -179,7 +179,7 @@ void launch_precond_apply(hipStream_t st, const float* X32, int ld, int npad, co
                           double* z, double* work, const double* done = nullptr);
 
 // k_cg.hip -- block-Jacobi conjugate gradients on the FP32-stored scaled S
-// (the mixed form's factor-free refinement).  A: lower tiles valid, camera
+// (the mixed form's factor-free inner solver).  A: lower tiles valid, camera
 // blocks on the diagonal in both triangles; vectors have npad entries.
 // Matrix-product workgroups: each takes a run of up to tpw tiles of one tile
 // row, wgtab[w] = (tile row i, run c), listed row by row; rowstart[i] is the
@@ -251,15 +251,18 @@ void launch_batch_lm_trial(hipStream_t st, int count, const gba_lm_options& opts
 
 // k_refine.hip
 // out = rhs - S x of the damped reduced camera system in FP64 (D (rhs - S x)
-// with dscale), with neither S nor W read: two passes that evaluate the
+// with dscale), with neither S nor W read: passes that evaluate the
 // Jacobians again at (cams, pts) -- z_p = V'^-1 (bp - sum W_f^T x) into z
-// (3P), then the camera sums; rot (4C doubles) receives the cameras'
-// rotation terms.  Problems switched off by pb.active or whose trial failed
-// (flags) are skipped.
+// (3P), then the camera sums per camera chunk into part (18 doubles per
+// chunk), then the rows; rot (4C doubles) receives the cameras' rotation
+// terms; mag (optional) receives per row the sum of the magnitudes of the
+// row's terms, scaled like out (the rounding of the evaluation is a small
+// multiple of 2^-53 times it).  x NULL: x = 0 (out = D rhs).  Problems
+// switched off by pb.active or whose trial failed (flags) are skipped.
 void launch_refine_residual(hipStream_t st, const DevProblem& pb, const double* cams, const double* pts,
                             const double* Vinv6, const double* VB, const double* U, const double* bc, double lambda,
-                            const double* x, double* z, double* rot, const double* dscale, double* out,
-                            const TrialFlags* flags);
+                            const double* x, double* z, double* rot, double* part, const double* dscale, double* out,
+                            double* mag, const TrialFlags* flags);
 // x_i += s_i d_i for i < n (s NULL: x += d)
 void launch_vec_axpy_scaled(hipStream_t st, int n, double* x, const double* d, const double* s);
 

@@ -14,7 +14,6 @@
 #include <chrono>
 #include <climits>
 #include <cmath>
-#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 
@@ -401,6 +400,12 @@ Solver::Solver(const gba_problem_view& prob, const gba_solver_config& cfg) : cfg
         D64_.alloc(2 * static_cast<size_t>(nblk_) * kTile * kTile);
     }
     rhs_.alloc(np);
+    if (mixed_) {
+        ref_res_.alloc(2 * np);  // the residual, then the magnitudes of its terms
+        ref_dx_.alloc(np);
+        ref_z_.alloc(3 * static_cast<size_t>(P));
+        ref_rot_.alloc(4 * static_cast<size_t>(C));
+    }
     y_.alloc(np);
     x_.alloc(np);
     dc_.alloc(np);
@@ -426,11 +431,13 @@ Solver::Solver(const gba_problem_view& prob, const gba_solver_config& cfg) : cfg
     if (mixed_) setup_cg();  // after every buffer its kernel arguments point at
     GBA_HIP(hipHostMalloc(&h_status_, sizeof(PassStatus), hipHostMallocDefault));
     std::memset(h_status_, 0, sizeof(PassStatus));
+    if (mixed_) GBA_HIP(hipHostMalloc(&h_res_, sizeof(double) * 2 * np, hipHostMallocDefault));
     // points without observations keep V = 0, bp = 0; padding of S is identity;
     // rhs padding stays zero
     launch_fill_double(st_, VB_.p, VB_.n, 0.0);
     launch_fill_double(st_, Vinv_.p, Vinv_.n, 0.0);
     launch_fill_double(st_, rhs_.p, rhs_.n, 0.0);
+    if (mixed_) launch_fill_double(st_, ref_res_.p, ref_res_.n, 0.0);
     launch_fill_double(st_, dc_.p, dc_.n, 0.0);
     launch_fill_double(st_, x_.p, x_.n, 0.0);
     if (mixed_) {
@@ -452,6 +459,7 @@ Solver::Solver(const gba_problem_view& prob, const gba_solver_config& cfg) : cfg
 Solver::~Solver() {
     if (st_) (void)hipStreamSynchronize(st_);
     if (h_status_) (void)hipHostFree(h_status_);
+    if (h_res_) (void)hipHostFree(h_res_);
     if (st_) (void)hipStreamDestroy(st_);
 }
 
@@ -604,16 +612,17 @@ static void check_cg_args(const CgArgs& g) {
         throw Error("block-Jacobi refinement: inconsistent shape");
 }
 
-int Solver::cg_solve(double tol, int max_it, double* rel, bool* converged) {
+int Solver::cg_solve(const double* b, double* x, double tol, int max_it, int* last, double* rel, bool* converged) {
     CgArgs g = cg_;
+    g.b = b;
+    g.x = x;
     g.tol2 = tol * tol;
     g.max_it = max_it;
     check_cg_args(g);
-    launch_cg_block_inverse(st_, S32low_.p, pb_.npad, pb_.C, cg_minv_.p, flags_.p);
     launch_cg_start(st_, g);
     // steps k = 0 .. max_it (the launch of step max_it only ends the loop),
-    // the first batch sized by the last refinement's count, then batches of 8
-    int enq = 0, batch = std::max(4, cg_last_ + std::max(2, cg_last_ / 8));
+    // the first batch sized by the last such solve's count, then batches of 8
+    int enq = 0, batch = std::max(4, *last + std::max(2, *last / 8));
     for (;;) {
         const int nb = std::min(batch, max_it + 1 - enq);
         for (int k = enq; k < enq + nb; ++k) launch_cg_step(st_, g, k);
@@ -631,7 +640,7 @@ int Solver::cg_solve(double tol, int max_it, double* rel, bool* converged) {
     if (!(bb > 0.0)) return 0;
     *rel = std::sqrt(h.pcg_rr / bb);
     *converged = !h.pcg_stop && h.pcg_rr <= g.tol2 * bb;
-    if (*converged) cg_last_ = h.pcg_it;
+    if (*converged) *last = h.pcg_it;
     return h.pcg_it;
 }
 
@@ -684,37 +693,106 @@ void Solver::enqueue_trial_linear_algebra(double lambda, std::vector<double>* ho
                           sync_.p, flags_.p, env_ ? first_.p : nullptr);
         return;
     }
-    // mixed: the camera system D S D x = D rhs is solved in FP64 by CG against
-    // the FP32-assembled D S D, preconditioned by its camera blocks or by its
-    // FP32 factor
+    mixed_solve(lambda, out);
+}
+
+// Rounding level of the FP64 residual evaluation relative to the magnitudes
+// of its terms: 32 units of 2^-53 (per-lane sums of a few terms, then a
+// tree of about a dozen levels).
+constexpr double kResidualFloor = 32.0 * 0x1p-53;
+// Inner tolerances of the FP64 refinement: the first solve when corrections
+// follow, and each correction relative to its right-hand side.  At C=356 the
+// FP32-stored system's solution has an FP64 residual of about 3e-8 at lambda
+// 1e-4, so solving it further than 1e-8 buys nothing, and one correction to
+// 1e-2 takes that to about 1e-11 (measured).
+constexpr double kInnerTol = 1e-8;
+constexpr double kCorrTol = 1e-2;
+
+void Solver::mixed_solve(double lambda, TrialOut& out) {
+    const int ld = pb_.npad;
+    const size_t np = static_cast<size_t>(pb_.npad);
     const double tol = cfg_.refine_tol > 0 ? cfg_.refine_tol : 1e-10;
-    if (cg_active()) {
-        const int jmax = cfg_.jacobi_max_iters > 0 ? cfg_.jacobi_max_iters : 200;
-        bool conv = false;
-        out.refine_iters = cg_solve(tol, jmax, &out.refine_rel, &conv);
-        if (conv) {
-            launch_vec_scale(st_, pb_.n, pb_.npad, x_.p, dscale_.p, dc_.p);
-            return;
-        }
-        // missed the tolerance within the cap: this trial and the rest of the
-        // solve are refined with the FP32 factor (its copy of S from the lower one)
-        cg_off_ = true;
-        GBA_HIP(hipMemcpyAsync(S32_.p, S32low_.p, sizeof(float) * S32_.n, hipMemcpyDeviceToDevice, st_));
-    }
-    out.refined_by_factor = true;
-    enqueue_factor();
+    const int jmax = cfg_.jacobi_max_iters > 0 ? cfg_.jacobi_max_iters : 200;
     const int max_it = cfg_.refine_max_iters > 0 ? cfg_.refine_max_iters : 12;
-    // first batch: the previous trial's step count (the counts of one solve
-    // differ by about one); GBA_PCG_BATCH=1 in the environment enqueues one
-    // step per synchronisation
+    const int corrections = cfg_.refine_corrections == 0 ? 1 : std::max(0, cfg_.refine_corrections);
+    // GBA_PCG_BATCH=1 in the environment enqueues one PCG step per
+    // synchronisation (the default: the previous trial's step count)
     static const bool one_step = [] {
         const char* e = std::getenv("GBA_PCG_BATCH");
         return e && std::atoi(e) == 1;
     }();
-    out.refine_iters = pcg_solve(st_, S32low_.p, precond(), ld, pb_.npad, rhs_.p, x_.p, pcgw_.p, flags_.p,
-                                 status_.p, h_status_, tol, max_it, &out.refine_rel, one_step ? 1 : last_refine_);
-    if (out.refine_iters > 0) last_refine_ = out.refine_iters;
+    bool factor = !cg_active();
+    if (factor) {
+        out.refined_by_factor = true;
+        enqueue_factor();
+    } else {
+        launch_cg_block_inverse(st_, S32low_.p, pb_.npad, pb_.C, cg_minv_.p, flags_.p);
+    }
+    // inner solve of A d = b on the FP32-stored system to tolerance t
+    // relative to |b|; false when the block-Jacobi CG missed it within its cap
+    auto inner = [&](const double* b, double* d, double t, bool first) {
+        bool conv = true;
+        double rel = 0.0;
+        if (!factor) {
+            out.refine_iters += cg_solve(b, d, t, jmax, first ? &cg_last_ : &cg_last_corr_, &rel, &conv);
+        } else {
+            int& last = first ? last_refine_ : pcg_last_corr_;
+            const int it = pcg_solve(st_, S32low_.p, precond(), ld, pb_.npad, b, d, pcgw_.p, flags_.p, status_.p,
+                                     h_status_, t, max_it, &rel, one_step ? 1 : last);
+            out.refine_iters += it;
+            if (it > 0) last = it;
+        }
+        return conv;
+    };
+    // First solve, of the system the Schur kernels formed (S and rhs from
+    // FP32 W): to 1e-8 when corrections follow (the FP32 system is itself
+    // only that close to the FP64 one), to tol when none do.
+    const double tin = corrections > 0 ? std::max(tol, kInnerTol) : tol;
+    if (!inner(rhs_.p, x_.p, tin, true) && !factor) {
+        // missed the tolerance within the cap: this trial and the rest of the
+        // solve are refined with the FP32 factor (its copy of S from the lower one)
+        cg_off_ = true;
+        factor = true;
+        out.refined_by_factor = true;
+        GBA_HIP(hipMemcpyAsync(S32_.p, S32low_.p, sizeof(float) * S32_.n, hipMemcpyDeviceToDevice, st_));
+        enqueue_factor();
+        inner(rhs_.p, x_.p, tin, true);
+    }
+    const TrialFlags f = h_status_->flags;
     launch_vec_scale(st_, pb_.n, pb_.npad, x_.p, dscale_.p, dc_.p);
+    const double bb = h_status_->pcg_bb;  // |D rhs|^2
+    if (f.nonspd || f.chol_fail || f.solve_err || !(bb > 0.0)) return;  // no step to refine
+    // FP64 iterative refinement: the residual D (rhs - S dc) of the FP64
+    // system (Jacobians evaluated again, W not read) and the magnitudes of its
+    // terms, norms on the host; the correction A d = D r to kCorrTol relative
+    // to |D r| on the FP32 system, x += d.  The chunk partials reuse the
+    // linearisation's Gram buffer (55 doubles per camera chunk).
+    double rr_prev = bb;
+    for (int k = 0;; ++k) {
+        launch_refine_residual(st_, pb_, cams_[cur_].p, pts_[cur_].p, Vinv_.p, VB_.p, U_.p, bc_.p, lambda, dc_.p,
+                               ref_z_.p, ref_rot_.p, gram_.p, dscale_.p, ref_res_.p, ref_res_.p + np, flags_.p);
+        GBA_HIP(hipMemcpyAsync(h_res_, ref_res_.p, sizeof(double) * 2 * np, hipMemcpyDeviceToHost, st_));
+        GBA_HIP(hipGetLastError());
+        GBA_HIP(hipStreamSynchronize(st_));
+        double rr = 0.0, mm = 0.0;
+        for (size_t i = 0; i < np; ++i) {
+            rr += h_res_[i] * h_res_[i];
+            mm += h_res_[np + i] * h_res_[np + i];
+        }
+        out.refine_outer++;
+        // the residual cannot be evaluated below the rounding of its terms
+        // (kResidualFloor |terms|); a refinement whose residual reaches that
+        // floor, or stops shrinking, has gone as far as FP64 can tell
+        const double floor2 = kResidualFloor * kResidualFloor * mm;
+        out.refine_rel = std::sqrt(rr / bb);
+        out.refine_floor = std::max(out.refine_floor, std::sqrt(floor2 / bb));
+        const bool done = !(rr > tol * tol * bb) || !(rr > floor2) || (k > 0 && !(rr < 0.25 * rr_prev));
+        rr_prev = rr;
+        if (done || k >= corrections) break;
+        inner(ref_res_.p, ref_dx_.p, kCorrTol, false);
+        launch_vec_axpy(st_, pb_.npad, x_.p, ref_dx_.p);
+        launch_vec_scale(st_, pb_.n, pb_.npad, x_.p, dscale_.p, dc_.p);
+    }
 }
 
 void Solver::enqueue_factor() {
@@ -948,6 +1026,9 @@ void Solver::solve(const gba_lm_options& opts, gba_lm_result* res, double t_entr
     res->iterations = 0;
     res->trials = 0;
     res->refine_iterations = 0;
+    res->refine_outer_steps = 0;
+    res->refine_floor_trials = 0;
+    res->max_refine_floor = 0.0;
     res->max_refine_residual = 0.0;
     res->refine_factor_trials = 0;
     cg_off_ = false;
@@ -964,6 +1045,9 @@ void Solver::solve(const gba_lm_options& opts, gba_lm_result* res, double t_entr
             TrialOut t = trial(lambda);
             res->trials++;
             res->refine_iterations += t.refine_iters;
+            res->refine_outer_steps += t.refine_outer;
+            if (t.refine_rel > (cfg_.refine_tol > 0 ? cfg_.refine_tol : 1e-10)) res->refine_floor_trials++;
+            res->max_refine_floor = std::max(res->max_refine_floor, t.refine_floor);
             if (t.refined_by_factor) res->refine_factor_trials++;
             if (!t.flags.nonspd && !t.flags.chol_fail && !t.flags.solve_err)
                 res->max_refine_residual = std::max(res->max_refine_residual, t.refine_rel);

@@ -144,8 +144,10 @@ struct TrialOut {
     double trial_sse = 0.0;
     double denom = 0.0;
     unsigned long long anomalies = 0;
-    int refine_iters = 0;
-    double refine_rel = 0.0;
+    int refine_iters = 0;     // mixed: inner CG / PCG steps, all solves of the trial
+    double refine_rel = 0.0;  // mixed: final |D (rhs - S x)| / |D rhs| of the FP64 system
+    int refine_outer = 0;     // mixed: FP64 residual evaluations
+    double refine_floor = 0.0;  // mixed: rounding level of that residual, relative to |D rhs|
     bool refined_by_factor = false;  // mixed: the FP32 factor preconditioned this trial
 };
 
@@ -192,13 +194,25 @@ private:
     void sync_status();
     // fac_copy: also write the FP32 factor copy of the scaled S (mixed; the
     // block-Jacobi refinement reads only the lower copy)
+    // mixed: the scaled S into the lower copy (the inner solver's operator),
+    // and into the factor copy when fac_copy
     void enqueue_schur(double lambda, bool fac_copy = true);
     void enqueue_factor();  // the dense factor of S (and X = L^-1 in mixed precision)
-    // Mixed, block-Jacobi refinement (k_cg.hip): solves the scaled system into
-    // x_; returns the steps, *rel = |r| / |b|, *converged when |r| <= tol |b|
-    // (or the trial failed before the solve, which needs no refinement).
+    // Mixed, block-Jacobi CG (k_cg.hip) on the FP32-stored scaled system:
+    // solves A x = b (A the scaled S, camera-block inverses from
+    // launch_cg_block_inverse of the trial); returns the steps, *rel = |r| /
+    // |b|, *converged when |r| <= tol |b| (or the trial failed before the
+    // solve, which needs no refinement); *last sizes the first batch of steps
+    // and receives the count.
     void setup_cg();
-    int cg_solve(double tol, int max_it, double* rel, bool* converged);
+    int cg_solve(const double* b, double* x, double tol, int max_it, int* last, double* rel, bool* converged);
+    // Mixed: the inner solve of the FP32-stored system (block-Jacobi CG, or
+    // PCG with the FP32 factor), then FP64 iterative refinement: the residual
+    // of the FP64 reduced camera system (k_refine.hip) against dc_, each
+    // correction solved by the same inner solver, until |D (rhs - S dc)| <=
+    // tol |D rhs|, the corrections are spent, or the residual stops
+    // shrinking.
+    void mixed_solve(double lambda, TrialOut& out);
     bool cg_active() const { return cg_mode_ && !cg_off_; }
 
     gba_solver_config cfg_;
@@ -234,7 +248,14 @@ private:
     // block-Jacobi refinement: chosen for the handle (cg_mode_), switched off
     // for the rest of a solve after a trial missed the tolerance (cg_off_)
     bool cg_mode_ = false, cg_off_ = false;
-    int cg_last_ = 40;  // steps of the last block-Jacobi refinement
+    int cg_last_ = 40;       // steps of the last block-Jacobi first solve
+    int cg_last_corr_ = 12;  // ... and of its last correction solve
+    int pcg_last_corr_ = 2;
+    // FP64 refinement: the residual D (rhs - S dc) and the magnitudes of its
+    // terms (2 npad), the correction, the point terms and camera rotation
+    // terms of the residual passes; host copy of the residual
+    DevBuf<double> ref_res_, ref_dx_, ref_z_, ref_rot_;
+    double* h_res_ = nullptr;
     CgArgs cg_{};
     DevBuf<int2> cg_wgtab_;
     DevBuf<int> cg_rowstart_;

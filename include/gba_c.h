@@ -69,13 +69,24 @@ typedef struct gba_lm_result {
                                      finished this problem (device clock) */
     double setup_time_s;          /* allocation, upload, observation sort */
     int refine_iterations;        /* mixed precision: PCG steps, summed over trials */
-    double max_refine_residual;   /* mixed precision: worst final |S x - b| / |b| of the PCG,
-                                     S the FP32-stored reduced camera matrix (built from
-                                     FP32 W), products accumulated in FP64; not a residual
-                                     of the FP64 system (about 2e-8 against it, measured) */
+    double max_refine_residual;   /* mixed precision: worst final |D (b - S x)| / |D b| of the
+                                     FP64 reduced camera system (S and b in FP64, evaluated
+                                     without W from the Jacobians; D the diagonal scaling),
+                                     after the FP64 iterative refinement */
     int refine_factor_trials;     /* mixed precision: trials refined with the FP32 factor
                                      (all of them with GBA_PRECOND_FACTOR; with block Jacobi,
                                      those after a trial missed the tolerance) */
+    int refine_outer_steps;       /* mixed precision: residuals of the FP64 reduced camera
+                                     system evaluated (one per trial plus one per correction
+                                     solve), summed over trials */
+    int refine_floor_trials;      /* mixed precision: trials whose refinement stopped above the
+                                     tolerance because the residual reached the rounding level
+                                     of its FP64 evaluation (or stopped shrinking) */
+    double max_refine_floor;      /* mixed precision: worst rounding level of the FP64 residual,
+                                     relative to |D b|: 32 x 2^-53 x |sum of the magnitudes of
+                                     its terms|.  Near a minimum the residual is a small
+                                     difference of large terms and cannot be evaluated, in
+                                     FP64, below this */
 } gba_lm_result;
 
 /* Arithmetic of the reduced camera solve. */
@@ -97,8 +108,10 @@ enum gba_obs_storage {
 /* Solver configuration beyond gba_lm_options (no reference counterpart). */
 typedef struct gba_solver_config {
     int precision;          /* enum gba_precision */
-    double refine_tol;      /* mixed: PCG relative residual target on the FP32-stored S
-                               (default 1e-10) */
+    double refine_tol;      /* mixed: relative residual target of the FP64 iterative
+                               refinement, |D (b - S x)| <= tol |D b| on the FP64 reduced
+                               camera system (default 1e-10); with no corrections, the
+                               inner solve's target on the FP32-stored system */
     int refine_max_iters;   /* mixed: PCG step cap per trial (default 12) */
     double* trace;          /* optional, GBA_TRACE_STRIDE doubles per trial */
     int trace_capacity;     /* trials that fit in trace */
@@ -115,6 +128,11 @@ typedef struct gba_solver_config {
                                that misses refine_tol within it is refined with the FP32
                                factor instead, and so is the rest of the solve
                                (0: default 200) */
+    int refine_corrections; /* mixed: FP64 iterative-refinement corrections per trial, each
+                               solving the residual of the FP64 system (evaluated without W,
+                               from the Jacobians) on the FP32-stored one; 0: default (1);
+                               negative: none, the FP64 residual is still evaluated once and
+                               reported */
 } gba_solver_config;
 
 /* Preconditioner of the mixed form's FP64 refinement (no reference

@@ -135,11 +135,13 @@ def test_one_and_two_camera_problems_match_oracle(gba, orc, C):
 
 @pytest.mark.parametrize("cfg,lam", [(2, 1e-4), (1, 1e-3)])
 def test_schur_step_mixed_close_to_fp64(gba, orc, cfg, lam):
-    """The mixed step solves the reduced camera system assembled from FP32 W
-    and stored in FP32, by FP64 PCG to 1e-10 relative residual of that
-    system.  Against the FP64 system (the oracle's S and rhs, the reference's
-    src/lm.cu:515-604) the step carries the FP32 rounding of W and S, amplified
-    by the condition of S: this test measures that gap and bounds it."""
+    """The mixed step: the reduced camera system assembled from FP32 W and
+    stored in FP32 is solved in FP64, then one FP64 iterative-refinement
+    correction solves the residual of the FP64 system (evaluated from the
+    Jacobians, without W).  Against the FP64 system (the oracle's S and rhs,
+    the reference's src/lm.cu:515-604) the step's residual is then far below
+    the FP32 system's distance from it (about 2e-8 here): this test measures
+    it and bounds it."""
     prob = gba.generate(cfg, 1)
     g = gba.schur_step(prob, lam, gba.PRECISION_MIXED, want_S=True)
     o = orc.schur_step(prob, orc.linearize(prob), lam, want_S=True)
@@ -150,9 +152,10 @@ def test_schur_step_mixed_close_to_fp64(gba, orc, cfg, lam):
     res64 = np.linalg.norm(S64 @ g["dc"] - o["rhs"]) / np.linalg.norm(o["rhs"])
     gap = arel(g["dc"], o["dc"])
     print(f"cfg {cfg}: mixed step vs FP64 system: relative residual {res64:.2e}, dc gap {gap:.2e}")
-    # measured on MI355X: residual 2.4e-8 (cfg 1 and 2), dc gap 1.0e-5 / 2.9e-5
-    assert res64 < 1e-7
-    assert gap < 2e-4
+    # measured on MI355X: residual 1.1e-10 / 6.4e-10 (cfg 2 / 1); 2.4e-8 for
+    # the unrefined FP32-system solution
+    assert res64 < 2e-9
+    assert gap < 1e-5
     assert abs(g["denom"] - o["denom"]) <= 1e-4 * abs(o["denom"])
 
 
@@ -355,7 +358,10 @@ def test_lm_mixed_reaches_fp64_cost(gba):
     r32 = gba.solve_bal_gpu(p32, opts, gba.PRECISION_MIXED)
     assert r32.termination == "CONVERGENCE"
     assert abs(r32.final_cost - r64.final_cost) <= 1e-6 * r64.final_cost
-    assert r32.max_refine_residual <= 1e-8
+    # the FP64 system's residual after one correction per trial; every trial
+    # evaluated it at least once
+    assert r32.max_refine_residual <= 1e-7
+    assert r32.refine_outer_steps >= r32.trials
 
 
 def test_lm_ragged_problem_matches_oracle(gba, orc):
@@ -404,8 +410,8 @@ def test_launch_bounds_gate_on_gpu(gba):
 def test_mixed_refinement_preconditioners_agree(gba, monkeypatch):
     """The mixed refinement preconditioned by the factor's triangular solves
     (what large matrices use) against the explicit inverse (small ones): both
-    solve each trial's FP32-stored system to the same residual, so the LM
-    trajectory is the same."""
+    solve each trial's FP32-stored system and refine against the FP64 one to
+    the same tolerances, so the LM trajectory is the same."""
     prob = heavy(gba)
     opts = gba.LMOptions(initial_lambda=1e-3)
     monkeypatch.setenv("GBA_PRECOND", "inverse")
@@ -417,7 +423,7 @@ def test_mixed_refinement_preconditioners_agree(gba, monkeypatch):
     assert s_tri["status"] == s_inv["status"] == "accept"
     assert arel(s_tri["dc"], s_inv["dc"]) < 1e-8
     assert tri.termination == inv.termination and tri.iterations == inv.iterations
-    assert tri.max_refine_residual <= 1e-10
+    assert tri.max_refine_residual <= 1e-7 and inv.max_refine_residual <= 1e-7
     assert abs(tri.final_cost - inv.final_cost) <= 1e-9 * inv.final_cost
 
 
