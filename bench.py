@@ -7,9 +7,12 @@ HBM (the solver handle is created, sorted and uploaded before the timed
 region).  The plain FP64 solve defines the target cost C64; the time-to-final-
 cost of a solve is the elapsed time from solve entry until the first accepted
 LM iterate whose cost is <= C64 * (1 + 1e-6).  `value` is the median over the
-timed mixed-precision steps (the library's default refinement of the mixed
-form: block-Jacobi CG at this size); the mixed form refined with its FP32
-MFMA factor is timed beside it (`mixed_factor`).  Prints one JSON line.  `including_setup`
+timed mixed-precision steps (the library's defaults for the mixed form: its
+inner solver on the FP32-stored system is the block-Jacobi CG at this size,
+then one FP64 iterative-refinement correction per trial against the FP64
+system's residual); the mixed form with its FP32 MFMA factor as the inner
+solver (`mixed_factor`) and without the correction (`mixed_no_correction`, the
+FP64 residual only evaluated) are timed beside it.  Prints one JSON line.  `including_setup`
 gives the same measure for one-shot solves that allocate, sort and upload
 inside the clock (the reference's total_time_s convention).
 
@@ -336,9 +339,13 @@ def main():
     s64 = g.Solver(prob, g.PRECISION_FP64)
     s32 = g.Solver(prob, g.PRECISION_MIXED)  # the library's choice of refinement (block Jacobi here)
     s32f = g.Solver(prob, g.PRECISION_MIXED, precond=g.PRECOND_FACTOR)
+    # the mixed form without its FP64 correction (the FP64 system's residual
+    # evaluated and reported only): what the refinement costs
+    s32n = g.Solver(prob, g.PRECISION_MIXED, refine_corrections=-1)
     jacobi = s32.precond() == g.PRECOND_JACOBI
     k64 = ("edge_point", "edge_camera", "schur", "factor", "solve", "backsub")
-    k32 = (("edge_point", "edge_camera", "schur", "cg_setup", "cg_step", "backsub") if jacobi else k64)
+    k32 = (("edge_point", "edge_camera", "schur", "cg_setup", "cg_step", "backsub") if jacobi else k64) + \
+        ("refine_residual",)
     if args.mode == "kernels":
         out = {"mode": "kernels", "config": {"cameras": C, "points": P, "observations": O}}
         for name, s, ks in (("fp64", s64, k64), ("mixed", s32, k32), ("mixed_factor", s32f, k64)):
@@ -360,6 +367,7 @@ def main():
         s64.solve(opts)
         s32.solve(opts)
         s32f.solve(opts)
+        s32n.solve(opts)
     torch.cuda.synchronize()
 
     # FP64 reference solves (also the target cost): at least five samples
@@ -378,6 +386,8 @@ def main():
     # the other refinement of the mixed form (FP32 MFMA factor), at least five samples
     r32f = [s32f.solve(opts) for _ in range(max(5, args.steps))]
     t32f = [time_to_cost(r, target) for r in r32f]
+    r32n = [s32n.solve(opts) for _ in range(max(5, args.steps))]
+    t32n = [time_to_cost(r, target) for r in r32n]
 
     # the same measure including allocation, sort and upload (one-shot
     # solve_bal_gpu calls, as the reference's total_time_s counts them)
@@ -396,16 +406,20 @@ def main():
     trials32 = statistics.median([r.trials for r in r32])
     lins32 = statistics.median([r.iterations for r in r32])
     refine = statistics.median([r.refine_iterations for r in r32])
+    outer = statistics.median([r.refine_outer_steps for r in r32])
     km = kern["mixed"]
     kf = kern["mixed_factor"]
     # kernel time per mixed solve (HIP-event averages x launches per solve)
     share = {"edge_point": lins32 * km["edge_point"], "edge_camera": lins32 * km["edge_camera"],
              "schur": trials32 * km["schur"], "backsub": trials32 * km["backsub"]}
     if jacobi:
-        share["cg_refinement"] = trials32 * km["cg_setup"] + (refine + trials32) * km["cg_step"]
+        # a CG solve runs one step past its last counted one (the launch
+        # that ends it); the first solve and the correction each
+        share["cg_refinement"] = trials32 * km["cg_setup"] + (refine + (outer or trials32)) * km["cg_step"]
     else:
         share["factor"] = trials32 * km["factor"]
         share["solve"] = (trials32 + refine) * km["solve"]
+    share["fp64_residual"] = outer * km["refine_residual"]
     dominant = max(share, key=share.get)
     w_bytes = 108  # FP32 W in the mixed form
     # edge accumulate = point pass + camera pass (one linearisation)
@@ -474,9 +488,12 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": ("f32 W/S + f64 block-Jacobi CG on the f32-stored S, f64 cost (mixed); "
-                  "f32 MFMA factor-preconditioned leg and f64 reference leg beside it") if jacobi else
-                 "f32 W/S/factor + f64 PCG on the f32-stored S, f64 cost (mixed); f64 reference leg",
+        "dtype": ("f32 W/S + f64 block-Jacobi CG on the f32-stored S, one f64 iterative-refinement "
+                  "correction against the f64 system (matrix-free residual), f64 cost (mixed); "
+                  "f32 MFMA factor-preconditioned leg, no-correction leg and f64 reference leg beside it")
+                 if jacobi else
+                 "f32 W/S/factor + f64 PCG on the f32-stored S, one f64 iterative-refinement correction against "
+                 "the f64 system, f64 cost (mixed); f64 reference leg",
         "data": "synthetic (seeded Snavely generator, seed %d)" % args.seed,
         "config": {"workload": f"LM solve C={C} P={P} O={O} (preset {args.config}), "
                                f"lambda0={args.lambda0}, rel_decrease_tol=1e-6",
@@ -491,8 +508,10 @@ def main():
                   "solve_s": round(statistics.median([r.solve_time_s for r in r32]), 6),
                   "final_cost": r32[0].final_cost, "iterations": r32[0].iterations,
                   "trials": r32[0].trials, "refine_iterations": r32[0].refine_iterations,
-                  "max_refine_residual_f32_system": r32[0].max_refine_residual,
-                  "refinement": "block-Jacobi CG" if jacobi else "FP32 factor PCG",
+                  "refine_outer_steps": r32[0].refine_outer_steps,
+                  "max_refine_residual_fp64_system": r32[0].max_refine_residual,
+                  "refine_floor_trials": r32[0].refine_floor_trials,
+                  "inner_solver": "block-Jacobi CG" if jacobi else "FP32 factor PCG",
                   "refine_factor_trials": r32[0].refine_factor_trials,
                   "termination": r32[0].termination,
                   "all_s": [round(t, 6) for t in t32]},
@@ -500,7 +519,12 @@ def main():
                          "best_s": round(min(t32f), 6), "worst_s": round(max(t32f), 6),
                          "final_cost": r32f[0].final_cost, "trials": r32f[0].trials,
                          "refine_iterations": r32f[0].refine_iterations,
-                         "max_refine_residual_f32_system": r32f[0].max_refine_residual},
+                         "max_refine_residual_fp64_system": r32f[0].max_refine_residual},
+        "mixed_no_correction": {"time_to_final_cost_s": round(statistics.median(t32n), 6), "samples": len(t32n),
+                                "best_s": round(min(t32n), 6), "worst_s": round(max(t32n), 6),
+                                "final_cost": r32n[0].final_cost, "trials": r32n[0].trials,
+                                "refine_iterations": r32n[0].refine_iterations,
+                                "max_refine_residual_fp64_system": r32n[0].max_refine_residual},
         "including_setup": once,
         "speedup_mixed_vs_fp64": round(statistics.median(t64) / value, 3) if value == value else None,
         "kernels_ms": {k: {kk: round(vv, 4) for kk, vv in v.items()} for k, v in kern.items()},

@@ -27,7 +27,7 @@ PRECOND_AUTO, PRECOND_FACTOR, PRECOND_JACOBI = 0, 1, 2  # enum gba_mixed_precond
 TRACE_STRIDE = 6
 TRIAL_STATUS = {0: "accept", 1: "reject", 2: "nonspd_v", 3: "chol_fail", 4: "denom", 5: "nonfinite"}
 KERNELS = {"edge_point": 0, "edge_camera": 1, "schur": 2, "factor": 3, "solve": 4, "backsub": 5,
-           "cost": 6, "cg_step": 7, "cg_setup": 8}
+           "cost": 6, "cg_step": 7, "cg_setup": 8, "refine_residual": 9}
 
 
 class ProblemView(ctypes.Structure):

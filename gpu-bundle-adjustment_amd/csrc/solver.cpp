@@ -963,6 +963,12 @@ double Solver::time_kernel(int kernel, double lambda, int reps) {
             case GBA_KERNEL_CG_STEP:
                 launch_cg_step(st_, cgt, cg_k++);
                 break;
+            case GBA_KERNEL_REFINE_RESIDUAL:
+                if (!mixed_) throw Error("the FP64 refinement residual is a mixed-precision kernel");
+                launch_refine_residual(st_, pb_, cams_[cur_].p, pts_[cur_].p, Vinv_.p, VB_.p, U_.p, bc_.p, lambda,
+                                       dc_.p, ref_z_.p, ref_rot_.p, gram_.p, dscale_.p, ref_res_.p,
+                                       ref_res_.p + pb_.npad, flags_.p);
+                break;
             default:
                 throw Error("unknown kernel id");
         }

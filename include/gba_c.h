@@ -195,7 +195,9 @@ enum gba_kernel_id {
     GBA_KERNEL_COST = 6,        /* residual-only cost pass */
     GBA_KERNEL_CG_STEP = 7,     /* mixed, block Jacobi: one refinement step (matrix product
                                    over S + vector update) */
-    GBA_KERNEL_CG_SETUP = 8     /* mixed, block Jacobi: inverses of the camera blocks */
+    GBA_KERNEL_CG_SETUP = 8,    /* mixed, block Jacobi: inverses of the camera blocks */
+    GBA_KERNEL_REFINE_RESIDUAL = 9 /* mixed: one residual of the FP64 reduced camera system
+                                      (matrix-free: point pass, camera chunks, rows) */
 };
 int gba_solver_time_kernel(gba_solver* s, int kernel, double lambda, int reps, double* avg_ms);
 /* Shape facts used for roofline accounting: sum_p k_p(k_p+1)/2, matrix
