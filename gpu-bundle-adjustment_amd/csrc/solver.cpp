@@ -405,6 +405,7 @@ Solver::Solver(const gba_problem_view& prob, const gba_solver_config& cfg) : cfg
         ref_dx_.alloc(np);
         ref_z_.alloc(3 * static_cast<size_t>(P));
         ref_rot_.alloc(4 * static_cast<size_t>(C));
+        neg_one_.alloc(np);
     }
     y_.alloc(np);
     x_.alloc(np);
@@ -431,13 +432,16 @@ Solver::Solver(const gba_problem_view& prob, const gba_solver_config& cfg) : cfg
     if (mixed_) setup_cg();  // after every buffer its kernel arguments point at
     GBA_HIP(hipHostMalloc(&h_status_, sizeof(PassStatus), hipHostMallocDefault));
     std::memset(h_status_, 0, sizeof(PassStatus));
-    if (mixed_) GBA_HIP(hipHostMalloc(&h_res_, sizeof(double) * 2 * np, hipHostMallocDefault));
+    if (mixed_) GBA_HIP(hipHostMalloc(&h_res_, sizeof(double) * 4 * np, hipHostMallocDefault));
     // points without observations keep V = 0, bp = 0; padding of S is identity;
     // rhs padding stays zero
     launch_fill_double(st_, VB_.p, VB_.n, 0.0);
     launch_fill_double(st_, Vinv_.p, Vinv_.n, 0.0);
     launch_fill_double(st_, rhs_.p, rhs_.n, 0.0);
-    if (mixed_) launch_fill_double(st_, ref_res_.p, ref_res_.n, 0.0);
+    if (mixed_) {
+        launch_fill_double(st_, ref_res_.p, ref_res_.n, 0.0);
+        launch_fill_double(st_, neg_one_.p, neg_one_.n, -1.0);
+    }
     launch_fill_double(st_, dc_.p, dc_.n, 0.0);
     launch_fill_double(st_, x_.p, x_.n, 0.0);
     if (mixed_) {
@@ -764,22 +768,61 @@ void Solver::mixed_solve(double lambda, TrialOut& out) {
     if (f.nonspd || f.chol_fail || f.solve_err || !(bb > 0.0)) return;  // no step to refine
     // FP64 iterative refinement: the residual D (rhs - S dc) of the FP64
     // system (Jacobians evaluated again, W not read) and the magnitudes of its
-    // terms, norms on the host; the correction A d = D r to kCorrTol relative
-    // to |D r| on the FP32 system, x += d.  The chunk partials reuse the
-    // linearisation's Gram buffer (55 doubles per camera chunk).
-    double rr_prev = bb;
-    for (int k = 0;; ++k) {
+    // terms into host slot k % 2 (norms read after the next synchronisation);
+    // the correction A d = D r to kCorrTol relative to |D r| on the FP32
+    // system, x += d.  The chunk partials reuse the linearisation's Gram
+    // buffer (55 doubles per camera chunk).  The first correction is enqueued
+    // without a host round trip (the first solve's residual is far above any
+    // tolerance it could meet: about 3e-8 at best, measured).
+    auto residual = [&](int slot) {
         launch_refine_residual(st_, pb_, cams_[cur_].p, pts_[cur_].p, Vinv_.p, VB_.p, U_.p, bc_.p, lambda, dc_.p,
                                ref_z_.p, ref_rot_.p, gram_.p, dscale_.p, ref_res_.p, ref_res_.p + np, flags_.p);
-        GBA_HIP(hipMemcpyAsync(h_res_, ref_res_.p, sizeof(double) * 2 * np, hipMemcpyDeviceToHost, st_));
-        GBA_HIP(hipGetLastError());
-        GBA_HIP(hipStreamSynchronize(st_));
-        double rr = 0.0, mm = 0.0;
-        for (size_t i = 0; i < np; ++i) {
-            rr += h_res_[i] * h_res_[i];
-            mm += h_res_[np + i] * h_res_[np + i];
-        }
+        GBA_HIP(hipMemcpyAsync(h_res_ + 2 * np * slot, ref_res_.p, sizeof(double) * 2 * np, hipMemcpyDeviceToHost,
+                               st_));
         out.refine_outer++;
+    };
+    auto norms = [&](int slot, double* rr, double* mm) {
+        const double* h = h_res_ + 2 * np * slot;
+        *rr = *mm = 0.0;
+        for (size_t i = 0; i < np; ++i) {
+            *rr += h[i] * h[i];
+            *mm += h[np + i] * h[np + i];
+        }
+    };
+    auto correct = [&](double sign) {
+        if (sign > 0) inner(ref_res_.p, ref_dx_.p, kCorrTol, false);
+        launch_vec_axpy_scaled(st_, pb_.npad, x_.p, ref_dx_.p, sign > 0 ? nullptr : neg_one_.p);
+        launch_vec_scale(st_, pb_.n, pb_.npad, x_.p, dscale_.p, dc_.p);
+    };
+    residual(0);
+    double rr = 0.0, mm = 0.0, rr_prev = bb;
+    int slot = 0;
+    if (corrections > 0) {
+        correct(+1.0);
+        residual(1);
+    }
+    GBA_HIP(hipGetLastError());
+    GBA_HIP(hipStreamSynchronize(st_));
+    norms(0, &rr, &mm);
+    if (corrections > 0) {
+        rr_prev = rr;
+        double rr1 = 0.0, mm1 = 0.0;
+        norms(1, &rr1, &mm1);
+        if (!(rr1 < rr)) {
+            // the correction did not help (the FP32 system's error along the
+            // least determined directions is of the order of the damping):
+            // take it back and keep the first solve's residual
+            correct(-1.0);
+            out.refine_floor = std::max(out.refine_floor, kResidualFloor * std::sqrt(mm / bb));
+            out.refine_rel = std::sqrt(rr / bb);
+            return;
+        }
+        rr = rr1;
+        mm = mm1;
+        slot = 1;
+    }
+    // further corrections (refine_corrections > 1), one round trip each
+    for (int k = corrections > 0 ? 1 : 0;; ++k) {
         // the residual cannot be evaluated below the rounding of its terms
         // (kResidualFloor |terms|); a refinement whose residual reaches that
         // floor, or stops shrinking, has gone as far as FP64 can tell
@@ -787,11 +830,14 @@ void Solver::mixed_solve(double lambda, TrialOut& out) {
         out.refine_rel = std::sqrt(rr / bb);
         out.refine_floor = std::max(out.refine_floor, std::sqrt(floor2 / bb));
         const bool done = !(rr > tol * tol * bb) || !(rr > floor2) || (k > 0 && !(rr < 0.25 * rr_prev));
-        rr_prev = rr;
         if (done || k >= corrections) break;
-        inner(ref_res_.p, ref_dx_.p, kCorrTol, false);
-        launch_vec_axpy(st_, pb_.npad, x_.p, ref_dx_.p);
-        launch_vec_scale(st_, pb_.n, pb_.npad, x_.p, dscale_.p, dc_.p);
+        rr_prev = rr;
+        correct(+1.0);
+        slot = 1 - slot;
+        residual(slot);
+        GBA_HIP(hipGetLastError());
+        GBA_HIP(hipStreamSynchronize(st_));
+        norms(slot, &rr, &mm);
     }
 }
 

@@ -255,7 +255,8 @@ private:
     // terms (2 npad), the correction, the point terms and camera rotation
     // terms of the residual passes; host copy of the residual
     DevBuf<double> ref_res_, ref_dx_, ref_z_, ref_rot_;
-    double* h_res_ = nullptr;
+    DevBuf<double> neg_one_;  // npad entries of -1 (x -= d)
+    double* h_res_ = nullptr;  // two slots of residual + magnitudes (4 npad)
     CgArgs cg_{};
     DevBuf<int2> cg_wgtab_;
     DevBuf<int> cg_rowstart_;
