@@ -13,10 +13,10 @@
 // indices, parameters) instead of W.  An observation whose residual or
 // products are not finite contributes nothing (its W is 0).  The rotation
 // terms (sine, cosine, square root, divisions) depend on the camera only:
-// the point pass reads them from a per-camera table, the camera pass
-// evaluates them once per wavefront.  The camera pass runs over the camera
-// chunks (a wavefront each) with per-chunk partial sums, summed per row in a
-// fixed order: no atomics, one result for one input.
+// both passes read them from a per-camera table.  The camera pass runs over
+// the camera chunks (a workgroup each, one observation per lane) with
+// per-chunk partial sums, summed per row in a fixed order: no atomics, one
+// result for one input.
 #include <hip/hip_runtime.h>
 
 #include "device_common.hpp"
@@ -120,57 +120,64 @@ __global__ __launch_bounds__(256) void k_refine_points(DevProblem pb, const doub
 }
 
 // Camera pass over the camera chunks (runs of at most 256 observations of
-// one camera, one wavefront each): the chunk's sums of Jc^T (Jp z_p) and of
-// their magnitudes (18 values) into part.
+// one camera), one workgroup of four wavefronts per chunk and one observation
+// per lane: Jc^T (Jp z_p) and its magnitudes, wavefront sums, then the four
+// wavefronts' sums combined in order in LDS: 18 values per chunk into part.
 __global__ __launch_bounds__(256) void k_refine_cam_chunks(DevProblem pb, const double* __restrict__ cams,
+                                                           const double* __restrict__ rot,
                                                            const double* __restrict__ pts,
                                                            const double* __restrict__ z, double* __restrict__ part,
                                                            const TrialFlags* __restrict__ flags) {
-    const int lane = lane_id();
-    const int chunk = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (chunk >= pb.num_chunks) return;
+    static_assert(kCamChunk == 256, "one observation per thread of a 256-thread workgroup");
+    __shared__ double s_part[4][18];
+    const int chunk = blockIdx.x;
+    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
     const int4 ch = pb.chunks[chunk];
     const int a = ch.x;
     const int q = prob_of_cam(pb, a);
     if (prob_off(pb, q) || trial_failed(flags + q)) return;
-    const double* cam = cams + 9 * static_cast<size_t>(a);
-    const RotTerms rt = rot_terms(cam);
-    double acc[9], amag[9];
+    double t[9];
 #pragma unroll
-    for (int i = 0; i < 9; ++i) acc[i] = amag[i] = 0.0;
-    for (int idx = ch.y + lane; idx < ch.z; idx += kWave) {
+    for (int i = 0; i < 9; ++i) t[i] = 0.0;
+    const int idx = ch.y + tid;
+    if (idx < ch.z) {
         const int e = pb.cam_list[idx];
         const int p = pb.spt[e];
         const double2 o = obs_px(pb, e, a);
+        const double4 t4 = reinterpret_cast<const double4*>(rot)[a];
+        const RotTerms rt{t4.x, t4.y, t4.z, t4.w != 0.0};
         double r[2];
         EdgeLin el;
-        snavely_project<true>(cam, rt, pts + 3 * static_cast<size_t>(p), o.x, o.y, r, el);
-        if (!(finite_d(r[0]) && finite_d(r[1]))) continue;
-        const double w2 = robust_w2(pb.huber, r);
-        const double* zp = z + 3 * static_cast<size_t>(p);
-        double jz[2], t[9];
-        edge_jp_z(el, Vec3{zp[0], zp[1], zp[2]}, jz);
-        edge_jcT_t(el, w2 * jz[0], w2 * jz[1], t);
-        bool ok = true;
+        snavely_project<true>(cams + 9 * static_cast<size_t>(a), rt, pts + 3 * static_cast<size_t>(p), o.x, o.y, r,
+                              el);
+        if (finite_d(r[0]) && finite_d(r[1])) {
+            const double w2 = robust_w2(pb.huber, r);
+            const double* zp = z + 3 * static_cast<size_t>(p);
+            double jz[2];
+            edge_jp_z(el, Vec3{zp[0], zp[1], zp[2]}, jz);
+            edge_jcT_t(el, w2 * jz[0], w2 * jz[1], t);
+            bool ok = true;
 #pragma unroll
-        for (int i = 0; i < 9; ++i) ok = ok && finite_d(t[i]);
-        if (!ok) continue;
+            for (int i = 0; i < 9; ++i) ok = ok && finite_d(t[i]);
+            if (!ok) {
 #pragma unroll
-        for (int i = 0; i < 9; ++i) {
-            acc[i] += t[i];
-            amag[i] += fabs(t[i]);
+                for (int i = 0; i < 9; ++i) t[i] = 0.0;
+            }
         }
     }
-    double* out = part + 18 * static_cast<size_t>(chunk);
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
-        const double t = wave_sum(acc[i]);
-        const double m = wave_sum(amag[i]);
+        const double s = wave_sum(t[i]);
+        const double m = wave_sum(fabs(t[i]));
         if (lane == 0) {
-            out[i] = t;
-            out[9 + i] = m;
+            s_part[w][i] = s;
+            s_part[w][9 + i] = m;
         }
     }
+    __syncthreads();
+    if (tid < 18)
+        part[18 * static_cast<size_t>(chunk) + tid] =
+            ((s_part[0][tid] + s_part[1][tid]) + s_part[2][tid]) + s_part[3][tid];
 }
 
 // One thread per camera row: the row's chunk sums in order, then
@@ -226,14 +233,13 @@ void launch_refine_residual(hipStream_t st, const DevProblem& pb, const double* 
                             const double* x, double* z, double* rot, double* part, const double* dscale, double* out,
                             double* mag, const TrialFlags* flags) {
     const int grid = (pb.num_units + 3) / 4;
-    if (x && pb.C > 0)
-        hipLaunchKernelGGL(k_rot_terms, dim3((pb.C + 255) / 256), dim3(256), 0, st, pb.C, cams, rot);
+    if (pb.C > 0) hipLaunchKernelGGL(k_rot_terms, dim3((pb.C + 255) / 256), dim3(256), 0, st, pb.C, cams, rot);
     if (grid > 0)
         hipLaunchKernelGGL(k_refine_points, dim3(grid), dim3(256), 0, st, pb, cams, rot, pts, Vinv6, VB, x, z,
                            flags);
     if (pb.num_chunks > 0)
-        hipLaunchKernelGGL(k_refine_cam_chunks, dim3((pb.num_chunks + 3) / 4), dim3(256), 0, st, pb, cams, pts, z,
-                           part, flags);
+        hipLaunchKernelGGL(k_refine_cam_chunks, dim3(pb.num_chunks), dim3(256), 0, st, pb, cams, rot, pts, z, part,
+                           flags);
     if (pb.C > 0)
         hipLaunchKernelGGL(k_refine_cam_rows, dim3((9 * pb.C + 255) / 256), dim3(256), 0, st, pb, U, bc, lambda, x,
                            part, dscale, out, mag, flags);
