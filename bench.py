@@ -471,9 +471,22 @@ def main():
                   "achieved": round(schur_gbs, 1), "peak": PEAK["hbm_gbs"], "unit": "GB/s",
                   "frac": round(schur_gbs / PEAK["hbm_gbs"], 4), "traffic": traffic_of("k_schur_C356_traffic.json"),
                   "basis": f"O*108 + 72P + n^2/2*4 = {schur_bytes} B per assembly over {km['schur']:.4f} ms"}
+    # one residual of the FP64 reduced camera system (k_refine.hip: rotation
+    # terms, point pass, camera chunks, rows): per observation 16 B of pixel
+    # and 8 B of indices in each of the two passes, per point X, z (written
+    # and read) and V'^-1 / bp, per camera its parameters, U, bc, x and the
+    # residual
+    res_bytes = O * (16 + 8) * 2 + P * (24 * 2 + 24 * 2 + 48 + 72) + C * (72 + 648 + 72 * 3)
+    res_gbs = res_bytes / (km["refine_residual"] * 1e-3) / 1e9
+    roof_res = {"kernel": "FP64 residual of the reduced camera system (k_refine.hip), mixed form's refinement",
+                "bound": "hbm", "achieved": round(res_gbs, 1), "peak": PEAK["hbm_gbs"], "unit": "GB/s",
+                "frac": round(res_gbs / PEAK["hbm_gbs"], 4), "traffic": None,
+                "basis": f"2 O (16 + 8) + P (48 + 48 + 120) + C (936) = {res_bytes} B per residual over "
+                         f"{km['refine_residual']:.4f} ms; the Jacobians are evaluated again in FP64 and the camera "
+                         f"pass gathers X and z at random, so the kernels run well below the HBM rate"}
     # the bench line's roofline is the kernel family with the largest share of
     # the measured mixed solve; the others follow as roofline_<name>
-    entries = {"schur": roof_schur, "cg_refinement": roof_cg, "factor": roof_factor}
+    entries = {"schur": roof_schur, "cg_refinement": roof_cg, "factor": roof_factor, "fp64_residual": roof_res}
     dom_key = dominant if dominant in entries and entries[dominant] else ("cg_refinement" if jacobi else "factor")
     roof = dict(entries[dom_key], largest_share=dominant)
 

@@ -118,3 +118,26 @@ def test_fp64_refinement_corrections_lower_the_fp64_residual(gba):
     assert one.refine_outer_steps > one.trials
     assert 0.0 < one.max_refine_residual < 0.1 * none.max_refine_residual
     assert abs(one.final_cost - none.final_cost) <= 1e-6 * none.final_cost
+
+
+@pytest.mark.parametrize("cfg,lam", [(2, 1e-4), (1, 1e-3)])
+def test_reported_fp64_residual_matches_oracle_system(gba, orc, cfg, lam):
+    """max_refine_residual is |D (rhs - S dc)| / |D rhs| of the FP64 system,
+    evaluated on the device without W.  Recomputed here from the oracle's S
+    and rhs (the reference's src/lm.cu:515-604) and D = (diag(U) + lambda
+    diag'(U))^-1/2 for the first trial's step (the step schur_step returns),
+    it matches what a one-trial solve reports."""
+    prob = gba.generate(cfg, 1)
+    g = gba.schur_step(prob, lam, gba.PRECISION_MIXED)
+    assert g["status"] == "accept"
+    lin = orc.linearize(prob)
+    o = orc.schur_step(prob, lin, lam, want_S=True)
+    S = np.tril(o["S"]) + np.tril(o["S"], -1).T
+    d = np.concatenate([np.diag(u) for u in lin["U"]])
+    D = 1.0 / np.sqrt(d + lam * np.maximum(d, 1e-12))
+    ref = np.linalg.norm(D * (o["rhs"] - S @ g["dc"])) / np.linalg.norm(D * o["rhs"])
+    r = gba.Solver(prob, gba.PRECISION_MIXED).solve(gba.LMOptions(initial_lambda=lam, max_iterations=1))
+    assert r.trials == 1 and r.iterations == 1
+    print(f"cfg {cfg}: reported FP64 residual {r.max_refine_residual:.3e}, recomputed {ref:.3e}")
+    assert ref < 1e-9
+    assert abs(r.max_refine_residual - ref) <= 0.05 * ref + 1e-13
