@@ -209,6 +209,9 @@ BatchSolver::BatchSolver(const gba_problem_view* probs, int count, bool autotune
     pb_.cam_chunk_off = cam_chunk_off_.p;
     pb_.cam_prob = cam_prob_.p;
     pb_.pt_prob = pt_prob_.p;
+    build_cam_order_copies(st_, O, cam_list_.p, spt_.p, spx_.p, cspt_, cspx_);
+    pb_.cspt = cspt_.p;
+    pb_.cspx = cspx_.p;
     cams_.alloc(9 * static_cast<size_t>(C));
     tcams_.alloc(9 * static_cast<size_t>(C));
     pts_.alloc(3 * static_cast<size_t>(P));

@@ -74,7 +74,8 @@ private:
     int num_stasks_ = 0;
 
     DevBuf<int> scam_, spt_, pt_off_, cam_list_, cam_off_, cam_chunk_off_, cam_prob_, pt_prob_;
-    DevBuf<double2> spx_;
+    DevBuf<double2> spx_, cspx_;
+    DevBuf<int> cspt_;  // camera-ordered copies for the camera passes
     DevBuf<int2> units_, spairs_;
     DevBuf<int4> chunks_;
     DevBuf<SchurTask> stasks_;

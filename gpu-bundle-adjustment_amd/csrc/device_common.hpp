@@ -59,7 +59,18 @@ struct DevProblem {
     // from the sorted layout; the diagonal Schur blocks then add the cross
     // terms of those observations)
     int repeated_obs;
+    // Camera-ordered copies (entry i for observation cam_list[i]): its point
+    // and its pixel, read contiguously by the camera passes instead of
+    // through cam_list (null: read through cam_list; cspx is null with FP16
+    // pixel storage)
+    const int* cspt;
+    const double2* cspx;
 };
+
+// Point of the observation at camera-ordered position idx.
+__device__ __forceinline__ int cam_obs_point(const DevProblem& pb, int idx) {
+    return pb.cspt ? pb.cspt[idx] : pb.spt[pb.cam_list[idx]];
+}
 
 // Huber loss on the squared residual norm s = |r|^2, as the reference's
 // Ceres harness applies HuberLoss(delta) (benchmarks/bench_ceres.cpp:120-131):
@@ -100,6 +111,11 @@ __device__ __forceinline__ double2 obs_px(const DevProblem& pb, int e, int c) {
         return make_double2(static_cast<double>(hx) * s, static_cast<double>(hy) * s);
     }
     return pb.spx[e];
+}
+
+// Pixel of the observation at camera-ordered position idx (camera c).
+__device__ __forceinline__ double2 cam_obs_px(const DevProblem& pb, int idx, int c) {
+    return pb.cspx ? pb.cspx[idx] : obs_px(pb, pb.cam_list[idx], c);
 }
 
 // The 27 values of W record e, widened to FP64, read as 16-byte vectors.

@@ -146,4 +146,24 @@ void launch_offset_obs(hipStream_t st, int O, int count, const long long* o_off,
     hipLaunchKernelGGL(k_offset_obs, dim3((O + 255) / 256), dim3(256), 0, st, O, count, o_off, c0, p0, cam, pt);
 }
 
+__global__ void k_cam_order_copies(int O, const int* __restrict__ cam_list, const int* __restrict__ spt,
+                                   const double2* __restrict__ spx, int* __restrict__ cspt,
+                                   double2* __restrict__ cspx) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= O) return;
+    const int e = cam_list[i];
+    cspt[i] = spt[e];
+    if (cspx) cspx[i] = spx[e];
+}
+
+void build_cam_order_copies(hipStream_t st, int O, const int* cam_list, const int* spt, const double2* spx,
+                            DevBuf<int>& cspt, DevBuf<double2>& cspx) {
+    cspt.alloc(std::max(1, O));
+    if (spx) cspx.alloc(std::max(1, O));
+    if (O > 0)
+        hipLaunchKernelGGL(k_cam_order_copies, dim3((O + 255) / 256), dim3(256), 0, st, O, cam_list, spt, spx, cspt.p,
+                           spx ? cspx.p : nullptr);
+    GBA_HIP(hipGetLastError());
+}
+
 }  // namespace gba_amd

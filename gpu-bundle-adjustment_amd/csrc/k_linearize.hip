@@ -146,9 +146,8 @@ __global__ __launch_bounds__(256) void k_lin_cameras(DevProblem pb, const double
 #pragma unroll
         for (int k = 0; k < 18; ++k) Jc[k] = 0.0;
         if (i < ch.z) {
-            const int ed = pb.cam_list[i];
-            const double2 o = obs_px(pb, ed, ch.x);
-            const int p = pb.spt[ed];
+            const double2 o = cam_obs_px(pb, i, ch.x);
+            const int p = cam_obs_point(pb, i);
             const bool ok = snavely_eval<true>(cam, pts + 3 * p, o.x, o.y, r, Jc, Jp);
             if (ok) robustify(pb.huber, r, Jc, nullptr);
             if (!ok) {

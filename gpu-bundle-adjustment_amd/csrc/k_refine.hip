@@ -141,9 +141,8 @@ __global__ __launch_bounds__(256) void k_refine_cam_chunks(DevProblem pb, const 
     for (int i = 0; i < 9; ++i) t[i] = 0.0;
     const int idx = ch.y + tid;
     if (idx < ch.z) {
-        const int e = pb.cam_list[idx];
-        const int p = pb.spt[e];
-        const double2 o = obs_px(pb, e, a);
+        const int p = cam_obs_point(pb, idx);
+        const double2 o = cam_obs_px(pb, idx, a);
         const double4 t4 = reinterpret_cast<const double4*>(rot)[a];
         const RotTerms rt{t4.x, t4.y, t4.z, t4.w != 0.0};
         double r[2];

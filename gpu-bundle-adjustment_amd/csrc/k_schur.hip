@@ -227,7 +227,7 @@ __global__ __launch_bounds__(256) void k_schur_diag(
     for (int i = 0; i < 9; ++i) racc[i] = 0.0;
     for (int idx = pb.cam_off[a] + tid; idx < pb.cam_off[a + 1]; idx += 256) {
         const int e = pb.cam_list[idx];
-        const int p = pb.spt[e];
+        const int p = cam_obs_point(pb, idx);
         const double* vi = Vinv6 + 6 * static_cast<size_t>(p);
         const double v00 = vi[0], v10 = vi[1], v11 = vi[2], v20 = vi[3], v21 = vi[4], v22 = vi[5];
         const double* bpp = VB + 9 * static_cast<size_t>(p) + 6;

@@ -369,6 +369,9 @@ Solver::Solver(const gba_problem_view& prob, const gba_solver_config& cfg) : cfg
     pb_.cam_off = cam_off_.p;
     pb_.chunks = chunks_.p;
     pb_.cam_chunk_off = cam_chunk_off_.p;
+    build_cam_order_copies(st_, O, cam_list_.p, spt_.p, spx_.p, cspt_, cspx_);
+    pb_.cspt = cspt_.p;
+    pb_.cspx = spx_.p ? cspx_.p : nullptr;
     for (int i = 0; i < 2; ++i) {
         cams_[i].alloc(9 * static_cast<size_t>(C));
         pts_[i].alloc(3 * static_cast<size_t>(P));

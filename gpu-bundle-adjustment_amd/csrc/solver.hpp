@@ -49,6 +49,10 @@ void layout_chunks(const std::vector<int>& cam_off, std::vector<int4>& chunks, s
 // positions by camera, ascending within a camera), from the caller's arrays
 // already on the device; *repeated = 1 when some point is observed more than
 // once by one camera.  Allocates the outputs; synchronises st.
+// Camera-ordered copies of the observations' points and pixels (cspt[i] =
+// spt[cam_list[i]], likewise cspx; spx NULL: no pixel copy), enqueued on st.
+void build_cam_order_copies(hipStream_t st, int O, const int* cam_list, const int* spt, const double2* spx,
+                            DevBuf<int>& cspt, DevBuf<double2>& cspx);
 void build_layout_device(hipStream_t st, int C, int P, int O, const int* d_cam, const int* d_pt,
                          const double* d_px, DevBuf<int>& perm, DevBuf<int>& scam, DevBuf<int>& spt,
                          DevBuf<double2>* spx, DevBuf<int>& pt_off, DevBuf<int>& cam_off, DevBuf<int>& cam_list,
@@ -219,6 +223,8 @@ private:
     bool mixed_;
     bool sep_diag_ = true;  // diagonal Schur blocks + rhs from launch_schur_diag
     DevProblem pb_{};
+    DevBuf<int> cspt_;       // camera-ordered copies for the camera passes
+    DevBuf<double2> cspx_;
     int nblk_ = 0;
     hipStream_t st_ = nullptr;
     std::vector<double> init_cams_, init_pts_;
