@@ -60,7 +60,7 @@ def test_cfg3_mixed_reaches_fp64_cost(gba, cfg3):
     block-Jacobi CG; the FP32 factor is the other) reach the FP64 final cost
     within the 1e-6 target, with the same trial count.  The FP64 system's
     residual after the one correction per trial: about 1e-11 in the first
-    trials, up to about 1e-6 in the last ones (lambda near 4e-7, where the FP32
+    trials, up to a few times 1e-6 in the last ones (lambda near 4e-7, where the FP32
     system's error along the least determined directions is of the order of
     the damping: measured, and reported by the solve)."""
     opts = gba.LMOptions(initial_lambda=1e-4)
