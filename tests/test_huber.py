@@ -136,3 +136,30 @@ def test_gpu_robust_mixed_reaches_fp64_cost(gba):
     r32 = gba.solve_bal_gpu(p32, opts, gba.PRECISION_MIXED, huber_delta=d)
     assert r32.termination == r64.termination
     assert abs(r32.final_cost - r64.final_cost) <= 1e-6 * r64.final_cost
+
+
+@pytest.mark.gpu
+def test_gpu_robust_mixed_refinement_residual_matches_oracle(gba, orc):
+    """The mixed form's FP64 refinement with the Huber loss: its residual uses
+    the linearisation's robust weights (the square of the IRLS weight on
+    W = Jc^T Jp).  The first trial's step (the change of the cameras after a
+    one-iteration solve) against the oracle's robust S and rhs (src/lm.cu's
+    system with bench_ceres.cpp's HuberLoss) recomputed on the host: the
+    residual the solve reports, to the first digits.  A wrong weight would
+    make the refinement's residual that of another operator."""
+    d, lam = 3.0, 1e-4
+    p = with_outliers(gba, 2)
+    s = gba.Solver(p, gba.PRECISION_MIXED, huber_delta=d)
+    r = s.solve(gba.LMOptions(initial_lambda=lam, max_iterations=1))
+    assert r.trials == 1 and r.iterations == 1
+    cams, _ = s.params()
+    dc = (cams - p.cameras).reshape(-1)
+    lin = orc.linearize(p, huber_delta=d)
+    o = orc.schur_step(p, lin, lam, want_S=True)
+    S = np.tril(o["S"]) + np.tril(o["S"], -1).T
+    dd = np.concatenate([np.diag(u) for u in lin["U"]])
+    D = 1.0 / np.sqrt(dd + lam * np.maximum(dd, 1e-12))
+    ref = np.linalg.norm(D * (o["rhs"] - S @ dc)) / np.linalg.norm(D * o["rhs"])
+    print(f"Huber: reported FP64 residual {r.max_refine_residual:.3e}, recomputed {ref:.3e}")
+    assert ref < 1e-8
+    assert abs(r.max_refine_residual - ref) <= 0.05 * ref + 1e-12
