@@ -92,9 +92,11 @@ typedef struct gba_lm_result {
 /* Arithmetic of the reduced camera solve. */
 enum gba_precision {
     GBA_PRECISION_FP64 = 0,  /* plain drop-in: FP64 throughout */
-    GBA_PRECISION_MIXED = 1  /* FP32 W and S; FP64 PCG on the FP32-stored S, preconditioned
-                                by its FP32 MFMA factor or by its camera blocks (enum
-                                gba_mixed_precond); FP64 cost, gain ratio,
+    GBA_PRECISION_MIXED = 1  /* FP32 W and S; the FP32-stored S solved in FP64 by CG
+                                preconditioned by its FP32 MFMA factor or by its camera
+                                blocks (enum gba_mixed_precond), then FP64 iterative
+                                refinement against the FP64 system's residual
+                                (refine_corrections); FP64 cost, gain ratio,
                                 back-substitution */
 };
 
