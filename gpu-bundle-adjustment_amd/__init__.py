@@ -58,7 +58,7 @@ class SolverConfigC(ctypes.Structure):
                 ("refine_max_iters", ctypes.c_int), ("trace", _dp), ("trace_capacity", ctypes.c_int),
                 ("obs_storage", ctypes.c_int), ("huber_delta", ctypes.c_double), ("factor_layout", ctypes.c_int),
                 ("mixed_precond", ctypes.c_int), ("jacobi_max_iters", ctypes.c_int),
-                ("refine_corrections", ctypes.c_int)]
+                ("refine_corrections", ctypes.c_int), ("cg_deflation", ctypes.c_int)]
 
 
 class SynthSpecC(ctypes.Structure):
@@ -248,10 +248,13 @@ class LMResult:
 
 
 def _config(precision, trace_capacity=0, refine_tol=1e-10, refine_max_iters=12, obs_storage=OBS_FP64,
-            huber_delta=0.0, layout=None, precond=None, jacobi_max_iters=None, refine_corrections=None):
+            huber_delta=0.0, layout=None, precond=None, jacobi_max_iters=None, refine_corrections=None,
+            deflation=None):
     c = SolverConfigC()
     lib().gba_default_config(ctypes.byref(c))
     c.precision = precision
+    if deflation is not None:
+        c.cg_deflation = 0 if deflation else -1
     if refine_corrections is not None:
         c.refine_corrections = refine_corrections
     if precond is not None:
@@ -299,10 +302,12 @@ class Solver:
     """Device-resident solver handle: upload once, solve repeatedly."""
 
     def __init__(self, prob, precision=PRECISION_FP64, trace_capacity=1024, obs_storage=OBS_FP64,
-                 huber_delta=0.0, layout=None, precond=None, jacobi_max_iters=None, refine_corrections=None):
+                 huber_delta=0.0, layout=None, precond=None, jacobi_max_iters=None, refine_corrections=None,
+                 deflation=None):
         self._cfg, self._trace = _config(precision, trace_capacity, obs_storage=obs_storage,
                                          huber_delta=huber_delta, layout=layout, precond=precond,
-                                         jacobi_max_iters=jacobi_max_iters, refine_corrections=refine_corrections)
+                                         jacobi_max_iters=jacobi_max_iters, refine_corrections=refine_corrections,
+                                         deflation=deflation)
         self._h = ctypes.c_void_p()
         self._prob = prob
         v = prob.view()

@@ -185,7 +185,13 @@ void launch_precond_apply(hipStream_t st, const float* X32, int ld, int npad, co
 // row, wgtab[w] = (tile row i, run c), listed row by row; rowstart[i] is the
 // index of tile row i's first run (nblk + 1 entries).  Work arrays: rp
 // (nwg_m x 64), cp (nblk (nblk + 1) / 2 x 64), dpart (nwg_m), gpart
-// (4 nwg_v), st (16).
+// (kCgPartials nwg_v x 2), st (16).
+// Deflation by the similarity gauge (zd non-null): zd and azd hold the
+// normalised gauge basis Z of the scaled system and A Z (npad x 8, the eighth
+// column zero), einv the 7x7 (Z^T A Z)^-1 (all zero when the basis is
+// unusable, which turns the deflation off), zbpart the start's partial sums
+// of Z^T b and b.b (nwg_v x 8).
+constexpr int kCgPartials = 10;  // per vector workgroup: gamma, r.r, (A Z)^T u
 struct CgArgs {
     const float* A;
     int ld, npad, nblk, ncam, tpw;
@@ -196,16 +202,31 @@ struct CgArgs {
     double *x, *r, *u, *p, *s;
     const double* minv;  // ncam x 81: inverse of each camera block
     double *rp, *cp, *dpart, *gpart, *st;
+    const double *zd, *azd, *einv;
+    double* zbpart;
     PassStatus* status;
     const TrialFlags* flags;
     double tol2;
     int max_it;
 };
+// Work of the deflation set-up: unnormalised basis (npad x 8), partial
+// column norms (nwg_v x 8), partials of A Z per run of tiles
+// (nblk x cg_gauge_runs(nblk) x 64 x 8) and of Z^T A Z (nblk x 49).
+struct CgDeflWork {
+    double *zraw, *npart, *apart, *epart;
+    double *zd, *azd, *einv;
+};
 int cg_vector_groups(int npad);
+int cg_gauge_runs(int nblk);
 // minv[c] = A_cc^-1 (FP64 Cholesky of the FP32 block); a block that is not
 // SPD raises chol_fail.
 void launch_cg_block_inverse(hipStream_t st, const float* A, int ld, int ncam, double* minv, TrialFlags* flags);
-// x = 0, r = b, u = M^-1 r (clears the done state).
+// Once per trial, after the Schur assembly: the gauge basis of the scaled
+// system at cameras cams (internal order), A Z and (Z^T A Z)^-1.
+void launch_cg_deflation_setup(hipStream_t st, const CgArgs& g, const double* cams, const double* dscale,
+                               const CgDeflWork& w);
+// x = 0, r = b, u = M^-1 r (clears the done state); with the deflation
+// x = Z (Z^T A Z)^-1 Z^T b, r = b - A x.
 void launch_cg_start(hipStream_t st, const CgArgs& g);
 // Step k (0, 1, ... in order): the matrix product and the vector update;
 // both do nothing once the iteration has stopped.

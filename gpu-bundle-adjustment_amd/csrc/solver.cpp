@@ -577,7 +577,7 @@ void Solver::setup_cg() {
     cg_rp_.alloc(static_cast<size_t>(nwg_m) * kTile);
     cg_cp_.alloc(static_cast<size_t>(nblk) * (nblk + 1) / 2 * kTile);
     cg_dpart_.alloc(static_cast<size_t>(nwg_m));
-    cg_gpart_.alloc(4 * static_cast<size_t>(nwg_v));
+    cg_gpart_.alloc(2 * kCgPartials * static_cast<size_t>(nwg_v));
     cg_st_.alloc(16);
     launch_fill_double(st_, cg_st_.p, cg_st_.n, 0.0);
     launch_fill_double(st_, cg_gpart_.p, cg_gpart_.n, 0.0);
@@ -606,6 +606,30 @@ void Solver::setup_cg() {
     g.st = cg_st_.p;
     g.status = status_.p;
     g.flags = flags_.p;
+    // deflation by the similarity gauge (on unless the configuration turns it
+    // off, or GBA_CG_DEFLATE=0 in the environment)
+    bool defl = cfg_.cg_deflation >= 0;
+    if (const char* env = std::getenv("GBA_CG_DEFLATE")) defl = std::atoi(env) != 0;
+    g.zd = g.azd = g.einv = nullptr;
+    g.zbpart = nullptr;
+    if (defl) {
+        const size_t nv = static_cast<size_t>(nwg_v), nb = static_cast<size_t>(nblk);
+        const size_t na = nb * static_cast<size_t>(cg_gauge_runs(nblk)) * kTile * 8;
+        cg_defl_.alloc(3 * 8 * np + 2 * 8 * nv + 49 * nb + 64 + na);
+        double* q = cg_defl_.p;
+        cg_dw_.zraw = q;
+        cg_dw_.zd = q + 8 * np;
+        cg_dw_.azd = q + 16 * np;
+        cg_dw_.npart = q + 24 * np;
+        g.zbpart = q + 24 * np + 8 * nv;
+        cg_dw_.epart = q + 24 * np + 16 * nv;
+        cg_dw_.einv = cg_dw_.epart + 49 * nb;
+        cg_dw_.apart = cg_dw_.einv + 64;
+        launch_fill_double(st_, cg_defl_.p, cg_defl_.n, 0.0);
+        g.zd = cg_dw_.zd;
+        g.azd = cg_dw_.azd;
+        g.einv = cg_dw_.einv;
+    }
 }
 
 // Every device pointer of the block-Jacobi kernels' arguments must be set
@@ -615,6 +639,9 @@ static void check_cg_args(const CgArgs& g) {
                           g.rp, g.cp, g.dpart, g.gpart, g.st, g.status, g.flags};
     for (const void* q : ptrs)
         if (!q) throw Error("block-Jacobi refinement: unset device buffer");
+    // the deflation's buffers are all set or all null
+    const bool any = g.zd || g.azd || g.einv || g.zbpart, all = g.zd && g.azd && g.einv && g.zbpart;
+    if (any && !all) throw Error("block-Jacobi refinement: partly set deflation buffers");
     if (g.nwg_m <= 0 || g.nwg_v <= 0 || g.tpw <= 0 || g.nblk * kTile != g.npad || 9 * g.ncam > g.npad)
         throw Error("block-Jacobi refinement: inconsistent shape");
 }
@@ -734,6 +761,7 @@ void Solver::mixed_solve(double lambda, TrialOut& out) {
         enqueue_factor();
     } else {
         launch_cg_block_inverse(st_, S32low_.p, pb_.npad, pb_.C, cg_minv_.p, flags_.p);
+        if (cg_.zd) launch_cg_deflation_setup(st_, cg_, cams_[cur_].p, dscale_.p, cg_dw_);
     }
     // inner solve of A d = b on the FP32-stored system to tolerance t
     // relative to |b|; false when the block-Jacobi CG missed it within its cap
@@ -1008,6 +1036,7 @@ double Solver::time_kernel(int kernel, double lambda, int reps) {
                 break;
             case GBA_KERNEL_CG_SETUP:
                 launch_cg_block_inverse(st_, S32low_.p, pb_.npad, pb_.C, cg_minv_.p, flags_.p);
+                if (cg_.zd) launch_cg_deflation_setup(st_, cg_, cams_[cur_].p, dscale_.p, cg_dw_);
                 break;
             case GBA_KERNEL_CG_STEP:
                 launch_cg_step(st_, cgt, cg_k++);
@@ -1035,6 +1064,7 @@ double Solver::time_kernel(int kernel, double lambda, int reps) {
         cgt.tol2 = 0.0;
         cgt.max_it = INT_MAX;
         launch_cg_block_inverse(st_, S32low_.p, pb_.npad, pb_.C, cg_minv_.p, flags_.p);
+        if (cgt.zd) launch_cg_deflation_setup(st_, cgt, cams_[cur_].p, dscale_.p, cg_dw_);
         launch_cg_start(st_, cgt);
     }
     if (kernel == GBA_KERNEL_FACTOR) {

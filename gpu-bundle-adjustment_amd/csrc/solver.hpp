@@ -267,6 +267,9 @@ private:
     DevBuf<int2> cg_wgtab_;
     DevBuf<int> cg_rowstart_;
     DevBuf<double> cg_minv_, cg_vec_, cg_rp_, cg_cp_, cg_dpart_, cg_gpart_, cg_st_;
+    // gauge deflation of the block-Jacobi CG (cg_.zd set when on)
+    DevBuf<double> cg_defl_;
+    CgDeflWork cg_dw_{};
     // mixed: precondition by the factor's triangular solves instead of the
     // explicit inverse (large matrices; see kInversePrecondBlk)
     bool precond_trsv_ = false;

@@ -135,6 +135,10 @@ typedef struct gba_solver_config {
                                from the Jacobians) on the FP32-stored one; 0: default (1);
                                negative: none, the FP64 residual is still evaluated once and
                                reported */
+    int cg_deflation;       /* mixed, block-Jacobi refinement: 0 (default) deflates the CG
+                               iteration by the seven directions of the similarity gauge
+                               (the near-null space of the damped S); negative: plain
+                               block-Jacobi CG */
 } gba_solver_config;
 
 /* Preconditioner of the mixed form's FP64 refinement (no reference
