@@ -238,6 +238,15 @@ __device__ __forceinline__ double wave_sum(double v) {
     for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, kWave);
     return v;
 }
+// N independent wavefront sums, one butterfly level for all of them at a
+// time so that their cross-lane exchanges overlap.
+template <int N>
+__device__ __forceinline__ void wave_sum_n(double* v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1)
+#pragma unroll
+        for (int i = 0; i < N; ++i) v[i] += __shfl_xor(v[i], d, kWave);
+}
 __device__ __forceinline__ unsigned wave_sum_u(unsigned v) {
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, kWave);

@@ -386,17 +386,21 @@ __global__ __launch_bounds__(256) void k_cg_vector(CgArgs g, int k) {
                 for (int j = 0; j < kGauge; ++j) gz[j] += rec[kPgZ + j];
             }
         }
-        ga = wave_sum(ga);
-        const double da = wave_sum(strided_partial(g.dpart, g.nwg_m, 1, tid, 256));
-        if (lane == 0) {
-            s_g[w] = ga;
-            s_d[w] = da;
-        }
-        if (defl) {
+        double red[2 + kGauge];
+        red[0] = ga;
+        red[1] = strided_partial(g.dpart, g.nwg_m, 1, tid, 256);
 #pragma unroll
-            for (int j = 0; j < kGauge; ++j) {
-                const double t = wave_sum(gz[j]);
-                if (lane == 0) s_z[w][j] = t;
+        for (int j = 0; j < kGauge; ++j) red[2 + j] = gz[j];
+        if (defl)
+            wave_sum_n<2 + kGauge>(red);
+        else
+            wave_sum_n<2>(red);
+        if (lane == 0) {
+            s_g[w] = red[0];
+            s_d[w] = red[1];
+            if (defl) {
+#pragma unroll
+                for (int j = 0; j < kGauge; ++j) s_z[w][j] = red[2 + j];
             }
         }
         __syncthreads();
@@ -495,17 +499,23 @@ __global__ __launch_bounds__(256) void k_cg_vector(CgArgs g, int k) {
         }
         g.u[row] = u;
     }
-    const double gsum = wave_sum(r * u), rsum = wave_sum(r * r);
+    // gamma, r.r and g = (A Z)^T u partials of this workgroup
+    double red[2 + kGauge];
+    red[kPgGam] = r * u;
+    red[kPgRR] = r * r;
+#pragma unroll
+    for (int j = 0; j < kGauge; ++j) red[kPgZ + j] = azr[j] * u;
+    if (defl)
+        wave_sum_n<2 + kGauge>(red);
+    else
+        wave_sum_n<2>(red);
     double* gp = g.gpart + kCgPartials * (static_cast<size_t>(g.nwg_v) * ((k + 1) & 1) + blockIdx.x);
     if (lane == 0) {
-        gp[kPgGam] = gsum;
-        gp[kPgRR] = rsum;
-    }
-    if (defl) {
+        gp[kPgGam] = red[kPgGam];
+        gp[kPgRR] = red[kPgRR];
+        if (defl) {
 #pragma unroll
-        for (int j = 0; j < kGauge; ++j) {
-            const double t = wave_sum(azr[j] * u);
-            if (lane == 0) gp[kPgZ + j] = t;
+            for (int j = 0; j < kGauge; ++j) gp[kPgZ + j] = red[kPgZ + j];
         }
     }
 }
@@ -576,31 +586,19 @@ __global__ __launch_bounds__(64) void k_cg_gauge(int npad, int ncam, const doubl
     }
 }
 
-// Norm of basis column j from the gauge kernel's partials: every workgroup
-// sums them in the same order.
-__device__ __forceinline__ double gauge_col_scale(const double* __restrict__ npart, int nwg_v, int j) {
-    double sum = 0.0;
-    if (j < kGauge)
-        for (int q = 0; q < nwg_v; ++q) sum += npart[8 * static_cast<size_t>(q) + j];
-    return sum > 0.0 ? 1.0 / sqrt(sum) : 0.0;
-}
-
-// A Z, partial over a run of kGaugeRun tiles: workgroup (run c, tile row I)
-// takes tiles J = kGaugeRun c ... of row I (A_IJ = A_JI^T above the
-// diagonal, the diagonal tile's lower triangle mirrored), the basis rows of
-// tile J staged in LDS with the column normalisation applied; the next
-// tile and basis rows are loaded while the current ones are used.
+// A Z (unnormalised basis), partial over a run of kGaugeRun tiles:
+// workgroup (run c, tile row I) takes tiles J = kGaugeRun c ... of row I
+// (A_IJ = A_JI^T above the diagonal, the diagonal tile's lower triangle
+// mirrored), the basis rows of tile J staged in LDS; the next tile and basis
+// rows are loaded while the current ones are used.
 constexpr int kGaugeRun = 4;
-__global__ __launch_bounds__(256) void k_cg_gauge_product(const float* __restrict__ A, int ld, int nblk, int nwg_v,
+__global__ __launch_bounds__(256) void k_cg_gauge_product(const float* __restrict__ A, int ld, int nblk,
                                                           const double* __restrict__ zraw,
-                                                          const double* __restrict__ npart,
                                                           double* __restrict__ apart) {
     __shared__ float s_a[kTile][kTile + 1];
     __shared__ double s_zj[kTile][8];
-    __shared__ double s_sc[8];
     const int tid = threadIdx.x, I = blockIdx.y, run = blockIdx.x;
     const int J0 = kGaugeRun * run, J1 = min(J0 + kGaugeRun, nblk);
-    if (tid < 8) s_sc[tid] = gauge_col_scale(npart, nwg_v, tid);
     auto tile = [&](int J) {
         return J <= I ? A + static_cast<size_t>(I * kTile) * ld + J * kTile
                       : A + static_cast<size_t>(J * kTile) * ld + I * kTile;
@@ -622,7 +620,7 @@ __global__ __launch_bounds__(256) void k_cg_gauge_product(const float* __restric
 #pragma unroll
     for (int j = 0; j < kGauge; ++j) acc[j] = 0.0;
     for (int J = J0; J < J1; ++J) {
-        __syncthreads();  // the previous tile is consumed (and s_sc is visible)
+        __syncthreads();  // the previous tile is consumed
 #pragma unroll
         for (int q = 0; q < 16; ++q) {
             const int e = tid + 256 * q;
@@ -631,7 +629,7 @@ __global__ __launch_bounds__(256) void k_cg_gauge_product(const float* __restric
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             const int t = tid + 256 * h;
-            s_zj[t >> 3][t & 7] = zt[h] * s_sc[t & 7];
+            s_zj[t >> 3][t & 7] = zt[h];
         }
         __syncthreads();
         if (J + 1 < J1) {
@@ -671,14 +669,30 @@ __global__ __launch_bounds__(256) void k_cg_gauge_product(const float* __restric
     }
 }
 
-// Tile row I: A Z from the runs' partials (in run order), the normalised
-// basis rows, and the row's share of Z^T A Z.
+// Tile row I: the column normalisation (every workgroup sums the norm
+// partials in the same order), A Z from the runs' partials (in run order),
+// both scaled, and the row's share of Z^T A Z.
 __global__ __launch_bounds__(256) void k_cg_gauge_rows(int nruns, int nwg_v, const double* __restrict__ zraw,
                                                        const double* __restrict__ npart,
                                                        const double* __restrict__ apart, double* __restrict__ zd,
                                                        double* __restrict__ azd, double* __restrict__ epart) {
     __shared__ double s_zi[kTile][8], s_az[kTile][8];
+    __shared__ double s_np[32][8];
+    __shared__ double s_sc[8];
     const int tid = threadIdx.x, I = blockIdx.x;
+    {
+        const int j = tid & 7, part = tid >> 3;
+        double sum = 0.0;
+        for (int q = part; q < nwg_v; q += 32) sum += npart[8 * static_cast<size_t>(q) + j];
+        s_np[part][j] = sum;
+    }
+    __syncthreads();
+    if (tid < 8) {
+        double sum = 0.0;
+        for (int part = 0; part < 32; ++part) sum += s_np[part][tid];
+        s_sc[tid] = (tid < kGauge && sum > 0.0) ? 1.0 / sqrt(sum) : 0.0;
+    }
+    __syncthreads();
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
         const int t = tid + 256 * h, rr = t >> 3, j = t & 7;
@@ -691,8 +705,8 @@ __global__ __launch_bounds__(256) void k_cg_gauge_rows(int nruns, int nwg_v, con
             for (int u = 0; u < 4; ++u) az += x[u];
         }
         const size_t row = static_cast<size_t>(I) * kTile + rr;
-        const double zv = j < kGauge ? zraw[8 * row + j] * gauge_col_scale(npart, nwg_v, j) : 0.0;
-        if (j >= kGauge) az = 0.0;
+        const double zv = zraw[8 * row + j] * s_sc[j];
+        az *= s_sc[j];
         zd[8 * row + j] = zv;
         azd[8 * row + j] = az;
         s_zi[rr][j] = zv;
@@ -715,11 +729,7 @@ __global__ __launch_bounds__(64) void k_cg_gauge_finish(int nblk, const double* 
                                                         double* __restrict__ einv) {
     __shared__ double s_e[kGauge * kGauge];
     const int tid = threadIdx.x;
-    if (tid < kGauge * kGauge) {
-        double sum = 0.0;
-        for (int I = 0; I < nblk; ++I) sum += epart[static_cast<size_t>(I) * (kGauge * kGauge) + tid];
-        s_e[tid] = sum;
-    }
+    if (tid < kGauge * kGauge) s_e[tid] = strided_partial(epart + tid, nblk, kGauge * kGauge, 0, 1);
     __syncthreads();
     if (tid != 0) return;
     double l[kGauge][kGauge], m[kGauge][kGauge];
@@ -781,8 +791,8 @@ void launch_cg_deflation_setup(hipStream_t st, const CgArgs& g, const double* ca
                                const CgDeflWork& w) {
     hipLaunchKernelGGL(k_cg_gauge, dim3(g.nwg_v), dim3(64), 0, st, g.npad, g.ncam, cams, dscale, w.zraw, w.npart);
     const int nruns = cg_gauge_runs(g.nblk);
-    hipLaunchKernelGGL(k_cg_gauge_product, dim3(nruns, g.nblk), dim3(256), 0, st, g.A, g.ld, g.nblk, g.nwg_v,
-                       w.zraw, w.npart, w.apart);
+    hipLaunchKernelGGL(k_cg_gauge_product, dim3(nruns, g.nblk), dim3(256), 0, st, g.A, g.ld, g.nblk, w.zraw,
+                       w.apart);
     hipLaunchKernelGGL(k_cg_gauge_rows, dim3(g.nblk), dim3(256), 0, st, nruns, g.nwg_v, w.zraw, w.npart, w.apart,
                        w.zd, w.azd, w.epart);
     hipLaunchKernelGGL(k_cg_gauge_finish, dim3(1), dim3(64), 0, st, g.nblk, w.epart, w.einv);
