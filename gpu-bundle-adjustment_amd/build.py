@@ -23,6 +23,7 @@ SOURCES = [
     "k_dense.hip",
     "k_precond.hip",
     "k_cg.hip",
+    "k_heavy.hip",
     "k_refine.hip",
     "k_backsub.hip",
     "k_batch.hip",

@@ -65,6 +65,10 @@ struct DevProblem {
     // pixel storage)
     const int* cspt;
     const double2* cspx;
+    // Points with at least heavy_k observations (0: none) are left out of the
+    // pair and diagonal-block sums; their share of S is one dense product
+    // (k_heavy.hip)
+    int heavy_k;
 };
 
 // Point of the observation at camera-ordered position idx.

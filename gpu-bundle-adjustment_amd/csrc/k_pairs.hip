@@ -7,7 +7,9 @@
 // in [lo, hi); its slots qoff[t] .. qoff[t] + n_a - 1 are camera a's
 // observations in index order (slot j <-> cam_list[cam_off[a] + j]), so the
 // pairs of a task come grouped by e (ascending point) and ordered by f.  Two passes over the same loops, one thread
-// per slot: count per slot, exclusive scan (write positions), fill.
+// per slot: count per slot, exclusive scan (write positions), fill.  Points
+// with at least heavy_k observations (heavy_k > 0) have no pairs: their share
+// of S is formed densely (k_heavy.hip).
 #include <hipcub/hipcub.hpp>
 
 #include "device_common.hpp"
@@ -26,7 +28,7 @@ __global__ __launch_bounds__(256) void k_pairs(int ntask, const int4* __restrict
                                                const long long* __restrict__ qoff, long long nslots,
                                                const int* __restrict__ cam_off, const int* __restrict__ cam_list,
                                                const int* __restrict__ spt, const int* __restrict__ pt_off,
-                                               const int* __restrict__ scam, long long* __restrict__ cnt,
+                                               const int* __restrict__ scam, int heavy_k, long long* __restrict__ cnt,
                                                const long long* __restrict__ pos, int2* __restrict__ pairs) {
     const long long q = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
     if (!kFill && q == nslots) cnt[nslots] = 0;
@@ -44,8 +46,9 @@ __global__ __launch_bounds__(256) void k_pairs(int ntask, const int4* __restrict
     const int p = spt[e];
     const int f1 = pt_off[p + 1];
     long long w = kFill ? pos[q] : 0;
+    const int f0 = (heavy_k > 0 && f1 - pt_off[p] >= heavy_k) ? f1 : pt_off[p];
     // the point's observations are ordered by camera
-    for (int f = pt_off[p]; f < f1; ++f) {
+    for (int f = f0; f < f1; ++f) {
         const int b = scam[f];
         if (b >= d.z) break;
         if (b < d.y) continue;
@@ -72,11 +75,12 @@ size_t pair_scan_work(long long n) {
 
 void launch_pair_count(hipStream_t st, int ntask, const int4* tq, const long long* qoff, long long nslots,
                        const int* cam_off, const int* cam_list, const int* spt, const int* pt_off,
-                       const int* scam, long long* cnt) {
+                       const int* scam, int heavy_k, long long* cnt) {
     if (ntask <= 0) return;
     const unsigned blocks = static_cast<unsigned>((nslots + 1 + 255) / 256);
     hipLaunchKernelGGL(k_pairs<false>, dim3(blocks), dim3(256), 0, st, ntask, tq, qoff, nslots, cam_off, cam_list,
-                       spt, pt_off, scam, cnt, static_cast<const long long*>(nullptr), static_cast<int2*>(nullptr));
+                       spt, pt_off, scam, heavy_k, cnt, static_cast<const long long*>(nullptr),
+                       static_cast<int2*>(nullptr));
 }
 
 void launch_pair_scan(hipStream_t st, const long long* cnt, long long* pos, long long n, void* work,
@@ -92,11 +96,11 @@ void launch_pair_bounds(hipStream_t st, int ntask, const long long* qoff, const 
 
 void launch_pair_fill(hipStream_t st, int ntask, const int4* tq, const long long* qoff, long long nslots,
                       const int* cam_off, const int* cam_list, const int* spt, const int* pt_off, const int* scam,
-                      const long long* pos, int2* pairs) {
+                      int heavy_k, const long long* pos, int2* pairs) {
     if (ntask <= 0 || nslots <= 0) return;
     const unsigned blocks = static_cast<unsigned>((nslots + 255) / 256);
     hipLaunchKernelGGL(k_pairs<true>, dim3(blocks), dim3(256), 0, st, ntask, tq, qoff, nslots, cam_off, cam_list,
-                       spt, pt_off, scam, static_cast<long long*>(nullptr), pos, pairs);
+                       spt, pt_off, scam, heavy_k, static_cast<long long*>(nullptr), pos, pairs);
 }
 
 }  // namespace gba_amd

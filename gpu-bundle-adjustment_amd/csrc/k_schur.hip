@@ -236,23 +236,28 @@ __global__ __launch_bounds__(256) void k_schur_diag(
         const double vb2 = v20 * bpp[0] + v21 * bpp[1] + v22 * bpp[2];
         double wv[27];
         load_w(W, static_cast<size_t>(e), wv);
+        // a heavy point's S terms come from the dense product (k_heavy.hip)
+        const bool heavy = pb.heavy_k > 0 && pb.pt_off[p + 1] - pb.pt_off[p] >= pb.heavy_k;
+        if (!heavy) {
 #pragma unroll
-        for (int i = 0; i < 9; ++i) {
-            const double t0 = wv[3 * i] * v00 + wv[3 * i + 1] * v10 + wv[3 * i + 2] * v20;
-            const double t1 = wv[3 * i] * v10 + wv[3 * i + 1] * v11 + wv[3 * i + 2] * v21;
-            const double t2 = wv[3 * i] * v20 + wv[3 * i + 1] * v21 + wv[3 * i + 2] * v22;
+            for (int i = 0; i < 9; ++i) {
+                const double t0 = wv[3 * i] * v00 + wv[3 * i + 1] * v10 + wv[3 * i + 2] * v20;
+                const double t1 = wv[3 * i] * v10 + wv[3 * i + 1] * v11 + wv[3 * i + 2] * v21;
+                const double t2 = wv[3 * i] * v20 + wv[3 * i + 1] * v21 + wv[3 * i + 2] * v22;
 #pragma unroll
-            for (int j = 0; j <= i; ++j)
-                acc[i * (i + 1) / 2 + j] += t0 * wv[3 * j] + t1 * wv[3 * j + 1] + t2 * wv[3 * j + 2];
-            racc[i] += wv[3 * i] * vb0 + wv[3 * i + 1] * vb1 + wv[3 * i + 2] * vb2;
+                for (int j = 0; j <= i; ++j)
+                    acc[i * (i + 1) / 2 + j] += t0 * wv[3 * j] + t1 * wv[3 * j + 1] + t2 * wv[3 * j + 2];
+            }
         }
+#pragma unroll
+        for (int i = 0; i < 9; ++i) racc[i] += wv[3 * i] * vb0 + wv[3 * i + 1] * vb1 + wv[3 * i + 2] * vb2;
         // a point observed more than once by camera a: the pairs e != f of
         // that run (adjacent in the point's camera-ordered list) also land
         // on S_aa, W_e V'^-1 W_f^T (src/lm.cu:515-559 takes every f with
         // camera(f) <= camera(e)).  Rare, so it stays out of the registers
         // of the common path: rows of W_e and W_f read from memory, sums
         // into s_dup with LDS atomics.
-        if (!pb.repeated_obs) continue;
+        if (!pb.repeated_obs || heavy) continue;
         const int k0 = pb.pt_off[p], k1 = pb.pt_off[p + 1];
         if ((e > k0 && pb.scam[e - 1] == a) || (e + 1 < k1 && pb.scam[e + 1] == a)) {
             int f = e;

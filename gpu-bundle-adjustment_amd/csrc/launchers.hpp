@@ -77,14 +77,25 @@ void launch_diag_scale(hipStream_t st, int n, int npad, const double* U, double 
 size_t pair_scan_work(long long n);
 void launch_pair_count(hipStream_t st, int ntask, const int4* tq, const long long* qoff, long long nslots,
                        const int* cam_off, const int* cam_list, const int* spt, const int* pt_off,
-                       const int* scam, long long* cnt);
+                       const int* scam, int heavy_k, long long* cnt);
 void launch_pair_scan(hipStream_t st, const long long* cnt, long long* pos, long long n, void* work,
                       size_t work_bytes);
 void launch_pair_bounds(hipStream_t st, int ntask, const long long* qoff, const long long* pos,
                         long long* bounds);
 void launch_pair_fill(hipStream_t st, int ntask, const int4* tq, const long long* qoff, long long nslots,
                       const int* cam_off, const int* cam_list, const int* spt, const int* pt_off, const int* scam,
-                      const long long* pos, int2* pairs);
+                      int heavy_k, const long long* pos, int2* pairs);
+
+// k_heavy.hip -- the heavy points' share of S (pb.heavy_k): Yt (k-major,
+// kp x ld, zero outside the observed blocks) receives D W_{a,p} L_p for heavy
+// point hpts[h] in columns 3h .. 3h + 2 (FP32 with W32 and Yt32, else FP64);
+// then S -= Y Y^T over the lower tiles (FP32 into S32 and, when given, the
+// second copy S32b; FP64 into S64).
+void launch_heavy_y(hipStream_t st, const DevProblem& pb, const int* hpts, int nh, const double* W64,
+                    const float* W32, const double* Vinv6, const double* dscale, double* Yt64, float* Yt32,
+                    int ld);
+void launch_heavy_syrk(hipStream_t st, int nblk, int kp, int ld, const double* Yt64, const float* Yt32, double* S64,
+                       float* S32, float* S32b, const TrialFlags* flags);
 
 // k_layout.hip: a batch's concatenated observations from problem-local to
 // batch camera / point indices (problem q owns observations [o_off[q],

@@ -43,7 +43,7 @@ void upload(DevBuf<T>& b, const std::vector<T>& v, hipStream_t st) {
 
 void schur_pairs_device(hipStream_t st, const std::vector<int4>& tq, const std::vector<int>& cam_off,
                         const int* d_cam_off, const int* d_cam_list, const int* d_spt, const int* d_pt_off,
-                        const int* d_scam, std::vector<long long>& bounds, DevBuf<int2>& pairs) {
+                        const int* d_scam, std::vector<long long>& bounds, DevBuf<int2>& pairs, int heavy_k) {
     const int ntask = static_cast<int>(tq.size());
     bounds.assign(static_cast<size_t>(ntask) + 1, 0);
     pairs.release();
@@ -62,7 +62,8 @@ void schur_pairs_device(hipStream_t st, const std::vector<int4>& tq, const std::
     d_bounds.alloc(static_cast<size_t>(ntask) + 1);
     const size_t wb = pair_scan_work(nslots + 1);
     work.alloc(std::max<size_t>(wb, 1));
-    launch_pair_count(st, ntask, d_tq.p, d_qoff.p, nslots, d_cam_off, d_cam_list, d_spt, d_pt_off, d_scam, cnt.p);
+    launch_pair_count(st, ntask, d_tq.p, d_qoff.p, nslots, d_cam_off, d_cam_list, d_spt, d_pt_off, d_scam, heavy_k,
+                      cnt.p);
     launch_pair_scan(st, cnt.p, pos.p, nslots + 1, work.p, wb);
     launch_pair_bounds(st, ntask, d_qoff.p, pos.p, d_bounds.p);
     GBA_HIP(hipMemcpyAsync(bounds.data(), d_bounds.p, sizeof(long long) * bounds.size(), hipMemcpyDeviceToHost,
@@ -72,7 +73,7 @@ void schur_pairs_device(hipStream_t st, const std::vector<int4>& tq, const std::
     if (bounds[ntask] > 0) {
         pairs.alloc(static_cast<size_t>(bounds[ntask]));
         launch_pair_fill(st, ntask, d_tq.p, d_qoff.p, nslots, d_cam_off, d_cam_list, d_spt, d_pt_off, d_scam,
-                         pos.p, pairs.p);
+                         heavy_k, pos.p, pairs.p);
     }
     GBA_HIP(hipGetLastError());
     GBA_HIP(hipStreamSynchronize(st));  // the temporaries are released on return
@@ -81,14 +82,14 @@ void schur_pairs_device(hipStream_t st, const std::vector<int4>& tq, const std::
 void build_schur_tasks_device(hipStream_t st, int C, int cb, const std::vector<int>& cam_off,
                               const int* d_cam_off, const int* d_cam_list, const int* d_spt,
                               const int* d_pt_off, const int* d_scam, std::vector<SchurTask>& tasks,
-                              DevBuf<int2>& pairs, bool with_diag) {
+                              DevBuf<int2>& pairs, bool with_diag, int heavy_k) {
     // tasks ordered by camera a, then column chunk
     const int dg = with_diag ? 1 : 0;
     std::vector<int4> tq;
     for (int a = 0; a < C; ++a)
         for (int lo = 0; lo < a + dg; lo += cb) tq.push_back(make_int4(a, lo, std::min(lo + cb, a + dg), 0));
     std::vector<long long> hb;
-    schur_pairs_device(st, tq, cam_off, d_cam_off, d_cam_list, d_spt, d_pt_off, d_scam, hb, pairs);
+    schur_pairs_device(st, tq, cam_off, d_cam_off, d_cam_list, d_spt, d_pt_off, d_scam, hb, pairs, heavy_k);
     const int ntask = static_cast<int>(tq.size());
     tasks.resize(static_cast<size_t>(ntask));
     for (int t = 0; t < ntask; ++t)
@@ -350,10 +351,58 @@ Solver::Solver(const gba_problem_view& prob, const gba_solver_config& cfg) : cfg
         pb_.cam_scale = cam_scale_.p;
     }
     upload(units_, units, st_);
+    // Heavy points: those seen at least heavy_k times, when they hold a fifth
+    // or more of the observation pairs, are assembled by one dense product on
+    // MFMA (k_heavy.hip) instead of the pair lists.  The threshold C / 3
+    // (32 at least) weighs the dense product's 9C x 9C x 3 multiply-adds per
+    // point against the k(k-1)/2 pairs of the sparse kernels (measured
+    // crossover at C=356 between k = 64 and k = 128).  GBA_SCHUR_HEAVY_K in
+    // the environment sets the threshold (0: off) and skips the share test.
+    {
+        int kh = std::max(32, (C + 2) / 3);
+        bool forced = false;
+        if (const char* env = std::getenv("GBA_SCHUR_HEAVY_K")) {
+            kh = std::max(0, std::atoi(env));
+            forced = true;
+        }
+        if (!sep_diag_ || env_) kh = 0;  // the rhs terms need the diagonal kernel
+        std::vector<int> hp;
+        if (kh > 0) {
+            double all = 0.0, heavy = 0.0;
+            for (int p = 0; p < P; ++p) {
+                const double k = pt_off[p + 1] - pt_off[p];
+                all += k * (k - 1) / 2;
+                if (k >= kh) {
+                    heavy += k * (k - 1) / 2;
+                    hp.push_back(p);
+                }
+            }
+            const size_t kp = (3 * hp.size() + 15) / 16 * 16;
+            const size_t bytes = kp * static_cast<size_t>(pb_.npad) * (mixed_ ? 4 : 8);
+            if (hp.empty() || (!forced && heavy < 0.2 * all) || bytes > (size_t{1} << 30)) {
+                kh = 0;
+                hp.clear();
+            }
+        }
+        pb_.heavy_k = kh;
+        nheavy_ = static_cast<int>(hp.size());
+        heavy_kp_ = (3 * nheavy_ + 15) / 16 * 16;
+        if (nheavy_ > 0) {
+            upload(hpts_, hp, st_);
+            const size_t ny = static_cast<size_t>(heavy_kp_) * pb_.npad;
+            if (mixed_) {
+                Yt32_.alloc(ny);
+                GBA_HIP(hipMemsetAsync(Yt32_.p, 0, sizeof(float) * ny, st_));
+            } else {
+                Yt64_.alloc(ny);
+                GBA_HIP(hipMemsetAsync(Yt64_.p, 0, sizeof(double) * ny, st_));
+            }
+        }
+    }
     // Schur task and pair lists, built on the device from the sorted layout
     std::vector<SchurTask> stasks;
     build_schur_tasks_device(st_, C, sshape_.cb, cam_off, cam_off_.p, cam_list_.p, spt_.p, pt_off_.p, scam_.p,
-                             stasks, spairs_, !sep_diag_);
+                             stasks, spairs_, !sep_diag_, pb_.heavy_k);
     num_stasks_ = static_cast<int>(stasks.size());
     for (SchurTask& t : stasks) t.ld = pb_.npad;
     upload(stasks_, stasks, st_);
@@ -541,6 +590,12 @@ void Solver::enqueue_schur(double lambda, bool fac_copy) {
         launch_schur(st_, sshape_, pb_, stasks_.p, num_stasks_, spairs_.p, W64_.p, W32_.p, Vinv_.p,
                  VB_.p, U_.p, bc_.p, lambda, mixed_ ? dscale_.p : nullptr, S64_.p, sfac,
                  slow, pb_.npad, rhs_.p, flags_.p);
+    if (nheavy_ > 0) {
+        // after both kernels above have written every block of S
+        launch_heavy_y(st_, pb_, hpts_.p, nheavy_, W64_.p, W32_.p, Vinv_.p, mixed_ ? dscale_.p : nullptr,
+                       Yt64_.p, Yt32_.p, pb_.npad);
+        launch_heavy_syrk(st_, nblk_, heavy_kp_, pb_.npad, Yt64_.p, Yt32_.p, S64_.p, sfac, slow, flags_.p);
+    }
 }
 
 void Solver::setup_cg() {

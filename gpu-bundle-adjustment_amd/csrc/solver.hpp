@@ -68,10 +68,11 @@ void encode_obs_fp16(const gba_problem_view& prob, float* cam_scale, uint16_t* p
 // camera(e) = a, camera(f) in [lo, hi), grouped by e (ascending point) and
 // ordered by f, built on the device from the sorted layout (d_*; cam_off on
 // the host too); task t's pairs are [bounds[t], bounds[t + 1]) of pairs
-// (allocated here).  Synchronises st.
+// (allocated here); points with at least heavy_k observations (heavy_k > 0)
+// contribute no pairs.  Synchronises st.
 void schur_pairs_device(hipStream_t st, const std::vector<int4>& tq, const std::vector<int>& cam_off,
                         const int* d_cam_off, const int* d_cam_list, const int* d_spt, const int* d_pt_off,
-                        const int* d_scam, std::vector<long long>& bounds, DevBuf<int2>& pairs);
+                        const int* d_scam, std::vector<long long>& bounds, DevBuf<int2>& pairs, int heavy_k = 0);
 // Schur task and pair lists of one problem for chunks of cb camera columns:
 // task (a, c) holds the pairs (e, f) with camera(e) = a and camera(f) in
 // [c cb, min((c + 1) cb, a + with_diag)); every (a, c) gets a task, empty or
@@ -81,7 +82,7 @@ void schur_pairs_device(hipStream_t st, const std::vector<int4>& tq, const std::
 void build_schur_tasks_device(hipStream_t st, int C, int cb, const std::vector<int>& cam_off,
                               const int* d_cam_off, const int* d_cam_list, const int* d_spt,
                               const int* d_pt_off, const int* d_scam, std::vector<SchurTask>& tasks,
-                              DevBuf<int2>& pairs, bool with_diag);
+                              DevBuf<int2>& pairs, bool with_diag, int heavy_k = 0);
 
 // FP64 preconditioned conjugate gradients on A x = b, A symmetric given by the
 // lower triangle of an FP32 matrix, preconditioned by (L L^T)^-1 (PcgPrecond:
@@ -236,6 +237,12 @@ private:
     DevBuf<SchurTask> stasks_;
     DevBuf<int2> spairs_;
     int num_stasks_ = 0;
+    // heavy points (pb_.heavy_k > 0): their list, the padded width of Y (3 per
+    // point) and Y itself, k-major (FP32 in the mixed form)
+    DevBuf<int> hpts_;
+    int nheavy_ = 0, heavy_kp_ = 0;
+    DevBuf<double> Yt64_;
+    DevBuf<float> Yt32_;
     DevBuf<double2> spx_;
     DevBuf<unsigned> spx16_;  // FP16 storage: packed pixel pair per sorted observation
     DevBuf<float> cam_scale_;
