@@ -345,7 +345,7 @@ def main():
     jacobi = s32.precond() == g.PRECOND_JACOBI
     k64 = ("edge_point", "edge_camera", "schur", "factor", "solve", "backsub")
     k32 = (("edge_point", "edge_camera", "schur", "cg_setup", "cg_step", "backsub") if jacobi else k64) + \
-        ("refine_residual",)
+        ("refine_residual",) + (("heavy_syrk",) if s32.heavy()[1] > 0 else ())
     if args.mode == "kernels":
         out = {"mode": "kernels", "config": {"cameras": C, "points": P, "observations": O}}
         for name, s, ks in (("fp64", s64, k64), ("mixed", s32, k32), ("mixed_factor", s32f, k64)):
@@ -467,10 +467,25 @@ def main():
                             f"over {km['cg_step']:.4f} ms"}
     schur_bytes = O * w_bytes + 72 * P + (n * n // 2) * 4
     schur_gbs = schur_bytes / (km["schur"] * 1e-3) / 1e9
-    roof_schur = {"kernel": "Schur assembly (k_schur_diag + k_schur), mixed form", "bound": "hbm",
-                  "achieved": round(schur_gbs, 1), "peak": PEAK["hbm_gbs"], "unit": "GB/s",
+    roof_schur = {"kernel": "Schur assembly (k_schur_diag + k_schur + k_heavy_y + k_heavy_syrk), mixed form",
+                  "bound": "hbm", "achieved": round(schur_gbs, 1), "peak": PEAK["hbm_gbs"], "unit": "GB/s",
                   "frac": round(schur_gbs / PEAK["hbm_gbs"], 4), "traffic": traffic_of("k_schur_C356_traffic.json"),
                   "basis": f"O*108 + 72P + n^2/2*4 = {schur_bytes} B per assembly over {km['schur']:.4f} ms"}
+    # the heavy points' dense product S -= Y Y^T inside the assembly (FP32
+    # MFMA): 2 x 64 x 64 x width flops per live lower tile, over its own
+    # HIP-event time
+    roof_syrk = None
+    hk, hn, hw = s32.heavy()
+    if hn > 0 and "heavy_syrk" in km:
+        nblk = npad // 64
+        syrk_flops = 2.0 * (nblk * (nblk + 1) // 2) * 64 * 64 * hw
+        syrk_tflops = syrk_flops / (km["heavy_syrk"] * 1e-3) / 1e12
+        roof_syrk = {"kernel": "heavy points' dense product S -= Y Y^T (k_heavy_syrk), mixed form", "bound": "mfma",
+                     "achieved": round(syrk_tflops, 3), "peak": PEAK["fp32_tflops"], "unit": "TFLOP/s",
+                     "frac": round(syrk_tflops / PEAK["fp32_tflops"], 4), "traffic": None,
+                     "basis": f"{hn} points seen >= {hk} times, width {hw}: 2 x 64^2 x {hw} flop x "
+                              f"{nblk * (nblk + 1) // 2} lower tiles = {syrk_flops:.4e} flop per launch over "
+                              f"{km['heavy_syrk']:.4f} ms"}
     # one residual of the FP64 reduced camera system (k_refine.hip: rotation
     # terms, point pass, camera chunks, rows): per observation 16 B of pixel
     # and 8 B of indices in each of the two passes, per point X, z (written
@@ -486,7 +501,8 @@ def main():
                          f"pass gathers X and z at random, so the kernels run well below the HBM rate"}
     # the bench line's roofline is the kernel family with the largest share of
     # the measured mixed solve; the others follow as roofline_<name>
-    entries = {"schur": roof_schur, "cg_refinement": roof_cg, "factor": roof_factor, "fp64_residual": roof_res}
+    entries = {"schur": roof_schur, "cg_refinement": roof_cg, "factor": roof_factor, "fp64_residual": roof_res,
+               "heavy_syrk": roof_syrk}
     dom_key = dominant if dominant in entries and entries[dominant] else ("cg_refinement" if jacobi else "factor")
     roof = dict(entries[dom_key], largest_share=dominant)
 

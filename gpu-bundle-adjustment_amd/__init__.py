@@ -27,7 +27,7 @@ PRECOND_AUTO, PRECOND_FACTOR, PRECOND_JACOBI = 0, 1, 2  # enum gba_mixed_precond
 TRACE_STRIDE = 6
 TRIAL_STATUS = {0: "accept", 1: "reject", 2: "nonspd_v", 3: "chol_fail", 4: "denom", 5: "nonfinite"}
 KERNELS = {"edge_point": 0, "edge_camera": 1, "schur": 2, "factor": 3, "solve": 4, "backsub": 5,
-           "cost": 6, "cg_step": 7, "cg_setup": 8, "refine_residual": 9}
+           "cost": 6, "cg_step": 7, "cg_setup": 8, "refine_residual": 9, "heavy_syrk": 10}
 
 
 class ProblemView(ctypes.Structure):
@@ -83,6 +83,7 @@ _PROTOS = {
                                               ctypes.c_int, _dp]),
     "gba_solver_stats": (ctypes.c_int, [ctypes.c_void_p, _dp, _ip, _ip]),
     "gba_solver_layout": (ctypes.c_int, [ctypes.c_void_p, _ip, _dp, _dp]),
+    "gba_solver_heavy": (ctypes.c_int, [ctypes.c_void_p, _ip, _ip, _ip]),
     "gba_solver_mixed_precond": (ctypes.c_int, [ctypes.c_void_p, _ip]),
     "gba_solver_destroy": (None, [ctypes.c_void_p]),
     "gba_batch_create": (ctypes.c_int, [ctypes.POINTER(ProblemView), ctypes.c_int,
@@ -336,6 +337,12 @@ class Solver:
         pc, n, npad = ctypes.c_double(), ctypes.c_int(), ctypes.c_int()
         _check(lib().gba_solver_stats(self._h, ctypes.byref(pc), ctypes.byref(n), ctypes.byref(npad)))
         return {"pair_count": pc.value, "n": n.value, "n_padded": npad.value}
+
+    def heavy(self):
+        """(heavy_k, heavy points, inner width of the dense product) of the Schur assembly."""
+        k, n, w = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _check(lib().gba_solver_heavy(self._h, ctypes.byref(k), ctypes.byref(n), ctypes.byref(w)))
+        return k.value, n.value, w.value
 
     def layout(self):
         """(LAYOUT_DENSE or LAYOUT_ENVELOPE, envelope work / dense work, camera-graph density)."""

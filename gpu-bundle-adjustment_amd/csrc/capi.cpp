@@ -149,6 +149,18 @@ int gba_solver_stats(gba_solver* s, double* pair_count, int* n, int* n_padded) {
     });
 }
 
+int gba_solver_heavy(gba_solver* s, int* heavy_k, int* heavy_points, int* width) {
+    return guarded([&] {
+        need(s, "gba_solver_heavy: null handle");
+        int k = 0, np = 0, w = 0;
+        s->s->heavy(&k, &np, &w);
+        if (heavy_k) *heavy_k = k;
+        if (heavy_points) *heavy_points = np;
+        if (width) *width = w;
+        return 0;
+    });
+}
+
 int gba_solver_layout(gba_solver* s, int* layout, double* factor_work_ratio, double* graph_density) {
     return guarded([&] {
         need(s, "gba_solver_layout: null handle");

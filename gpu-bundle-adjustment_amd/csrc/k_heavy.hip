@@ -73,64 +73,94 @@ __global__ __launch_bounds__(256) void k_heavy_y(DevProblem pb, const int* __res
     }
 }
 
-// S -= Y Y^T over the lower tiles: tile (I, J), I >= J, one workgroup each
-// (blockIdx = I (I + 1) / 2 + J).  The four wavefronts take the four 32x32
-// quarters of the tile, each a 2x2 set of 16x16 MFMA accumulators over k in
-// steps of four: lane l supplies A[l % 16][l / 16] = Y[row][k0 + l / 16] and
-// B[l / 16][l % 16] = Y[col][k0 + l / 16], i.e. 16 consecutive entries of one
-// k row of Yt -- coalesced loads straight from memory, four k steps of them
-// issued ahead of their MFMAs.  kp is a multiple of 16 (zero columns pad it).
-// A diagonal tile updates its lower triangle and the upper halves of the 9x9
-// camera blocks inside it (the layout every consumer of S reads); S2, when
-// given, is a second copy receiving the same values.
+// S -= Y Y^T over the lower tiles, 128x128 per workgroup: block (I2, J2),
+// I2 >= J2 (blockIdx = I2 (I2 + 1) / 2 + J2), covering the 64x64 tiles
+// (2 I2 + qr, 2 J2 + qc); wavefront w = 2 qr + qc forms that tile as a 4x4
+// set of 16x16 MFMA accumulators (a tile above the diagonal or past the last
+// tile row is skipped).  k runs in blocks of 16: the block's rows of Yt for
+// the 128 rows and the 128 columns (k-major, so 16 k rows of 128 consecutive
+// entries: 8 per thread, vector loads) are staged in LDS once for the four
+// wavefronts, the next block's loads issued before the current block's
+// MFMAs.  Fragments: lane l supplies A[l % 16][l / 16] = Y[row][k0 + l / 16]
+// and B[l / 16][l % 16] = Y[col][k0 + l / 16].  Yt has kp rows (a multiple of
+// 16) of ldy entries (a multiple of 128, zero past the matrix).  A diagonal
+// tile updates its lower triangle and the upper halves of the 9x9 camera
+// blocks inside it (the layout every consumer of S reads); S2, when given,
+// is a second copy receiving the same values.
+constexpr int kSyrkPad = 8;  // LDS row padding (bank spread across k rows)
 template <typename T>
-__global__ __launch_bounds__(256) void k_heavy_syrk(const T* __restrict__ Yt, int kp, int ld, T* __restrict__ S,
-                                                    T* __restrict__ S2, const TrialFlags* __restrict__ flags) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_heavy_syrk(
+    const T* __restrict__ Yt, int kp, int ldy, int nblk, int ld,
+                                                    T* __restrict__ S, T* __restrict__ S2,
+                                                    const TrialFlags* __restrict__ flags) {
+    __shared__ T s_a[16][128 + kSyrkPad], s_b[16][128 + kSyrkPad];
     if (flags->nonspd) return;
     typedef typename Acc4<T>::type acc_t;
     const int t = blockIdx.x;
-    int I = static_cast<int>((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
-    while ((I + 1) * (I + 2) / 2 <= t) ++I;
-    while (I * (I + 1) / 2 > t) --I;
-    const int J = t - I * (I + 1) / 2;
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int wr = w >> 1, wc = w & 1;
-    const int kk = lane >> 4, ii = lane & 15;
-    const T* ya = Yt + static_cast<size_t>(kk) * ld + kTile * I + 32 * wr + ii;
-    const T* yb = Yt + static_cast<size_t>(kk) * ld + kTile * J + 32 * wc + ii;
-    acc_t acc[2][2];
+    int I2 = static_cast<int>((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+    while ((I2 + 1) * (I2 + 2) / 2 <= t) ++I2;
+    while (I2 * (I2 + 1) / 2 > t) --I2;
+    const int J2 = t - I2 * (I2 + 1) / 2;
+    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    const int qr = w >> 1, qc = w & 1;
+    const int ti = 2 * I2 + qr, tj = 2 * J2 + qc;  // this wavefront's tile
+    const bool live = ti < nblk && tj < nblk && ti >= tj;
+    // staging: thread -> (k row lk, 8 entries from column c8)
+    const int lk = tid >> 4, c8 = 8 * (tid & 15);
+    const T* ga = Yt + static_cast<size_t>(lk) * ldy + 128 * I2 + c8;
+    const T* gb = Yt + static_cast<size_t>(lk) * ldy + 128 * J2 + c8;
+    T ra[8], rb[8];
+    auto fetch = [&](int k0) {
+        const size_t o = static_cast<size_t>(k0) * ldy;
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int b = 0; b < 2; ++b) acc[a][b] = acc_t{0, 0, 0, 0};
-    const size_t step = static_cast<size_t>(4) * ld;
-    for (int k0 = 0; k0 < kp; k0 += 16) {
-        T av[4][2], bv[4][2];
-#pragma unroll
-        for (int s = 0; s < 4; ++s) {
-            const size_t o = static_cast<size_t>(k0) * ld + s * step;
-            av[s][0] = ya[o];
-            av[s][1] = ya[o + 16];
-            bv[s][0] = yb[o];
-            bv[s][1] = yb[o + 16];
+        for (int u = 0; u < 8; ++u) {
+            ra[u] = ga[o + u];
+            rb[u] = gb[o + u];
         }
+    };
+    acc_t acc[4][4];
 #pragma unroll
-        for (int s = 0; s < 4; ++s)
+    for (int a = 0; a < 4; ++a)
 #pragma unroll
-            for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 4; ++b) acc[a][b] = acc_t{0, 0, 0, 0};
+    const int kk = lane >> 4, ii = lane & 15;
+    fetch(0);
+    for (int k0 = 0; k0 < kp; k0 += 16) {
+        __syncthreads();  // the previous block is consumed
 #pragma unroll
-                for (int b = 0; b < 2; ++b) acc[a][b] = mfma16(av[s][a], bv[s][b], acc[a][b]);
+        for (int u = 0; u < 8; ++u) {
+            s_a[lk][c8 + u] = ra[u];
+            s_b[lk][c8 + u] = rb[u];
+        }
+        __syncthreads();
+        if (k0 + 16 < kp) fetch(k0 + 16);
+        if (live) {
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                T av[4], bv[4];
+#pragma unroll
+                for (int a = 0; a < 4; ++a) {
+                    av[a] = s_a[4 * s + kk][64 * qr + 16 * a + ii];
+                    bv[a] = s_b[4 * s + kk][64 * qc + 16 * a + ii];
+                }
+#pragma unroll
+                for (int a = 0; a < 4; ++a)
+#pragma unroll
+                    for (int b = 0; b < 4; ++b) acc[a][b] = mfma16(av[a], bv[b], acc[a][b]);
+            }
+        }
     }
+    if (!live) return;
     const int g = lane >> 4, c16 = lane & 15;
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < 4; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b)
+        for (int b = 0; b < 4; ++b)
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                const int r = 32 * wr + 16 * a + acc_row<T>(g, q), c = 32 * wc + 16 * b + c16;
-                const int R = kTile * I + r, Cc = kTile * J + c;
-                if (I == J && r < c && R / 9 != Cc / 9) continue;
+                const int r = 16 * a + acc_row<T>(g, q), c = 16 * b + c16;
+                const int R = kTile * ti + r, Cc = kTile * tj + c;
+                if (ti == tj && r < c && R / 9 != Cc / 9) continue;
                 const size_t o = static_cast<size_t>(R) * ld + Cc;
                 S[o] = S[o] - acc[a][b][q];
                 if (S2) S2[o] = S2[o] - acc[a][b][q];
@@ -151,14 +181,16 @@ void launch_heavy_y(hipStream_t st, const DevProblem& pb, const int* hpts, int n
                            Yt64, ld);
 }
 
-void launch_heavy_syrk(hipStream_t st, int nblk, int kp, int ld, const double* Yt64, const float* Yt32, double* S64,
-                       float* S32, float* S32b, const TrialFlags* flags) {
-    const int ntiles = nblk * (nblk + 1) / 2;
-    if (ntiles <= 0 || kp <= 0) return;
+void launch_heavy_syrk(hipStream_t st, int nblk, int kp, int ldy, int ld, const double* Yt64, const float* Yt32,
+                       double* S64, float* S32, float* S32b, const TrialFlags* flags) {
+    const int n2 = (nblk + 1) / 2;
+    const int nbk = n2 * (n2 + 1) / 2;
+    if (nbk <= 0 || kp <= 0) return;
     if (Yt32)
-        hipLaunchKernelGGL(k_heavy_syrk<float>, dim3(ntiles), dim3(256), 0, st, Yt32, kp, ld, S32, S32b, flags);
+        hipLaunchKernelGGL(k_heavy_syrk<float>, dim3(nbk), dim3(256), 0, st, Yt32, kp, ldy, nblk, ld, S32, S32b,
+                           flags);
     else
-        hipLaunchKernelGGL(k_heavy_syrk<double>, dim3(ntiles), dim3(256), 0, st, Yt64, kp, ld, S64,
+        hipLaunchKernelGGL(k_heavy_syrk<double>, dim3(nbk), dim3(256), 0, st, Yt64, kp, ldy, nblk, ld, S64,
                            static_cast<double*>(nullptr), flags);
 }
 

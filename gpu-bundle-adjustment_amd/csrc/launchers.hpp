@@ -87,15 +87,16 @@ void launch_pair_fill(hipStream_t st, int ntask, const int4* tq, const long long
                       int heavy_k, const long long* pos, int2* pairs);
 
 // k_heavy.hip -- the heavy points' share of S (pb.heavy_k): Yt (k-major,
-// kp x ld, zero outside the observed blocks) receives D W_{a,p} L_p for heavy
-// point hpts[h] in columns 3h .. 3h + 2 (FP32 with W32 and Yt32, else FP64);
-// then S -= Y Y^T over the lower tiles (FP32 into S32 and, when given, the
+// kp x ldy with ldy = npad rounded up to 128, zero outside the observed
+// blocks) receives D W_{a,p} L_p for heavy point hpts[h] in columns
+// 3h .. 3h + 2 (FP32 with W32 and Yt32, else FP64); then S -= Y Y^T over the
+// lower tiles of S (leading dimension ld; FP32 into S32 and, when given, the
 // second copy S32b; FP64 into S64).
 void launch_heavy_y(hipStream_t st, const DevProblem& pb, const int* hpts, int nh, const double* W64,
                     const float* W32, const double* Vinv6, const double* dscale, double* Yt64, float* Yt32,
-                    int ld);
-void launch_heavy_syrk(hipStream_t st, int nblk, int kp, int ld, const double* Yt64, const float* Yt32, double* S64,
-                       float* S32, float* S32b, const TrialFlags* flags);
+                    int ldy);
+void launch_heavy_syrk(hipStream_t st, int nblk, int kp, int ldy, int ld, const double* Yt64, const float* Yt32,
+                       double* S64, float* S32, float* S32b, const TrialFlags* flags);
 
 // k_layout.hip: a batch's concatenated observations from problem-local to
 // batch camera / point indices (problem q owns observations [o_off[q],

@@ -353,13 +353,16 @@ Solver::Solver(const gba_problem_view& prob, const gba_solver_config& cfg) : cfg
     upload(units_, units, st_);
     // Heavy points: those seen at least heavy_k times, when they hold a fifth
     // or more of the observation pairs, are assembled by one dense product on
-    // MFMA (k_heavy.hip) instead of the pair lists.  The threshold C / 3
-    // (32 at least) weighs the dense product's 9C x 9C x 3 multiply-adds per
-    // point against the k(k-1)/2 pairs of the sparse kernels (measured
-    // crossover at C=356 between k = 64 and k = 128).  GBA_SCHUR_HEAVY_K in
-    // the environment sets the threshold (0: off) and skips the share test.
+    // MFMA (k_heavy.hip) instead of the pair lists.  The dense product costs
+    // (9C)^2 x 3 / 2 multiply-adds per point at about half the MFMA peak,
+    // whatever the point's k; the pair kernels cost k(k-1)/2 pairs.  Measured
+    // at C=356 (assembly per trial, mixed / FP64): none 0.93 / 1.29 ms,
+    // k >= 64 1.03 / 1.74, k >= 96 0.83 / 1.33, k >= 128 0.76 / 1.21,
+    // k >= 160 0.755 / 1.15, k >= 200 0.79 / 1.13; hence 0.45 C (32 at
+    // least).  GBA_SCHUR_HEAVY_K in the environment sets the threshold
+    // (0: off) and skips the share test.
     {
-        int kh = std::max(32, (C + 2) / 3);
+        int kh = std::max(32, (9 * C + 19) / 20);
         bool forced = false;
         if (const char* env = std::getenv("GBA_SCHUR_HEAVY_K")) {
             kh = std::max(0, std::atoi(env));
@@ -378,7 +381,7 @@ Solver::Solver(const gba_problem_view& prob, const gba_solver_config& cfg) : cfg
                 }
             }
             const size_t kp = (3 * hp.size() + 15) / 16 * 16;
-            const size_t bytes = kp * static_cast<size_t>(pb_.npad) * (mixed_ ? 4 : 8);
+            const size_t bytes = kp * static_cast<size_t>(pb_.npad + 127) * (mixed_ ? 4 : 8);
             if (hp.empty() || (!forced && heavy < 0.2 * all) || bytes > (size_t{1} << 30)) {
                 kh = 0;
                 hp.clear();
@@ -387,9 +390,10 @@ Solver::Solver(const gba_problem_view& prob, const gba_solver_config& cfg) : cfg
         pb_.heavy_k = kh;
         nheavy_ = static_cast<int>(hp.size());
         heavy_kp_ = (3 * nheavy_ + 15) / 16 * 16;
+        heavy_ld_ = (pb_.npad + 127) / 128 * 128;
         if (nheavy_ > 0) {
             upload(hpts_, hp, st_);
-            const size_t ny = static_cast<size_t>(heavy_kp_) * pb_.npad;
+            const size_t ny = static_cast<size_t>(heavy_kp_) * heavy_ld_;
             if (mixed_) {
                 Yt32_.alloc(ny);
                 GBA_HIP(hipMemsetAsync(Yt32_.p, 0, sizeof(float) * ny, st_));
@@ -593,8 +597,9 @@ void Solver::enqueue_schur(double lambda, bool fac_copy) {
     if (nheavy_ > 0) {
         // after both kernels above have written every block of S
         launch_heavy_y(st_, pb_, hpts_.p, nheavy_, W64_.p, W32_.p, Vinv_.p, mixed_ ? dscale_.p : nullptr,
-                       Yt64_.p, Yt32_.p, pb_.npad);
-        launch_heavy_syrk(st_, nblk_, heavy_kp_, pb_.npad, Yt64_.p, Yt32_.p, S64_.p, sfac, slow, flags_.p);
+                       Yt64_.p, Yt32_.p, heavy_ld_);
+        launch_heavy_syrk(st_, nblk_, heavy_kp_, heavy_ld_, pb_.npad, Yt64_.p, Yt32_.p, S64_.p, sfac, slow,
+                          flags_.p);
     }
 }
 
@@ -1060,6 +1065,13 @@ double Solver::time_kernel(int kernel, double lambda, int reps) {
             case GBA_KERNEL_SCHUR:
                 enqueue_schur(lambda);
                 break;
+            case GBA_KERNEL_HEAVY_SYRK: {
+                float* sfac = mixed_ ? (cg_active() ? S32low_.p : S32_.p) : nullptr;
+                float* slow = mixed_ && !cg_active() ? S32low_.p : nullptr;
+                launch_heavy_syrk(st_, nblk_, heavy_kp_, heavy_ld_, pb_.npad, Yt64_.p, Yt32_.p, S64_.p, sfac, slow,
+                                  flags_.p);
+                break;
+            }
             case GBA_KERNEL_FACTOR:
                 // mixed: factor + explicit inverse (what a trial runs before PCG)
                 enqueue_factor();
@@ -1113,6 +1125,8 @@ double Solver::time_kernel(int kernel, double lambda, int reps) {
     (void)trial(lambda);
     if ((kernel == GBA_KERNEL_CG_STEP || kernel == GBA_KERNEL_CG_SETUP) && !cg_active())
         throw Error("the block-Jacobi refinement is not in use on this handle");
+    if (kernel == GBA_KERNEL_HEAVY_SYRK && nheavy_ == 0)
+        throw Error("this handle's Schur assembly has no heavy points");
     if (kernel == GBA_KERNEL_CG_STEP) {
         check_cg_args(cgt);
         // steps that never stop: no tolerance, no cap

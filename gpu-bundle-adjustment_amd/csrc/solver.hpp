@@ -190,6 +190,11 @@ public:
     }
     int n() const { return pb_.n; }
     // mixed refinement preconditioner of the handle (enum gba_mixed_precond)
+    void heavy(int* k, int* np, int* w) const {
+        *k = pb_.heavy_k;
+        *np = nheavy_;
+        *w = heavy_kp_;
+    }
     int mixed_precond() const { return !mixed_ ? 0 : (cg_mode_ ? GBA_PRECOND_JACOBI : GBA_PRECOND_FACTOR); }
     int npad() const { return pb_.npad; }
 
@@ -240,7 +245,7 @@ private:
     // heavy points (pb_.heavy_k > 0): their list, the padded width of Y (3 per
     // point) and Y itself, k-major (FP32 in the mixed form)
     DevBuf<int> hpts_;
-    int nheavy_ = 0, heavy_kp_ = 0;
+    int nheavy_ = 0, heavy_kp_ = 0, heavy_ld_ = 0;
     DevBuf<double> Yt64_;
     DevBuf<float> Yt32_;
     DevBuf<double2> spx_;

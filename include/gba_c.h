@@ -201,9 +201,12 @@ enum gba_kernel_id {
     GBA_KERNEL_COST = 6,        /* residual-only cost pass */
     GBA_KERNEL_CG_STEP = 7,     /* mixed, block Jacobi: one refinement step (matrix product
                                    over S + vector update) */
-    GBA_KERNEL_CG_SETUP = 8,    /* mixed, block Jacobi: inverses of the camera blocks */
-    GBA_KERNEL_REFINE_RESIDUAL = 9 /* mixed: one residual of the FP64 reduced camera system
+    GBA_KERNEL_CG_SETUP = 8,    /* mixed, block Jacobi: inverses of the camera blocks and
+                                   the gauge deflation's basis, A Z and (Z^T A Z)^-1 */
+    GBA_KERNEL_REFINE_RESIDUAL = 9, /* mixed: one residual of the FP64 reduced camera system
                                       (matrix-free: point pass, camera chunks, rows) */
+    GBA_KERNEL_HEAVY_SYRK = 10  /* the heavy points' dense product S -= Y Y^T alone (part of
+                                   GBA_KERNEL_SCHUR; a handle without heavy points: error) */
 };
 int gba_solver_time_kernel(gba_solver* s, int kernel, double lambda, int reps, double* avg_ms);
 /* Shape facts used for roofline accounting: sum_p k_p(k_p+1)/2, matrix
@@ -217,6 +220,13 @@ int gba_solver_layout(gba_solver* s, int* layout, double* factor_work_ratio, dou
 /* The preconditioner of the handle's mixed refinement (enum gba_mixed_precond:
  * FACTOR or JACOBI, AUTO resolved; 0 for a plain FP64 handle). */
 int gba_solver_mixed_precond(gba_solver* s, int* mode);
+/* The heavy points of the handle's Schur assembly (no reference counterpart;
+ * the reference sums every observation pair, src/lm.cu:515-559): points seen
+ * at least heavy_k times (0: none) whose share of S is one dense MFMA product
+ * instead of pair sums, their count, and the product's padded inner width
+ * (3 per point, rounded up to 16).  GBA_SCHUR_HEAVY_K in the environment sets
+ * heavy_k when the handle is created (0: off).  Any output may be NULL. */
+int gba_solver_heavy(gba_solver* s, int* heavy_k, int* heavy_points, int* width);
 void gba_solver_destroy(gba_solver* s);
 
 /* ---- batched solve (no reference counterpart) --------------------------- */

@@ -71,3 +71,17 @@ def test_heavy_lm_matches_oracle(gba, orc, monkeypatch):
     assert abs(r.final_cost - o["final_cost"]) <= 1e-9 * o["final_cost"]
     m = gba.solve_bal_gpu(prob.copy(), opts, precision=gba.PRECISION_MIXED)
     assert abs(m.final_cost - o["final_cost"]) <= 1e-6 * o["final_cost"]
+
+
+def test_heavy_handle_query_and_timer(gba, monkeypatch):
+    prob = gba.generate(2, 1)
+    monkeypatch.setenv("GBA_SCHUR_HEAVY_K", "8")
+    s = gba.Solver(prob, gba.PRECISION_MIXED)
+    k, n, w = s.heavy()
+    assert k == 8 and n == int((np.bincount(prob.obs_point) >= 8).sum()) and w % 16 == 0 and 3 * n <= w < 3 * n + 16
+    assert s.time_kernel("heavy_syrk", 1e-4, 3) > 0.0
+    monkeypatch.setenv("GBA_SCHUR_HEAVY_K", "0")
+    s0 = gba.Solver(prob, gba.PRECISION_FP64)
+    assert s0.heavy() == (0, 0, 0)
+    with pytest.raises(RuntimeError):
+        s0.time_kernel("heavy_syrk", 1e-4, 1)
