@@ -85,3 +85,19 @@ def test_heavy_handle_query_and_timer(gba, monkeypatch):
     assert s0.heavy() == (0, 0, 0)
     with pytest.raises(RuntimeError):
         s0.time_kernel("heavy_syrk", 1e-4, 1)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_heavy_huber_lm_matches_oracle(gba, orc, monkeypatch, prec):
+    """With the Huber loss the heavy points' Y is formed from the IRLS-weighted
+    W and V' like the pair sums: the robust LM trajectory is the oracle's."""
+    prob = gba.generate(2, 1)
+    monkeypatch.setenv("GBA_SCHUR_HEAVY_K", "8")
+    opts = gba.LMOptions(initial_lambda=1e-4)
+    p = gba.PRECISION_FP64 if prec == "fp64" else gba.PRECISION_MIXED
+    r = gba.solve_bal_gpu(prob.copy(), opts, precision=p, huber_delta=1.0)
+    o = orc.lm_solve(prob.copy(), initial_lambda=1e-4, huber_delta=1.0)
+    assert r.termination == o["termination"]
+    assert abs(r.final_cost - o["final_cost"]) <= (1e-9 if prec == "fp64" else 1e-6) * o["final_cost"]
+    if prec == "fp64":
+        assert r.iterations == o["iterations"]
