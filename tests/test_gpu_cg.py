@@ -169,3 +169,23 @@ def test_cg_gauge_deflation(gba, which):
     assert d.refine_iterations < 0.7 * p.refine_iterations
     d2 = ds.solve(opts)  # fixed-order sums: repeatable bit for bit
     assert d2.final_cost == d.final_cost and d2.refine_iterations == d.refine_iterations
+
+
+def test_cg_deflation_rank_deficient_basis_is_plain_cg(gba):
+    """One camera: its scale column lies in the span of its translation
+    columns, so Z^T A Z is singular, fails the pivot test and leaves
+    E^-1 = 0 -- the deflated kernels then compute exactly the plain block-
+    Jacobi iterates (x_0 = 0, r_0 = b, mu = 0), bit for bit."""
+    p = gba.generate(2, 1)
+    keep = p.obs_camera < 1
+    used = np.unique(p.obs_point[keep])
+    remap = -np.ones(p.num_points, np.int32)
+    remap[used] = np.arange(used.size, dtype=np.int32)
+    q = gba.BALProblem(p.cameras[:1].copy(), p.points[used].copy(), p.obs_camera[keep].copy(),
+                       remap[p.obs_point[keep]].copy(), p.obs_pixel[keep].copy())
+    opts = gba.LMOptions(initial_lambda=1e-4, max_iterations=5)
+    d = gba.Solver(q, gba.PRECISION_MIXED, precond=gba.PRECOND_JACOBI).solve(opts)
+    pl = gba.Solver(q, gba.PRECISION_MIXED, precond=gba.PRECOND_JACOBI, deflation=False).solve(opts)
+    assert d.refine_iterations == pl.refine_iterations > 0
+    assert d.final_cost == pl.final_cost and d.iterations == pl.iterations
+    assert np.array_equal(d.trace[:, [1, 2, 4]], pl.trace[:, [1, 2, 4]])
